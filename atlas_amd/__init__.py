@@ -1,0 +1,194 @@
+"""atlas_amd -- Python binding of libatlas_mi355x (the MI355X Level-3 library).
+
+The product is the C-ABI shared library ``atlas_amd/libatlas_mi355x.so``
+(cblas_/clapack_ entry points plus the device-pointer atlmx_ extensions in
+include/atlmx.h).  This module only loads it with ctypes and offers thin
+numpy (host pointers, through the cblas/clapack ABI) and torch (device
+pointers, through atlmx_*) wrappers.  There is no fallback: if the library
+is missing, every call raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libatlas_mi355x.so")
+
+RowMajor, ColMajor = 101, 102
+NoTrans, Trans, ConjTrans = 111, 112, 113
+Upper, Lower = 121, 122
+NonUnit, Unit = 131, 132
+Left, Right = 141, 142
+
+PROF_GEMM, PROF_PANEL, PROF_TRSM_INV, PROF_LASWP, PROF_POTF2, PROF_MISC = range(6)
+
+_lib = None
+
+
+def lib():
+    """Load libatlas_mi355x.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libatlas_mi355x.so is not built: run `python -m atlas_amd.build`"
+                               " (or __graft_entry__.build())")
+        L = C.CDLL(LIBPATH, mode=C.RTLD_GLOBAL)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+_I, _D, _F, _P, _L = C.c_int, C.c_double, C.c_float, C.c_void_p, C.c_long
+
+
+def _declare(L):
+    for p, t in (("d", _D), ("s", _F)):
+        f = getattr(L, "cblas_%sgemm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I, t, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%strsm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%ssyrk" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, t, _P, _I, t, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_i%samax" % p)
+        f.argtypes = [_I, _P, _I]
+        f.restype = _I
+        f = getattr(L, "clapack_%sgetrf" % p)
+        f.argtypes = [_I, _I, _I, _P, _I, _P]
+        f.restype = _I
+        f = getattr(L, "clapack_%sgetrs" % p)
+        f.argtypes = [_I, _I, _I, _I, _P, _I, _P, _P, _I]
+        f.restype = _I
+        f = getattr(L, "clapack_%spotrf" % p)
+        f.argtypes = [_I, _I, _I, _P, _I]
+        f.restype = _I
+        f = getattr(L, "clapack_%spotrs" % p)
+        f.argtypes = [_I, _I, _I, _I, _P, _I, _P, _I]
+        f.restype = _I
+        f = getattr(L, "clapack_%sgesv" % p)
+        f.argtypes = [_I, _I, _I, _P, _I, _P, _P, _I]
+        f.restype = _I
+        f = getattr(L, "clapack_%sposv" % p)
+        f.argtypes = [_I, _I, _I, _I, _P, _I, _P, _I]
+        f.restype = _I
+        # device-pointer API
+        f = getattr(L, "atlmx_%sgemm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, t, _P, _I, _P, _I, t, _P, _I]
+        f.restype = None
+        f = getattr(L, "atlmx_%strsm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I]
+        f.restype = None
+        f = getattr(L, "atlmx_%ssyrk" % p)
+        f.argtypes = [_I, _I, _I, _I, t, _P, _I, t, _P, _I]
+        f.restype = None
+        f = getattr(L, "atlmx_%sgetrf" % p)
+        f.argtypes = [_I, _I, _P, _I, _P]
+        f.restype = _I
+        f = getattr(L, "atlmx_%sgetrs" % p)
+        f.argtypes = [_I, _I, _I, _P, _I, _P, _P, _I]
+        f.restype = None
+        f = getattr(L, "atlmx_%spotrf" % p)
+        f.argtypes = [_I, _I, _P, _I]
+        f.restype = _I
+        f = getattr(L, "atlmx_%spotrs" % p)
+        f.argtypes = [_I, _I, _I, _P, _I, _P, _I]
+        f.restype = None
+    L.atlmx_init.restype = _I
+    L.atlmx_set_stream.argtypes = [_P]
+    L.atlmx_get_stream.restype = _P
+    L.atlmx_prof_enable.argtypes = [_I]
+    L.atlmx_prof_query.argtypes = [_I, _P, _P, _P]
+    L.atlmx_prof_query.restype = _I
+
+
+def _pre(dt):
+    dt = np.dtype(dt)
+    if dt == np.float64:
+        return "d"
+    if dt == np.float32:
+        return "s"
+    raise TypeError("unsupported dtype %s" % dt)
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _ld(a):
+    if a.flags.f_contiguous:
+        return max(a.shape[0], 1)
+    assert a.strides[0] == a.itemsize, "need column-major storage"
+    return a.strides[1] // a.itemsize
+
+
+# --------------------------------------------------------------------------
+# host (numpy, column-major) wrappers of the ATLAS C ABI
+# --------------------------------------------------------------------------
+def gemm(ta, tb, alpha, A, B, beta, Cm):
+    """Cm (Fortran-ordered, in place) = alpha op(A) op(B) + beta Cm."""
+    M, N = Cm.shape
+    K = A.shape[1] if ta == NoTrans else A.shape[0]
+    getattr(lib(), "cblas_%sgemm" % _pre(Cm.dtype))(
+        ColMajor, ta, tb, M, N, K, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(Cm),
+        _ld(Cm))
+    return Cm
+
+
+def trsm(side, uplo, ta, diag, alpha, A, B):
+    M, N = B.shape
+    getattr(lib(), "cblas_%strsm" % _pre(B.dtype))(
+        ColMajor, side, uplo, ta, diag, M, N, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B))
+    return B
+
+
+def getrf(A):
+    M, N = A.shape
+    ipiv = np.zeros(max(min(M, N), 1), dtype=np.int32)
+    info = getattr(lib(), "clapack_%sgetrf" % _pre(A.dtype))(ColMajor, M, N, _ptr(A), _ld(A),
+                                                             _ptr(ipiv))
+    return info, ipiv[:min(M, N)]
+
+
+def getrs(LU, ipiv, B, trans=NoTrans):
+    N = LU.shape[0]
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    return getattr(lib(), "clapack_%sgetrs" % _pre(B.dtype))(
+        ColMajor, trans, N, B.shape[1], _ptr(LU), _ld(LU), _ptr(ipiv), _ptr(B), _ld(B))
+
+
+def potrf(A, uplo=Lower):
+    return getattr(lib(), "clapack_%spotrf" % _pre(A.dtype))(ColMajor, uplo, A.shape[0],
+                                                             _ptr(A), _ld(A))
+
+
+def potrs(L, B, uplo=Lower):
+    return getattr(lib(), "clapack_%spotrs" % _pre(B.dtype))(
+        ColMajor, uplo, L.shape[0], B.shape[1], _ptr(L), _ld(L), _ptr(B), _ld(B))
+
+
+# --------------------------------------------------------------------------
+# device (torch) helpers
+# --------------------------------------------------------------------------
+def use_torch_stream():
+    """Order library work on torch's current CUDA(HIP) stream."""
+    import torch
+    lib().atlmx_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream))
+
+
+def dptr(t):
+    return C.c_void_p(t.data_ptr())
+
+
+def prof_enable(on=True):
+    lib().atlmx_prof_enable(1 if on else 0)
+
+
+def prof_query(cls):
+    ms = C.c_double(0)
+    n = C.c_long(0)
+    w = C.c_double(0)
+    lib().atlmx_prof_query(cls, C.byref(ms), C.byref(n), C.byref(w))
+    return ms.value, n.value, w.value

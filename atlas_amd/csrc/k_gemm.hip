@@ -1,0 +1,287 @@
+/*
+ * k_gemm.hip -- gfx950 MFMA GEMM: C = alpha*op(A)*op(B) + beta*C.
+ *
+ * Replaces ATLAS's access-major engine (src/blas/ammm/ATL_ammm.c:12-356,
+ * the loop nests of ATL_iploops.c:19-735 and the generated ammkern_t
+ * microkernels, include/atlas_amm.h:46) and its pthread decomposition
+ * (src/threads/blas/level3/ATL_tammm_G.c:211-420).  Instead of packing
+ * op(A)/op(B) into access-major blocks with a copy pass, each workgroup
+ * streams its K-slices of op(A) and op(B) from HBM straight into a
+ * double-buffered LDS tile (the "copy" of ATL_iploops.c:397-415 is the
+ * global->LDS load), and the C block stays in registers until one fused
+ * alpha/beta epilogue (the blk2c of ATL_iploops.c:416-417).  Tile
+ * scheduling is on the GPU: one workgroup per 128x128 C tile, numbered so
+ * that each of the 8 XCDs works on a compact group of tiles.
+ *
+ * Tiling (both precisions): 256 threads = 4 wavefronts in a 2x2 grid; each
+ * wavefront owns a 64x64 block of C = 4x4 MFMA tiles of 16x16
+ * (v_mfma_f64_16x16x4f64 or v_mfma_f32_16x16x4f32).  LDS holds op(A) as
+ * [m][k] and op(B) as [n][k] (k contiguous, row padded), so each lane reads
+ * its four K values of an MFMA operand with one 16/32-byte LDS load.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "kernels.h"
+
+namespace {
+
+constexpr int BM = ATLMX_GEMM_BM;
+constexpr int BN = ATLMX_GEMM_BN;
+constexpr int BK = 16;
+constexpr int NT = 256;
+constexpr int GROUP_M = 8;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4 acc_t;
+  typedef d2 vec_t;                 /* one 16-byte global load */
+  static constexpr int VEC = 2;
+  static constexpr int PADK = 2;    /* LDS row = 18 doubles = 144 B */
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  /* v_mfma_f64_16x16x4f64: lane l, element r holds C[(l>>4) + 4r][l&15] */
+  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct Mfma<float> {
+  typedef f4 acc_t;
+  typedef f4 vec_t;
+  static constexpr int VEC = 4;
+  static constexpr int PADK = 4;    /* LDS row = 20 floats = 80 B */
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  /* v_mfma_f32_16x16x4f32: lane l, element r holds C[4(l>>4) + r][l&15] */
+  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T>
+struct GemmArgs {
+  int M, N, K;
+  T alpha, beta;
+  const T *A; long lda;
+  const T *B; long ldb;
+  T *C; long ldc;
+  int tri;
+  int tiles_m, tiles_n;
+  int per_xcd;            /* tiles per XCD slot in the remapped numbering */
+  int vec_a, vec_b;       /* 16-byte vector loads allowed */
+  const int *abort_flag;
+};
+
+/* Load one BKxBM (or BKxBN) slice of an operand into registers.
+ * KCONT: the operand's K index is contiguous in memory (op(A) with A
+ * transposed, or op(B) with B not transposed).  rows = extent in M (or N).
+ * Element (r, k) of the slice lives at P[k + r*ld] when KCONT, else at
+ * P[r + k*ld]. */
+template <typename T, bool KCONT, int ROWS>
+struct SliceLoader {
+  typedef Mfma<T> MF;
+  static constexpr int VEC = MF::VEC;
+  static constexpr int NV = ROWS * BK / VEC / NT;   /* vectors per thread */
+  typename MF::vec_t v[NV];
+
+  __device__ void load(const T *P, long ld, int r0, int k0, int rows, int K,
+                       bool vec_ok, int tid) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int lin = tid + NT * i;
+      int r, k;
+      if (KCONT) { r = lin / (BK / VEC); k = (lin % (BK / VEC)) * VEC; }
+      else       { k = lin / (ROWS / VEC); r = (lin % (ROWS / VEC)) * VEC; }
+      const int gr = r0 + r, gk = k0 + k;
+      if (KCONT) {
+        if (vec_ok && gr < rows && gk + VEC <= K) {
+          v[i] = *reinterpret_cast<const typename MF::vec_t *>(P + (size_t)gr * ld + gk);
+        } else {
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            v[i][e] = (gr < rows && gk + e < K) ? P[(size_t)gr * ld + gk + e] : T(0);
+        }
+      } else {
+        if (vec_ok && gk < K && gr + VEC <= rows) {
+          v[i] = *reinterpret_cast<const typename MF::vec_t *>(P + (size_t)gk * ld + gr);
+        } else {
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            v[i][e] = (gk < K && gr + e < rows) ? P[(size_t)gk * ld + gr + e] : T(0);
+        }
+      }
+    }
+  }
+
+  /* S is the [ROWS][BK+PADK] LDS tile */
+  __device__ void store(T *S, int tid) const {
+    constexpr int LDK = BK + MF::PADK;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+      const int lin = tid + NT * i;
+      if (KCONT) {
+        const int r = lin / (BK / VEC), k = (lin % (BK / VEC)) * VEC;
+        *reinterpret_cast<typename MF::vec_t *>(S + r * LDK + k) = v[i];
+      } else {
+        const int k = lin / (ROWS / VEC), r = (lin % (ROWS / VEC)) * VEC;
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) S[(r + e) * LDK + k] = v[i][e];
+      }
+    }
+  }
+};
+
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> g) {
+  typedef Mfma<T> MF;
+  typedef typename MF::acc_t acc_t;
+  constexpr int LDK = BK + MF::PADK;
+  constexpr bool A_KCONT = (TA != KX_NOTRANS);   /* op(A)[m][k] = A[k + m*lda] */
+  constexpr bool B_KCONT = (TB == KX_NOTRANS);   /* op(B)[k][n] = B[k + n*ldb] */
+
+  if (g.abort_flag && *g.abort_flag) return;
+
+  /* ---- tile scheduling: XCD-compact numbering, then grouped raster ---- */
+  const int bid = blockIdx.x;
+  const int tile = (bid & 7) * g.per_xcd + (bid >> 3);
+  const int ntiles = g.tiles_m * g.tiles_n;
+  if (tile >= ntiles) return;
+  const int per_group = GROUP_M * g.tiles_n;
+  const int group = tile / per_group;
+  const int first_m = group * GROUP_M;
+  const int gsz = min(g.tiles_m - first_m, GROUP_M);
+  const int tm = first_m + (tile % per_group) % gsz;
+  const int tn = (tile % per_group) / gsz;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  if (g.tri == KX_LOWER && m0 + BM - 1 < n0) return;   /* tile above diagonal */
+  if (g.tri == KX_UPPER && n0 + BN - 1 < m0) return;   /* tile below diagonal */
+
+  __shared__ __attribute__((aligned(16))) T As[2][BM * LDK];
+  __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDK];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+  SliceLoader<T, A_KCONT, BM> la;
+  SliceLoader<T, B_KCONT, BN> lb;
+  const int nk = (g.K + BK - 1) / BK;
+
+  if (nk > 0) {
+    la.load(g.A, g.lda, m0, 0, g.M, g.K, g.vec_a, tid);
+    lb.load(g.B, g.ldb, n0, 0, g.N, g.K, g.vec_b, tid);
+    la.store(As[0], tid);
+    lb.store(Bs[0], tid);
+  }
+  __syncthreads();
+
+  const int kq = (lane >> 4) * 4;        /* this lane's 4 K values in a slice */
+  const int lr = lane & 15;
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) {
+      la.load(g.A, g.lda, m0, (kt + 1) * BK, g.M, g.K, g.vec_a, tid);
+      lb.load(g.B, g.ldb, n0, (kt + 1) * BK, g.N, g.K, g.vec_b, tid);
+    }
+    T a[4][4], b[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const T *pa = &As[cur][(wm + i * 16 + lr) * LDK + kq];
+      const T *pb = &Bs[cur][(wn + i * 16 + lr) * LDK + kq];
+#pragma unroll
+      for (int s = 0; s < 4; ++s) { a[i][s] = pa[s]; b[i][s] = pb[s]; }
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(a[i][s], b[j][s], acc[i][j]);
+    if (kt + 1 < nk) {
+      la.store(As[cur ^ 1], tid);
+      lb.store(Bs[cur ^ 1], tid);
+    }
+    __syncthreads();
+  }
+
+  /* ---- fused epilogue: C = alpha*acc + beta*C ---- */
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + MF::row(lane, r);
+        const int col = n0 + wn + j * 16 + lr;
+        if (row < g.M && col < g.N) {
+          if (g.tri == KX_LOWER && row < col) continue;
+          if (g.tri == KX_UPPER && row > col) continue;
+          T *c = g.C + (size_t)col * g.ldc + row;
+          T v = g.alpha * acc[i][j][r];
+          if (g.beta != T(0)) v += g.beta * *c;
+          *c = v;
+        }
+      }
+}
+
+template <typename T>
+hipError_t launch_gemm(int ta, int tb, int M, int N, int K, T alpha, const T *A,
+                       long lda, const T *B, long ldb, T beta, T *C, long ldc,
+                       int tri, const int *abort_flag, hipStream_t s) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  if (ta != KX_NOTRANS && ta != KX_TRANS) return hipErrorInvalidValue;
+  if (tb != KX_NOTRANS && tb != KX_TRANS) return hipErrorInvalidValue;
+  if (K < 0 || ldc < M) return hipErrorInvalidValue;
+  if (K > 0 && (!A || !B)) return hipErrorInvalidValue;
+  if (K > 0 && lda < (ta == KX_NOTRANS ? M : K)) return hipErrorInvalidValue;
+  if (K > 0 && ldb < (tb == KX_NOTRANS ? K : N)) return hipErrorInvalidValue;
+  GemmArgs<T> g;
+  g.M = M; g.N = N; g.K = K; g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.tri = tri; g.abort_flag = abort_flag;
+  g.tiles_m = (M + BM - 1) / BM;
+  g.tiles_n = (N + BN - 1) / BN;
+  const long ntiles = (long)g.tiles_m * g.tiles_n;
+  if (ntiles > (1L << 30)) return hipErrorInvalidValue;
+  g.per_xcd = (int)((ntiles + 7) / 8);
+  constexpr int VEC = Mfma<T>::VEC;
+  g.vec_a = (((uintptr_t)A % 16) == 0) && (lda % VEC == 0);
+  g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % VEC == 0);
+  const dim3 grid((unsigned)(8 * g.per_xcd)), block(NT);
+  if (ta == KX_NOTRANS && tb == KX_NOTRANS)
+    hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_NOTRANS>), grid, block, 0, s, g);
+  else if (ta == KX_NOTRANS)
+    hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_TRANS>), grid, block, 0, s, g);
+  else if (tb == KX_NOTRANS)
+    hipLaunchKernelGGL((gemm_kernel<T, KX_TRANS, KX_NOTRANS>), grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_kernel<T, KX_TRANS, KX_TRANS>), grid, block, 0, s, g);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" hipError_t atlmx_k_dgemm(int ta, int tb, int M, int N, int K, double alpha,
+                                    const double *A, long lda, const double *B, long ldb,
+                                    double beta, double *C, long ldc, int tri,
+                                    const int *abort_flag, hipStream_t s) {
+  return launch_gemm<double>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri,
+                             abort_flag, s);
+}
+
+extern "C" hipError_t atlmx_k_sgemm(int ta, int tb, int M, int N, int K, float alpha,
+                                    const float *A, long lda, const float *B, long ldb,
+                                    float beta, float *C, long ldc, int tri,
+                                    const int *abort_flag, hipStream_t s) {
+  return launch_gemm<float>(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri,
+                            abort_flag, s);
+}

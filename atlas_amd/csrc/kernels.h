@@ -1,0 +1,105 @@
+/*
+ * kernels.h -- the thin C-ABI between the C host layer and the gfx950 HIP
+ * kernels of libatlas_mi355x.  Every launcher takes device pointers, 64-bit
+ * leading dimensions and the stream to launch on, checks the launch shape
+ * against what its kernel assumes, and returns a hipError_t (0 = success).
+ * Matrices are column-major.
+ */
+#ifndef ATLMX_KERNELS_H
+#define ATLMX_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Operand codes used by the launchers (values of ATLAS include/cblas.h:5-10). */
+#define KX_NOTRANS 111
+#define KX_TRANS   112
+#define KX_UPPER   121
+#define KX_LOWER   122
+#define KX_NONUNIT 131
+#define KX_UNIT    132
+#define KX_LEFT    141
+#define KX_RIGHT   142
+
+/* C = alpha*op(A)*op(B) + beta*C on the MFMA GEMM kernel.
+ * tri: 0 = whole C; KX_LOWER / KX_UPPER = only elements on or below / above
+ * the diagonal of C are read and written (the SYRK shape).
+ * abort_flag: NULL, or a device int; the kernel does nothing when it is
+ * nonzero (lets a stopped Cholesky skip the rest of its queued work). */
+hipError_t atlmx_k_dgemm(int ta, int tb, int M, int N, int K, double alpha,
+                         const double *A, long lda, const double *B, long ldb,
+                         double beta, double *C, long ldc, int tri,
+                         const int *abort_flag, hipStream_t s);
+hipError_t atlmx_k_sgemm(int ta, int tb, int M, int N, int K, float alpha,
+                         const float *A, long lda, const float *B, long ldb,
+                         float beta, float *C, long ldc, int tri,
+                         const int *abort_flag, hipStream_t s);
+
+/* Largest row count one in-place GEMM tile holds: an in-place product
+ * B := inv(T)*B with M <= ATLMX_GEMM_BM rows (or B := B*inv(T) with
+ * N <= ATLMX_GEMM_BN columns) reads all of its input before any write. */
+#define ATLMX_GEMM_BM 128
+#define ATLMX_GEMM_BN 128
+
+/* Invert the nb x nb diagonal blocks of a triangular n x n matrix A into
+ * inv (nblk blocks of nb*nb, column-major, ld = nb, zero outside the
+ * triangle; the last block is padded with the identity).  nb = 64. */
+#define ATLMX_TRI_NB 64
+hipError_t atlmx_k_dtri_inv(int uplo, int diag, int n, const double *A,
+                            long lda, double *inv, const int *abort_flag,
+                            hipStream_t s);
+hipError_t atlmx_k_stri_inv(int uplo, int diag, int n, const float *A,
+                            long lda, float *inv, const int *abort_flag,
+                            hipStream_t s);
+
+/* B(M x N) *= alpha */
+hipError_t atlmx_k_dscal2d(int M, int N, double alpha, double *B, long ldb,
+                           hipStream_t s);
+hipError_t atlmx_k_sscal2d(int M, int N, float alpha, float *B, long ldb,
+                           hipStream_t s);
+
+/* LU panel: factor the m x w column panel P (ld) with partial pivoting.
+ * ipiv[c] receives the absolute pivot row (row0 + panel row); info (device
+ * int) gets row0+c+1 for the first exactly-zero pivot if it is still 0.
+ * scratch: device buffer of atlmx_k_panel_scratch_bytes(m, w) bytes, its
+ * first 4 bytes zeroed before the launch (barrier counter). */
+size_t atlmx_k_panel_scratch_bytes(int m, int w);
+hipError_t atlmx_k_dgetf2_panel(int m, int w, double *P, long ld, int row0,
+                                int *ipiv, int *info, void *scratch,
+                                hipStream_t s);
+hipError_t atlmx_k_sgetf2_panel(int m, int w, float *P, long ld, int row0,
+                                int *ipiv, int *info, void *scratch,
+                                hipStream_t s);
+
+/* Apply the row interchanges ipiv[k1..k2) (absolute row numbers, forward
+ * order when inc > 0, reverse when inc < 0) to the N columns of A. */
+hipError_t atlmx_k_dlaswp(int N, double *A, long lda, int k1, int k2,
+                          const int *ipiv, int inc, hipStream_t s);
+hipError_t atlmx_k_slaswp(int N, float *A, long lda, int k1, int k2,
+                          const int *ipiv, int inc, hipStream_t s);
+
+/* Cholesky of one n x n diagonal block (n <= 128) in place (lower or
+ * upper).  On a non-positive pivot sets *info = info_base + k + 1 (if still
+ * 0) and stops; does nothing when *info is already nonzero. */
+#define ATLMX_CHOL_NB 128
+hipError_t atlmx_k_dpotf2(int uplo, int n, double *A, long lda, int info_base,
+                          int *info, hipStream_t s);
+hipError_t atlmx_k_spotf2(int uplo, int n, float *A, long lda, int info_base,
+                          int *info, hipStream_t s);
+
+/* First index of max |x_i| (cblas_i?amax semantics) into *out. */
+hipError_t atlmx_k_idamax(int n, const double *x, int incx, int *out,
+                          hipStream_t s);
+hipError_t atlmx_k_isamax(int n, const float *x, int incx, int *out,
+                          hipStream_t s);
+
+/* Fill n ints with v (stream-ordered, no memset node). */
+hipError_t atlmx_k_fill_i32(int *p, int n, int v, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,517 @@
+/*
+ * lapack_tmpl.h -- precision template of the LAPACK layer: blocked
+ * right-looking LU and Cholesky on the GPU, getrs/potrs/gesv/posv, and the
+ * clapack C ABI.  Included by lapack.c with T=double/PFX=d and
+ * T=float/PFX=s.
+ */
+#define GETRF_DEV NM(PFX, getrf_dev)
+#define GETRS_DEV NM(PFX, getrs_dev)
+#define POTRF_DEV NM(PFX, potrf_dev)
+#define POTRS_DEV NM(PFX, potrs_dev)
+#define PANEL_REC NM(PFX, panel_rec)
+
+/* Width of the outer LU block (the K of the trailing GEMM) and of the
+ * cooperative panel leaf. */
+#define LU_NB 256
+#define LU_PW 32
+#define CHOL_NB ATLMX_CHOL_NB
+
+/* Factor the tall panel A[r0:M, r0:r0+w] with partial pivoting.
+ * Splits like src/lapack/ATL_getrfC.c:277-300 (left half, laswp + TRSM +
+ * GEMM on the right half, right half, back-swap of the left half) down to
+ * 32-wide leaves, which run as one cooperative kernel each (k_lu.hip). */
+static void PANEL_REC(atlmx_ctx *c, int M, T *A, long lda, int r0, int w, int *ipiv,
+                      int *info, void *scratch, const char *rout)
+{
+  if (w <= LU_PW) {
+    int pid;
+    RT_CHECK(rout, atlmx_k_fill_i32((int *)scratch, 1, 0, c->stream));
+    pid = rt_prof_begin(c, PROF_PANEL, (double)(M - r0) * w * 2.0 * sizeof(T));
+    RT_CHECK(rout, K_PANEL(M - r0, w, A + r0 + (size_t)r0 * lda, lda, r0, ipiv + r0, info,
+                           scratch, c->stream));
+    rt_prof_end(c, pid);
+    return;
+  } else {
+    int wl = (w / 2 / LU_PW) * LU_PW;
+    int pid;
+    if (wl < LU_PW) wl = LU_PW;
+    PANEL_REC(c, M, A, lda, r0, wl, ipiv, info, scratch, rout);
+    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
+    RT_CHECK(rout, K_LASWP(w - wl, A + (size_t)(r0 + wl) * lda, lda, r0, r0 + wl, ipiv, 1,
+                           c->stream));
+    rt_prof_end(c, pid);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, wl, w - wl, (T)1,
+                      A + r0 + (size_t)r0 * lda, lda, A + r0 + (size_t)(r0 + wl) * lda, lda,
+                      NULL, rout);
+    if (r0 + wl < M)
+      NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - r0 - wl, w - wl, wl, (T)-1,
+                        A + (r0 + wl) + (size_t)r0 * lda, lda,
+                        A + r0 + (size_t)(r0 + wl) * lda, lda, (T)1,
+                        A + (r0 + wl) + (size_t)(r0 + wl) * lda, lda, 0, NULL, rout);
+    PANEL_REC(c, M, A, lda, r0 + wl, w - wl, ipiv, info, scratch, rout);
+    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
+    RT_CHECK(rout, K_LASWP(wl, A + (size_t)r0 * lda, lda, r0 + wl, r0 + w, ipiv, 1,
+                           c->stream));
+    rt_prof_end(c, pid);
+  }
+}
+
+/* Blocked right-looking LU, A = P*L*U in place (column-major M x N).
+ * d_ipiv: device, min(M,N) absolute 0-based pivot rows (clapack
+ * convention, src/lapack/ATL_getrfC.c:299,325).  Returns info (1-based
+ * first exactly-zero pivot; the factorization still completes,
+ * interfaces/lapack/C/src/clapack_dgetrf.c:35-45).  Synchronizes. */
+int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const char *rout)
+{
+  const int MN = M < N ? M : N;
+  int j, info = 0, pid;
+  int *d_info;
+  void *scratch;
+  if (MN <= 0) return 0;
+  d_info = (int *)rt_ws(c, WS_INFO, 256, rout);
+  scratch = rt_ws(c, WS_PANEL, atlmx_k_panel_scratch_bytes(M, LU_PW), rout);
+  RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
+  for (j = 0; j < MN; j += LU_NB) {
+    const int jb = (MN - j < LU_NB) ? MN - j : LU_NB;
+    PANEL_REC(c, M, A, lda, j, jb, d_ipiv, d_info, scratch, rout);
+    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
+    if (j > 0) RT_CHECK(rout, K_LASWP(j, A, lda, j, j + jb, d_ipiv, 1, c->stream));
+    if (j + jb < N)
+      RT_CHECK(rout, K_LASWP(N - j - jb, A + (size_t)(j + jb) * lda, lda, j, j + jb, d_ipiv,
+                             1, c->stream));
+    rt_prof_end(c, pid);
+    if (j + jb < N) {
+      NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, jb, N - j - jb, (T)1,
+                        A + j + (size_t)j * lda, lda, A + j + (size_t)(j + jb) * lda, lda,
+                        NULL, rout);
+      if (j + jb < M)
+        NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - j - jb, N - j - jb, jb, (T)-1,
+                          A + (j + jb) + (size_t)j * lda, lda,
+                          A + j + (size_t)(j + jb) * lda, lda, (T)1,
+                          A + (j + jb) + (size_t)(j + jb) * lda, lda, 0, NULL, rout);
+    }
+  }
+  RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost,
+                                c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+/* src/lapack/ATL_getrs.c:8-89 (ColMajor).  NoTrans: P*B, L-unit solve, U
+ * solve; Trans: U^T solve, L^T-unit solve, P^T*B. */
+void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
+               const int *d_ipiv, T *B, long ldb, const char *rout)
+{
+  if (N <= 0 || NRHS <= 0) return;
+  if (trans == KX_NOTRANS) {
+    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, 1, c->stream));
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
+                      ldb, NULL, rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda,
+                      B, ldb, NULL, rout);
+  } else {
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda, B,
+                      ldb, NULL, rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
+                      ldb, NULL, rout);
+    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, -1, c->stream));
+  }
+}
+
+/* Blocked right-looking Cholesky (Lower: A = L*L^T, Upper: A = U^T*U).
+ * Replaces the recursion of src/lapack/ATL_potrfL.c:204-262 /
+ * ATL_potrfU.c:199-257: diagonal block in LDS (k_chol.hip), TRSM of the
+ * panel, SYRK trailing update on the GEMM kernel restricted to one
+ * triangle.  A non-positive pivot stops the factorization: the device info
+ * flag makes every queued kernel after it return at once.  Synchronizes;
+ * returns info. */
+int POTRF_DEV(atlmx_ctx *c, int uplo, int N, T *A, long lda, const char *rout)
+{
+  int j, info = 0, pid;
+  int *d_info;
+  if (N <= 0) return 0;
+  d_info = (int *)rt_ws(c, WS_INFO, 256, rout);
+  RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
+  for (j = 0; j < N; j += CHOL_NB) {
+    const int jb = (N - j < CHOL_NB) ? N - j : CHOL_NB;
+    const int nr = N - j - jb;
+    T *Ad = A + j + (size_t)j * lda;
+    pid = rt_prof_begin(c, PROF_POTF2, 0.0);
+    RT_CHECK(rout, K_POTF2(uplo, jb, Ad, lda, j, d_info, c->stream));
+    rt_prof_end(c, pid);
+    if (nr <= 0) break;
+    if (uplo == KX_LOWER) {
+      T *Ar = A + (j + jb) + (size_t)j * lda;
+      NM(PFX, trsm_dev)(c, KX_RIGHT, KX_LOWER, KX_TRANS, KX_NONUNIT, nr, jb, (T)1, Ad, lda,
+                        Ar, lda, d_info, rout);
+      NM(PFX, syrk_dev)(c, KX_LOWER, KX_NOTRANS, nr, jb, (T)-1, Ar, lda, (T)1,
+                        A + (j + jb) + (size_t)(j + jb) * lda, lda, d_info, rout);
+    } else {
+      T *Ac = A + j + (size_t)(j + jb) * lda;
+      NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, jb, nr, (T)1, Ad, lda,
+                        Ac, lda, d_info, rout);
+      NM(PFX, syrk_dev)(c, KX_UPPER, KX_TRANS, nr, jb, (T)-1, Ac, lda, (T)1,
+                        A + (j + jb) + (size_t)(j + jb) * lda, lda, d_info, rout);
+    }
+  }
+  RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost,
+                                c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+/* src/lapack/ATL_potrs.c:13-86 (ColMajor) */
+void POTRS_DEV(atlmx_ctx *c, int uplo, int N, int NRHS, const T *A, long lda, T *B,
+               long ldb, const char *rout)
+{
+  if (N <= 0 || NRHS <= 0) return;
+  if (uplo == KX_UPPER) {
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda, B,
+                      ldb, NULL, rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda,
+                      B, ldb, NULL, rout);
+  } else {
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda,
+                      B, ldb, NULL, rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda, B,
+                      ldb, NULL, rout);
+  }
+}
+
+/* ====================================================================== */
+/* public device-pointer API                                                */
+/* ====================================================================== */
+int NM3(atlmx_, PFX, getrf)(int M, int N, T *A, int lda, int *d_ipiv)
+{
+  return GETRF_DEV(rt_ctx(), M, N, A, lda, d_ipiv, "atlmx_getrf");
+}
+void NM3(atlmx_, PFX, getrs)(int trans, int N, int NRHS, const T *A, int lda,
+                             const int *d_ipiv, T *B, int ldb)
+{
+  GETRS_DEV(rt_ctx(), trans == KX_NOTRANS ? KX_NOTRANS : KX_TRANS, N, NRHS, A, lda, d_ipiv,
+            B, ldb, "atlmx_getrs");
+}
+int NM3(atlmx_, PFX, potrf)(int uplo, int N, T *A, int lda)
+{
+  return POTRF_DEV(rt_ctx(), uplo, N, A, lda, "atlmx_potrf");
+}
+void NM3(atlmx_, PFX, potrs)(int uplo, int N, int NRHS, const T *A, int lda, T *B, int ldb)
+{
+  POTRS_DEV(rt_ctx(), uplo, N, NRHS, A, lda, B, ldb, "atlmx_potrs");
+}
+
+/* ====================================================================== */
+/* ATLAS clapack C ABI                                                      */
+/* ====================================================================== */
+
+/* interfaces/lapack/C/src/clapack_dgetrf.c:13-88; RowMajor is the
+ * column-major factorization of the transposed view (src/lapack/
+ * ATL_getrf.c:12, ATL_getrfR.c). */
+int NM3(clapack_, PFX, getrf)(const enum CBLAS_ORDER Order, const int M, const int N,
+                              T *A, const int lda, int *ipiv)
+{
+  static const char *rout = "clapack_" STR(PFX) "getrf";
+  int ierr = 0, m, n, mn, info;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  int *dp;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (M < 0) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "M cannot be less than zero 0,; is set to %d.\n", M);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (Order == CblasColMajor) {
+    if (lda < M || lda < 1) {
+      ierr = -6;
+      cblas_xerbla(6, rout, "lda must be >= MAX(M,1): lda=%d M=%d\n", lda, M);
+    }
+  } else if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ierr) return ierr;
+  m = (Order == CblasColMajor) ? M : N;
+  n = (Order == CblasColMajor) ? N : M;
+  mn = m < n ? m : n;
+  if (mn == 0) return 0;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, m, n, lda, sizeof(T), 1, &dla, rout);
+  dp = rt_is_device(ipiv) ? ipiv : (int *)rt_ws(c, WS_IPIV, (size_t)mn * sizeof(int), rout);
+  info = GETRF_DEV(c, m, n, dA, dla, dp, rout);
+  rt_stage_out(c, A, dA, m, n, lda, dla, sizeof(T), rout);
+  if (dp != ipiv)
+    RT_CHECK(rout, hipMemcpyAsync(ipiv, dp, (size_t)mn * sizeof(int), hipMemcpyDeviceToHost,
+                                  c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+static const int *NM(PFX, stage_ipiv)(atlmx_ctx *c, const int *ipiv, int n, const char *rout)
+{
+  int *dp;
+  if (rt_is_device(ipiv)) return ipiv;
+  dp = (int *)rt_ws(c, WS_IPIV, (size_t)(n > 0 ? n : 1) * sizeof(int), rout);
+  RT_CHECK(rout, hipMemcpyAsync(dp, ipiv, (size_t)n * sizeof(int), hipMemcpyHostToDevice,
+                                c->stream));
+  return dp;
+}
+
+/* interfaces/lapack/C/src/clapack_dgetrs.c; RowMajor holds the factor of
+ * A^T, so it is the column-major solve with Trans flipped
+ * (src/lapack/ATL_getrs.c:60-88 spells the same with Right-side TRSMs). */
+int NM3(clapack_, PFX, getrs)(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE Trans,
+                              const int N, const int NRHS, const T *A, const int lda,
+                              const int *ipiv, T *B, const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "getrs";
+  int ierr = 0, t;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Trans != CblasNoTrans && Trans != CblasTrans && Trans != CblasConjTrans) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "Trans must be %d, %d, or %d, but is set to %d\n", CblasNoTrans,
+                 CblasTrans, CblasConjTrans, Trans);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (NRHS < 0) {
+    ierr = -4;
+    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ldb < N || ldb < 1) {
+    ierr = -9;
+    cblas_xerbla(9, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
+  }
+  if (ierr) return ierr;
+  if (N == 0 || NRHS == 0) return 0;
+  t = (Trans != CblasNoTrans);
+  if (Order == CblasRowMajor) t = !t;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, sizeof(T), 1, &dlb, rout);
+  GETRS_DEV(c, t ? KX_TRANS : KX_NOTRANS, N, NRHS, dA, dla,
+            NM(PFX, stage_ipiv)(c, ipiv, N, rout), dB, dlb, rout);
+  rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, sizeof(T), rout);
+  rt_sync(c, rout);
+  return 0;
+}
+
+/* interfaces/lapack/C/src/clapack_dpotrf.c; RowMajor swaps Upper and
+ * Lower (src/lapack/ATL_potrf.c:21-26). */
+int NM3(clapack_, PFX, potrf)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                              const int N, T *A, const int lda)
+{
+  static const char *rout = "clapack_" STR(PFX) "potrf";
+  int ierr = 0, low, info;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
+                 CblasLower, Uplo);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -5;
+    cblas_xerbla(5, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  info = POTRF_DEV(c, low ? KX_LOWER : KX_UPPER, N, dA, dla, rout);
+  rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
+  rt_sync(c, rout);
+  return info;
+}
+
+/* interfaces/lapack/C/src/clapack_dpotrs.c */
+int NM3(clapack_, PFX, potrs)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                              const int N, const int NRHS, const T *A, const int lda, T *B,
+                              const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "potrs";
+  int ierr = 0, low;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
+                 CblasLower, Uplo);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (NRHS < 0) {
+    ierr = -4;
+    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ldb < N || ldb < 1) {
+    ierr = -8;
+    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
+  }
+  if (ierr) return ierr;
+  if (N == 0 || NRHS == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, sizeof(T), 1, &dlb, rout);
+  POTRS_DEV(c, low ? KX_LOWER : KX_UPPER, N, NRHS, dA, dla, dB, dlb, rout);
+  rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, sizeof(T), rout);
+  rt_sync(c, rout);
+  return 0;
+}
+
+/* interfaces/lapack/C/src/clapack_dgesv.c:29-62 */
+int NM3(clapack_, PFX, gesv)(const enum CBLAS_ORDER Order, const int N, const int NRHS,
+                             T *A, const int lda, int *ipiv, T *B, const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "gesv";
+  int ierr = 0, info;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  int *dp;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (N < 0) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (NRHS < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -5;
+    cblas_xerbla(5, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ldb < N || ldb < 1) {
+    ierr = -8;
+    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
+  }
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  dp = rt_is_device(ipiv) ? ipiv : (int *)rt_ws(c, WS_IPIV, (size_t)N * sizeof(int), rout);
+  info = GETRF_DEV(c, N, N, dA, dla, dp, rout);
+  if (!info && NRHS > 0) {
+    dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, sizeof(T), 1, &dlb, rout);
+    GETRS_DEV(c, (Order == CblasRowMajor) ? KX_TRANS : KX_NOTRANS, N, NRHS, dA, dla, dp,
+              dB, dlb, rout);
+    rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, sizeof(T), rout);
+  }
+  rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
+  if (dp != ipiv)
+    RT_CHECK(rout, hipMemcpyAsync(ipiv, dp, (size_t)N * sizeof(int), hipMemcpyDeviceToHost,
+                                  c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+/* interfaces/lapack/C/src/clapack_dposv.c:19-58 */
+int NM3(clapack_, PFX, posv)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                             const int N, const int NRHS, T *A, const int lda, T *B,
+                             const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "posv";
+  int ierr = 0, info, low;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
+                 CblasLower, Uplo);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (NRHS < 0) {
+    ierr = -4;
+    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ldb < N || ldb < 1) {
+    ierr = -8;
+    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
+  }
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  info = POTRF_DEV(c, low ? KX_LOWER : KX_UPPER, N, dA, dla, rout);
+  if (!info && NRHS > 0) {
+    dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, sizeof(T), 1, &dlb, rout);
+    POTRS_DEV(c, low ? KX_LOWER : KX_UPPER, N, NRHS, dA, dla, dB, dlb, rout);
+    rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, sizeof(T), rout);
+  }
+  rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
+  rt_sync(c, rout);
+  return info;
+}
+
+#undef GETRF_DEV
+#undef GETRS_DEV
+#undef POTRF_DEV
+#undef POTRS_DEV
+#undef PANEL_REC

@@ -1,0 +1,68 @@
+/*
+ * rt.h -- host runtime of libatlas_mi355x (C): per-thread HIP stream and
+ * device workspaces, host<->device staging, error handling, and the
+ * optional per-launch profiler.
+ */
+#ifndef ATLMX_RT_H
+#define ATLMX_RT_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+#include "kernels.h"
+
+/* workspace slots; each grows independently and is reused across calls */
+enum {
+  WS_TRI_INV = 0,   /* TRSM inverted diagonal blocks           */
+  WS_PANEL,         /* LU panel exchange buffers + counter     */
+  WS_INFO,          /* device info / pivot scratch             */
+  WS_STAGE_A,       /* staging of host operands                */
+  WS_STAGE_B,
+  WS_STAGE_C,
+  WS_IPIV,
+  WS_MP0,           /* mixed-precision solver buffers          */
+  WS_MP1,
+  WS_MP2,
+  WS_MP3,
+  WS_MP4,
+  WS_NSLOTS
+};
+
+typedef struct atlmx_ctx {
+  int dev;
+  hipStream_t stream;       /* stream every launch of this thread uses */
+  hipStream_t own;          /* the library's stream for this thread    */
+  void *ws[WS_NSLOTS];
+  size_t ws_bytes[WS_NSLOTS];
+} atlmx_ctx;
+
+atlmx_ctx *rt_ctx(void);
+
+/* Abort through cblas_xerbla's convention (message on stderr, exit(-1)) on
+ * any HIP failure: a GPU error never returns garbage. */
+void rt_fail(const char *rout, const char *what, hipError_t e);
+#define RT_CHECK(rout, call)                                    \
+  do {                                                          \
+    hipError_t e_ = (call);                                     \
+    if (e_ != hipSuccess) rt_fail((rout), #call, e_);           \
+  } while (0)
+
+int rt_is_device(const void *p);
+void *rt_ws(atlmx_ctx *c, int slot, size_t bytes, const char *rout);
+void rt_sync(atlmx_ctx *c, const char *rout);
+
+/* Stage a rows x cols column-major operand.  A device pointer is returned
+ * as is (*dld = ld); a host operand is copied (when copy_in) into
+ * workspace slot `slot` with a padded leading dimension. */
+void *rt_stage_in(atlmx_ctx *c, int slot, const void *h, int rows, int cols, long ld,
+                  size_t esz, int copy_in, long *dld, const char *rout);
+void rt_stage_out(atlmx_ctx *c, void *h, const void *d, int rows, int cols, long ld,
+                  long dld, size_t esz, const char *rout);
+
+/* ---- profiler: hipEvents around selected launches on the ctx stream ---- */
+enum { PROF_GEMM = 0, PROF_PANEL, PROF_TRSM_INV, PROF_LASWP, PROF_POTF2, PROF_MISC,
+       PROF_NCLASS };
+int rt_prof_on(void);
+int rt_prof_begin(atlmx_ctx *c, int cls, double work);
+void rt_prof_end(atlmx_ctx *c, int id);
+
+#endif

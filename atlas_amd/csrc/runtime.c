@@ -1,0 +1,194 @@
+/*
+ * runtime.c -- host runtime of libatlas_mi355x.
+ *
+ * ATLAS has no device and no handles: each call mallocs its workspace and
+ * frees it (src/blas/ammm/ATL_ammmKMNK.c:53-65,105), and when a malloc
+ * fails it recurses on a smaller problem (ATL_ammm.c:293-356).  Here each
+ * host thread gets one HIP stream and a set of grow-only device workspaces
+ * that persist across calls (hipMalloc is far too slow to do per call), so
+ * the library is callable from any host thread with no visible global
+ * mutable state (SURVEY boundary contract, "Threading").
+ */
+#include <pthread.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "rt.h"
+#include "../../include/cblas.h"
+
+static __thread atlmx_ctx *tls_ctx = NULL;
+
+void rt_fail(const char *rout, const char *what, hipError_t e)
+{
+  cblas_xerbla(0, rout, "GPU failure in %s: %s (%d)\n", what, hipGetErrorString(e), (int)e);
+  abort();   /* a user cblas_xerbla that returns must not let us continue */
+}
+
+atlmx_ctx *rt_ctx(void)
+{
+  atlmx_ctx *c = tls_ctx;
+  hipError_t e;
+  if (c) return c;
+  c = (atlmx_ctx *)calloc(1, sizeof(*c));
+  if (!c) { fprintf(stderr, "atlas_mi355x: out of host memory\n"); exit(-1); }
+  e = hipGetDevice(&c->dev);
+  if (e != hipSuccess) rt_fail("atlas_mi355x", "hipGetDevice", e);
+  e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
+  if (e != hipSuccess) rt_fail("atlas_mi355x", "hipStreamCreateWithFlags", e);
+  c->stream = c->own;
+  tls_ctx = c;
+  return c;
+}
+
+int rt_is_device(const void *p)
+{
+  hipPointerAttribute_t a;
+  hipError_t e;
+  if (!p) return 0;
+  memset(&a, 0, sizeof(a));
+  e = hipPointerGetAttributes(&a, p);
+  if (e != hipSuccess) { (void)hipGetLastError(); return 0; }
+  return a.type == hipMemoryTypeDevice || a.type == hipMemoryTypeManaged ||
+         a.type == hipMemoryTypeUnified || a.isManaged;
+}
+
+void rt_sync(atlmx_ctx *c, const char *rout)
+{
+  RT_CHECK(rout, hipStreamSynchronize(c->stream));
+}
+
+void *rt_ws(atlmx_ctx *c, int slot, size_t bytes, const char *rout)
+{
+  if (bytes == 0) bytes = 256;
+  if (c->ws_bytes[slot] < bytes) {
+    size_t nb = bytes + bytes / 8;
+    nb = (nb + 4095) & ~(size_t)4095;
+    if (c->ws[slot]) {
+      /* the old buffer may still be in use by queued work */
+      RT_CHECK(rout, hipStreamSynchronize(c->stream));
+      RT_CHECK(rout, hipFree(c->ws[slot]));
+      c->ws[slot] = NULL;
+      c->ws_bytes[slot] = 0;
+    }
+    RT_CHECK(rout, hipMalloc(&c->ws[slot], nb));
+    c->ws_bytes[slot] = nb;
+  }
+  return c->ws[slot];
+}
+
+void *rt_stage_in(atlmx_ctx *c, int slot, const void *h, int rows, int cols, long ld,
+                  size_t esz, int copy_in, long *dld, const char *rout)
+{
+  long pad;
+  void *d;
+  if (rt_is_device(h)) { *dld = ld; return (void *)h; }
+  pad = ((long)(rows > 0 ? rows : 1) + 15) & ~15L;
+  d = rt_ws(c, slot, (size_t)pad * (size_t)(cols > 0 ? cols : 1) * esz, rout);
+  *dld = pad;
+  if (copy_in && rows > 0 && cols > 0)
+    RT_CHECK(rout, hipMemcpy2DAsync(d, (size_t)pad * esz, h, (size_t)ld * esz,
+                                    (size_t)rows * esz, (size_t)cols, hipMemcpyHostToDevice,
+                                    c->stream));
+  return d;
+}
+
+void rt_stage_out(atlmx_ctx *c, void *h, const void *d, int rows, int cols, long ld,
+                  long dld, size_t esz, const char *rout)
+{
+  if (h == d || rows <= 0 || cols <= 0) return;
+  RT_CHECK(rout, hipMemcpy2DAsync(h, (size_t)ld * esz, d, (size_t)dld * esz,
+                                  (size_t)rows * esz, (size_t)cols, hipMemcpyDeviceToHost,
+                                  c->stream));
+}
+
+/* ---------------------------------------------------------------------- */
+/* profiler                                                                 */
+/* ---------------------------------------------------------------------- */
+#define PROF_MAX 65536
+typedef struct { hipEvent_t a, b; int cls; double work; } prof_rec;
+static __thread prof_rec *prof_buf = NULL;
+static __thread int prof_n = 0;
+static __thread int prof_enabled = 0;
+
+int rt_prof_on(void) { return prof_enabled; }
+
+int rt_prof_begin(atlmx_ctx *c, int cls, double work)
+{
+  prof_rec *r;
+  if (!prof_enabled || prof_n >= PROF_MAX) return -1;
+  r = &prof_buf[prof_n];
+  if (!r->a) {
+    RT_CHECK("atlmx_prof", hipEventCreate(&r->a));
+    RT_CHECK("atlmx_prof", hipEventCreate(&r->b));
+  }
+  r->cls = cls;
+  r->work = work;
+  RT_CHECK("atlmx_prof", hipEventRecord(r->a, c->stream));
+  return prof_n++;
+}
+
+void rt_prof_end(atlmx_ctx *c, int id)
+{
+  if (id < 0) return;
+  RT_CHECK("atlmx_prof", hipEventRecord(prof_buf[id].b, c->stream));
+}
+
+/* ---- public runtime entry points (declared in include/atlmx.h) ---- */
+int atlmx_init(void)
+{
+  (void)rt_ctx();
+  return 0;
+}
+
+void atlmx_set_stream(void *stream)
+{
+  atlmx_ctx *c = rt_ctx();
+  c->stream = stream ? (hipStream_t)stream : c->own;
+}
+
+void *atlmx_get_stream(void)
+{
+  return (void *)rt_ctx()->stream;
+}
+
+void atlmx_synchronize(void)
+{
+  atlmx_ctx *c = rt_ctx();
+  rt_sync(c, "atlmx_synchronize");
+}
+
+void atlmx_prof_enable(int on)
+{
+  if (on && !prof_buf) {
+    prof_buf = (prof_rec *)calloc(PROF_MAX, sizeof(prof_rec));
+    if (!prof_buf) return;
+  }
+  prof_enabled = on ? 1 : 0;
+  prof_n = 0;
+}
+
+/* Sum of event-timed durations (ms), launch count and summed algorithmic
+ * work (flops or bytes, as each launcher reports) of one launch class. */
+int atlmx_prof_query(int cls, double *ms_total, long *launches, double *work_total)
+{
+  atlmx_ctx *c = rt_ctx();
+  double ms = 0.0, w = 0.0;
+  long n = 0;
+  int i;
+  if (prof_n > 0) rt_sync(c, "atlmx_prof_query");
+  for (i = 0; i < prof_n; ++i) {
+    float t = 0.0f;
+    if (prof_buf[i].cls != cls) continue;
+    RT_CHECK("atlmx_prof_query", hipEventElapsedTime(&t, prof_buf[i].a, prof_buf[i].b));
+    ms += t;
+    w += prof_buf[i].work;
+    n++;
+  }
+  if (ms_total) *ms_total = ms;
+  if (launches) *launches = n;
+  if (work_total) *work_total = w;
+  return 0;
+}
+
+void atlmx_prof_reset(void) { prof_n = 0; }

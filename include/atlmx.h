@@ -1,0 +1,61 @@
+/*
+ * atlmx.h -- extensions of libatlas_mi355x beyond ATLAS's C ABI.
+ *
+ * None of these has a counterpart in ATLAS.  They serve callers whose data
+ * already lives in GPU memory (HBM-resident pipelines, PyTorch, the
+ * benchmarks): every atlmx_* routine takes DEVICE pointers, column-major
+ * storage, ATLAS's enum values (include/cblas.h), and is ordered on the
+ * calling thread's stream (atlmx_set_stream), returning before the work
+ * finishes unless it says "synchronizes".
+ */
+#ifndef ATLMX_H
+#define ATLMX_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime ------------------------------------------------------------ */
+int   atlmx_init(void);                 /* optional; creates this thread's stream */
+void  atlmx_set_stream(void *hip_stream); /* NULL: the library's own stream      */
+void *atlmx_get_stream(void);
+void  atlmx_synchronize(void);
+
+/* ---- profiler: hipEvent timing of every launch of a class on this thread's
+ * stream while enabled.  Classes: 0 GEMM (work = 2*M*N*K flops), 1 LU panel
+ * (work = bytes loaded+stored), 2 TRSM diagonal inversion, 3 laswp,
+ * 4 Cholesky diagonal block, 5 other. ---- */
+void atlmx_prof_enable(int on);
+void atlmx_prof_reset(void);
+int  atlmx_prof_query(int cls, double *ms_total, long *launches, double *work_total);
+
+/* ---- BLAS-3 on device pointers ----------------------------------------- */
+void atlmx_dgemm(int TA, int TB, int M, int N, int K, double alpha, const double *A,
+                 int lda, const double *B, int ldb, double beta, double *C, int ldc);
+void atlmx_sgemm(int TA, int TB, int M, int N, int K, float alpha, const float *A,
+                 int lda, const float *B, int ldb, float beta, float *C, int ldc);
+void atlmx_dtrsm(int Side, int Uplo, int TA, int Diag, int M, int N, double alpha,
+                 const double *A, int lda, double *B, int ldb);
+void atlmx_strsm(int Side, int Uplo, int TA, int Diag, int M, int N, float alpha,
+                 const float *A, int lda, float *B, int ldb);
+void atlmx_dsyrk(int Uplo, int Trans, int N, int K, double alpha, const double *A,
+                 int lda, double beta, double *C, int ldc);
+void atlmx_ssyrk(int Uplo, int Trans, int N, int K, float alpha, const float *A, int lda,
+                 float beta, float *C, int ldc);
+
+/* ---- LAPACK on device pointers (ipiv: device int array, 0-based) -------- */
+int  atlmx_dgetrf(int M, int N, double *A, int lda, int *d_ipiv);    /* synchronizes */
+int  atlmx_sgetrf(int M, int N, float *A, int lda, int *d_ipiv);     /* synchronizes */
+void atlmx_dgetrs(int Trans, int N, int NRHS, const double *A, int lda, const int *d_ipiv,
+                  double *B, int ldb);
+void atlmx_sgetrs(int Trans, int N, int NRHS, const float *A, int lda, const int *d_ipiv,
+                  float *B, int ldb);
+int  atlmx_dpotrf(int Uplo, int N, double *A, int lda);              /* synchronizes */
+int  atlmx_spotrf(int Uplo, int N, float *A, int lda);               /* synchronizes */
+void atlmx_dpotrs(int Uplo, int N, int NRHS, const double *A, int lda, double *B, int ldb);
+void atlmx_spotrs(int Uplo, int N, int NRHS, const float *A, int lda, float *B, int ldb);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,130 @@
+"""GPU parity of LU / Cholesky / solves through the clapack ABI against the
+oracle: pivot vectors bit-identical to the recursive ATL_getrfC restatement
+up to the first near-tied column, and ATLAS's tester residuals (lutst,
+llttst, slvtst: threshold 100, bin/lutst.c:398, bin/slvtst.c:75)."""
+import numpy as np
+import pytest
+
+import atlas_amd as X
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(99)
+TIE = 1e-9
+
+
+def _f(a):
+    return np.asfortranarray(a)
+
+
+def _pivots_agree(gpu, orc, gaps):
+    """Equal up to the first column whose top-2 candidates are near-tied."""
+    n = len(orc)
+    stop = n
+    near = np.nonzero(gaps[:n] < TIE)[0]
+    if near.size:
+        stop = int(near[0])
+    return np.array_equal(gpu[:stop], orc[:stop]), stop
+
+
+@pytest.mark.parametrize("M,N,lda", [(1, 1, 1), (5, 5, 5), (31, 31, 40), (33, 33, 33),
+                                     (100, 100, 100), (257, 257, 260), (300, 200, 300),
+                                     (200, 300, 200), (600, 600, 600), (1030, 1030, 1030)])
+def test_getrf_lutst_and_pivots(M, N, lda):
+    A0 = O.gegen(M, N, lda)                 # lutst input, seed M*N+lda
+    A = A0.copy(order="F")
+    info, ipiv = X.getrf(A[:M, :N])           # M x N view with leading dimension lda
+    assert info == 0
+    assert np.all(A[M:, :] == -2.56e9)        # lda gap untouched
+    Ao = A0.copy(order="F")
+    gaps = np.ones(min(M, N))
+    _, opiv = O.getrf(Ao[:M, :N], gaps=gaps)
+    ok, stop = _pivots_agree(ipiv, opiv, gaps)
+    assert ok, (stop, np.nonzero(ipiv != opiv)[0][:5])
+    r = O.lu_resid(A[:M, :N], ipiv, A0[:M, :N])
+    assert r < 100, r
+
+
+def test_getrf_float():
+    n = 500
+    A0 = O.gegen(n, n, dtype=np.float32)
+    A = A0.copy(order="F")
+    info, ipiv = X.getrf(A)
+    assert info == 0
+    assert O.lu_resid(A, ipiv, A0) < 100
+
+
+def test_getrf_singular_column_info():
+    n = 200
+    A = _f(RNG.uniform(-1, 1, (n, n)))
+    A[:, 77] = 0.0
+    info, ipiv = X.getrf(A)
+    assert info == 78
+    assert np.isfinite(A).all()
+
+
+@pytest.mark.parametrize("nrhs", [1, 64])
+@pytest.mark.parametrize("trans", [X.NoTrans, X.Trans])
+def test_getrs_slvtst(nrhs, trans):
+    n = 700
+    A0 = O.gegen(n, n)
+    Xc = O.gegen(n, nrhs)
+    B = _f((A0 if trans == X.NoTrans else A0.T) @ Xc)
+    A = A0.copy(order="F")
+    info, ipiv = X.getrf(A)
+    assert info == 0
+    X.getrs(A, ipiv, B, trans=trans)
+    assert O.slv_resid(O.genrm1(A0), Xc, B) < 100
+
+
+def test_gesv_rowmajor():
+    n, nrhs = 300, 3
+    A0 = np.ascontiguousarray(RNG.uniform(-1, 1, (n, n)))
+    Xc = RNG.uniform(-1, 1, (n, nrhs))
+    Bt = np.ascontiguousarray((A0 @ Xc).T)           # one RHS per row, ldb = n
+    A = A0.copy()
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = X.lib().clapack_dgesv(X.RowMajor, n, nrhs, A.ctypes.data, n, ipiv.ctypes.data,
+                                 Bt.ctypes.data, n)
+    assert info == 0
+    assert np.allclose(Bt.T, Xc, atol=1e-9)
+
+
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+@pytest.mark.parametrize("N,lda", [(1, 1), (4, 4), (127, 130), (128, 128), (129, 129),
+                                   (300, 310), (1000, 1000)])
+def test_potrf_llttst(uplo, N, lda):
+    A0 = O.lltgen(uplo, N, lda)            # llttst input, seed N*1029+lda
+    A = A0.copy(order="F")
+    assert X.potrf(A[:N, :N], uplo) == 0
+    unref = np.triu(np.ones((N, N), bool), 1) if uplo == X.Lower else np.tril(np.ones((N, N), bool), -1)
+    assert np.all(A[:N, :N][unref] == A0[:N, :N][unref])       # other triangle untouched
+    assert np.all(A[N:, :] == A0[N:, :])
+    assert O.llt_resid(uplo, A[:N, :N], A0[:N, :N]) < 100
+
+
+def test_potrf_not_spd_info():
+    n = 400
+    S = O.lltgen(X.Lower, n)
+    S[250, 250] = -1.0
+    A = S.copy(order="F")
+    assert X.potrf(A, X.Lower) == 251
+    assert O.potrf(S.copy(order="F"), O.LOWER) == 251
+
+
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+def test_potrs_posv(uplo):
+    n, nrhs = 500, 4
+    S = O.lltgen(uplo, n)
+    Sf = np.tril(S) + np.tril(S, -1).T if uplo == X.Lower else np.triu(S) + np.triu(S, 1).T
+    Xc = O.gegen(n, nrhs)
+    B = _f(Sf @ Xc)
+    A = S.copy(order="F")
+    assert X.potrf(A, uplo) == 0
+    X.potrs(A, B, uplo)
+    assert O.slv_resid(O.genrm1(_f(Sf)), Xc, B) < 100
+    A = S.copy(order="F")
+    B = _f(Sf @ Xc)
+    info = X.lib().clapack_dposv(X.ColMajor, uplo, n, nrhs, A.ctypes.data, n, B.ctypes.data, n)
+    assert info == 0
+    assert O.slv_resid(O.genrm1(_f(Sf)), Xc, B) < 100
