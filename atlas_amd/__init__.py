@@ -102,6 +102,10 @@ def _declare(L):
     L.atlmx_prof_enable.argtypes = [_I]
     L.atlmx_prof_query.argtypes = [_I, _P, _P, _P]
     L.atlmx_prof_query.restype = _I
+    L.atlmx_dsgesv.argtypes = [_I, _I, _P, _I, _P, _I, _P, _I, _I, _P]
+    L.atlmx_dsgesv.restype = _I
+    L.atlmx_dbackward_error.argtypes = [_I, _I, _P, _I, _P, _I, _P, _I]
+    L.atlmx_dbackward_error.restype = _D
 
 
 def _pre(dt):
@@ -178,12 +182,33 @@ def use_torch_stream():
     lib().atlmx_set_stream(C.c_void_p(torch.cuda.current_stream().cuda_stream))
 
 
+class IRInfo(C.Structure):
+    """atlmx_ir_info (include/atlmx.h)"""
+    _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
+                ("factor_info", C.c_int), ("backward_error", C.c_double)]
+
+
+def dsgesv_dev(A, B, Xout, iter_max=30):
+    """FP64-accurate solve of A X = B with an FP32 LU + FP64 refinement.
+    A (n x n), B and Xout (n x nrhs) are column-major float64 CUDA tensors given
+    as their row-major transposes (A.T storage)."""
+    n = A.shape[0]
+    nrhs = B.shape[0]
+    st = IRInfo()
+    info = lib().atlmx_dsgesv(n, nrhs, dptr(A), n, dptr(B), n, dptr(Xout), n, iter_max,
+                              C.byref(st))
+    return info, st
+
+
 def dptr(t):
     return C.c_void_p(t.data_ptr())
 
 
-def prof_enable(on=True):
-    lib().atlmx_prof_enable(1 if on else 0)
+def prof_enable(mask=-1):
+    """mask: bit c records launch class c (True/-1 = all, 0/False = off)."""
+    if mask is True:
+        mask = -1
+    lib().atlmx_prof_enable(int(mask))
 
 
 def prof_query(cls):

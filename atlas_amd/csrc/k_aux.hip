@@ -1,6 +1,6 @@
 /*
  * k_aux.hip -- the small kernels around the GEMM: triangular diagonal-block
- * inversion (TRSM), 2-D scaling, row interchanges (laswp), iamax, fills.
+ * inversion (TRSM), 2-D scaling, iamax, fills.
  */
 #include <hip/hip_runtime.h>
 #include <float.h>
@@ -18,57 +18,60 @@ constexpr int TNB = ATLMX_TRI_NB;
  * block is inverted once, so the TRSM becomes a chain of MFMA GEMMs
  * (X_k = inv(A_kk) * B_k, then the trailing update).
  *
- * One 64-thread workgroup per block; thread j builds column j of the
- * inverse by substitution, the block and the growing inverse live in LDS.
+ * One wavefront per 64x64 block, four blocks per workgroup.  Lane j builds
+ * column j of the inverse by substitution with the column held in
+ * registers (the loops are fully unrolled); the triangle is read from LDS
+ * as a broadcast.  Lanes j > i compute x_i = 0 automatically, so all lanes
+ * run the same instruction stream.
  * ---------------------------------------------------------------------- */
 template <typename T>
-__global__ __launch_bounds__(TNB) void tri_inv_kernel(int lower, int unit, int n,
+__global__ __launch_bounds__(256) void tri_inv_kernel(int lower, int unit, int n,
                                                       const T *__restrict__ A, long lda,
                                                       T *__restrict__ inv,
                                                       const int *abort_flag) {
   if (abort_flag && *abort_flag) return;
-  __shared__ T L[TNB][TNB + 1];
-  __shared__ T X[TNB][TNB + 1];
-  const int blk = blockIdx.x;
-  const int j = threadIdx.x;
+  __shared__ T L[4][TNB][TNB + 1];
+  const int wv = threadIdx.x >> 6, j = threadIdx.x & 63;
+  const int nblk = (n + TNB - 1) / TNB;
+  const int blk = blockIdx.x * 4 + wv;
   const int b0 = blk * TNB;
-  const int nb = min(TNB, n - b0);
-  /* load the block's triangle, identity-padded beyond nb */
-  for (int i = 0; i < TNB; ++i) {
+  const int nb = (blk < nblk) ? min(TNB, n - b0) : 0;
+  /* coalesced load: lane = row */
+  for (int c = 0; c < TNB; ++c) {
+    const int i = j;
     T v = T(0);
-    const bool in = lower ? (i >= j) : (i <= j);
-    if (i < nb && j < nb) {
-      if (in) v = A[(size_t)(b0 + j) * lda + (b0 + i)];
-      if (i == j && unit) v = T(1);
-    } else if (i == j) {
+    const bool in = lower ? (i >= c) : (i <= c);
+    if (i < nb && c < nb) {
+      if (in) v = A[(size_t)(b0 + c) * lda + (b0 + i)];
+      if (i == c && unit) v = T(1);
+    } else if (i == c) {
       v = T(1);
     }
-    L[i][j] = in ? v : T(0);
-    X[i][j] = T(0);
+    L[wv][i][c] = in ? v : T(0);
   }
   __syncthreads();
+  if (blk >= nblk) return;
+  T x[TNB];
   if (lower) {
-    /* x_i = (delta_ij - sum_{k<i} L_ik x_k) / L_ii, i = j..63 */
+#pragma unroll
     for (int i = 0; i < TNB; ++i) {
-      if (i >= j) {
-        T s = (i == j) ? T(1) : T(0);
-        for (int k = j; k < i; ++k) s -= L[i][k] * X[k][j];
-        X[i][j] = s / L[i][i];
-      }
+      T s = (i == j) ? T(1) : T(0);
+#pragma unroll
+      for (int k = 0; k < i; ++k) s -= L[wv][i][k] * x[k];
+      x[i] = s / L[wv][i][i];
     }
   } else {
-    /* x_i = (delta_ij - sum_{k>i} U_ik x_k) / U_ii, i = j..0 */
+#pragma unroll
     for (int i = TNB - 1; i >= 0; --i) {
-      if (i <= j) {
-        T s = (i == j) ? T(1) : T(0);
-        for (int k = i + 1; k <= j; ++k) s -= L[i][k] * X[k][j];
-        X[i][j] = s / L[i][i];
-      }
+      T s = (i == j) ? T(1) : T(0);
+#pragma unroll
+      for (int k = i + 1; k < TNB; ++k) s -= L[wv][i][k] * x[k];
+      x[i] = s / L[wv][i][i];
     }
   }
-  __syncthreads();
-  T *out = inv + (size_t)blk * TNB * TNB;
-  for (int i = 0; i < TNB; ++i) out[(size_t)j * TNB + i] = X[i][j];
+  T *out = inv + (size_t)blk * TNB * TNB + (size_t)j * TNB;
+#pragma unroll
+  for (int i = 0; i < TNB; ++i) out[i] = x[i];
 }
 
 template <typename T>
@@ -79,27 +82,6 @@ __global__ void scal2d_kernel(int M, int N, T alpha, T *B, long ldb) {
     const long j = idx / M, i = idx - j * M;
     T *p = B + (size_t)j * ldb + i;
     *p = (alpha == T(0)) ? T(0) : *p * alpha;
-  }
-}
-
-/* Row interchanges on N columns, one thread per column (src/lapack/
- * ATL_laswp.c:7-110 walks the same swaps over 32-column chunks). */
-template <typename T>
-__global__ void laswp_kernel(int N, T *A, long lda, int k1, int k2, const int *ipiv,
-                             int inc) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= N) return;
-  T *col = A + (size_t)c * lda;
-  if (inc > 0) {
-    for (int k = k1; k < k2; ++k) {
-      const int p = ipiv[k];
-      if (p != k) { const T t = col[k]; col[k] = col[p]; col[p] = t; }
-    }
-  } else {
-    for (int k = k2 - 1; k >= k1; --k) {
-      const int p = ipiv[k];
-      if (p != k) { const T t = col[k]; col[k] = col[p]; col[p] = t; }
-    }
   }
 }
 
@@ -149,8 +131,8 @@ hipError_t launch_tri_inv(int uplo, int diag, int n, const T *A, long lda, T *in
   if (n <= 0) return hipSuccess;
   if (lda < n || !A || !inv) return hipErrorInvalidValue;
   const int nblk = (n + TNB - 1) / TNB;
-  hipLaunchKernelGGL(tri_inv_kernel<T>, dim3(nblk), dim3(TNB), 0, s, uplo == KX_LOWER,
-                     diag == KX_UNIT, n, A, lda, inv, abort_flag);
+  hipLaunchKernelGGL(tri_inv_kernel<T>, dim3((nblk + 3) / 4), dim3(256), 0, s,
+                     uplo == KX_LOWER, diag == KX_UNIT, n, A, lda, inv, abort_flag);
   return hipGetLastError();
 }
 
@@ -163,16 +145,6 @@ hipError_t launch_scal2d(int M, int N, T alpha, T *B, long ldb, hipStream_t s) {
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(scal2d_kernel<T>, dim3((unsigned)blocks), dim3(256), 0, s, M, N,
                      alpha, B, ldb);
-  return hipGetLastError();
-}
-
-template <typename T>
-hipError_t launch_laswp(int N, T *A, long lda, int k1, int k2, const int *ipiv, int inc,
-                        hipStream_t s) {
-  if (N <= 0 || k2 <= k1) return hipSuccess;
-  if (k1 < 0 || lda < k2) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(laswp_kernel<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, A, lda,
-                     k1, k2, ipiv, inc);
   return hipGetLastError();
 }
 
@@ -203,14 +175,6 @@ hipError_t atlmx_k_dscal2d(int M, int N, double alpha, double *B, long ldb, hipS
 }
 hipError_t atlmx_k_sscal2d(int M, int N, float alpha, float *B, long ldb, hipStream_t s) {
   return launch_scal2d<float>(M, N, alpha, B, ldb, s);
-}
-hipError_t atlmx_k_dlaswp(int N, double *A, long lda, int k1, int k2, const int *ipiv,
-                          int inc, hipStream_t s) {
-  return launch_laswp<double>(N, A, lda, k1, k2, ipiv, inc, s);
-}
-hipError_t atlmx_k_slaswp(int N, float *A, long lda, int k1, int k2, const int *ipiv,
-                          int inc, hipStream_t s) {
-  return launch_laswp<float>(N, A, lda, k1, k2, ipiv, inc, s);
 }
 hipError_t atlmx_k_idamax(int n, const double *x, int incx, int *out, hipStream_t s) {
   return launch_iamax<double>(n, x, incx, out, s);

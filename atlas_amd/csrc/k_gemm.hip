@@ -27,9 +27,15 @@ namespace {
 
 constexpr int BM = ATLMX_GEMM_BM;
 constexpr int BN = ATLMX_GEMM_BN;
-constexpr int BK = 16;
 constexpr int NT = 256;
-constexpr int GROUP_M = 8;
+#ifndef ATLMX_GEMM_BK
+#define ATLMX_GEMM_BK 32
+#endif
+#ifndef ATLMX_GEMM_GROUP_M
+#define ATLMX_GEMM_GROUP_M 8
+#endif
+constexpr int BKD = ATLMX_GEMM_BK;
+constexpr int GROUP_M = ATLMX_GEMM_GROUP_M;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -44,8 +50,8 @@ template <> struct Mfma<double> {
   __device__ static acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
-  /* v_mfma_f64_16x16x4f64: lane l, element r holds C[(l>>4) + 4r][l&15] */
-  __device__ static int row(int lane, int r) { return (lane >> 4) + 4 * r; }
+  /* v_mfma_f64_16x16x4f64: lane l, element r holds D[(l>>4) + 4r][l&15] */
+  __device__ static int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
 };
 template <> struct Mfma<float> {
   typedef f4 acc_t;
@@ -55,8 +61,8 @@ template <> struct Mfma<float> {
   __device__ static acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-  /* v_mfma_f32_16x16x4f32: lane l, element r holds C[4(l>>4) + r][l&15] */
-  __device__ static int row(int lane, int r) { return 4 * (lane >> 4) + r; }
+  /* v_mfma_f32_16x16x4f32: lane l, element r holds D[4(l>>4) + r][l&15] */
+  __device__ static int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 template <typename T>
@@ -78,7 +84,7 @@ struct GemmArgs {
  * transposed, or op(B) with B not transposed).  rows = extent in M (or N).
  * Element (r, k) of the slice lives at P[k + r*ld] when KCONT, else at
  * P[r + k*ld]. */
-template <typename T, bool KCONT, int ROWS>
+template <typename T, bool KCONT, int ROWS, int BK>
 struct SliceLoader {
   typedef Mfma<T> MF;
   static constexpr int VEC = MF::VEC;
@@ -132,7 +138,7 @@ struct SliceLoader {
   }
 };
 
-template <typename T, int TA, int TB>
+template <typename T, int TA, int TB, int BK = BKD>
 __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> g) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
@@ -172,8 +178,8 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> g) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  SliceLoader<T, A_KCONT, BM> la;
-  SliceLoader<T, B_KCONT, BN> lb;
+  SliceLoader<T, A_KCONT, BM, BK> la;
+  SliceLoader<T, B_KCONT, BN, BK> lb;
   const int nk = (g.K + BK - 1) / BK;
 
   if (nk > 0) {
@@ -184,7 +190,11 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> g) {
   }
   __syncthreads();
 
-  const int kq = (lane >> 4) * 4;        /* this lane's 4 K values in a slice */
+  /* MFMA k-slot (lane>>4) of sub-step s holds K index 4*(lane>>4) + s of a
+   * 16-wide K group, so each lane reads its 4 K values with one LDS load.
+   * The product is formed transposed, D = B^T * A^T, so a lane's output
+   * elements run down a column of C (coalesced stores). */
+  const int kq = (lane >> 4) * 4;
   const int lr = lane & 15;
   for (int kt = 0; kt < nk; ++kt) {
     const int cur = kt & 1;
@@ -192,20 +202,23 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> g) {
       la.load(g.A, g.lda, m0, (kt + 1) * BK, g.M, g.K, g.vec_a, tid);
       lb.load(g.B, g.ldb, n0, (kt + 1) * BK, g.N, g.K, g.vec_b, tid);
     }
-    T a[4][4], b[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const T *pa = &As[cur][(wm + i * 16 + lr) * LDK + kq];
-      const T *pb = &Bs[cur][(wn + i * 16 + lr) * LDK + kq];
+    for (int kg = 0; kg < BK / 16; ++kg) {
+      T a[4][4], b[4][4];
 #pragma unroll
-      for (int s = 0; s < 4; ++s) { a[i][s] = pa[s]; b[i][s] = pb[s]; }
+      for (int i = 0; i < 4; ++i) {
+        const T *pa = &As[cur][(wm + i * 16 + lr) * LDK + kg * 16 + kq];
+        const T *pb = &Bs[cur][(wn + i * 16 + lr) * LDK + kg * 16 + kq];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) { a[i][s] = pa[s]; b[i][s] = pb[s]; }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j][s], a[i][s], acc[i][j]);
     }
-#pragma unroll
-    for (int s = 0; s < 4; ++s)
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(a[i][s], b[j][s], acc[i][j]);
     if (kt + 1 < nk) {
       la.store(As[cur ^ 1], tid);
       lb.store(Bs[cur ^ 1], tid);
@@ -220,8 +233,9 @@ __global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> g) {
     for (int j = 0; j < 4; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + MF::row(lane, r);
-        const int col = n0 + wn + j * 16 + lr;
+        /* acc[i][j] holds D = (C tile)^T: D[n][m] */
+        const int row = m0 + wm + i * 16 + lr;
+        const int col = n0 + wn + j * 16 + MF::drow(lane, r);
         if (row < g.M && col < g.N) {
           if (g.tri == KX_LOWER && row < col) continue;
           if (g.tri == KX_UPPER && row > col) continue;

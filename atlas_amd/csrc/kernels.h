@@ -61,25 +61,48 @@ hipError_t atlmx_k_dscal2d(int M, int N, double alpha, double *B, long ldb,
 hipError_t atlmx_k_sscal2d(int M, int N, float alpha, float *B, long ldb,
                            hipStream_t s);
 
-/* LU panel: factor the m x w column panel P (ld) with partial pivoting.
- * ipiv[c] receives the absolute pivot row (row0 + panel row); info (device
- * int) gets row0+c+1 for the first exactly-zero pivot if it is still 0.
- * scratch: device buffer of atlmx_k_panel_scratch_bytes(m, w) bytes, its
- * first 4 bytes zeroed before the launch (barrier counter). */
+/* LU panel: factor the m x w column panel P (ld), w <= ATLMX_PANEL_MAXW
+ * (w <= ATLMX_PANEL_MAXW/2 when m > ATLMX_PANEL_TALL), m <= 2*65536, with
+ * partial pivoting, in one cooperative launch.  ipiv[c] receives the
+ * absolute pivot row (row0 + panel row); info (device int) gets row0+c+1
+ * for the first exactly-zero pivot if it is still 0.  scratch: device
+ * buffer of atlmx_k_panel_scratch_bytes() bytes, initialised once with
+ * atlmx_k_panel_scratch_init(); consecutive launches on one scratch must
+ * alternate set = 0, 1, 0, ... starting from 0 after the init. */
+#define ATLMX_PANEL_MAXW 64
+#define ATLMX_PANEL_TALL 65536
 size_t atlmx_k_panel_scratch_bytes(int m, int w);
+hipError_t atlmx_k_panel_scratch_init(void *scratch, hipStream_t s);
 hipError_t atlmx_k_dgetf2_panel(int m, int w, double *P, long ld, int row0,
-                                int *ipiv, int *info, void *scratch,
+                                int *ipiv, int *info, void *scratch, int set,
                                 hipStream_t s);
 hipError_t atlmx_k_sgetf2_panel(int m, int w, float *P, long ld, int row0,
-                                int *ipiv, int *info, void *scratch,
+                                int *ipiv, int *info, void *scratch, int set,
                                 hipStream_t s);
 
-/* Apply the row interchanges ipiv[k1..k2) (absolute row numbers, forward
- * order when inc > 0, reverse when inc < 0) to the N columns of A. */
+/* Apply the row interchanges ipiv[k1..k2) (absolute row numbers below
+ * rows_end, forward order when inc > 0, reverse when inc < 0) to the N
+ * columns of A.  Up to 256 columns with rows_end-k1 <= 16384 replay the
+ * swaps in LDS (one workgroup per column); otherwise one thread per
+ * column. */
 hipError_t atlmx_k_dlaswp(int N, double *A, long lda, int k1, int k2,
-                          const int *ipiv, int inc, hipStream_t s);
+                          const int *ipiv, int inc, int rows_end, hipStream_t s);
 hipError_t atlmx_k_slaswp(int N, float *A, long lda, int k1, int k2,
-                          const int *ipiv, int inc, hipStream_t s);
+                          const int *ipiv, int inc, int rows_end, hipStream_t s);
+
+/* Forward interchanges k1..k2 (k2-k1 <= 2*ATLMX_PANEL_MAXW) of rows below
+ * rows_end (rows_end-k1 <= ATLMX_LASWP_PLAN_MAXROWS) as a move list:
+ * plan[0] = count, plan[2+2i] = destination row, plan[3+2i] = source row;
+ * plan holds ATLMX_LASWP_PLAN_INTS ints.  apply moves the rows in columns
+ * [c0, c1) minus [s0, s1). */
+#define ATLMX_LASWP_PLAN_MAXROWS 32768
+#define ATLMX_LASWP_PLAN_INTS (2 + 8 * ATLMX_PANEL_MAXW)
+hipError_t atlmx_k_laswp_plan(int k1, int k2, int rows_end, const int *ipiv,
+                              int *plan, hipStream_t s);
+hipError_t atlmx_k_dlaswp_apply(double *A, long lda, int c0, int c1, int s0,
+                                int s1, const int *plan, hipStream_t s);
+hipError_t atlmx_k_slaswp_apply(float *A, long lda, int c0, int c1, int s0,
+                                int s1, const int *plan, hipStream_t s);
 
 /* Cholesky of one n x n diagonal block (n <= 128) in place (lower or
  * upper).  On a non-positive pivot sets *info = info_base + k + 1 (if still

@@ -40,17 +40,20 @@ void ssyrk_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, float alpha,
 #define PFX d
 #define K_PANEL atlmx_k_dgetf2_panel
 #define K_LASWP atlmx_k_dlaswp
+#define K_LASWP_APPLY atlmx_k_dlaswp_apply
 #define K_POTF2 atlmx_k_dpotf2
 #include "lapack_tmpl.h"
 #undef T
 #undef PFX
 #undef K_PANEL
 #undef K_LASWP
+#undef K_LASWP_APPLY
 #undef K_POTF2
 
 #define T float
 #define PFX s
 #define K_PANEL atlmx_k_sgetf2_panel
 #define K_LASWP atlmx_k_slaswp
+#define K_LASWP_APPLY atlmx_k_slaswp_apply
 #define K_POTF2 atlmx_k_spotf2
 #include "lapack_tmpl.h"

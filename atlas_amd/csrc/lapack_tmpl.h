@@ -9,12 +9,38 @@
 #define POTRF_DEV NM(PFX, potrf_dev)
 #define POTRS_DEV NM(PFX, potrs_dev)
 #define PANEL_REC NM(PFX, panel_rec)
+#define LASWP_ROWS NM(PFX, laswp_rows)
 
 /* Width of the outer LU block (the K of the trailing GEMM) and of the
  * cooperative panel leaf. */
-#define LU_NB 256
-#define LU_PW 32
+#define LU_NB 128
+#define LU_PW ATLMX_PANEL_MAXW
 #define CHOL_NB ATLMX_CHOL_NB
+
+/* Forward interchanges k1..k2 applied to columns [c0, c1) minus [s0, s1):
+ * as one move list for the whole column range when it fits the plan
+ * kernel, else per-column swaps. */
+static void LASWP_ROWS(atlmx_ctx *c, T *A, long lda, int M, int k1, int k2, const int *ipiv,
+                       int c0, int c1, int s0, int s1, const char *rout)
+{
+  int pid;
+  if (k2 <= k1 || c1 <= c0) return;
+  pid = rt_prof_begin(c, PROF_LASWP, 0.0);
+  if (k2 - k1 <= 2 * ATLMX_PANEL_MAXW && M - k1 <= ATLMX_LASWP_PLAN_MAXROWS) {
+    int *plan = (int *)rt_ws(c, WS_PLAN, ATLMX_LASWP_PLAN_INTS * sizeof(int), rout);
+    RT_CHECK(rout, atlmx_k_laswp_plan(k1, k2, M, ipiv, plan, c->stream));
+    RT_CHECK(rout, K_LASWP_APPLY(A, lda, c0, c1, s0, s1, plan, c->stream));
+  } else {
+    const int e0 = (s0 < c1) ? s0 : c1, b1 = (s1 > c0) ? s1 : c0;
+    if (e0 > c0)
+      RT_CHECK(rout, K_LASWP(e0 - c0, A + (size_t)c0 * lda, lda, k1, k2, ipiv, 1, M,
+                             c->stream));
+    if (c1 > b1)
+      RT_CHECK(rout, K_LASWP(c1 - b1, A + (size_t)b1 * lda, lda, k1, k2, ipiv, 1, M,
+                             c->stream));
+  }
+  rt_prof_end(c, pid);
+}
 
 /* Factor the tall panel A[r0:M, r0:r0+w] with partial pivoting.
  * Splits like src/lapack/ATL_getrfC.c:277-300 (left half, laswp + TRSM +
@@ -23,23 +49,21 @@
 static void PANEL_REC(atlmx_ctx *c, int M, T *A, long lda, int r0, int w, int *ipiv,
                       int *info, void *scratch, const char *rout)
 {
-  if (w <= LU_PW) {
+  const int leaf = (M - r0 > ATLMX_PANEL_TALL) ? LU_PW / 2 : LU_PW;
+  if (w <= leaf) {
     int pid;
-    RT_CHECK(rout, atlmx_k_fill_i32((int *)scratch, 1, 0, c->stream));
     pid = rt_prof_begin(c, PROF_PANEL, (double)(M - r0) * w * 2.0 * sizeof(T));
     RT_CHECK(rout, K_PANEL(M - r0, w, A + r0 + (size_t)r0 * lda, lda, r0, ipiv + r0, info,
-                           scratch, c->stream));
+                           scratch, c->panel_set, c->stream));
+    c->panel_set ^= 1;
     rt_prof_end(c, pid);
     return;
   } else {
-    int wl = (w / 2 / LU_PW) * LU_PW;
-    int pid;
-    if (wl < LU_PW) wl = LU_PW;
+    int wl = (w / 2 / 32) * 32;
+    if (wl < 32) wl = 32;
+    if (wl > leaf) wl = leaf;
     PANEL_REC(c, M, A, lda, r0, wl, ipiv, info, scratch, rout);
-    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
-    RT_CHECK(rout, K_LASWP(w - wl, A + (size_t)(r0 + wl) * lda, lda, r0, r0 + wl, ipiv, 1,
-                           c->stream));
-    rt_prof_end(c, pid);
+    LASWP_ROWS(c, A, lda, M, r0, r0 + wl, ipiv, r0 + wl, r0 + w, 0, 0, rout);
     NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, wl, w - wl, (T)1,
                       A + r0 + (size_t)r0 * lda, lda, A + r0 + (size_t)(r0 + wl) * lda, lda,
                       NULL, rout);
@@ -49,10 +73,7 @@ static void PANEL_REC(atlmx_ctx *c, int M, T *A, long lda, int r0, int w, int *i
                         A + r0 + (size_t)(r0 + wl) * lda, lda, (T)1,
                         A + (r0 + wl) + (size_t)(r0 + wl) * lda, lda, 0, NULL, rout);
     PANEL_REC(c, M, A, lda, r0 + wl, w - wl, ipiv, info, scratch, rout);
-    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
-    RT_CHECK(rout, K_LASWP(wl, A + (size_t)r0 * lda, lda, r0 + wl, r0 + w, ipiv, 1,
-                           c->stream));
-    rt_prof_end(c, pid);
+    LASWP_ROWS(c, A, lda, M, r0 + wl, r0 + w, ipiv, r0, r0 + wl, 0, 0, rout);
   }
 }
 
@@ -70,16 +91,17 @@ int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const cha
   if (MN <= 0) return 0;
   d_info = (int *)rt_ws(c, WS_INFO, 256, rout);
   scratch = rt_ws(c, WS_PANEL, atlmx_k_panel_scratch_bytes(M, LU_PW), rout);
+  if (c->panel_buf != scratch) {
+    RT_CHECK(rout, atlmx_k_panel_scratch_init(scratch, c->stream));
+    c->panel_buf = scratch;
+    c->panel_set = 0;
+  }
   RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
+  (void)pid;
   for (j = 0; j < MN; j += LU_NB) {
     const int jb = (MN - j < LU_NB) ? MN - j : LU_NB;
     PANEL_REC(c, M, A, lda, j, jb, d_ipiv, d_info, scratch, rout);
-    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
-    if (j > 0) RT_CHECK(rout, K_LASWP(j, A, lda, j, j + jb, d_ipiv, 1, c->stream));
-    if (j + jb < N)
-      RT_CHECK(rout, K_LASWP(N - j - jb, A + (size_t)(j + jb) * lda, lda, j, j + jb, d_ipiv,
-                             1, c->stream));
-    rt_prof_end(c, pid);
+    LASWP_ROWS(c, A, lda, M, j, j + jb, d_ipiv, 0, N, j, j + jb, rout);
     if (j + jb < N) {
       NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, jb, N - j - jb, (T)1,
                         A + j + (size_t)j * lda, lda, A + j + (size_t)(j + jb) * lda, lda,
@@ -104,7 +126,7 @@ void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
 {
   if (N <= 0 || NRHS <= 0) return;
   if (trans == KX_NOTRANS) {
-    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, 1, c->stream));
+    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, 1, N, c->stream));
     NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
                       ldb, NULL, rout);
     NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda,
@@ -114,7 +136,7 @@ void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
                       ldb, NULL, rout);
     NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
                       ldb, NULL, rout);
-    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, -1, c->stream));
+    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, -1, N, c->stream));
   }
 }
 
@@ -515,3 +537,4 @@ int NM3(clapack_, PFX, posv)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO
 #undef POTRF_DEV
 #undef POTRS_DEV
 #undef PANEL_REC
+#undef LASWP_ROWS

@@ -19,6 +19,7 @@ enum {
   WS_STAGE_B,
   WS_STAGE_C,
   WS_IPIV,
+  WS_PLAN,          /* row-interchange move lists               */
   WS_MP0,           /* mixed-precision solver buffers          */
   WS_MP1,
   WS_MP2,
@@ -30,9 +31,11 @@ enum {
 typedef struct atlmx_ctx {
   int dev;
   hipStream_t stream;       /* stream every launch of this thread uses */
-  hipStream_t own;          /* the library's stream for this thread    */
+  hipStream_t own;          /* default stream of this thread (0: legacy default) */
   void *ws[WS_NSLOTS];
   size_t ws_bytes[WS_NSLOTS];
+  void *panel_buf;          /* LU panel exchange buffer last initialised */
+  int panel_set;            /* header set the next panel launch uses     */
 } atlmx_ctx;
 
 atlmx_ctx *rt_ctx(void);

@@ -7,7 +7,7 @@
  * host thread gets one HIP stream and a set of grow-only device workspaces
  * that persist across calls (hipMalloc is far too slow to do per call), so
  * the library is callable from any host thread with no visible global
- * mutable state (SURVEY boundary contract, "Threading").
+ * mutable state.
  */
 #include <pthread.h>
 #include <stdint.h>
@@ -34,9 +34,11 @@ atlmx_ctx *rt_ctx(void)
   if (!c) { fprintf(stderr, "atlas_mi355x: out of host memory\n"); exit(-1); }
   e = hipGetDevice(&c->dev);
   if (e != hipSuccess) rt_fail("atlas_mi355x", "hipGetDevice", e);
-  e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking);
-  if (e != hipSuccess) rt_fail("atlas_mi355x", "hipStreamCreateWithFlags", e);
-  c->stream = c->own;
+  /* Default: the legacy default stream, so device operands written by the
+   * caller on that stream (or any blocking stream) are ready before the
+   * library reads them. */
+  c->own = 0;
+  c->stream = 0;
   tls_ctx = c;
   return c;
 }
@@ -109,14 +111,14 @@ void rt_stage_out(atlmx_ctx *c, void *h, const void *d, int rows, int cols, long
 typedef struct { hipEvent_t a, b; int cls; double work; } prof_rec;
 static __thread prof_rec *prof_buf = NULL;
 static __thread int prof_n = 0;
-static __thread int prof_enabled = 0;
+static __thread int prof_enabled = 0;   /* bit mask of recorded classes */
 
 int rt_prof_on(void) { return prof_enabled; }
 
 int rt_prof_begin(atlmx_ctx *c, int cls, double work)
 {
   prof_rec *r;
-  if (!prof_enabled || prof_n >= PROF_MAX) return -1;
+  if (!(prof_enabled & (1 << cls)) || prof_n >= PROF_MAX) return -1;
   r = &prof_buf[prof_n];
   if (!r->a) {
     RT_CHECK("atlmx_prof", hipEventCreate(&r->a));
@@ -141,10 +143,11 @@ int atlmx_init(void)
   return 0;
 }
 
+/* NULL selects the legacy default stream. */
 void atlmx_set_stream(void *stream)
 {
   atlmx_ctx *c = rt_ctx();
-  c->stream = stream ? (hipStream_t)stream : c->own;
+  c->stream = (hipStream_t)stream;
 }
 
 void *atlmx_get_stream(void)
@@ -158,13 +161,14 @@ void atlmx_synchronize(void)
   rt_sync(c, "atlmx_synchronize");
 }
 
-void atlmx_prof_enable(int on)
+/* mask: bit c records launch class c (0 = off; 1 = GEMM only; -1 = all) */
+void atlmx_prof_enable(int mask)
 {
-  if (on && !prof_buf) {
+  if (mask && !prof_buf) {
     prof_buf = (prof_rec *)calloc(PROF_MAX, sizeof(prof_rec));
     if (!prof_buf) return;
   }
-  prof_enabled = on ? 1 : 0;
+  prof_enabled = mask;
   prof_n = 0;
 }
 

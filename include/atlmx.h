@@ -17,15 +17,16 @@ extern "C" {
 
 /* ---- runtime ------------------------------------------------------------ */
 int   atlmx_init(void);                 /* optional; creates this thread's stream */
-void  atlmx_set_stream(void *hip_stream); /* NULL: the library's own stream      */
+void  atlmx_set_stream(void *hip_stream); /* NULL: the legacy default stream (the default) */
 void *atlmx_get_stream(void);
 void  atlmx_synchronize(void);
 
-/* ---- profiler: hipEvent timing of every launch of a class on this thread's
- * stream while enabled.  Classes: 0 GEMM (work = 2*M*N*K flops), 1 LU panel
+/* ---- profiler: hipEvent timing of every launch of the enabled classes on
+ * this thread's stream.  Classes: 0 GEMM (work = 2*M*N*K flops), 1 LU panel
  * (work = bytes loaded+stored), 2 TRSM diagonal inversion, 3 laswp,
- * 4 Cholesky diagonal block, 5 other. ---- */
-void atlmx_prof_enable(int on);
+ * 4 Cholesky diagonal block, 5 other.  mask bit c enables class c (0 = off,
+ * -1 = all); enabling clears earlier records. ---- */
+void atlmx_prof_enable(int mask);
 void atlmx_prof_reset(void);
 int  atlmx_prof_query(int cls, double *ms_total, long *launches, double *work_total);
 
@@ -54,6 +55,24 @@ int  atlmx_dpotrf(int Uplo, int N, double *A, int lda);              /* synchron
 int  atlmx_spotrf(int Uplo, int N, float *A, int lda);               /* synchronizes */
 void atlmx_dpotrs(int Uplo, int N, int NRHS, const double *A, int lda, double *B, int ldb);
 void atlmx_spotrs(int Uplo, int N, int NRHS, const float *A, int lda, float *B, int ldb);
+
+/* ---- FP64-accurate solve from a low-precision LU + FP64 refinement ------
+ * A (N x N), B (N x NRHS), X (N x NRHS) are FP64 device arrays; A and B are
+ * not modified.  The LU is done in FP32; the residual, the update and the
+ * stopping test (LAPACK dsgesv's: ||r|| <= ||x|| ||A|| eps sqrt(N)) are
+ * FP64.  Falls back to the FP64 LU if refinement does not converge in
+ * iter_max steps.  Synchronizes; returns 0 or the FP64 LU's info. */
+typedef struct {
+  int iterations;          /* refinement steps taken (max over RHS groups)  */
+  int converged;           /* stopping test met without fallback            */
+  int fallback;            /* solved by the FP64 LU instead                 */
+  int factor_info;         /* info of the low-precision factorization       */
+  double backward_error;   /* max_j ||b-Ax||inf / (||A||inf ||x||inf + ||b||inf) */
+} atlmx_ir_info;
+int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
+                 double *X, int ldx, int iter_max, atlmx_ir_info *info);
+double atlmx_dbackward_error(int N, int NRHS, const double *A, int lda, const double *B,
+                             int ldb, const double *X, int ldx);   /* synchronizes */
 
 #ifdef __cplusplus
 }
