@@ -18,6 +18,7 @@
 #define K_TRI_INV atlmx_k_dtri_inv
 #define K_SCAL atlmx_k_dscal2d
 #define K_IAMAX atlmx_k_idamax
+#define K_TRSV atlmx_k_dtrsv
 #include "blas3_tmpl.h"
 #undef T
 #undef PFX
@@ -25,6 +26,7 @@
 #undef K_TRI_INV
 #undef K_SCAL
 #undef K_IAMAX
+#undef K_TRSV
 
 #define T float
 #define PFX s
@@ -32,4 +34,5 @@
 #define K_TRI_INV atlmx_k_stri_inv
 #define K_SCAL atlmx_k_sscal2d
 #define K_IAMAX atlmx_k_isamax
+#define K_TRSV atlmx_k_strsv
 #include "blas3_tmpl.h"

@@ -43,7 +43,8 @@ void GEMM_DEV(atlmx_ctx *c, int ta, int tb, int M, int N, int K, T alpha, const 
  * utrsm diagonal solves with GEMM updates.  Here the 64x64 diagonal blocks
  * are inverted in one launch, then per diagonal block one in-place GEMM
  * (X_k = op(inv A_kk) * B_k, a single 128-row tile, so it reads B_k before
- * writing it) and one GEMM update of the rest of B.
+ * writing it) and one GEMM update of the rest of B.  Left-side solves with
+ * at most ATLMX_TRSV_MAXRHS columns run as one flag-chained launch.
  * ---------------------------------------------------------------------- */
 void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, T alpha,
               const T *A, long lda, T *B, long ldb, const int *abort_flag,
@@ -67,7 +68,15 @@ void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, 
   RT_CHECK(rout, K_TRI_INV(uplo, diag, na, A, lda, inv, abort_flag, c->stream));
   rt_prof_end(c, pid);
 
-  if (side == KX_LEFT) {
+  if (side == KX_LEFT && N <= ATLMX_TRSV_MAXRHS && !abort_flag) {
+    /* few right-hand sides: the solve is bound by reading the factor once,
+     * so one flag-chained launch replaces 2*nblk small GEMMs */
+    int *ctl = (int *)rt_ws(c, WS_TRSV, (size_t)(nblk + 1) * sizeof(int), rout);
+    RT_CHECK(rout, atlmx_k_fill_i32(ctl, nblk + 1, 0, c->stream));
+    pid = rt_prof_begin(c, PROF_TRSV, (double)na * na / 2 * sizeof(T));
+    RT_CHECK(rout, K_TRSV(uplo, tinv, M, N, A, lda, inv, B, ldb, ctl, c->stream));
+    rt_prof_end(c, pid);
+  } else if (side == KX_LEFT) {
     const int fwd = opA_lower;
     for (step = 0; step < nblk; ++step) {
       const int k = fwd ? step : nblk - 1 - step;

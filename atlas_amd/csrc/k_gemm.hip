@@ -28,13 +28,9 @@ namespace {
 constexpr int BM = ATLMX_GEMM_BM;
 constexpr int BN = ATLMX_GEMM_BN;
 constexpr int NT = 256;
-#ifndef ATLMX_GEMM_BK
-#define ATLMX_GEMM_BK 32
-#endif
 #ifndef ATLMX_GEMM_GROUP_M
 #define ATLMX_GEMM_GROUP_M 8
 #endif
-constexpr int BKD = ATLMX_GEMM_BK;
 constexpr int GROUP_M = ATLMX_GEMM_GROUP_M;
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -46,6 +42,7 @@ template <> struct Mfma<double> {
   typedef d4 acc_t;
   typedef d2 vec_t;                 /* one 16-byte global load */
   static constexpr int VEC = 2;
+  static constexpr int BK = 16;     /* 2 x 36 KB LDS: two workgroups per CU */
   static constexpr int PADK = 2;    /* LDS row = 18 doubles = 144 B */
   __device__ static acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -57,6 +54,7 @@ template <> struct Mfma<float> {
   typedef f4 acc_t;
   typedef f4 vec_t;
   static constexpr int VEC = 4;
+  static constexpr int BK = 16;     /* 2 x 20 KB LDS: three workgroups per CU */
   static constexpr int PADK = 4;    /* LDS row = 20 floats = 80 B */
   __device__ static acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -138,10 +136,15 @@ struct SliceLoader {
   }
 };
 
-template <typename T, int TA, int TB, int BK = BKD>
-__global__ __launch_bounds__(NT) void gemm_kernel(GemmArgs<T> g) {
+/* Two workgroups per CU (at most 256 registers per lane, 2 x 72 KB LDS):
+ * one workgroup's prologue, barriers and C epilogue overlap the other's
+ * MFMA stream, which matters most for the short-K trailing updates of the
+ * LU and Cholesky. */
+template <typename T, int TA, int TB>
+__global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
+  constexpr int BK = MF::BK;
   constexpr int LDK = BK + MF::PADK;
   constexpr bool A_KCONT = (TA != KX_NOTRANS);   /* op(A)[m][k] = A[k + m*lda] */
   constexpr bool B_KCONT = (TB == KX_NOTRANS);   /* op(B)[k][n] = B[k + n*ldb] */

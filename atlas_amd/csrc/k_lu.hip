@@ -31,6 +31,7 @@
 #include <float.h>
 #include <stdint.h>
 #include "kernels.h"
+#include "wave_argmax.h"
 
 namespace {
 
@@ -94,18 +95,18 @@ __device__ inline PanelSlots slots_view(void *base) {
   return s;
 }
 
-/* The panel rows of one workgroup live in LDS, column-major with a padded
- * column stride (R+1), so column c of all rows is a conflict-free read and
- * a whole row is a conflict-free gather. */
+/* The panel rows of one workgroup live in LDS, row-major with a padded row
+ * stride (W+1): a thread's own row is contiguous (its rank-1 update streams
+ * through it), and column c of all rows is a stride-(W+1) gather. */
 template <typename T, int W, int R>
-__global__ __launch_bounds__(NT) void getf2_panel_kernel(int m, int w, T *P, long ld,
+__global__ __launch_bounds__(NT) void getf2_panel_lds_kernel(int m, int w, T *P, long ld,
                                                          int row0, int *ipiv, int *info,
                                                          void *scratch, int set) {
   constexpr int RPT = R / NT;
-  constexpr int LDR = R + 1;
+  constexpr int LDW = W + 1;   /* odd stride: lanes hit distinct LDS banks */
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
-  T *rows = (T *)smem;                         /* [W][LDR] */
-  __shared__ T s_wv[NT / 64];
+  T *rows = (T *)smem;                         /* [R][LDW] */
+  __shared__ double s_wv[NT / 64];
   __shared__ int s_wi[NT / 64];
   __shared__ T s_prow[W], s_drow[W];
   __shared__ int s_piv;
@@ -129,81 +130,78 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(int m, int w, T *P, lon
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int r = rbase + q * NT + tid;
-      rows[k * LDR + q * NT + tid] = (r < m) ? P[(size_t)k * ld + r] : T(0);
+      rows[(q * NT + tid) * LDW + k] = (r < m) ? P[(size_t)k * ld + r] : T(0);
     }
   __syncthreads();
 
   for (int c = 0; c < w; ++c) {
     const int par = c & 1;
     /* ---- local arg-max over my rows >= c ---- */
-    T bv = T(-1);
-    int bi = 0x7fffffff;
+    double bv = -2.0;
+    int bi = 0x7fffffff, bz = 0;
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int r = rbase + q * NT + tid;
       if (r < m && r >= c) {
-        T v = rows[c * LDR + q * NT + tid];
-        v = v < T(0) ? -v : v;
-        if (!(v == v)) v = T(-1);
-        amax_combine(bv, bi, v, r);
+        double v = (double)rows[(q * NT + tid) * LDW + c];
+        v = v < 0.0 ? -v : v;
+        if (!(v == v)) v = -1.0;
+        atlmx::argmax_combine(bv, bi, bz, v, r, 0);
       }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const T v2 = __shfl_xor(bv, off);
-      const int i2 = __shfl_xor(bi, off);
-      amax_combine(bv, bi, v2, i2);
-    }
+    atlmx::wave_argmax(bv, bi, bz);
     if (lane == 0) { s_wv[wave] = bv; s_wi[wave] = bi; }
     __syncthreads();
 
     if (wave == 0) {
-      T wv = s_wv[0];
+      double wv = s_wv[0];
       int wi = s_wi[0];
 #pragma unroll
-      for (int k = 1; k < NT / 64; ++k) amax_combine(wv, wi, s_wv[k], s_wi[k]);
+      for (int k = 1; k < NT / 64; ++k) atlmx::argmax_combine(wv, wi, bz, s_wv[k], s_wi[k], 0);
+      if (wv < 0.0) wi = 0x7fffffff;    /* only NaN or no rows: nothing to offer */
       /* ---- publish: one coherent store per lane, then the header ---- */
       if (lane < w) {
         if (wi != 0x7fffffff)
           st_coh(&S.crow[((size_t)par * MAXG + g) * MAXW + lane],
-                 (double)rows[lane * LDR + (wi - rbase)]);
-        if (g == 0) st_coh(&S.drow[(size_t)par * MAXW + lane], (double)rows[lane * LDR + c]);
+                 (double)rows[(wi - rbase) * LDW + lane]);
+        if (g == 0) st_coh(&S.drow[(size_t)par * MAXW + lane], (double)rows[c * LDW + lane]);
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      /* the row must reach the coherence point before the header can:
+       * wait for the stores' acknowledgements (vmcnt = 0) */
+      __builtin_amdgcn_s_waitcnt(0);
       if (lane == 0) {
         st_coh(&hv[(size_t)c * MAXG + g], __double_as_longlong((double)wv));
         st_coh(&hi[(size_t)c * MAXG + g], (long long)wi);
       }
       /* ---- poll the G headers, reduce, fetch the pivot row ---- */
-      double v = -1.0;
+      double v = -2.0;
       int i = 0x7fffffff, gi = 0;
       for (int k0 = 0; k0 < G; k0 += 64) {
         const int k = k0 + lane;
-        double v2 = -1.0;
-        int i2 = 0x7fffffff;
         if (k < G) {
           long long hx, hy;
           do {
             hx = ld_coh(&hv[(size_t)c * MAXG + k]);
             hy = ld_coh(&hi[(size_t)c * MAXG + k]);
           } while (hx == SENT || hy == SENT);
-          v2 = __longlong_as_double(hx);
-          i2 = (int)hy;
+          atlmx::argmax_combine(v, i, gi, __longlong_as_double(hx), (int)hy, k);
         }
-        if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; gi = k; }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double v2 = __shfl_xor(v, off);
-        const int i2 = __shfl_xor(i, off);
-        const int g2 = __shfl_xor(gi, off);
-        if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; gi = g2; }
-      }
+      atlmx::wave_argmax(v, i, gi);
       /* all |a| zero or NaN: iamax answers the first row, i.e. c */
       const int p = (v > 0.0) ? i : c;
       if (lane < w) {
         const T dr = (T)ld_coh(&S.drow[(size_t)par * MAXW + lane]);
-        s_drow[lane] = dr;
-        s_prow[lane] =
+        const T pr =
             (p == c) ? dr : (T)ld_coh(&S.crow[((size_t)par * MAXG + gi) * MAXW + lane]);
+        s_drow[lane] = dr;
+        s_prow[lane] = pr;
+        /* swap the whole panel rows c and p where this workgroup holds them,
+         * one lane per column (p != c implies a nonzero pivot) */
+        if (p != c) {
+          if (p >= rbase && p < rbase + R) rows[(p - rbase) * LDW + lane] = dr;
+          if (c >= rbase && c < rbase + R) rows[(c - rbase) * LDW + lane] = pr;
+        }
       }
       if (lane == 0) {
         s_piv = p;
@@ -215,22 +213,17 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(int m, int w, T *P, lon
     const int p = s_piv;
     const T pv = s_prow[c];
     if (pv != T(0)) {
-      /* swap the whole panel rows c and p (each may live in any workgroup) */
-      if (p != c && tid < w) {
-        if (p >= rbase && p < rbase + R) rows[tid * LDR + (p - rbase)] = s_drow[tid];
-        if (c >= rbase && c < rbase + R) rows[tid * LDR + (c - rbase)] = s_prow[tid];
-      }
-      __syncthreads();
       const bool recip = (pv < T(0) ? -pv : pv) >= Lim<T>::safmin();
       const T rcp = T(1) / pv;
 #pragma unroll
       for (int q = 0; q < RPT; ++q) {
         const int r = rbase + q * NT + tid;
+        T *row = rows + (q * NT + tid) * LDW;
         if (r < m && r > c) {
-          T *row = rows + q * NT + tid;
-          const T l = recip ? row[c * LDR] * rcp : row[c * LDR] / pv;
-          row[c * LDR] = l;
-          for (int k = c + 1; k < w; ++k) row[k * LDR] -= l * s_prow[k];
+          const T l = recip ? row[c] * rcp : row[c] / pv;
+          row[c] = l;
+#pragma unroll 8
+          for (int k = c + 1; k < w; ++k) row[k] -= l * s_prow[k];
         }
       }
     } else if (g == 0 && tid == 0) {
@@ -244,7 +237,7 @@ __global__ __launch_bounds__(NT) void getf2_panel_kernel(int m, int w, T *P, lon
 #pragma unroll
     for (int q = 0; q < RPT; ++q) {
       const int r = rbase + q * NT + tid;
-      if (r < m) P[(size_t)k * ld + r] = rows[k * LDR + q * NT + tid];
+      if (r < m) P[(size_t)k * ld + r] = rows[(q * NT + tid) * LDW + k];
     }
 }
 
@@ -257,14 +250,14 @@ __global__ void slots_init_kernel(long long *h, size_t n) {
 template <typename T, int W, int R>
 hipError_t launch_wr(int G, void **args, hipStream_t s) {
   static int attr_set = 0;
-  const int lds = W * (R + 1) * (int)sizeof(T);
+  const int lds = R * (W + 1) * (int)sizeof(T);
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute((const void *)getf2_panel_kernel<T, W, R>,
+    hipError_t e = hipFuncSetAttribute((const void *)getf2_panel_lds_kernel<T, W, R>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, lds);
     if (e != hipSuccess) return e;
     attr_set = 1;
   }
-  return hipLaunchCooperativeKernel((const void *)getf2_panel_kernel<T, W, R>, dim3(G),
+  return hipLaunchCooperativeKernel((const void *)getf2_panel_lds_kernel<T, W, R>, dim3(G),
                                     dim3(NT), args, (unsigned)lds, s);
 }
 

@@ -1,0 +1,211 @@
+/*
+ * k_trsv.hip -- triangular solve with a few right-hand sides, in ONE
+ * launch: the HBM-bound case of TRSM (getrs/potrs with NRHS = 1..4 reads
+ * the n^2/2 factor once).
+ *
+ * ATLAS runs this case through the same recursive TRSM as the many-RHS
+ * case (src/blas/ulevel3/ATL_trsm.c:13-228).  Here each 64-row block of the
+ * solution is owned by one workgroup.  Workgroups take block indices from
+ * an atomic counter in solve order, so every block a workgroup waits for
+ * was claimed by a workgroup that is already running (no deadlock, no
+ * cooperative launch needed).  Block k streams its row (or, transposed, its
+ * column) of 64x64 tiles of A, subtracting A_kj * x_j as each x_j is
+ * published, then multiplies by the inverted diagonal block (from
+ * atlmx_k_?tri_inv) and publishes x_k with a flag.  Published solution
+ * values and flags move with device-coherent (sc1) accesses.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "kernels.h"
+
+namespace {
+
+constexpr int TB = ATLMX_TRI_NB;   /* 64 */
+constexpr int NT = 256;
+constexpr int MAXR = ATLMX_TRSV_MAXRHS;
+
+__device__ inline double ldc(const double *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline float ldc(const float *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void stc(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void stc(float *p, float v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+/* op(A) lower (forward) when lower != trans.  X (n x nrhs, ldx) holds B on
+ * entry and the solution on exit.  ctl[0] = block counter, ctl[1 + k] =
+ * flag of block k (all zero on entry).  inv: inverted diagonal blocks. */
+template <typename T>
+__global__ __launch_bounds__(NT) void trsv_kernel(int n, int nrhs, int lower, int trans,
+                                                  const T *__restrict__ A, long lda,
+                                                  const T *__restrict__ inv, T *X, long ldx,
+                                                  int *ctl) {
+  __shared__ int s_idx;
+  __shared__ T s_part[4][TB][MAXR];
+  __shared__ T s_acc[TB][MAXR];
+  const int tid = threadIdx.x;
+  const int nblk = (n + TB - 1) / TB;
+  const bool fwd = (lower != 0) != (trans != 0);
+  if (tid == 0) s_idx = atomicAdd(&ctl[0], 1);
+  __syncthreads();
+  const int order = s_idx;
+  if (order >= nblk) return;
+  const int k = fwd ? order : nblk - 1 - order;
+  const int r0 = k * TB, rb = min(TB, n - r0);
+  const int ln = tid & 63, q = tid >> 6;
+
+  if (!trans) {
+    /* thread (row i = ln, column group q): coalesced down the columns */
+    T acc[MAXR];
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) acc[r] = T(0);
+    for (int s = 0; s < order; ++s) {
+      const int j = fwd ? s : nblk - 1 - s;
+      const int c0 = j * TB, cb = min(TB, n - c0);
+      /* the tile of A does not depend on x_j: load it before waiting */
+      T a[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int cc = q * 16 + t;
+        a[t] = (ln < rb && cc < cb) ? A[(size_t)(c0 + cc) * lda + (r0 + ln)] : T(0);
+      }
+      if (tid == 0)
+        while (__hip_atomic_load(&ctl[1 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+          __builtin_amdgcn_s_sleep(1);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int cc = q * 16 + t;
+        if (cc < cb) {
+#pragma unroll
+          for (int r = 0; r < MAXR; ++r)
+            if (r < nrhs) acc[r] += a[t] * ldc(&X[(size_t)r * ldx + c0 + cc]);
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r) s_part[q][ln][r] = acc[r];
+  } else {
+    /* thread (lane cc = ln of each tile, column group q): column segments of
+     * A are contiguous, so lanes read consecutive elements; the 64 lanes'
+     * partial dot products are added once at the end */
+    T acc[16][MAXR];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) acc[t][r] = T(0);
+    for (int s = 0; s < order; ++s) {
+      const int j = fwd ? s : nblk - 1 - s;
+      const int c0 = j * TB, cb = min(TB, n - c0);
+      /* op(A)[r0+i][c0+ln] = A[c0+ln][r0+i]; loaded before waiting */
+      T a[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const int i = q * 16 + t;
+        a[t] = (ln < cb && i < rb) ? A[(size_t)(r0 + i) * lda + (c0 + ln)] : T(0);
+      }
+      if (tid == 0)
+        while (__hip_atomic_load(&ctl[1 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+          __builtin_amdgcn_s_sleep(1);
+      __syncthreads();
+      T xv[MAXR];
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r)
+        xv[r] = (ln < cb && r < nrhs) ? ldc(&X[(size_t)r * ldx + c0 + ln]) : T(0);
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) acc[t][r] += a[t] * xv[r];
+    }
+    /* sum each (column i, rhs r) over the 64 lanes */
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) {
+        T v = acc[t][r];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        acc[t][r] = v;
+      }
+    if (ln == 0) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) s_part[0][q * 16 + t][r] = acc[t][r];
+    }
+    if (ln < 3) {
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) s_part[1 + ln][q * 16 + t][r] = T(0);
+    }
+  }
+  __syncthreads();
+  if (q == 0) {
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r)
+      s_acc[ln][r] = (ln < rb && r < nrhs)
+                         ? ldc(&X[(size_t)r * ldx + r0 + ln]) -
+                               (s_part[0][ln][r] + s_part[1][ln][r] + s_part[2][ln][r] +
+                                s_part[3][ln][r])
+                         : T(0);
+  }
+  __syncthreads();
+  /* x_k = op(inv_k) * acc: thread (i, q) sums its quarter of row i */
+  const T *iv = inv + (size_t)k * TB * TB;
+  T y[MAXR];
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) y[r] = T(0);
+  if (ln < rb) {
+    for (int t = 0; t < 16; ++t) {
+      const int cc = q * 16 + t;
+      const T v = trans ? iv[(size_t)ln * TB + cc] : iv[(size_t)cc * TB + ln];
+#pragma unroll
+      for (int r = 0; r < MAXR; ++r) y[r] += v * s_acc[cc][r];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < MAXR; ++r) s_part[q][ln][r] = y[r];
+  __syncthreads();
+  if (q == 0 && ln < rb) {
+#pragma unroll
+    for (int r = 0; r < MAXR; ++r)
+      if (r < nrhs)
+        stc(&X[(size_t)r * ldx + r0 + ln],
+            s_part[0][ln][r] + s_part[1][ln][r] + s_part[2][ln][r] + s_part[3][ln][r]);
+  }
+  /* x_k must reach the coherence point before the flag: wait for the
+   * stores' acknowledgements (vmcnt = 0), then the barrier orders the flag */
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  if (tid == 0) __hip_atomic_store(&ctl[1 + k], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename T>
+hipError_t launch_trsv(int uplo, int trans, int n, int nrhs, const T *A, long lda,
+                       const T *inv, T *X, long ldx, int *ctl, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  if (nrhs > MAXR || lda < n || ldx < n || !ctl) return hipErrorInvalidValue;
+  const int nblk = (n + TB - 1) / TB;
+  hipLaunchKernelGGL(trsv_kernel<T>, dim3(nblk), dim3(NT), 0, s, n, nrhs, uplo == KX_LOWER,
+                     trans == KX_TRANS, A, lda, inv, X, ldx, ctl);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" hipError_t atlmx_k_dtrsv(int uplo, int trans, int n, int nrhs, const double *A,
+                                    long lda, const double *inv, double *X, long ldx, int *ctl,
+                                    hipStream_t s) {
+  return launch_trsv<double>(uplo, trans, n, nrhs, A, lda, inv, X, ldx, ctl, s);
+}
+extern "C" hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const float *A,
+                                    long lda, const float *inv, float *X, long ldx, int *ctl,
+                                    hipStream_t s) {
+  return launch_trsv<float>(uplo, trans, n, nrhs, A, lda, inv, X, ldx, ctl, s);
+}

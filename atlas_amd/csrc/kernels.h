@@ -55,6 +55,18 @@ hipError_t atlmx_k_stri_inv(int uplo, int diag, int n, const float *A,
                             long lda, float *inv, const int *abort_flag,
                             hipStream_t s);
 
+/* op(A) X = B for a few right-hand sides (nrhs <= ATLMX_TRSV_MAXRHS) in
+ * one launch; X holds B on entry.  inv: the inverted 64x64 diagonal blocks
+ * of A from atlmx_k_?tri_inv (with the same uplo/diag).  ctl: device ints,
+ * 1 + ceil(n/64) of them, zero on entry. */
+#define ATLMX_TRSV_MAXRHS 4
+hipError_t atlmx_k_dtrsv(int uplo, int trans, int n, int nrhs, const double *A,
+                         long lda, const double *inv, double *X, long ldx,
+                         int *ctl, hipStream_t s);
+hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const float *A,
+                         long lda, const float *inv, float *X, long ldx,
+                         int *ctl, hipStream_t s);
+
 /* B(M x N) *= alpha */
 hipError_t atlmx_k_dscal2d(int M, int N, double alpha, double *B, long ldb,
                            hipStream_t s);

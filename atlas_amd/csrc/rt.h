@@ -20,6 +20,7 @@ enum {
   WS_STAGE_C,
   WS_IPIV,
   WS_PLAN,          /* row-interchange move lists               */
+  WS_TRSV,          /* few-RHS triangular solve counter + flags */
   WS_MP0,           /* mixed-precision solver buffers          */
   WS_MP1,
   WS_MP2,
@@ -63,7 +64,7 @@ void rt_stage_out(atlmx_ctx *c, void *h, const void *d, int rows, int cols, long
 
 /* ---- profiler: hipEvents around selected launches on the ctx stream ---- */
 enum { PROF_GEMM = 0, PROF_PANEL, PROF_TRSM_INV, PROF_LASWP, PROF_POTF2, PROF_MISC,
-       PROF_NCLASS };
+       PROF_TRSV, PROF_NCLASS };
 int rt_prof_on(void);
 int rt_prof_begin(atlmx_ctx *c, int cls, double work);
 void rt_prof_end(atlmx_ctx *c, int id);

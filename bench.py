@@ -181,7 +181,7 @@ def main():
     X.prof_enable(-1)
     tmp = time_steps(mp_step, args.steps, 0, torch)
     classes = {}
-    for cls, name in enumerate(["gemm", "panel", "trsm_inv", "laswp", "potf2"]):
+    for cls, name in enumerate(["gemm", "panel", "trsm_inv", "laswp", "potf2", "misc", "trsv"]):
         ms, cnt, work = X.prof_query(cls)
         if cnt:
             classes[name] = {"ms_per_step": round(ms / args.steps, 3),

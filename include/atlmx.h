@@ -24,7 +24,8 @@ void  atlmx_synchronize(void);
 /* ---- profiler: hipEvent timing of every launch of the enabled classes on
  * this thread's stream.  Classes: 0 GEMM (work = 2*M*N*K flops), 1 LU panel
  * (work = bytes loaded+stored), 2 TRSM diagonal inversion, 3 laswp,
- * 4 Cholesky diagonal block, 5 other.  mask bit c enables class c (0 = off,
+ * 4 Cholesky diagonal block, 5 other, 6 few-right-hand-side triangular
+ * solve (work = bytes of the factor read).  mask bit c enables class c (0 = off,
  * -1 = all); enabling clears earlier records. ---- */
 void atlmx_prof_enable(int mask);
 void atlmx_prof_reset(void);

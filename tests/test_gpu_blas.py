@@ -115,6 +115,40 @@ def test_trsm_all_variants_vs_oracle(dt):
             assert np.allclose(Xg, Xo, rtol=tol, atol=tol)
 
 
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_trsm_few_rhs_vs_oracle(dt):
+    """Left-side solves with 1..4 right-hand sides (the one-launch
+    flag-chained path): every uplo/trans/diag, ragged block counts, a gap
+    between M and ldb that must stay untouched, and a block count above the
+    number of workgroups one wave of the chip holds."""
+    for uplo, ta, diag in itertools.product([X.Upper, X.Lower], [X.NoTrans, X.Trans],
+                                            [X.NonUnit, X.Unit]):
+        for M, N in ((1, 1), (63, 2), (64, 4), (65, 3), (700, 1), (20000, 2)):
+            if M == 20000 and (diag == X.Unit or dt == np.float32):
+                continue
+            # off-diagonal entries O(1/M) keep the systems well conditioned
+            # at every size; a unit diagonal holds a value that must not be read
+            A = _gen((M, M), dt) * np.asarray(1.0 / M, dt)
+            A[np.diag_indices(M)] = np.asarray(1000.0, dt) if diag == X.Unit else \
+                np.asarray(1.0, dt) + np.abs(RNG.uniform(-0.5, 0.5, M)).astype(dt)
+            B0 = _gen((M, N), dt)
+            Bbuf = _f(np.full((M + 3, N), 7.0, dt))
+            Bbuf[:M] = B0
+            Xg = X.trsm(X.Left, uplo, ta, diag, 0.75, A, Bbuf[:M])
+            assert np.all(Bbuf[M:] == 7.0)
+            r = O.trsm_resid(X.Left, uplo, ta, diag, 0.75, A, Xg, B0)
+            assert r < 1.0, (uplo, ta, diag, M, N, r)
+            if M <= 700:
+                Xo = O.trsm(X.Left, uplo, ta, diag, 0.75, A, B0.copy(order="F"))
+                tol = 1e-10 if dt == np.float64 else 2e-3
+                assert np.allclose(Xg, Xo, rtol=tol, atol=tol)
+            # the many-RHS path (GEMM chain) on the same columns agrees
+            Bw = _f(np.concatenate([B0, _gen((M, 5), dt)], axis=1))
+            Xw = X.trsm(X.Left, uplo, ta, diag, 0.75, A, Bw)
+            tol = 1e-9 if dt == np.float64 else 1e-2
+            assert np.allclose(Xg, Xw[:, :N], rtol=tol, atol=tol)
+
+
 def test_syrk_lower_upper_vs_oracle():
     N, K = 150, 70
     A = _gen((N, K), np.float64)
