@@ -102,6 +102,8 @@ def _declare(L):
     L.atlmx_prof_enable.argtypes = [_I]
     L.atlmx_prof_query.argtypes = [_I, _P, _P, _P]
     L.atlmx_prof_query.restype = _I
+    L.atlmx_prof_query2.argtypes = [_I, _P, _P, _P, _P]
+    L.atlmx_prof_query2.restype = _I
     L.atlmx_dsgesv.argtypes = [_I, _I, _P, _I, _P, _I, _P, _I, _I, _P]
     L.atlmx_dsgesv.restype = _I
     L.atlmx_dbackward_error.argtypes = [_I, _I, _P, _I, _P, _I, _P, _I]
@@ -211,9 +213,13 @@ def prof_enable(mask=-1):
     lib().atlmx_prof_enable(int(mask))
 
 
-def prof_query(cls):
+def prof_query(cls, with_bytes=False):
+    """(ms, launches, work[, algorithmic bytes]) summed over one launch class."""
     ms = C.c_double(0)
     n = C.c_long(0)
     w = C.c_double(0)
-    lib().atlmx_prof_query(cls, C.byref(ms), C.byref(n), C.byref(w))
+    by = C.c_double(0)
+    lib().atlmx_prof_query2(cls, C.byref(ms), C.byref(n), C.byref(w), C.byref(by))
+    if with_bytes:
+        return ms.value, n.value, w.value, by.value
     return ms.value, n.value, w.value

@@ -30,7 +30,9 @@ void GEMM_DEV(atlmx_ctx *c, int ta, int tb, int M, int N, int K, T alpha, const 
   if (M <= 0 || N <= 0) return;
   if ((alpha == (T)0 || K == 0) && beta == (T)1) return;
   if (alpha == (T)0) K = 0;   /* ATL_drefgemm.c: alpha == 0 only scales C */
-  pid = rt_prof_begin(c, PROF_GEMM, 2.0 * (double)M * N * K);
+  pid = rt_prof_begin2(c, PROF_GEMM, 2.0 * (double)M * N * K,
+                      ((double)M * K + (double)K * N +
+                       (double)M * N * (beta != (T)0 ? 2 : 1)) * sizeof(T));
   RT_CHECK(rout, K_GEMM(ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, tri,
                         abort_flag, c->stream));
   rt_prof_end(c, pid);

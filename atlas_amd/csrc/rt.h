@@ -67,6 +67,7 @@ enum { PROF_GEMM = 0, PROF_PANEL, PROF_TRSM_INV, PROF_LASWP, PROF_POTF2, PROF_MI
        PROF_TRSV, PROF_NCLASS };
 int rt_prof_on(void);
 int rt_prof_begin(atlmx_ctx *c, int cls, double work);
+int rt_prof_begin2(atlmx_ctx *c, int cls, double work, double bytes);
 void rt_prof_end(atlmx_ctx *c, int id);
 
 #endif

@@ -108,7 +108,7 @@ void rt_stage_out(atlmx_ctx *c, void *h, const void *d, int rows, int cols, long
 /* profiler                                                                 */
 /* ---------------------------------------------------------------------- */
 #define PROF_MAX 65536
-typedef struct { hipEvent_t a, b; int cls; double work; } prof_rec;
+typedef struct { hipEvent_t a, b; int cls; double work, bytes; } prof_rec;
 static __thread prof_rec *prof_buf = NULL;
 static __thread int prof_n = 0;
 static __thread int prof_enabled = 0;   /* bit mask of recorded classes */
@@ -116,6 +116,11 @@ static __thread int prof_enabled = 0;   /* bit mask of recorded classes */
 int rt_prof_on(void) { return prof_enabled; }
 
 int rt_prof_begin(atlmx_ctx *c, int cls, double work)
+{
+  return rt_prof_begin2(c, cls, work, 0.0);
+}
+
+int rt_prof_begin2(atlmx_ctx *c, int cls, double work, double bytes)
 {
   prof_rec *r;
   if (!(prof_enabled & (1 << cls)) || prof_n >= PROF_MAX) return -1;
@@ -126,6 +131,7 @@ int rt_prof_begin(atlmx_ctx *c, int cls, double work)
   }
   r->cls = cls;
   r->work = work;
+  r->bytes = bytes;
   RT_CHECK("atlmx_prof", hipEventRecord(r->a, c->stream));
   return prof_n++;
 }
@@ -176,8 +182,16 @@ void atlmx_prof_enable(int mask)
  * work (flops or bytes, as each launcher reports) of one launch class. */
 int atlmx_prof_query(int cls, double *ms_total, long *launches, double *work_total)
 {
+  return atlmx_prof_query2(cls, ms_total, launches, work_total, NULL);
+}
+
+/* As atlmx_prof_query, plus the summed algorithmic HBM bytes (each operand
+ * read once, each output written once) where the launcher states them. */
+int atlmx_prof_query2(int cls, double *ms_total, long *launches, double *work_total,
+                      double *bytes_total)
+{
   atlmx_ctx *c = rt_ctx();
-  double ms = 0.0, w = 0.0;
+  double ms = 0.0, w = 0.0, by = 0.0;
   long n = 0;
   int i;
   if (prof_n > 0) rt_sync(c, "atlmx_prof_query");
@@ -187,11 +201,13 @@ int atlmx_prof_query(int cls, double *ms_total, long *launches, double *work_tot
     RT_CHECK("atlmx_prof_query", hipEventElapsedTime(&t, prof_buf[i].a, prof_buf[i].b));
     ms += t;
     w += prof_buf[i].work;
+    by += prof_buf[i].bytes;
     n++;
   }
   if (ms_total) *ms_total = ms;
   if (launches) *launches = n;
   if (work_total) *work_total = w;
+  if (bytes_total) *bytes_total = by;
   return 0;
 }
 

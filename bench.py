@@ -68,6 +68,24 @@ def make_system(n, kappa, nrhs, seed, torch, X):
     return A, b
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` on `workload` as last measured with
+    two rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE, each scaled by the
+    same counter's reading for a 1 GiB device copy in the same run), kept as
+    `kernel=... workload=... traffic_bytes_per_launch=...` lines in
+    profiles/traffic.txt.  None when that file holds no such line."""
+    path = os.path.join(ROOT, "profiles", "traffic.txt")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None
+    for ln in lines:
+        kv = dict(f.split("=", 1) for f in ln.split() if "=" in f)
+        if kv.get("kernel") == kernel and kv.get("workload") == workload:
+            return float(kv["traffic_bytes_per_launch"])
+    return None
+
+
 def time_steps(fn, steps, warmup, torch):
     for _ in range(warmup):
         fn()
@@ -117,6 +135,9 @@ def main():
     ap.add_argument("--gemm-n", type=int, default=8192)
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="one warm mixed solve plus a 1 GiB device copy (the byte-counter "
+                         "calibration) only: the run rocprofv3 --pmc passes wrap")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -130,6 +151,24 @@ def main():
     X.use_torch_stream()
     n, nrhs = args.n, args.nrhs
     log = lambda *a: print(*a, file=sys.stderr, flush=True)   # noqa: E731
+
+    if args.profile_run:
+        # uniform[-1,1) A: the same launch sequence as the Haar system (the
+        # byte counters depend on shapes, not values) without the 2 QRs'
+        # ~10^5 small dispatches that a counter pass serialises
+        g = torch.Generator(device="cuda")
+        g.manual_seed(args.seed)
+        A = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        b = torch.rand(nrhs, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+        xmp = torch.empty_like(b)
+        X.dsgesv_dev(A, b, xmp)
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)                  # reads 1 GiB, writes 1 GiB
+        X.dsgesv_dev(A, b, xmp)
+        torch.cuda.synchronize()
+        log("profile run done")
+        return
 
     # ---- DGEMM %peak (n=8192, one launch, HIP events on the library stream)
     gn = args.gemm_n
@@ -177,17 +216,19 @@ def main():
     def mp_step():
         info[0], st[0] = X.dsgesv_dev(A, b, xmp)
 
-    tmp = time_steps(mp_step, 1, args.warmup, torch)     # warm
+    tmp = time_steps(mp_step, args.steps, args.warmup, torch)
+    # the same steps again with a HIP event pair around every library launch
+    # (on the library's stream): per-kernel-class times and the roofline
     X.prof_enable(-1)
-    tmp = time_steps(mp_step, args.steps, 0, torch)
+    tmp_prof = time_steps(mp_step, args.steps, 0, torch)
     classes = {}
     for cls, name in enumerate(["gemm", "panel", "trsm_inv", "laswp", "potf2", "misc", "trsv"]):
-        ms, cnt, work = X.prof_query(cls)
+        ms, cnt, work, nbytes = X.prof_query(cls, with_bytes=True)
         if cnt:
             classes[name] = {"ms_per_step": round(ms / args.steps, 3),
                              "launches_per_step": cnt // args.steps}
             if cls == X.PROF_GEMM:
-                gemm_ms, gemm_cnt, gemm_work = ms, cnt, work
+                gemm_ms, gemm_cnt, gemm_work, gemm_bytes = ms, cnt, work, nbytes
     X.prof_enable(0)
     bemp = L.atlmx_dbackward_error(n, nrhs, X.dptr(A), n, X.dptr(b), n, X.dptr(xmp), n)
     gmp = dgesv_flops(n, nrhs) / tmp / 1e9
@@ -205,8 +246,10 @@ def main():
     roofline = {"bound": "mfma", "kernel": "gemm_kernel<float> (sgetrf trailing updates)",
                 "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32_MFMA / 1e12,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_MFMA, 4),
-                "traffic": None, "launches": gemm_cnt,
-                "avg_launch_ms": round(gemm_avg_ms, 4)}
+                "traffic": measured_traffic("gemm_kernel<float>", "sgetrf_n%d" % n),
+                "launches": gemm_cnt,
+                "avg_launch_ms": round(gemm_avg_ms, 4),
+                "algorithmic_bytes_per_launch": round(gemm_bytes / gemm_cnt)}
 
     cpu = None
     if not args.no_cpu:
@@ -228,7 +271,7 @@ def main():
         "mixed_vs_fp64": round(gmp / g64, 3),
         "dgemm_n": gn, "dgemm_tflops": round(dgemm_tflops, 3),
         "dgemm_pct_peak": round(100 * dgemm_tflops * 1e12 / PEAK_FP64_MFMA, 2),
-        "kernel_time_per_step": classes,
+        "kernel_time_per_step": classes, "profiled_ms_per_step": round(tmp_prof * 1e3, 3),
         "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -30,6 +30,10 @@ void  atlmx_synchronize(void);
 void atlmx_prof_enable(int mask);
 void atlmx_prof_reset(void);
 int  atlmx_prof_query(int cls, double *ms_total, long *launches, double *work_total);
+/* ... plus summed algorithmic HBM bytes (each operand read once, each output
+ * written once) for the classes whose launchers state them (GEMM). */
+int  atlmx_prof_query2(int cls, double *ms_total, long *launches, double *work_total,
+                       double *bytes_total);
 
 /* ---- BLAS-3 on device pointers ----------------------------------------- */
 void atlmx_dgemm(int TA, int TB, int M, int N, int K, double alpha, const double *A,
