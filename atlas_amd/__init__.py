@@ -107,6 +107,8 @@ def _declare(L):
     L.atlmx_dsgesv.argtypes = [_I, _I, _P, _I, _P, _I, _P, _I, _I, _P]
     L.atlmx_dsgesv.restype = _I
     L.atlmx_dbackward_error.argtypes = [_I, _I, _P, _I, _P, _I, _P, _I]
+    L.atlmx_set_mixed_method.argtypes = [_I]
+    L.atlmx_set_mixed_method.restype = None
     L.atlmx_dbackward_error.restype = _D
 
 
@@ -187,7 +189,17 @@ def use_torch_stream():
 class IRInfo(C.Structure):
     """atlmx_ir_info (include/atlmx.h)"""
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
-                ("factor_info", C.c_int), ("backward_error", C.c_double)]
+                ("factor_info", C.c_int), ("backward_error", C.c_double),
+                ("method", C.c_int)]
+
+
+MIXED_AUTO, MIXED_TSLU, MIXED_GEPP = 0, 1, 2
+
+
+def set_mixed_method(m):
+    """0 tournament-pivoting FP32 LU, then partial-pivoting FP32 LU, then
+    FP64; 1 tournament then FP64; 2 partial pivoting then FP64."""
+    lib().atlmx_set_mixed_method(int(m))
 
 
 def dsgesv_dev(A, B, Xout, iter_max=30):

@@ -1,0 +1,612 @@
+/*
+ * k_batched.hip -- thousands of small, uneven-size problems in ONE
+ * persistent launch (no ATLAS counterpart: ATLAS has no batched interface;
+ * each instance here follows the same algorithms as its single-problem
+ * routines, src/lapack/ATL_potrfL.c:104-262, ATL_getrfC.c:46-361,
+ * ATL_potrs.c:13-86, ATL_getrs.c:8-89, and GEMM's C = alpha op(A) op(B) +
+ * beta C of interfaces/blas/C/src/cblas_dgemm.c:16-134).
+ *
+ * A persistent grid of workgroups takes work items from one atomic counter
+ * in the order the host planned (largest first), so uneven sizes balance
+ * themselves.  A factor+solve instance is one workgroup's work:
+ *
+ *   posv (lower Cholesky): right-looking, NB-column panels: the diagonal
+ *     block is factored in LDS, the panel below is solved row by row
+ *     against it, and the lower triangle of the trailing matrix is updated
+ *     in 64 x 64 tiles (each thread a 4 x 4 piece, the panel's rows staged
+ *     in LDS).  The right-hand sides are solved in LDS.
+ *   gesv (LU, partial pivoting): the (n-k) x NB panel lives in LDS while it
+ *     is factored with partial pivoting (workgroup arg-max per column, ties
+ *     to the lower row, NaN never wins, like cblas_idamax), its interchanges
+ *     are applied to the other columns, U12 is solved column by column and
+ *     the trailing matrix is updated in 64 x 64 tiles.
+ *
+ * The GEMM work item is one 64 x 64 tile of one instance.
+ */
+#include <hip/hip_runtime.h>
+#include <float.h>
+#include <math.h>
+#include "kernels.h"
+
+namespace {
+
+constexpr int NT = 256;
+constexpr int TS = 64;                 /* trailing-update / GEMM tile */
+constexpr int MAXR = ATLMX_BATCH_MAXRHS;
+
+template <typename T> __device__ inline T dsqrt(T x);
+template <> __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
+template <> __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
+
+/* C(r0.., c0..) -= P_r(rows r0.., K) * P_c(rows c0.., K)^T for one 64 x 64 tile,
+ * restricted to r >= c when lower_only.  P_r, P_c: column-major with ld.
+ * mr, mc: rows/cols of the tile inside the matrix. */
+template <typename T>
+__device__ void tile_update_nt(T *C, long ldc, const T *Pr, const T *Pc, long ldp, int K, int mr,
+                               int mc, bool lower_only, int r0, int c0, T *s_a, T *s_b) {
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int kc = min(16, K - k0);
+    __syncthreads();
+    /* s_a[kk][r] = Pr[r + (k0+kk)*ldp]; s_b[kk][c] likewise */
+    for (int e = tid; e < 16 * TS; e += NT) {
+      const int kk = e / TS, rr = e % TS;
+      s_a[kk * TS + rr] = (kk < kc && rr < mr) ? Pr[(size_t)(k0 + kk) * ldp + rr] : T(0);
+      s_b[kk * TS + rr] = (kk < kc && rr < mc) ? Pc[(size_t)(k0 + kk) * ldp + rr] : T(0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = s_a[kk * TS + ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = s_b[kk * TS + tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = ty * 4 + i, c = tx * 4 + j;
+      if (r < mr && c < mc && (!lower_only || r0 + r >= c0 + c))
+        C[(size_t)c * ldc + r] -= acc[i][j];
+    }
+}
+
+/* C(tile) -= L(rows, K) * U(K, cols): L column-major ld ldl, U column-major ld ldu */
+template <typename T>
+__device__ void tile_update_nn(T *C, long ldc, const T *L, long ldl, const T *U, long ldu, int K,
+                               int mr, int mc, T *s_a, T *s_b) {
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int kc = min(16, K - k0);
+    __syncthreads();
+    for (int e = tid; e < 16 * TS; e += NT) {
+      const int kk = e / TS, rr = e % TS;
+      s_a[kk * TS + rr] = (kk < kc && rr < mr) ? L[(size_t)(k0 + kk) * ldl + rr] : T(0);
+      /* U(k0+kk, cc): kk fastest for the global read, cc = e / 16 */
+      const int cc = e / 16, k2 = e % 16;
+      s_b[k2 * TS + cc] = (k2 < kc && cc < mc) ? U[(size_t)cc * ldu + k0 + k2] : T(0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = s_a[kk * TS + ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = s_b[kk * TS + tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = ty * 4 + i, c = tx * 4 + j;
+      if (r < mr && c < mc) C[(size_t)c * ldc + r] -= acc[i][j];
+    }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Cholesky factor + solve of one instance by one workgroup                */
+/* ---------------------------------------------------------------------- */
+template <typename T, int NB>
+__device__ void wg_posv(int n, T *A, long lda, int nrhs, T *B, long ldb, int *info, T *s_d,
+                        T *s_a, T *s_b, T *s_x, int *s_flag) {
+  const int tid = threadIdx.x;
+  for (int k = 0; k < n; k += NB) {
+    const int kb = min(NB, n - k);
+    /* ---- diagonal block: potf2 in LDS (ATL_potrfL.c's leaf, unblocked) ---- */
+    __syncthreads();
+    for (int e = tid; e < NB * NB; e += NT) {
+      const int r = e % NB, c = e / NB;
+      s_d[c * (NB + 1) + r] = (r < kb && c < kb && r >= c) ? A[(size_t)(k + c) * lda + k + r] : T(0);
+    }
+    if (tid == 0) *s_flag = 0;
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+      if (tid == 0) {
+        const T d = s_d[j * (NB + 1) + j];
+        if (!(d > T(0))) *s_flag = j + 1;
+        else s_d[j * (NB + 1) + j] = dsqrt(d);
+      }
+      __syncthreads();
+      if (*s_flag) break;
+      const T djj = s_d[j * (NB + 1) + j];
+      if (tid > j && tid < kb) s_d[j * (NB + 1) + tid] /= djj;
+      __syncthreads();
+      for (int e = tid; e < NB * NB; e += NT) {
+        const int r = e % NB, c = e / NB;
+        if (c > j && r >= c && r < kb)
+          s_d[c * (NB + 1) + r] -= s_d[j * (NB + 1) + r] * s_d[j * (NB + 1) + c];
+      }
+      __syncthreads();
+    }
+    if (*s_flag) {
+      if (tid == 0) *info = k + *s_flag;
+      return;                               /* ATL_potrfL.c:238-240: stop */
+    }
+    for (int e = tid; e < NB * NB; e += NT) {
+      const int r = e % NB, c = e / NB;
+      if (r < kb && c < kb && r >= c) A[(size_t)(k + c) * lda + k + r] = s_d[c * (NB + 1) + r];
+    }
+    /* ---- panel below: row i solves x L_kk^T = a ---- */
+    for (int i = k + kb + tid; i < n; i += NT) {
+      T x[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) x[c] = (c < kb) ? A[(size_t)(k + c) * lda + i] : T(0);
+#pragma unroll
+      for (int c = 0; c < NB; ++c) {
+        if (c < kb) {
+          T v = x[c];
+#pragma unroll
+          for (int l = 0; l < c; ++l) v -= x[l] * s_d[l * (NB + 1) + c];
+          x[c] = v / s_d[c * (NB + 1) + c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < kb) A[(size_t)(k + c) * lda + i] = x[c];
+    }
+    __syncthreads();
+    /* ---- trailing lower triangle: A22 -= P P^T in 64 x 64 tiles ---- */
+    const int m0 = k + kb, m = n - m0;
+    const int nt = (m + TS - 1) / TS;
+    const T *P = A + (size_t)k * lda;
+    for (int tj = 0; tj < nt; ++tj)
+      for (int ti = tj; ti < nt; ++ti) {
+        const int r0 = m0 + ti * TS, c0 = m0 + tj * TS;
+        tile_update_nt(A + (size_t)c0 * lda + r0, lda, P + r0, P + c0, lda, kb,
+                       min(TS, n - r0), min(TS, n - c0), ti == tj, r0, c0, s_a, s_b);
+      }
+  }
+  if (tid == 0) *info = 0;
+  if (nrhs <= 0) return;
+  /* ---- potrs: L y = b, L^T x = y, right-hand sides in LDS ---- */
+  __syncthreads();
+  for (int e = tid; e < n * nrhs; e += NT) s_x[e] = B[(size_t)(e / n) * ldb + e % n];
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {            /* forward, column-oriented */
+    const T d = A[(size_t)j * lda + j];
+    if (tid < nrhs) s_x[tid * n + j] /= d;
+    __syncthreads();
+    for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
+      const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
+      s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
+    }
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; --j) {       /* backward with L^T: dot products */
+    for (int r = 0; r < nrhs; ++r) {
+      T part = T(0);
+      for (int i = j + 1 + tid; i < n; i += NT) part += A[(size_t)j * lda + i] * s_x[r * n + i];
+      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
+      if ((tid & 63) == 0) s_a[(tid >> 6) * MAXR + r] = part;
+    }
+    __syncthreads();
+    if (tid < nrhs) {
+      const T s = s_a[tid] + s_a[MAXR + tid] + s_a[2 * MAXR + tid] + s_a[3 * MAXR + tid];
+      s_x[tid * n + j] = (s_x[tid * n + j] - s) / A[(size_t)j * lda + j];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * nrhs; e += NT) B[(size_t)(e / n) * ldb + e % n] = s_x[e];
+}
+
+/* ---------------------------------------------------------------------- */
+/* LU (partial pivoting) factor + solve of one instance by one workgroup   */
+/* ---------------------------------------------------------------------- */
+template <typename T, int NB>
+__device__ void wg_gesv(int n, T *A, long lda, int *ipiv, int nrhs, T *B, long ldb, int *info,
+                        T *s_p, T *s_a, T *s_b, T *s_x, T *s_v, int *s_i) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int first0 = 0;                          /* first exactly-zero pivot, 1-based */
+  for (int k = 0; k < n; k += NB) {
+    const int kb = min(NB, n - k), mp = n - k;
+    /* ---- panel rows k..n-1, columns k..k+kb-1 into LDS (s_p[c*mp + r]) ---- */
+    __syncthreads();
+    for (int e = tid; e < mp * kb; e += NT) {
+      const int r = e % mp, c = e / mp;
+      s_p[c * mp + r] = A[(size_t)(k + c) * lda + k + r];
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+      /* arg-max |p(r, j)|, r >= j: first maximum, NaN never wins */
+      T bv = T(-1);
+      int bi = 0x7fffffff;
+      for (int r = j + tid; r < mp; r += NT) {
+        const T v = fabs(s_p[j * mp + r]);
+        if (v > bv) { bv = v; bi = r; }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const T ov = __shfl_xor(bv, off);
+        const int oi = __shfl_xor(bi, off);
+        if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+      }
+      if (lane == 0) { s_v[wave] = bv; s_i[wave] = bi; }
+      __syncthreads();
+      if (tid == 0) {
+        T v = s_v[0];
+        int p = s_i[0];
+        for (int q = 1; q < NT / 64; ++q)
+          if (s_v[q] > v || (s_v[q] == v && s_i[q] < p)) { v = s_v[q]; p = s_i[q]; }
+        if (!(v > T(0))) p = j;            /* all zero/NaN: cblas_i?amax answers the first */
+        s_i[4] = p;
+      }
+      __syncthreads();
+      const int p = s_i[4];
+      if (tid == 0) ipiv[k + j] = k + p;
+      if (p != j && tid < kb) {
+        const T t = s_p[tid * mp + j];
+        s_p[tid * mp + j] = s_p[tid * mp + p];
+        s_p[tid * mp + p] = t;
+      }
+      __syncthreads();
+      const T piv = s_p[j * mp + j];
+      if (piv == T(0)) {
+        if (tid == 0 && !first0) first0 = k + j + 1;
+      } else {
+        for (int r = j + 1 + tid; r < mp; r += NT) {
+          const T l = s_p[j * mp + r] / piv;
+          s_p[j * mp + r] = l;
+          for (int c = j + 1; c < kb; ++c) s_p[c * mp + r] -= l * s_p[c * mp + j];
+        }
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < mp * kb; e += NT) {
+      const int r = e % mp, c = e / mp;
+      A[(size_t)(k + c) * lda + k + r] = s_p[c * mp + r];
+    }
+    /* ---- the panel's interchanges on the other columns ---- */
+    for (int c = tid; c < n; c += NT) {
+      if (c >= k && c < k + kb) continue;
+      T *col = A + (size_t)c * lda;
+      for (int j = 0; j < kb; ++j) {
+        const int p = ipiv[k + j];
+        if (p != k + j) { const T t = col[k + j]; col[k + j] = col[p]; col[p] = t; }
+      }
+    }
+    __syncthreads();
+    /* ---- U12 = L11^{-1} A12, one column per thread ---- */
+    for (int c = k + kb + tid; c < n; c += NT) {
+      T x[NB];
+#pragma unroll
+      for (int r = 0; r < NB; ++r) x[r] = (r < kb) ? A[(size_t)c * lda + k + r] : T(0);
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        if (r < kb) {
+          T v = x[r];
+#pragma unroll
+          for (int l = 0; l < r; ++l) v -= s_p[l * mp + r] * x[l];
+          x[r] = v;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r)
+        if (r < kb) A[(size_t)c * lda + k + r] = x[r];
+    }
+    __syncthreads();
+    /* ---- A22 -= L21 U12 ---- */
+    const int m0 = k + kb, m = n - m0;
+    const int nt = (m + TS - 1) / TS;
+    for (int tj = 0; tj < nt; ++tj)
+      for (int ti = 0; ti < nt; ++ti) {
+        const int r0 = m0 + ti * TS, c0 = m0 + tj * TS;
+        tile_update_nn(A + (size_t)c0 * lda + r0, lda, A + (size_t)k * lda + r0, lda,
+                       A + (size_t)c0 * lda + k, lda, kb, min(TS, n - r0), min(TS, n - c0),
+                       s_a, s_b);
+      }
+  }
+  if (tid == 0) *info = first0;
+  if (nrhs <= 0 || first0) return;
+  /* ---- getrs: P b, L y = b (unit), U x = y, in LDS ---- */
+  __syncthreads();
+  for (int e = tid; e < n * nrhs; e += NT) s_x[e] = B[(size_t)(e / n) * ldb + e % n];
+  __syncthreads();
+  if (tid < nrhs) {
+    for (int j = 0; j < n; ++j) {
+      const int p = ipiv[j];
+      if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
+    }
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
+      const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
+      s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
+    }
+    __syncthreads();
+  }
+  for (int j = n - 1; j >= 0; --j) {
+    const T d = A[(size_t)j * lda + j];
+    if (tid < nrhs) s_x[tid * n + j] /= d;
+    __syncthreads();
+    for (int e = tid; e < j * nrhs; e += NT) {
+      const int i = e % j, r = e / j;
+      s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n * nrhs; e += NT) B[(size_t)(e / n) * ldb + e % n] = s_x[e];
+}
+
+template <typename T>
+struct BatchArgs {
+  int count;
+  const int *order;              /* instance processing order */
+  const int *n;
+  T *const *A; const int *lda;
+  int *const *ipiv;              /* gesv */
+  int nrhs;
+  T *const *B; const int *ldb;
+  int *info;
+  int *counter;                  /* zero on entry; left at count + grid */
+};
+
+template <typename T, int NB>
+__global__ __launch_bounds__(NT) void posv_batched_kernel(BatchArgs<T> g) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  T *s_d = (T *)smem;                                  /* NB x (NB+1) */
+  T *s_a = s_d + NB * (NB + 1);                        /* 16 x 64 */
+  T *s_b = s_a + 16 * TS;                              /* 16 x 64 */
+  T *s_x = s_b + 16 * TS;                              /* n x nrhs */
+  __shared__ int s_item, s_flag;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(g.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= g.count) return;
+    const int i = g.order[it];
+    wg_posv<T, NB>(g.n[i], g.A[i], g.lda[i], g.nrhs, g.B ? g.B[i] : nullptr,
+                   g.ldb ? g.ldb[i] : 1, &g.info[i], s_d, s_a, s_b, s_x, &s_flag);
+  }
+}
+
+template <typename T, int NB>
+__global__ __launch_bounds__(NT) void gesv_batched_kernel(BatchArgs<T> g, int maxn) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
+  T *s_p = (T *)smem;                                  /* maxn x NB */
+  T *s_a = s_p + (size_t)maxn * NB;                    /* 16 x 64 */
+  T *s_b = s_a + 16 * TS;
+  T *s_x = s_b + 16 * TS;                              /* maxn x nrhs */
+  __shared__ T s_v[4];
+  __shared__ int s_i[5];
+  __shared__ int s_item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(g.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= g.count) return;
+    const int i = g.order[it];
+    wg_gesv<T, NB>(g.n[i], g.A[i], g.lda[i], g.ipiv[i], g.nrhs, g.B ? g.B[i] : nullptr,
+                   g.ldb ? g.ldb[i] : 1, &g.info[i], s_p, s_a, s_b, s_x, s_v, s_i);
+  }
+}
+
+/* ---------------------------------------------------------------------- */
+/* variable-size batched GEMM: one work item = one 64 x 64 tile           */
+/* ---------------------------------------------------------------------- */
+template <typename T>
+struct GemmBatchArgs {
+  int count, ta, tb;
+  const int *m, *n, *k;
+  T alpha, beta;
+  const T *const *A; const int *lda;
+  const T *const *B; const int *ldb;
+  T *const *C; const int *ldc;
+  const int *tile_start;         /* prefix sums of tiles per instance (count + 1) */
+  int *counter;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void gemm_batched_kernel(GemmBatchArgs<T> g) {
+  __shared__ T s_a[16 * TS], s_b[16 * TS];
+  __shared__ int s_item;
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const int total = g.tile_start[g.count];
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) s_item = atomicAdd(g.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= total) return;
+    /* instance: the last i with tile_start[i] <= it */
+    int lo = 0, hi = g.count - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (g.tile_start[mid] <= it) lo = mid; else hi = mid - 1;
+    }
+    const int i = lo, t = it - g.tile_start[i];
+    const int M = g.m[i], N = g.n[i], K = g.k[i];
+    const int tm = (M + TS - 1) / TS;
+    const int r0 = (t % tm) * TS, c0 = (t / tm) * TS;
+    const T *A = g.A[i], *B = g.B[i];
+    const long lda = g.lda[i], ldb = g.ldb[i], ldc = g.ldc[i];
+    T acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] = T(0);
+    for (int k0 = 0; k0 < K; k0 += 16) {
+      __syncthreads();
+      for (int e = tid; e < 16 * TS; e += NT) {
+        /* op(A)(r0+rr, k0+kk), op(B)(k0+kk, c0+cc) */
+        int rr, kk;
+        if (g.ta == KX_NOTRANS) { kk = e / TS; rr = e % TS; }
+        else                    { rr = e / 16; kk = e % 16; }
+        const int gr = r0 + rr, gk = k0 + kk;
+        s_a[kk * TS + rr] = (gr < M && gk < K)
+            ? (g.ta == KX_NOTRANS ? A[(size_t)gk * lda + gr] : A[(size_t)gr * lda + gk]) : T(0);
+        int cc, k2;
+        if (g.tb == KX_NOTRANS) { cc = e / 16; k2 = e % 16; }
+        else                    { k2 = e / TS; cc = e % TS; }
+        const int gc = c0 + cc, gk2 = k0 + k2;
+        s_b[k2 * TS + cc] = (gc < N && gk2 < K)
+            ? (g.tb == KX_NOTRANS ? B[(size_t)gc * ldb + gk2] : B[(size_t)gk2 * ldb + gc]) : T(0);
+      }
+      __syncthreads();
+#pragma unroll 4
+      for (int kk = 0; kk < 16; ++kk) {
+        T a[4], b[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a[q] = s_a[kk * TS + ty * 4 + q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) b[q] = s_b[kk * TS + tx * 4 + q];
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) acc[p][q] += a[p] * b[q];
+      }
+    }
+    T *C = g.C[i];
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = r0 + ty * 4 + p, c = c0 + tx * 4 + q;
+        if (r < M && c < N) {
+          T *pc = C + (size_t)c * ldc + r;
+          T v = g.alpha * acc[p][q];
+          if (g.beta != T(0)) v += g.beta * *pc;
+          *pc = v;
+        }
+      }
+  }
+}
+
+template <typename T, int NB>
+hipError_t launch_posv(const BatchArgs<T> &g, int maxn, int grid, hipStream_t s) {
+  const size_t lds = sizeof(T) * ((size_t)NB * (NB + 1) + 2 * 16 * TS + (size_t)maxn * MAXR);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void *)posv_batched_kernel<T, NB>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((posv_batched_kernel<T, NB>), dim3(grid), dim3(NT), lds, s, g);
+  return hipGetLastError();
+}
+
+template <typename T, int NB>
+hipError_t launch_gesv(const BatchArgs<T> &g, int maxn, int grid, hipStream_t s) {
+  const size_t lds = sizeof(T) * ((size_t)maxn * NB + 2 * 16 * TS + (size_t)maxn * MAXR);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipError_t e = hipFuncSetAttribute((const void *)gesv_batched_kernel<T, NB>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((gesv_batched_kernel<T, NB>), dim3(grid), dim3(NT), lds, s, g, maxn);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t batch_factor(int kind, int nb, int count, const int *order, const int *n, T *const *A,
+                        const int *lda, int *const *ipiv, int nrhs, T *const *B, const int *ldb,
+                        int *info, int *counter, int maxn, int grid, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (nrhs < 0 || nrhs > MAXR || maxn <= 0 || grid <= 0) return hipErrorInvalidValue;
+  BatchArgs<T> g;
+  g.count = count; g.order = order; g.n = n; g.A = A; g.lda = lda; g.ipiv = ipiv;
+  g.nrhs = nrhs; g.B = B; g.ldb = ldb; g.info = info; g.counter = counter;
+  if (kind == 0) {
+    if (nb == 16) return launch_posv<T, 16>(g, maxn, grid, s);
+    if (nb == 32) return launch_posv<T, 32>(g, maxn, grid, s);
+  } else {
+    if (nb == 16) return launch_gesv<T, 16>(g, maxn, grid, s);
+    if (nb == 32) return launch_gesv<T, 32>(g, maxn, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <typename T>
+hipError_t batch_gemm(int count, int ta, int tb, const int *m, const int *n, const int *k,
+                      T alpha, const T *const *A, const int *lda, const T *const *B,
+                      const int *ldb, T beta, T *const *C, const int *ldc,
+                      const int *tile_start, int *counter, int grid, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if ((ta != KX_NOTRANS && ta != KX_TRANS) || (tb != KX_NOTRANS && tb != KX_TRANS) ||
+      grid <= 0)
+    return hipErrorInvalidValue;
+  GemmBatchArgs<T> g;
+  g.count = count; g.ta = ta; g.tb = tb; g.m = m; g.n = n; g.k = k; g.alpha = alpha;
+  g.beta = beta; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.tile_start = tile_start; g.counter = counter;
+  hipLaunchKernelGGL(gemm_batched_kernel<T>, dim3(grid), dim3(NT), 0, s, g);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" hipError_t atlmx_k_dbatch_factor(int kind, int nb, int count, const int *order,
+                                            const int *n, double *const *A, const int *lda,
+                                            int *const *ipiv, int nrhs, double *const *B,
+                                            const int *ldb, int *info, int *counter, int maxn,
+                                            int grid, hipStream_t s) {
+  return batch_factor<double>(kind, nb, count, order, n, A, lda, ipiv, nrhs, B, ldb, info,
+                              counter, maxn, grid, s);
+}
+extern "C" hipError_t atlmx_k_sbatch_factor(int kind, int nb, int count, const int *order,
+                                            const int *n, float *const *A, const int *lda,
+                                            int *const *ipiv, int nrhs, float *const *B,
+                                            const int *ldb, int *info, int *counter, int maxn,
+                                            int grid, hipStream_t s) {
+  return batch_factor<float>(kind, nb, count, order, n, A, lda, ipiv, nrhs, B, ldb, info,
+                             counter, maxn, grid, s);
+}
+extern "C" hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+                                          const int *k, double alpha, const double *const *A,
+                                          const int *lda, const double *const *B,
+                                          const int *ldb, double beta, double *const *C,
+                                          const int *ldc, const int *tile_start, int *counter,
+                                          int grid, hipStream_t s) {
+  return batch_gemm<double>(count, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                            tile_start, counter, grid, s);
+}
+extern "C" hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+                                          const int *k, float alpha, const float *const *A,
+                                          const int *lda, const float *const *B, const int *ldb,
+                                          float beta, float *const *C, const int *ldc,
+                                          const int *tile_start, int *counter, int grid,
+                                          hipStream_t s) {
+  return batch_gemm<float>(count, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+                           tile_start, counter, grid, s);
+}

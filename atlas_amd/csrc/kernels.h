@@ -67,6 +67,62 @@ hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const float *A,
                          long lda, const float *inv, float *X, long ldx,
                          int *ctl, hipStream_t s);
 
+/* Tournament pivoting (FP32) on S = A^T (column-major, A(i,j) = S[j+i*lds],
+ * lds and k multiples of 4, S 16-byte aligned): chooses the w <=
+ * ATLMX_TSLU_W pivot rows of the panel rows k..n-1, columns k..k+w-1,
+ * writes the move list that brings them to rows k.. in pivot order
+ * (plan[0] = count <= 2*ATLMX_TSLU_W, then dst, src pairs), applies those
+ * moves to perm, and writes the w x w diagonal block's factor L\U (lu11)
+ * and inv(L11) (unit lower), inv(U11) (invl, invu), all ATLMX_TSLU_W x
+ * ATLMX_TSLU_W column-major, identity beyond w.  A zero or non-finite U11
+ * pivot sets *info = k + j + 1 (if 0); with *info nonzero on entry nothing
+ * is done.  ws: atlmx_k_tslu_ws_ints(n) ints, zero before the first call
+ * (each call leaves its counters zero). */
+#define ATLMX_TSLU_W 64
+size_t atlmx_k_tslu_ws_ints(int n);
+hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w, int *ws,
+                               int *plan, int *perm, float *lu11, float *invl, float *invu,
+                               int *info, hipStream_t s);
+/* columns dst <- src of S (rows of A) for the moves of plan, all nrows
+ * entries; then rows/entries k..k+w-1 (the diagonal block) <- lu11.
+ * Nothing is done when *info is nonzero. */
+hipError_t atlmx_k_tslu_swap(float *S, long lds, int nrows, const int *plan, int k, int w,
+                             const float *lu11, const int *info, hipStream_t s);
+/* Batched factor + solve in one persistent launch (k_batched.hip).
+ * kind 0: Cholesky (lower) + solve; kind 1: LU (partial pivoting) + solve.
+ * Instance i = order[it] for it = 0.. (taken from *counter, zero on entry):
+ * n[i], A[i] (lda[i]), ipiv[i] (kind 1, 0-based), nrhs <= ATLMX_BATCH_MAXRHS
+ * right-hand sides B[i] (ldb[i]; B may be NULL with nrhs 0), info[i].
+ * maxn bounds every n[i] (LDS sizing); nb: panel width 16 or 32; grid:
+ * persistent workgroups. */
+#define ATLMX_BATCH_MAXRHS 8
+hipError_t atlmx_k_dbatch_factor(int kind, int nb, int count, const int *order, const int *n,
+                                 double *const *A, const int *lda, int *const *ipiv, int nrhs,
+                                 double *const *B, const int *ldb, int *info, int *counter,
+                                 int maxn, int grid, hipStream_t s);
+hipError_t atlmx_k_sbatch_factor(int kind, int nb, int count, const int *order, const int *n,
+                                 float *const *A, const int *lda, int *const *ipiv, int nrhs,
+                                 float *const *B, const int *ldb, int *info, int *counter,
+                                 int maxn, int grid, hipStream_t s);
+/* C[i] = alpha op(A[i]) op(B[i]) + beta C[i] for count instances of sizes
+ * m[i] x n[i] x k[i]; tile_start: prefix sums of 64 x 64 tiles (count+1). */
+hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+                               const int *k, double alpha, const double *const *A,
+                               const int *lda, const double *const *B, const int *ldb,
+                               double beta, double *const *C, const int *ldc,
+                               const int *tile_start, int *counter, int grid, hipStream_t s);
+hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+                               const int *k, float alpha, const float *const *A,
+                               const int *lda, const float *const *B, const int *ldb,
+                               float beta, float *const *C, const int *ldc,
+                               const int *tile_start, int *counter, int grid, hipStream_t s);
+
+/* S = (float) A^T, perm = 0..n-1 */
+hipError_t atlmx_k_tslu_convert(int n, const double *A, long lda, float *S, long lds, int *perm,
+                                hipStream_t s);
+/* Y(i, :) = (float) X(perm[i], :) */
+hipError_t atlmx_k_tslu_gather(int n, int nrhs, const double *X, long ldx, const int *perm,
+                               float *Y, long ldy, hipStream_t s);
 /* B(M x N) *= alpha */
 hipError_t atlmx_k_dscal2d(int M, int N, double alpha, double *B, long ldb,
                            hipStream_t s);

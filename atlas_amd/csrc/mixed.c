@@ -2,12 +2,17 @@
  * mixed.c -- FP64-accurate dense solve from a low-precision LU plus FP64
  * iterative refinement (no ATLAS counterpart).
  *
- * The LU is done in FP32 on the MFMA GEMM (sgetrf of this library); only
- * the residual r = b - A*x, the update x += d and the stopping test are in
- * FP64.  The stopping test is LAPACK dsgesv's: ||r||_inf <= ||x||_inf *
- * ||A||_inf * eps * sqrt(n) for every right-hand side.  If it is not met
- * within iter_max steps (or the FP32 factor is singular) the solve falls
- * back to the FP64 LU.
+ * The LU is done in FP32 on the MFMA GEMM; only the residual r = b - A*x,
+ * the update x += d and the stopping test are in FP64.  The stopping test is
+ * LAPACK dsgesv's: ||r||_inf <= ||x||_inf * ||A||_inf * eps * sqrt(n) for
+ * every right-hand side.
+ *
+ * First choice: the FP32 LU with tournament pivoting of S = A^T
+ * (k_tslu.hip), whose panels need one reduction tree instead of one
+ * device-wide exchange per column and whose row interchanges are contiguous
+ * column moves of S.  If its refinement stalls or a pivot is zero, the FP32
+ * LU with partial pivoting of A itself is tried, and failing that the FP64
+ * LU.
  */
 #include <math.h>
 #include <stdio.h>
@@ -23,6 +28,22 @@ void dgetrs_dev(atlmx_ctx *c, int trans, int N, int NRHS, const double *A, long 
                 const int *d_ipiv, double *B, long ldb, const char *rout);
 void sgetrs_dev(atlmx_ctx *c, int trans, int N, int NRHS, const float *A, long lda,
                 const int *d_ipiv, float *B, long ldb, const char *rout);
+void sgemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, float alpha,
+               const float *A, long lda, const float *B, long ldb, float beta, float *C,
+               long ldc, int tri, const int *abort_flag, const char *rout);
+void strsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
+               float alpha, const float *A, long lda, float *B, long ldb,
+               const int *abort_flag, const char *rout);
+
+enum { MP_AUTO = 0, MP_TSLU = 1, MP_GEPP = 2, MP_FP64 = 3 };
+static int mp_method = MP_AUTO;
+
+/* 0: tournament-pivoting FP32 LU, then partial-pivoting FP32 LU, then
+ * FP64; 1: tournament only (then FP64); 2: partial pivoting only (then FP64). */
+void atlmx_set_mixed_method(int m)
+{
+  mp_method = (m == MP_TSLU || m == MP_GEPP) ? m : MP_AUTO;
+}
 
 static double host_max(atlmx_ctx *c, const double *d, int n, const char *rout)
 {
@@ -37,6 +58,149 @@ static double host_max(atlmx_ctx *c, const double *d, int n, const char *rout)
   return m;
 }
 
+/* FP32 LU with tournament pivoting of A, held as S = A^T (column-major,
+ * leading dimension lds), panels of W = ATLMX_TSLU_W columns:
+ *   1. tournament: choose the panel's W pivot rows, move list + perm; the
+ *      root also factors the chosen rows' W x W block (its own partial
+ *      pivoting elimination, in pivot order) and inverts L11 and U11;
+ *   2. move those rows of A (columns of S) over the whole matrix and write
+ *      the factored diagonal block;
+ *   3. L21 = A21 inv(U11) and U12 = inv(L11) A12 as in-place GEMMs on S
+ *      (each a single tile row/column of the GEMM kernel, so every
+ *      workgroup reads its operand in full before writing it);
+ *   4. A22 -= L21 U12, i.e. S22 -= U12^T L21^T.
+ * On return PA = LU with row i of PA = row perm[i] of A; L (unit) is the
+ * strict upper triangle of S transposed, U the lower triangle of S
+ * transposed.  Returns 0 or 1 + the first column with a zero/non-finite
+ * pivot (every launch after that returns at once). */
+static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
+                           const char *rout)
+{
+  const int W = ATLMX_TSLU_W;
+  const size_t wsi = atlmx_k_tslu_ws_ints(n);
+  float *lu11 = (float *)rt_ws(c, WS_NPINV, sizeof(float) * 3 * W * W + 256, rout);
+  float *invL = lu11 + (size_t)W * W, *invU = invL + (size_t)W * W;
+  int *d_info = (int *)(invU + (size_t)W * W);
+  int *tws = (int *)rt_ws(c, WS_TSLU, sizeof(int) * (wsi + 2 + 4 * W), rout);
+  int *plan = tws + wsi;
+  int info = 0, k, pid;
+  RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
+  RT_CHECK(rout, atlmx_k_fill_i32(tws, (int)wsi, 0, c->stream));
+  for (k = 0; k < n; k += W) {
+    const int w = (n - k < W) ? n - k : W;
+    const int m = n - k - w;
+    float *Skk = S + k + (size_t)k * lds;
+    pid = rt_prof_begin(c, PROF_PANEL, (double)(n - k) * w * sizeof(float));
+    RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k, w, tws, plan, perm, lu11, invL, invU,
+                                       d_info, c->stream));
+    rt_prof_end(c, pid);
+    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
+    RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, n, plan, k, w, lu11, d_info, c->stream));
+    rt_prof_end(c, pid);
+    if (m > 0) {
+      float *Lt = Skk + (size_t)w * lds;   /* L21^T: rows k..k+w-1, columns k+w.. of S */
+      float *Ut = Skk + w;                 /* U12^T: rows k+w.., columns k..k+w-1 of S */
+      sgemm_dev(c, KX_TRANS, KX_NOTRANS, w, m, w, 1.0f, invU, W, Lt, lds, 0.0f, Lt, lds, 0,
+                d_info, rout);
+      sgemm_dev(c, KX_NOTRANS, KX_TRANS, m, w, w, 1.0f, Ut, lds, invL, W, 0.0f, Ut, lds, 0,
+                d_info, rout);
+      sgemm_dev(c, KX_NOTRANS, KX_NOTRANS, m, m, w, -1.0f, Ut, lds, Lt, lds, 1.0f,
+                Lt + w, lds, 0, d_info, rout);
+    }
+  }
+  RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+/* a low-precision factorization and how to apply its inverse */
+typedef struct {
+  int kind;                 /* MP_TSLU or MP_GEPP */
+  int n;
+  float *As;                /* FP32 factor: S = A^T (MP_TSLU) or A, leading dim lda */
+  long lda;
+  const int *ipiv;          /* MP_GEPP: LAPACK interchanges */
+  const int *perm;          /* MP_TSLU: row permutation */
+  float *Rs;                /* FP32 right-hand-side staging, n x MP_MAXRHS */
+} lp_fact;
+
+/* X (=, or += when accumulate) M^-1 R for nb <= MP_MAXRHS columns, with the
+ * transforms in FP64 and the triangular solves in FP32 */
+static void lp_solve(atlmx_ctx *c, const lp_fact *f, int nb, const double *R, long ldr,
+                     double *X, long ldx, int accumulate, const char *rout)
+{
+  if (f->kind == MP_TSLU) {
+    /* PA = LU, L = (strict upper of S)^T + I, U = (lower of S)^T */
+    RT_CHECK(rout, atlmx_k_tslu_gather(f->n, nb, R, ldr, f->perm, f->Rs, f->n, c->stream));
+    strsm_dev(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_UNIT, f->n, nb, 1.0f, f->As, f->lda,
+              f->Rs, f->n, NULL, rout);
+    strsm_dev(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_NONUNIT, f->n, nb, 1.0f, f->As, f->lda,
+              f->Rs, f->n, NULL, rout);
+    if (accumulate)
+      RT_CHECK(rout, atlmx_k_axpy_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
+    else
+      RT_CHECK(rout, atlmx_k_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
+  } else {
+    RT_CHECK(rout, atlmx_k_d2s(f->n, nb, R, ldr, f->Rs, f->n, c->stream));
+    sgetrs_dev(c, KX_NOTRANS, f->n, nb, f->As, f->lda, f->ipiv, f->Rs, f->n, rout);
+    if (accumulate)
+      RT_CHECK(rout, atlmx_k_axpy_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
+    else
+      RT_CHECK(rout, atlmx_k_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
+  }
+}
+
+/* FP64 iterative refinement of every right-hand side with one low-precision
+ * factorization.  Stops a group when LAPACK dsgesv's test holds for all its
+ * columns, or when a step fails to halve some column's residual (the
+ * factorization is then not good enough).  Returns 1 when all converged. */
+static int refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const double *A, int lda,
+                  const double *B, int ldb, double *X, int ldx, int iter_max, double anrm,
+                  double cte, double *R, double *nrm, void *ws, atlmx_ir_info *st,
+                  const char *rout)
+{
+  double xn[MP_MAXRHS], rn[MP_MAXRHS], bn[MP_MAXRHS], rprev[MP_MAXRHS];
+  int r0, nb, it, j, conv = 0;
+  st->iterations = 0;
+  st->backward_error = 0.0;
+  for (r0 = 0; r0 < NRHS; r0 += MP_MAXRHS) {
+    const double *Bg = B + (size_t)r0 * ldb;
+    double *Xg = X + (size_t)r0 * ldx;
+    nb = (NRHS - r0 < MP_MAXRHS) ? NRHS - r0 : MP_MAXRHS;
+    RT_CHECK(rout, atlmx_k_dcolmax(N, nb, Bg, ldb, nrm + 64, c->stream));
+    RT_CHECK(rout, hipMemcpyAsync(bn, nrm + 64, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                  c->stream));
+    lp_solve(c, f, nb, Bg, ldb, Xg, ldx, 0, rout);
+    conv = 0;
+    for (it = 0; it <= iter_max; ++it) {
+      int stalled = 0;
+      RT_CHECK(rout, atlmx_k_dresid(N, N, nb, A, lda, Xg, ldx, Bg, ldb, R, N, ws, c->stream));
+      RT_CHECK(rout, atlmx_k_dcolmax(N, nb, Xg, ldx, nrm, c->stream));
+      RT_CHECK(rout, atlmx_k_dcolmax(N, nb, R, N, nrm + 32, c->stream));
+      RT_CHECK(rout, hipMemcpyAsync(xn, nrm, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                    c->stream));
+      RT_CHECK(rout, hipMemcpyAsync(rn, nrm + 32, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                    c->stream));
+      rt_sync(c, rout);
+      conv = 1;
+      for (j = 0; j < nb; ++j) {
+        if (!(rn[j] <= xn[j] * cte)) conv = 0;
+        if (it > 0 && !(rn[j] <= 0.5 * rprev[j]) && !(rn[j] <= xn[j] * cte)) stalled = 1;
+        rprev[j] = rn[j];
+      }
+      if (conv || stalled || it == iter_max) break;
+      lp_solve(c, f, nb, R, N, Xg, ldx, 1, rout);
+    }
+    if (it > st->iterations) st->iterations = it;
+    for (j = 0; j < nb; ++j) {
+      const double be = rn[j] / (anrm * xn[j] + bn[j]);
+      if (!(be <= st->backward_error)) st->backward_error = be;
+    }
+    if (!conv) return 0;
+  }
+  return 1;
+}
+
 int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
                  double *X, int ldx, int iter_max, atlmx_ir_info *st)
 {
@@ -45,10 +209,9 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
   const double eps = 1.1102230246251565e-16;   /* 2^-53, LAPACK dlamch('E') */
   double anrm, cte;
   double xn[MP_MAXRHS], rn[MP_MAXRHS], bn[MP_MAXRHS];
-  float *As, *Rs;
   double *R, *nrm, *ws;
-  int *ipiv;
-  int info, it, j, conv = 0, nb, r0;
+  int info = 0, j, conv = 0, nb, r0;
+  lp_fact f;
   atlmx_ir_info loc;
   if (!st) st = &loc;
   memset(st, 0, sizeof(*st));
@@ -57,62 +220,52 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
 
   ws = (double *)rt_ws(c, WS_MP3, atlmx_k_resid_ws_bytes(N, MP_MAXRHS), rout);
   nrm = (double *)rt_ws(c, WS_MP4, sizeof(double) * ((size_t)N + 128), rout);
+  R = (double *)rt_ws(c, WS_MP2, sizeof(double) * (size_t)N * MP_MAXRHS, rout);
   RT_CHECK(rout, atlmx_k_drowsum(N, N, A, lda, nrm, ws, c->stream));
   anrm = host_max(c, nrm, N, rout);
   cte = anrm * eps * sqrt((double)N);
 
-  As = (float *)rt_ws(c, WS_MP0, sizeof(float) * (size_t)N * N, rout);
-  ipiv = (int *)rt_ws(c, WS_IPIV, sizeof(int) * (size_t)N, rout);
-  Rs = (float *)rt_ws(c, WS_MP1, sizeof(float) * (size_t)N * MP_MAXRHS, rout);
-  R = (double *)rt_ws(c, WS_MP2, sizeof(double) * (size_t)N * MP_MAXRHS, rout);
-
-  RT_CHECK(rout, atlmx_k_d2s(N, N, A, lda, As, N, c->stream));
-  info = sgetrf_dev(c, N, N, As, N, ipiv, rout);
-  st->factor_info = info;
-
-  if (!info) {
-    /* the right-hand sides go through in groups of MP_MAXRHS */
-    for (r0 = 0; r0 < NRHS; r0 += MP_MAXRHS) {
-      const double *Bg = B + (size_t)r0 * ldb;
-      double *Xg = X + (size_t)r0 * ldx;
-      nb = (NRHS - r0 < MP_MAXRHS) ? NRHS - r0 : MP_MAXRHS;
-      RT_CHECK(rout, atlmx_k_dcolmax(N, nb, Bg, ldb, nrm, c->stream));
-      RT_CHECK(rout, hipMemcpyAsync(bn, nrm, sizeof(double) * nb, hipMemcpyDeviceToHost,
-                                    c->stream));
-      RT_CHECK(rout, atlmx_k_d2s(N, nb, Bg, ldb, Rs, N, c->stream));
-      sgetrs_dev(c, KX_NOTRANS, N, nb, As, N, ipiv, Rs, N, rout);
-      RT_CHECK(rout, atlmx_k_s2d(N, nb, Rs, N, Xg, ldx, c->stream));
-      conv = 0;
-      for (it = 0; it <= iter_max; ++it) {
-        RT_CHECK(rout, atlmx_k_dresid(N, N, nb, A, lda, Xg, ldx, Bg, ldb, R, N, ws,
-                                      c->stream));
-        RT_CHECK(rout, atlmx_k_dcolmax(N, nb, Xg, ldx, nrm, c->stream));
-        RT_CHECK(rout, atlmx_k_dcolmax(N, nb, R, N, nrm + 32, c->stream));
-        RT_CHECK(rout, hipMemcpyAsync(xn, nrm, sizeof(double) * nb, hipMemcpyDeviceToHost,
-                                      c->stream));
-        RT_CHECK(rout, hipMemcpyAsync(rn, nrm + 32, sizeof(double) * nb,
-                                      hipMemcpyDeviceToHost, c->stream));
-        rt_sync(c, rout);
-        conv = 1;
-        for (j = 0; j < nb; ++j) if (!(rn[j] <= xn[j] * cte)) conv = 0;
-        if (conv || it == iter_max) break;
-        RT_CHECK(rout, atlmx_k_d2s(N, nb, R, N, Rs, N, c->stream));
-        sgetrs_dev(c, KX_NOTRANS, N, nb, As, N, ipiv, Rs, N, rout);
-        RT_CHECK(rout, atlmx_k_axpy_s2d(N, nb, Rs, N, Xg, ldx, c->stream));
-      }
-      if (it > st->iterations) st->iterations = it;
-      for (j = 0; j < nb; ++j) {
-        const double be = rn[j] / (anrm * xn[j] + bn[j]);
-        if (!(be <= st->backward_error)) st->backward_error = be;
-      }
-      if (!conv) break;
-    }
-    st->converged = conv;
+  memset(&f, 0, sizeof(f));
+  f.n = N;
+  if (mp_method != MP_GEPP) {
+    /* tournament-pivoting LU of S = A^T */
+    int *perm = (int *)rt_ws(c, WS_IPIV, sizeof(int) * (size_t)N, rout);
+    f.kind = MP_TSLU;
+    f.lda = (N + 3) & ~3;
+    f.perm = perm;
+    f.As = (float *)rt_ws(c, WS_MP0, sizeof(float) * (size_t)f.lda * N, rout);
+    f.Rs = (float *)rt_ws(c, WS_MP1, sizeof(float) * (size_t)N * MP_MAXRHS, rout);
+    RT_CHECK(rout, atlmx_k_tslu_convert(N, A, lda, f.As, f.lda, perm, c->stream));
+    info = sgetrf_tslu_dev(c, N, f.As, f.lda, perm, rout);
+    st->factor_info = info;
+    st->method = MP_TSLU;
+    if (!info)
+      conv = refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm, ws,
+                    st, rout);
   }
-  if (info || !conv) {
+  if (!conv && mp_method != MP_TSLU) {
+    /* partial-pivoting FP32 LU of A itself */
+    int *ipiv = (int *)rt_ws(c, WS_IPIV, sizeof(int) * (size_t)N, rout);
+    f.kind = MP_GEPP;
+    f.lda = N;
+    f.ipiv = ipiv;
+    f.As = (float *)rt_ws(c, WS_MP0, sizeof(float) * (size_t)N * N, rout);
+    f.Rs = (float *)rt_ws(c, WS_MP1, sizeof(float) * (size_t)N * MP_MAXRHS, rout);
+    RT_CHECK(rout, atlmx_k_d2s(N, N, A, lda, f.As, N, c->stream));
+    info = sgetrf_dev(c, N, N, f.As, N, ipiv, rout);
+    st->factor_info = info;
+    st->method = MP_GEPP;
+    if (!info)
+      conv = refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm, ws,
+                    st, rout);
+  }
+  st->converged = conv;
+  if (!conv) {
     /* FP64 fallback on a copy of A (the input stays untouched) */
     double *Aw = (double *)rt_ws(c, WS_MP0, sizeof(double) * (size_t)N * N, rout);
+    int *ipiv = (int *)rt_ws(c, WS_IPIV, sizeof(int) * (size_t)N, rout);
     st->fallback = 1;
+    st->method = MP_FP64;
     RT_CHECK(rout, hipMemcpy2DAsync(Aw, sizeof(double) * (size_t)N, A, sizeof(double) * lda,
                                     sizeof(double) * (size_t)N, (size_t)N,
                                     hipMemcpyDeviceToDevice, c->stream));

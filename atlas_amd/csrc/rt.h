@@ -21,6 +21,8 @@ enum {
   WS_IPIV,
   WS_PLAN,          /* row-interchange move lists               */
   WS_TRSV,          /* few-RHS triangular solve counter + flags */
+  WS_NPINV,         /* no-pivot LU: inverted diagonal-block factors */
+  WS_TSLU,          /* tournament pivoting: tree + move list    */
   WS_MP0,           /* mixed-precision solver buffers          */
   WS_MP1,
   WS_MP2,

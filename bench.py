@@ -268,6 +268,9 @@ def main():
         "fp64_backward_error": be64,
         "mixed_dgesv_gflops": round(gmp, 2), "mixed_backward_error": bemp,
         "mixed_iterations": st[0].iterations, "mixed_fallback": st[0].fallback,
+        "mixed_factorization": {1: "fp32 LU, tournament pivoting",
+                                2: "fp32 partial-pivoting LU",
+                                3: "fp64 LU (fallback)"}.get(st[0].method, "?"),
         "mixed_vs_fp64": round(gmp / g64, 3),
         "dgemm_n": gn, "dgemm_tflops": round(dgemm_tflops, 3),
         "dgemm_pct_peak": round(100 * dgemm_tflops * 1e12 / PEAK_FP64_MFMA, 2),

@@ -65,15 +65,23 @@ void atlmx_spotrs(int Uplo, int N, int NRHS, const float *A, int lda, float *B, 
  * A (N x N), B (N x NRHS), X (N x NRHS) are FP64 device arrays; A and B are
  * not modified.  The LU is done in FP32; the residual, the update and the
  * stopping test (LAPACK dsgesv's: ||r|| <= ||x|| ||A|| eps sqrt(N)) are
- * FP64.  Falls back to the FP64 LU if refinement does not converge in
- * iter_max steps.  Synchronizes; returns 0 or the FP64 LU's info. */
+ * FP64.  First the FP32 LU with tournament pivoting (pivot rows of each
+ * 64-column panel chosen by a reduction tree of partial-pivoting
+ * eliminations); if its refinement stalls or a pivot is zero, the FP32 LU
+ * with partial pivoting, then the FP64 LU.  Synchronizes; returns 0 or the
+ * FP64 LU's info. */
 typedef struct {
   int iterations;          /* refinement steps taken (max over RHS groups)  */
   int converged;           /* stopping test met without fallback            */
   int fallback;            /* solved by the FP64 LU instead                 */
-  int factor_info;         /* info of the low-precision factorization       */
+  int factor_info;         /* info of the last low-precision factorization  */
   double backward_error;   /* max_j ||b-Ax||inf / (||A||inf ||x||inf + ||b||inf) */
+  int method;              /* 1 tournament-pivoting FP32, 2 partial-pivoting
+                              FP32, 3 FP64 */
 } atlmx_ir_info;
+/* 0 (default): tournament, then partial pivoting, then FP64; 1: tournament
+ * then FP64; 2: partial pivoting then FP64.  Process-wide. */
+void atlmx_set_mixed_method(int method);
 int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
                  double *X, int ldx, int iter_max, atlmx_ir_info *info);
 double atlmx_dbackward_error(int N, int NRHS, const double *A, int lda, const double *B,

@@ -1,0 +1,196 @@
+"""GPU checks of the mixed-precision solver (FP32 LU + FP64 refinement) and
+of its kernels: the tournament's pivot choice against a numpy restatement of
+the tournament, its move list and row moves, the root's diagonal-block
+factor against LU = (chosen rows) and X inv(X) = I, and every solve against the
+FP64 accuracy bar: normwise backward error <= n * eps64 and forward error
+against the known solution within kappa * n * eps64."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import atlas_amd as X
+
+pytestmark = pytest.mark.gpu
+EPS64 = 2.0 ** -52
+
+
+@pytest.fixture(scope="module")
+def torch():
+    import torch as t
+    X.use_torch_stream()
+    return t
+
+
+def _dev(t, a, dtype):
+    """column-major device copy of a numpy matrix (stored as its transpose)"""
+    return t.tensor(np.ascontiguousarray(np.asarray(a).T), dtype=dtype, device="cuda")
+
+
+def _host(tt):
+    return tt.cpu().numpy().T
+
+
+def _k(name, *args):
+    L = X.lib()
+    fn = getattr(L, name)
+    fn.restype = C.c_int
+    r = fn(*args)
+    assert r == 0, (name, r)
+
+
+def _tournament(A, k, w, leaf=256, fan=4):
+    """numpy restatement of the tournament (the checker of the GPU kernel):
+    partial-pivoting elimination in FP32 on 256-row leaves of the panel
+    A[k:, k:k+w], then on the original rows of up to 4 children's choices,
+    level by level; returns the root's chosen rows in pivot order."""
+    def gepp(rows):
+        P = A[rows, k:k + w].astype(np.float32).copy()
+        act = np.ones(len(rows), bool)
+        sel = []
+        for c in range(w):
+            if not act.any():
+                break
+            v = np.where(act, np.abs(P[:, c]), -1.0)
+            v = np.where(np.isnan(v), 0.0, v)
+            p = int(np.argmax(v))                   # first maximum
+            sel.append(rows[p])
+            act[p] = False
+            if P[p, c] != 0:
+                lcol = P[:, c] / P[p, c]
+                P[act, c + 1:] -= np.outer(lcol[act], P[p, c + 1:])
+        return sel
+    n = A.shape[0]
+    level = [gepp(list(range(s, min(s + leaf, n)))) for s in range(k, n, leaf)]
+    while len(level) > 1:
+        level = [gepp([r for ch in level[i:i + fan] for r in ch]) for i in range(0, len(level), fan)]
+    return level[0]
+
+
+def _tslu_select(torch, A, k, w):
+    n = A.shape[0]
+    lds = (n + 3) // 4 * 4
+    S = torch.zeros(n, lds, dtype=torch.float32, device="cuda")     # column j of S = S[j]
+    S[:, :n] = torch.tensor(A.astype(np.float32), device="cuda")    # S = A^T: S[j][i] = A[i][j]
+    L = X.lib()
+    L.atlmx_k_tslu_ws_ints.restype = C.c_size_t
+    nws = L.atlmx_k_tslu_ws_ints(n)
+    ws = torch.zeros(nws, dtype=torch.int32, device="cuda")
+    plan = torch.zeros(2 + 4 * 64, dtype=torch.int32, device="cuda")
+    perm = torch.arange(n, dtype=torch.int32, device="cuda")
+    blk = torch.zeros(3, 64, 64, dtype=torch.float32, device="cuda")   # lu11, invL, invU
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _k("atlmx_k_tslu_select", X.dptr(S), C.c_long(lds), n, k, w, X.dptr(ws), X.dptr(plan),
+       X.dptr(perm), X.dptr(blk[0]), X.dptr(blk[1]), X.dptr(blk[2]), X.dptr(info), None)
+    torch.cuda.synchronize()
+    return S, ws, plan, perm, blk, info
+
+
+@pytest.mark.parametrize("n,k,w", [(64, 0, 64), (200, 0, 64), (300, 64, 64), (1500, 0, 64),
+                                   (5000, 128, 64), (70, 64, 6), (20000, 0, 64)])
+def test_tournament_choice_and_moves(torch, n, k, w):
+    rng = np.random.default_rng(n + k)
+    A = rng.uniform(-1, 1, (n, n))
+    S, ws, dplan, dperm, blk, info = _tslu_select(torch, A, k, w)
+    plan, perm = dplan.cpu().numpy(), dperm.cpu().numpy()
+    want = _tournament(A, k, min(w, n - k))
+    ws_ = len(want)
+    assert int(info.item()) == 0
+    # perm: rows k.. in the tournament's pivot order, the rest a permutation
+    assert list(perm[k:k + ws_]) == want
+    assert sorted(perm) == list(range(n)) and list(perm[:k]) == list(range(k))
+    # the move list is the same permutation of rows (dst <- src)
+    cur = np.arange(n)
+    mv = plan[2:2 + 2 * plan[0]].reshape(-1, 2)
+    new = cur.copy()
+    new[mv[:, 0]] = cur[mv[:, 1]]
+    assert np.array_equal(new, perm) and plan[0] <= 2 * ws_
+    # the counters (one per tree node of the k = 0 tree) are left zero
+    nodes = sz = -(-n // 256)
+    while sz > 1:
+        sz = -(-sz // 4)
+        nodes += sz
+    assert int(ws[:nodes].abs().sum().item()) == 0
+    # the diagonal block: L11 U11 = the chosen rows' block, and the inverses
+    lu = blk[0].cpu().numpy().T.astype(np.float64)[:ws_, :ws_]
+    L11 = np.tril(lu, -1) + np.eye(ws_)
+    U11 = np.triu(lu)
+    D = A[want][:, k:k + ws_]
+    assert np.abs(L11 @ U11 - D).max() <= 1e-4 * max(1.0, np.abs(D).max())
+    assert np.abs(L11).max() <= 1.0 + 1e-6            # partial pivoting at the root
+    iL = blk[1].cpu().numpy().T.astype(np.float64)
+    iU = blk[2].cpu().numpy().T.astype(np.float64)
+    assert np.abs(iL[:ws_, :ws_] @ L11 - np.eye(ws_)).max() < 1e-4
+    assert np.abs(iU[:ws_, :ws_] @ U11 - np.eye(ws_)).max() < 1e-4 * np.linalg.cond(U11)
+    assert np.array_equal(iL[ws_:, ws_:], np.eye(64 - ws_))
+    # applying the moves to S moves whole rows of A and writes the factored block
+    _k("atlmx_k_tslu_swap", X.dptr(S), C.c_long(S.shape[1]), n, X.dptr(dplan), k, ws_,
+       X.dptr(blk[0]), X.dptr(info), None)
+    torch.cuda.synchronize()
+    Sh = S[:, :n].cpu().numpy()
+    want_rows = A[perm].astype(np.float32)
+    want_rows[k:k + ws_, k:k + ws_] = blk[0].cpu().numpy().T[:ws_, :ws_]
+    assert np.array_equal(Sh, want_rows)
+
+
+def test_tournament_zero_pivot_sets_info(torch):
+    n = 300
+    A = np.random.default_rng(1).uniform(-1, 1, (n, n))
+    A[:, 5] = 0.0                                  # column 5 of the panel is zero
+    S, ws, dplan, dperm, blk, info = _tslu_select(torch, A, 0, 64)
+    assert int(info.item()) == 6
+    # with info set, the next call does nothing (perm and counters untouched)
+    before = dperm.clone()
+    _k("atlmx_k_tslu_select", X.dptr(S), C.c_long(S.shape[1]), n, 64, 64, X.dptr(ws),
+       X.dptr(dplan), X.dptr(dperm), X.dptr(blk[0]), X.dptr(blk[1]), X.dptr(blk[2]),
+       X.dptr(info), None)
+    torch.cuda.synchronize()
+    assert int(ws[:3].abs().sum().item()) == 0 and int(info.item()) == 6   # 2 leaves + root
+    assert torch.equal(before, dperm)
+
+
+def _system(n, kappa, nrhs, seed):
+    """A = U diag(s) V^T (Haar U, V), x known, b = A x"""
+    rng = np.random.default_rng(seed)
+    Uq, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    Vq, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = kappa ** (-np.arange(n) / max(n - 1, 1))
+    A = (Uq * s) @ Vq.T
+    x = rng.uniform(-1, 1, (n, nrhs))
+    return A, x, A @ x
+
+
+@pytest.mark.parametrize("method,want", [(X.MIXED_AUTO, 1), (X.MIXED_TSLU, 1),
+                                         (X.MIXED_GEPP, 2)])
+@pytest.mark.parametrize("n,nrhs,kappa", [(1, 1, 1.0), (5, 2, 10.0), (130, 1, 1e4),
+                                          (1001, 3, 1e6), (1024, 17, 1e5)])
+def test_dsgesv_fp64_accuracy(torch, method, want, n, nrhs, kappa):
+    A, x, b = _system(n, kappa, nrhs, seed=n + nrhs)
+    dA = _dev(torch, A, torch.float64)
+    dB = _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    X.set_mixed_method(method)
+    try:
+        info, st = X.dsgesv_dev(dA, dB, dX)
+    finally:
+        X.set_mixed_method(X.MIXED_AUTO)
+    assert info == 0
+    assert st.converged == 1 and st.fallback == 0 and st.method == want
+    be = X.lib().atlmx_dbackward_error(n, nrhs, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64, be
+    assert abs(st.backward_error - be) <= 0.5 * be + 1e-300 or be < 1e-17
+    xs = _host(dX)
+    ferr = np.abs(xs - x).max() / np.abs(x).max()
+    assert ferr <= 10 * kappa * n * EPS64, ferr
+    assert np.array_equal(_host(dA), A) and np.array_equal(_host(dB), b)   # inputs untouched
+
+
+def test_dsgesv_falls_back_to_fp64_when_fp32_cannot_converge(torch):
+    n = 300
+    A, x, b = _system(n, 1e12, 1, seed=3)     # kappa * eps32 >> 1
+    dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    info, st = X.dsgesv_dev(dA, dB, dX)
+    assert info == 0 and st.fallback == 1 and st.method == 3
+    be = X.lib().atlmx_dbackward_error(n, 1, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64
