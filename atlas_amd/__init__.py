@@ -21,7 +21,8 @@ Upper, Lower = 121, 122
 NonUnit, Unit = 131, 132
 Left, Right = 141, 142
 
-PROF_GEMM, PROF_PANEL, PROF_TRSM_INV, PROF_LASWP, PROF_POTF2, PROF_MISC, PROF_TRSV = range(7)
+PROF_GEMM, PROF_PANEL, PROF_TRSM_INV, PROF_LASWP, PROF_POTF2, PROF_MISC, PROF_TRSV, \
+    PROF_UPDATE = range(8)
 
 _lib = None
 

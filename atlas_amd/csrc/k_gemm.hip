@@ -250,6 +250,89 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
       }
 }
 
+/* C += alpha * A * B for short K (the trailing update of a blocked LU):
+ * A (M x K) and B (K x N) both not transposed.  The C tile is loaded
+ * straight into the MFMA accumulators before the K loop, so its HBM latency
+ * overlaps the operand staging and the MFMAs accumulate onto it (alpha is
+ * applied to A as it is staged); each K slice of up to KB columns is staged
+ * whole, with one barrier pair per slice. */
+template <typename T, int KB>
+__global__ __launch_bounds__(NT, 2) void update_kernel(GemmArgs<T> g) {
+  typedef Mfma<T> MF;
+  typedef typename MF::acc_t acc_t;
+  constexpr int LDK = KB + MF::PADK;
+  if (g.abort_flag && *g.abort_flag) return;
+  const int bid = blockIdx.x;
+  const int tile = (bid & 7) * g.per_xcd + (bid >> 3);
+  const int ntiles = g.tiles_m * g.tiles_n;
+  if (tile >= ntiles) return;
+  const int per_group = GROUP_M * g.tiles_n;
+  const int group = tile / per_group;
+  const int first_m = group * GROUP_M;
+  const int gsz = min(g.tiles_m - first_m, GROUP_M);
+  const int tm = first_m + (tile % per_group) % gsz;
+  const int tn = (tile % per_group) / gsz;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  __shared__ __attribute__((aligned(16))) T As[BM * LDK];
+  __shared__ __attribute__((aligned(16))) T Bs[BN * LDK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int kq = (lane >> 4) * 4, lr = lane & 15;
+
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lr;
+        const int col = n0 + wn + j * 16 + MF::drow(lane, r);
+        acc[i][j][r] = (row < g.M && col < g.N) ? g.C[(size_t)col * g.ldc + row] : T(0);
+      }
+
+  for (int k0 = 0; k0 < g.K; k0 += KB) {
+    SliceLoader<T, false, BM, KB> la;
+    SliceLoader<T, true, BN, KB> lb;
+    la.load(g.A, g.lda, m0, k0, g.M, g.K, g.vec_a, tid);
+    lb.load(g.B, g.ldb, n0, k0, g.N, g.K, g.vec_b, tid);
+#pragma unroll
+    for (int i = 0; i < la.NV; ++i) la.v[i] *= g.alpha;
+    if (k0 > 0) __syncthreads();
+    la.store(As, tid);
+    lb.store(Bs, tid);
+    __syncthreads();
+#pragma unroll
+    for (int kg = 0; kg < KB / 16; ++kg) {
+      T a[4][4], b[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const T *pa = &As[(wm + i * 16 + lr) * LDK + kg * 16 + kq];
+        const T *pb = &Bs[(wn + i * 16 + lr) * LDK + kg * 16 + kq];
+#pragma unroll
+        for (int s = 0; s < 4; ++s) { a[i][s] = pa[s]; b[i][s] = pb[s]; }
+      }
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+          for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j][s], a[i][s], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lr;
+        const int col = n0 + wn + j * 16 + MF::drow(lane, r);
+        if (row < g.M && col < g.N) g.C[(size_t)col * g.ldc + row] = acc[i][j][r];
+      }
+}
+
 template <typename T>
 hipError_t launch_gemm(int ta, int tb, int M, int N, int K, T alpha, const T *A,
                        long lda, const T *B, long ldb, T beta, T *C, long ldc,
@@ -286,6 +369,28 @@ hipError_t launch_gemm(int ta, int tb, int M, int N, int K, T alpha, const T *A,
 }
 
 }  // namespace
+
+/* C += alpha A B, A and B not transposed, FP32, any K (staged 64 at a time) */
+extern "C" hipError_t atlmx_k_supdate(int M, int N, int K, float alpha, const float *A,
+                                      long lda, const float *B, long ldb, float *C, long ldc,
+                                      const int *abort_flag, hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+  if (lda < M || ldb < K || ldc < M || !A || !B || !C) return hipErrorInvalidValue;
+  GemmArgs<float> g;
+  g.M = M; g.N = N; g.K = K; g.alpha = alpha; g.beta = 1.0f;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.tri = 0; g.abort_flag = abort_flag;
+  g.tiles_m = (M + BM - 1) / BM;
+  g.tiles_n = (N + BN - 1) / BN;
+  const long ntiles = (long)g.tiles_m * g.tiles_n;
+  if (ntiles > (1L << 30)) return hipErrorInvalidValue;
+  g.per_xcd = (int)((ntiles + 7) / 8);
+  g.vec_a = (((uintptr_t)A % 16) == 0) && (lda % 4 == 0);
+  g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % 4 == 0);
+  hipLaunchKernelGGL((update_kernel<float, 64>), dim3((unsigned)(8 * g.per_xcd)), dim3(NT), 0, s,
+                     g);
+  return hipGetLastError();
+}
 
 extern "C" hipError_t atlmx_k_dgemm(int ta, int tb, int M, int N, int K, double alpha,
                                     const double *A, long lda, const double *B, long ldb,

@@ -123,6 +123,12 @@ hipError_t atlmx_k_tslu_convert(int n, const double *A, long lda, float *S, long
 /* Y(i, :) = (float) X(perm[i], :) */
 hipError_t atlmx_k_tslu_gather(int n, int nrhs, const double *X, long ldx, const int *perm,
                                float *Y, long ldy, hipStream_t s);
+/* C += alpha A B (FP32, A and B not transposed): the short-K trailing
+ * update, C read once into the MFMA accumulators */
+hipError_t atlmx_k_supdate(int M, int N, int K, float alpha, const float *A, long lda,
+                           const float *B, long ldb, float *C, long ldc, const int *abort_flag,
+                           hipStream_t s);
+
 /* B(M x N) *= alpha */
 hipError_t atlmx_k_dscal2d(int M, int N, double alpha, double *B, long ldb,
                            hipStream_t s);

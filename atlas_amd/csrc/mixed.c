@@ -62,7 +62,9 @@ static double host_max(atlmx_ctx *c, const double *d, int n, const char *rout)
  * leading dimension lds), panels of W = ATLMX_TSLU_W columns:
  *   1. tournament: choose the panel's W pivot rows, move list + perm; the
  *      root also factors the chosen rows' W x W block (its own partial
- *      pivoting elimination, in pivot order) and inverts L11 and U11;
+ *      pivoting elimination, in pivot order) and inverts L11 and U11.  From
+ *      the second panel on, this runs on a second stream while the previous
+ *      trailing update finishes (the panel's own columns are updated first);
  *   2. move those rows of A (columns of S) over the whole matrix and write
  *      the factored diagonal block;
  *   3. L21 = A21 inv(U11) and U12 = inv(L11) A12 as in-place GEMMs on S
@@ -83,32 +85,56 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
   int *d_info = (int *)(invU + (size_t)W * W);
   int *tws = (int *)rt_ws(c, WS_TSLU, sizeof(int) * (wsi + 2 + 4 * W), rout);
   int *plan = tws + wsi;
+  hipStream_t main = c->stream, aux = rt_aux(c, rout);
   int info = 0, k, pid;
-  RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
-  RT_CHECK(rout, atlmx_k_fill_i32(tws, (int)wsi, 0, c->stream));
+  RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, main));
+  RT_CHECK(rout, atlmx_k_fill_i32(tws, (int)wsi, 0, main));
+  pid = rt_prof_begin(c, PROF_PANEL, (double)n * W * sizeof(float));
+  RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, 0, W < n ? W : n, tws, plan, perm, lu11, invL,
+                                     invU, d_info, main));
+  rt_prof_end(c, pid);
   for (k = 0; k < n; k += W) {
     const int w = (n - k < W) ? n - k : W;
     const int m = n - k - w;
     float *Skk = S + k + (size_t)k * lds;
-    pid = rt_prof_begin(c, PROF_PANEL, (double)(n - k) * w * sizeof(float));
-    RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k, w, tws, plan, perm, lu11, invL, invU,
-                                       d_info, c->stream));
-    rt_prof_end(c, pid);
+    /* panel k's pivot rows were chosen (on `main` for k = 0, else on `aux`
+     * overlapping the previous trailing update, which `main` waited for) */
     pid = rt_prof_begin(c, PROF_LASWP, 0.0);
-    RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, n, plan, k, w, lu11, d_info, c->stream));
+    RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, n, plan, k, w, lu11, d_info, main));
     rt_prof_end(c, pid);
     if (m > 0) {
       float *Lt = Skk + (size_t)w * lds;   /* L21^T: rows k..k+w-1, columns k+w.. of S */
       float *Ut = Skk + w;                 /* U12^T: rows k+w.., columns k..k+w-1 of S */
+      const int wn = (m < W) ? m : W;      /* the next panel's columns of A */
       sgemm_dev(c, KX_TRANS, KX_NOTRANS, w, m, w, 1.0f, invU, W, Lt, lds, 0.0f, Lt, lds, 0,
                 d_info, rout);
       sgemm_dev(c, KX_NOTRANS, KX_TRANS, m, w, w, 1.0f, Ut, lds, invL, W, 0.0f, Ut, lds, 0,
                 d_info, rout);
-      sgemm_dev(c, KX_NOTRANS, KX_NOTRANS, m, m, w, -1.0f, Ut, lds, Lt, lds, 1.0f,
-                Lt + w, lds, 0, d_info, rout);
+      /* A22 -= L21 U12 as S22 -= U12^T L21^T: first the next panel's
+       * columns of A (the first wn rows of S22), so its tournament can run
+       * on `aux` while `main` updates the rest */
+      pid = rt_prof_begin2(c, PROF_UPDATE, 2.0 * m * wn * w,
+                           ((double)wn * w + (double)w * m + 2.0 * wn * m) * sizeof(float));
+      RT_CHECK(rout, atlmx_k_supdate(wn, m, w, -1.0f, Ut, lds, Lt, lds, Lt + w, lds, d_info,
+                                     main));
+      rt_prof_end(c, pid);
+      RT_CHECK(rout, hipEventRecord(c->ev[0], main));
+      RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
+      RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k + w, wn, tws, plan, perm, lu11, invL, invU,
+                                         d_info, aux));
+      RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
+      if (m > wn) {
+        pid = rt_prof_begin2(c, PROF_UPDATE, 2.0 * (m - wn) * m * w,
+                             ((double)(m - wn) * w + (double)w * m + 2.0 * (m - wn) * m) *
+                                 sizeof(float));
+        RT_CHECK(rout, atlmx_k_supdate(m - wn, m, w, -1.0f, Ut + wn, lds, Lt, lds, Lt + w + wn,
+                                       lds, d_info, main));
+        rt_prof_end(c, pid);
+      }
+      RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
     }
   }
-  RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, main));
   rt_sync(c, rout);
   return info;
 }

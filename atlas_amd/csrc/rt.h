@@ -39,6 +39,8 @@ typedef struct atlmx_ctx {
   size_t ws_bytes[WS_NSLOTS];
   void *panel_buf;          /* LU panel exchange buffer last initialised */
   int panel_set;            /* header set the next panel launch uses     */
+  hipStream_t aux;          /* second stream for lookahead work (lazily created) */
+  hipEvent_t ev[2];         /* ordering events between stream and aux   */
 } atlmx_ctx;
 
 atlmx_ctx *rt_ctx(void);
@@ -54,6 +56,8 @@ void rt_fail(const char *rout, const char *what, hipError_t e);
 
 int rt_is_device(const void *p);
 void *rt_ws(atlmx_ctx *c, int slot, size_t bytes, const char *rout);
+/* the context's second stream and its two events (created on first use) */
+hipStream_t rt_aux(atlmx_ctx *c, const char *rout);
 void rt_sync(atlmx_ctx *c, const char *rout);
 
 /* Stage a rows x cols column-major operand.  A device pointer is returned
@@ -66,7 +70,7 @@ void rt_stage_out(atlmx_ctx *c, void *h, const void *d, int rows, int cols, long
 
 /* ---- profiler: hipEvents around selected launches on the ctx stream ---- */
 enum { PROF_GEMM = 0, PROF_PANEL, PROF_TRSM_INV, PROF_LASWP, PROF_POTF2, PROF_MISC,
-       PROF_TRSV, PROF_NCLASS };
+       PROF_TRSV, PROF_UPDATE, PROF_NCLASS };
 int rt_prof_on(void);
 int rt_prof_begin(atlmx_ctx *c, int cls, double work);
 int rt_prof_begin2(atlmx_ctx *c, int cls, double work, double bytes);

@@ -55,6 +55,16 @@ int rt_is_device(const void *p)
          a.type == hipMemoryTypeUnified || a.isManaged;
 }
 
+hipStream_t rt_aux(atlmx_ctx *c, const char *rout)
+{
+  if (!c->aux) {
+    RT_CHECK(rout, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+    RT_CHECK(rout, hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming));
+    RT_CHECK(rout, hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming));
+  }
+  return c->aux;
+}
+
 void rt_sync(atlmx_ctx *c, const char *rout)
 {
   RT_CHECK(rout, hipStreamSynchronize(c->stream));

@@ -25,13 +25,14 @@ void  atlmx_synchronize(void);
  * this thread's stream.  Classes: 0 GEMM (work = 2*M*N*K flops), 1 LU panel
  * (work = bytes loaded+stored), 2 TRSM diagonal inversion, 3 laswp,
  * 4 Cholesky diagonal block, 5 other, 6 few-right-hand-side triangular
- * solve (work = bytes of the factor read).  mask bit c enables class c (0 = off,
+ * solve (work = bytes of the factor read), 7 short-K trailing update of the
+ * mixed solver's FP32 LU (work = flops).  mask bit c enables class c (0 = off,
  * -1 = all); enabling clears earlier records. ---- */
 void atlmx_prof_enable(int mask);
 void atlmx_prof_reset(void);
 int  atlmx_prof_query(int cls, double *ms_total, long *launches, double *work_total);
 /* ... plus summed algorithmic HBM bytes (each operand read once, each output
- * written once) for the classes whose launchers state them (GEMM). */
+ * written once) for the classes whose launchers state them (0 and 7). */
 int  atlmx_prof_query2(int cls, double *ms_total, long *launches, double *work_total,
                        double *bytes_total);
 

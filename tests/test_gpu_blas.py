@@ -191,3 +191,32 @@ def test_device_pointer_api_torch():
     X.lib().cblas_dgemm(X.ColMajor, X.NoTrans, X.NoTrans, M, N, K, 1.0, X.dptr(A), M,
                         X.dptr(B), K, 0.0, X.dptr(C2), M)
     assert torch.allclose(C2.T, A.T @ B.T, atol=1e-12)
+
+
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (130, 129, 64), (300, 200, 17), (256, 384, 100),
+                                   (64, 1000, 130)])
+def test_short_k_update_kernel_vs_numpy(M, N, K):
+    """atlmx_k_supdate (C += alpha A B, the mixed solver's trailing update)
+    against a float64 numpy product of the same float32 inputs."""
+    import ctypes as C
+    import torch
+    X.use_torch_stream()
+    rng = np.random.default_rng(M + N + K)
+    lda, ldb, ldc = M + 3, K + 1, M + 5
+    A = rng.uniform(-1, 1, (K, lda)).astype(np.float32)     # column-major M x K, ld lda
+    B = rng.uniform(-1, 1, (N, ldb)).astype(np.float32)
+    Cm = rng.uniform(-1, 1, (N, ldc)).astype(np.float32)
+    dA, dB, dC = (torch.tensor(x, device="cuda") for x in (A, B, Cm))
+    fn = X.lib().atlmx_k_supdate
+    fn.restype = C.c_int
+    for alpha in (-1.0, 0.5):
+        dC.copy_(torch.tensor(Cm, device="cuda"))
+        r = fn(M, N, K, C.c_float(alpha), X.dptr(dA), C.c_long(lda), X.dptr(dB), C.c_long(ldb),
+               X.dptr(dC), C.c_long(ldc), None, None)
+        assert r == 0
+        torch.cuda.synchronize()
+        got = dC.cpu().numpy()
+        want = Cm.T[:M].astype(np.float64) + alpha * (A.T[:M].astype(np.float64) @
+                                                      B.T[:K].astype(np.float64))
+        assert np.abs(got.T[:M] - want).max() < 1e-5 * K
+        assert np.array_equal(got.T[M:], Cm.T[M:])       # rows past M untouched
