@@ -58,23 +58,71 @@ static double host_max(atlmx_ctx *c, const double *d, int n, const char *rout)
   return m;
 }
 
+/* S22 -= Ut Lt on the rows [r0, r0+M) x columns [c0, c0+N) of S with the
+ * K = columns k0.. of Ut (S rows, A's U12^T) and rows k0.. of Lt (A's
+ * L21^T): the operands are S's own blocks, so this is one short-K update */
+static void tslu_update(atlmx_ctx *c, float *S, long lds, int r0, int M, int c0, int N,
+                        int k0, int K, const int *d_info, hipStream_t st, const char *rout)
+{
+  int pid;
+  if (M <= 0 || N <= 0 || K <= 0) return;
+  pid = rt_prof_begin2(c, PROF_UPDATE, 2.0 * M * N * K,
+                       ((double)M * K + (double)K * N + 2.0 * M * N) * sizeof(float));
+  RT_CHECK(rout, atlmx_k_supdate(M, N, K, -1.0f, S + r0 + (size_t)k0 * lds, lds,
+                                 S + k0 + (size_t)c0 * lds, lds, S + r0 + (size_t)c0 * lds, lds,
+                                 d_info, st));
+  rt_prof_end(c, pid);
+}
+
+/* the panel at k (width w) after its pivots were chosen: row moves with the
+ * factored diagonal block ... */
+static void tslu_panel_swap(atlmx_ctx *c, float *S, long lds, int n, int k, int w,
+                            const int *plan, const float *lu11, const int *d_info,
+                            const char *rout)
+{
+  int pid = rt_prof_begin(c, PROF_LASWP, 0.0);
+  RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, n, plan, k, w, lu11, d_info, c->stream));
+  rt_prof_end(c, pid);
+}
+
+/* ... then L21 = A21 inv(U11) and U12 = inv(L11) A12 over the m = n-k-w
+ * rows/columns beyond the block (in-place GEMMs on S, each a single tile
+ * row/column of the GEMM kernel, so every workgroup reads its operand in
+ * full before writing it) */
+static void tslu_panel_solve(atlmx_ctx *c, float *S, long lds, int n, int k, int w,
+                             const float *invL, const float *invU, const int *d_info,
+                             const char *rout)
+{
+  const int W = ATLMX_TSLU_W, m = n - k - w;
+  float *Skk = S + k + (size_t)k * lds;
+  if (m > 0) {
+    float *Lt = Skk + (size_t)w * lds;     /* L21^T: rows k..k+w-1, columns k+w.. of S */
+    float *Ut = Skk + w;                   /* U12^T: rows k+w.., columns k..k+w-1 of S */
+    sgemm_dev(c, KX_TRANS, KX_NOTRANS, w, m, w, 1.0f, invU, W, Lt, lds, 0.0f, Lt, lds, 0,
+              d_info, rout);
+    sgemm_dev(c, KX_NOTRANS, KX_TRANS, m, w, w, 1.0f, Ut, lds, invL, W, 0.0f, Ut, lds, 0,
+              d_info, rout);
+  }
+}
+
 /* FP32 LU with tournament pivoting of A, held as S = A^T (column-major,
- * leading dimension lds), panels of W = ATLMX_TSLU_W columns:
- *   1. tournament: choose the panel's W pivot rows, move list + perm; the
- *      root also factors the chosen rows' W x W block (its own partial
- *      pivoting elimination, in pivot order) and inverts L11 and U11.  From
- *      the second panel on, this runs on a second stream while the previous
- *      trailing update finishes (the panel's own columns are updated first);
- *   2. move those rows of A (columns of S) over the whole matrix and write
- *      the factored diagonal block;
- *   3. L21 = A21 inv(U11) and U12 = inv(L11) A12 as in-place GEMMs on S
- *      (each a single tile row/column of the GEMM kernel, so every
- *      workgroup reads its operand in full before writing it);
- *   4. A22 -= L21 U12, i.e. S22 -= U12^T L21^T.
- * On return PA = LU with row i of PA = row perm[i] of A; L (unit) is the
- * strict upper triangle of S transposed, U the lower triangle of S
- * transposed.  Returns 0 or 1 + the first column with a zero/non-finite
- * pivot (every launch after that returns at once). */
+ * leading dimension lds), in pairs of W = ATLMX_TSLU_W column panels P, Q:
+ *   1. P's pivot rows were chosen (tournament: move list + perm; the root
+ *      also factors the chosen W x W block in pivot order and inverts L11
+ *      and U11); move them over the whole matrix with the factored block,
+ *      then L21 = A21 inv(U11), U12 = inv(L11) A12;
+ *   2. Q's columns get P's update (K = W); choose Q's pivots and move its
+ *      rows; only then do the rows of Q's block right of Q get P's update,
+ *      so that every row moved by Q's interchanges carries the same updates
+ *      as the row it replaces; finish Q like P;
+ *   3. the next pair's first panel gets P's and Q's update together
+ *      (K = 2W); its pivots are chosen on a second stream while the rest of
+ *      the trailing matrix gets the same K = 2W update.
+ * In S, A22 -= L21 U12 reads S22 -= U12^T L21^T.  On return PA = LU with row
+ * i of PA = row perm[i] of A; L (unit) is the strict upper triangle of S
+ * transposed, U the lower triangle of S transposed.  Returns 0 or 1 + the
+ * first column with a zero/non-finite pivot (every launch after that
+ * returns at once). */
 static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
                            const char *rout)
 {
@@ -93,44 +141,34 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
   RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, 0, W < n ? W : n, tws, plan, perm, lu11, invL,
                                      invU, d_info, main));
   rt_prof_end(c, pid);
-  for (k = 0; k < n; k += W) {
-    const int w = (n - k < W) ? n - k : W;
-    const int m = n - k - w;
-    float *Skk = S + k + (size_t)k * lds;
-    /* panel k's pivot rows were chosen (on `main` for k = 0, else on `aux`
-     * overlapping the previous trailing update, which `main` waited for) */
-    pid = rt_prof_begin(c, PROF_LASWP, 0.0);
-    RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, n, plan, k, w, lu11, d_info, main));
+  for (k = 0; k < n; k += 2 * W) {
+    const int wp = (n - k < W) ? n - k : W;          /* panel P at k */
+    const int mp = n - k - wp;
+    const int kq = k + wp, wq = (mp < W) ? mp : W;   /* panel Q at kq */
+    const int mq = mp - wq;
+    tslu_panel_swap(c, S, lds, n, k, wp, plan, lu11, d_info, rout);
+    tslu_panel_solve(c, S, lds, n, k, wp, invL, invU, d_info, rout);
+    if (mp == 0) break;
+    /* 2. */
+    tslu_update(c, S, lds, kq, wq, kq, mp, k, wp, d_info, main, rout);
+    pid = rt_prof_begin(c, PROF_PANEL, (double)mp * wq * sizeof(float));
+    RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, kq, wq, tws, plan, perm, lu11, invL, invU,
+                                       d_info, main));
     rt_prof_end(c, pid);
-    if (m > 0) {
-      float *Lt = Skk + (size_t)w * lds;   /* L21^T: rows k..k+w-1, columns k+w.. of S */
-      float *Ut = Skk + w;                 /* U12^T: rows k+w.., columns k..k+w-1 of S */
-      const int wn = (m < W) ? m : W;      /* the next panel's columns of A */
-      sgemm_dev(c, KX_TRANS, KX_NOTRANS, w, m, w, 1.0f, invU, W, Lt, lds, 0.0f, Lt, lds, 0,
-                d_info, rout);
-      sgemm_dev(c, KX_NOTRANS, KX_TRANS, m, w, w, 1.0f, Ut, lds, invL, W, 0.0f, Ut, lds, 0,
-                d_info, rout);
-      /* A22 -= L21 U12 as S22 -= U12^T L21^T: first the next panel's
-       * columns of A (the first wn rows of S22), so its tournament can run
-       * on `aux` while `main` updates the rest */
-      pid = rt_prof_begin2(c, PROF_UPDATE, 2.0 * m * wn * w,
-                           ((double)wn * w + (double)w * m + 2.0 * wn * m) * sizeof(float));
-      RT_CHECK(rout, atlmx_k_supdate(wn, m, w, -1.0f, Ut, lds, Lt, lds, Lt + w, lds, d_info,
-                                     main));
-      rt_prof_end(c, pid);
+    tslu_panel_swap(c, S, lds, n, kq, wq, plan, lu11, d_info, rout);
+    tslu_update(c, S, lds, kq + wq, mq, kq, wq, k, wp, d_info, main, rout);
+    tslu_panel_solve(c, S, lds, n, kq, wq, invL, invU, d_info, rout);
+    if (mq == 0) break;
+    /* 3. */
+    {
+      const int k2 = kq + wq, w2 = (mq < W) ? mq : W;
+      tslu_update(c, S, lds, k2, w2, k2, mq, k, wp + wq, d_info, main, rout);
       RT_CHECK(rout, hipEventRecord(c->ev[0], main));
       RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
-      RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k + w, wn, tws, plan, perm, lu11, invL, invU,
+      RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k2, w2, tws, plan, perm, lu11, invL, invU,
                                          d_info, aux));
       RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
-      if (m > wn) {
-        pid = rt_prof_begin2(c, PROF_UPDATE, 2.0 * (m - wn) * m * w,
-                             ((double)(m - wn) * w + (double)w * m + 2.0 * (m - wn) * m) *
-                                 sizeof(float));
-        RT_CHECK(rout, atlmx_k_supdate(m - wn, m, w, -1.0f, Ut + wn, lds, Lt, lds, Lt + w + wn,
-                                       lds, d_info, main));
-        rt_prof_end(c, pid);
-      }
+      tslu_update(c, S, lds, k2 + w2, mq - w2, k2, mq, k, wp + wq, d_info, main, rout);
       RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
     }
   }
