@@ -236,3 +236,117 @@ def prof_query(cls, with_bytes=False):
     if with_bytes:
         return ms.value, n.value, w.value, by.value
     return ms.value, n.value, w.value
+
+
+# ---- batched plans (include/atlmx.h) ----------------------------------------
+class Batch:
+    """A batched plan over lists of CUDA tensors (each a column-major matrix
+    given as its row-major transpose, like the rest of this module).  Keeps
+    the device pointer arrays alive; run() replays the captured launch on
+    the library's stream; close() (or the context manager) frees the plan."""
+
+    def __init__(self, handle, keep):
+        self.h = handle
+        self._keep = keep
+
+    def run(self):
+        lib().atlmx_batch_run(self.h)
+        return self
+
+    def config(self):
+        nb, grid = C.c_int(0), C.c_int(0)
+        lib().atlmx_batch_config(self.h, C.byref(nb), C.byref(grid))
+        return nb.value, grid.value
+
+    def close(self):
+        if self.h:
+            lib().atlmx_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _ptr_array(tensors):
+    import torch
+    return torch.tensor([t.data_ptr() for t in tensors], dtype=torch.int64, device="cuda")
+
+
+def _ints(vals):
+    arr = (C.c_int * max(len(vals), 1))(*vals)
+    return arr
+
+
+def _setup_batch(L):
+    if getattr(L, "_batch_ready", False):
+        return
+    P = C.c_void_p
+    for pre in ("d", "s"):
+        f = getattr(L, "atlmx_%sposv_batch_plan" % pre)
+        f.argtypes = [C.c_int, P, P, P, C.c_int, P, P, P]
+        f.restype = P
+        f = getattr(L, "atlmx_%sgesv_batch_plan" % pre)
+        f.argtypes = [C.c_int, P, P, P, P, C.c_int, P, P, P]
+        f.restype = P
+        f = getattr(L, "atlmx_%sgemm_batch_plan" % pre)
+        t = C.c_double if pre == "d" else C.c_float
+        f.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P, t, P, P, P, P, t, P, P]
+        f.restype = P
+    L.atlmx_batch_run.argtypes = [P]
+    L.atlmx_batch_destroy.argtypes = [P]
+    L.atlmx_batch_config.argtypes = [P, P, P]
+    L._batch_ready = True
+
+
+def posv_batch(As, Bs, info):
+    """Plan Cholesky factor + solve of A_i X_i = B_i (A_i: n x n tensors,
+    B_i: nrhs x n tensors = column-major n x nrhs, info: int32 tensor)."""
+    L = lib()
+    _setup_batch(L)
+    pre = "d" if As[0].dtype.itemsize == 8 else "s"
+    n = [a.shape[0] for a in As]
+    nrhs = Bs[0].shape[0] if Bs else 0
+    pa, pb = _ptr_array(As), (_ptr_array(Bs) if Bs else None)
+    keep = (pa, pb, _ints(n), _ints(n))
+    h = getattr(L, "atlmx_%sposv_batch_plan" % pre)(
+        len(As), keep[2], dptr(pa), keep[3], nrhs, dptr(pb) if pb is not None else None,
+        keep[3], dptr(info))
+    return Batch(h, keep + (info,))
+
+
+def gesv_batch(As, ipivs, Bs, info):
+    """Plan LU (partial pivoting) factor + solve, ipivs: int32 tensors (n,)."""
+    L = lib()
+    _setup_batch(L)
+    pre = "d" if As[0].dtype.itemsize == 8 else "s"
+    n = [a.shape[0] for a in As]
+    nrhs = Bs[0].shape[0] if Bs else 0
+    pa, pp, pb = _ptr_array(As), _ptr_array(ipivs), (_ptr_array(Bs) if Bs else None)
+    keep = (pa, pp, pb, _ints(n), _ints(n))
+    h = getattr(L, "atlmx_%sgesv_batch_plan" % pre)(
+        len(As), keep[3], dptr(pa), keep[4], dptr(pp), nrhs,
+        dptr(pb) if pb is not None else None, keep[4], dptr(info))
+    return Batch(h, keep + (info,))
+
+
+def gemm_batch(ta, tb, alpha, As, Bs, beta, Cs):
+    """Plan C_i = alpha op(A_i) op(B_i) + beta C_i; tensors are column-major
+    matrices given as row-major transposes (shape (cols, rows))."""
+    L = lib()
+    _setup_batch(L)
+    pre = "d" if Cs[0].dtype.itemsize == 8 else "s"
+    m = [c.shape[1] for c in Cs]
+    nn = [c.shape[0] for c in Cs]
+    k = [(a.shape[0] if ta == NoTrans else a.shape[1]) for a in As]
+    lda = [a.shape[1] for a in As]
+    ldb = [b.shape[1] for b in Bs]
+    ldc = [c.shape[1] for c in Cs]
+    pa, pb, pc = _ptr_array(As), _ptr_array(Bs), _ptr_array(Cs)
+    keep = (pa, pb, pc, _ints(m), _ints(nn), _ints(k), _ints(lda), _ints(ldb), _ints(ldc))
+    h = getattr(L, "atlmx_%sgemm_batch_plan" % pre)(
+        ta, tb, len(Cs), keep[3], keep[4], keep[5], alpha, dptr(pa), keep[6], dptr(pb),
+        keep[7], beta, dptr(pc), keep[8])
+    return Batch(h, keep)

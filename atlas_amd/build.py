@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libatlas_mi355x.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 
-C_SOURCES = ["runtime.c", "xerbla.c", "blas3.c", "lapack.c", "mixed.c"]
+C_SOURCES = ["runtime.c", "xerbla.c", "blas3.c", "lapack.c", "mixed.c", "batched.c"]
 HIP_SOURCES = ["k_gemm.hip", "k_aux.hip", "k_lu.hip", "k_chol.hip", "k_mp.hip", "k_trsv.hip", "k_tslu.hip", "k_batched.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-function",

@@ -1,0 +1,454 @@
+/*
+ * batched.c -- host side of the batched interface (no ATLAS counterpart):
+ * plans, the autotuner and hipGraph capture for thousands of small,
+ * uneven-size GEMM, Cholesky and LU factor+solve instances that run in one
+ * persistent launch (k_batched.hip).
+ *
+ * A plan is made once for a batch shape: the host orders the instances
+ * largest first (so the persistent workgroups that take them from one
+ * atomic counter finish together), uploads the size and leading-dimension
+ * arrays, picks the launch configuration with the autotuner, and captures
+ * the counter reset (a fill kernel, never a memset node) and the persistent
+ * kernel into a hipGraph.  Running the plan replays that graph on the
+ * calling thread's stream.
+ *
+ * The autotuner times every candidate configuration -- panel width 16 or
+ * 32, one to three persistent workgroups per CU; for GEMM the workgroups per
+ * CU -- on a replica of up to AT_SAMPLE instances drawn evenly from the
+ * ordered batch (well-conditioned synthetic data of the same sizes, so
+ * tuning never touches the caller's arrays), and keeps the fastest.
+ * Results are cached for the process by (kind, precision, nrhs, size
+ * buckets), so later plans of a similar shape skip the timing.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "rt.h"
+#include "../../include/cblas.h"
+#include "../../include/atlmx.h"
+
+#define AT_SAMPLE 512
+#define AT_CACHE 64
+
+enum { BK_POSV = 0, BK_GESV = 1, BK_GEMM = 2 };
+
+struct atlmx_batch {
+  int kind, dbl, count, nrhs, maxn;
+  int nb, grid;                   /* launch configuration */
+  int *d_meta;                    /* order | n | lda | ldb  (| m | k | ldc | tile_start) */
+  int *d_counter;
+  void *const *A, *const *B, *const *C;
+  int *const *ipiv;
+  int *info;
+  int ta, tb;
+  double alpha, beta;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  int captured;
+};
+
+typedef struct { int kind, dbl, nrhs, b_max, b_mean, nb, grid; } at_entry;
+static at_entry at_cache[AT_CACHE];
+static int at_n = 0;
+
+static int cu_count(void)
+{
+  static int cus = 0;
+  if (!cus) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess)
+      cus = 256;
+    else
+      cus = p.multiProcessorCount;
+  }
+  return cus;
+}
+
+static int bucket(double v)
+{
+  int b = 0;
+  while (v > 16.0 && b < 30) { v /= 2.0; ++b; }
+  return b;
+}
+
+/* one launch of the plan's kernel with configuration (nb, grid) */
+static hipError_t launch(const atlmx_batch *p, int count, const int *meta, void *const *A,
+                         void *const *B, void *const *C, int *const *ipiv, int *info,
+                         int nb, int grid, hipStream_t s)
+{
+  const int *order = meta, *n = meta + count, *lda = meta + 2 * count, *ldb = meta + 3 * count;
+  hipError_t e = atlmx_k_fill_i32(p->d_counter, 1, 0, s);
+  if (e != hipSuccess) return e;
+  if (p->kind == BK_GEMM) {
+    const int *m = meta + 4 * count, *k = meta + 5 * count, *ldc = meta + 6 * count;
+    const int *ts = meta + 7 * count;
+    /* meta layout for GEMM: order | n | lda | ldb | m | k | ldc | tile_start */
+    if (p->dbl)
+      return atlmx_k_dbatch_gemm(count, p->ta, p->tb, order, m, n, k, p->alpha,
+                                 (const double *const *)A, lda, (const double *const *)B, ldb,
+                                 p->beta, (double *const *)C, ldc, ts, p->d_counter, grid, s);
+    return atlmx_k_sbatch_gemm(count, p->ta, p->tb, order, m, n, k, (float)p->alpha,
+                               (const float *const *)A, lda, (const float *const *)B, ldb,
+                               (float)p->beta, (float *const *)C, ldc, ts, p->d_counter, grid,
+                               s);
+  }
+  if (p->dbl)
+    return atlmx_k_dbatch_factor(p->kind, nb, count, order, n, (double *const *)A, lda, ipiv,
+                                 p->nrhs, (double *const *)B, B ? ldb : NULL, info,
+                                 p->d_counter, p->maxn, grid, s);
+  return atlmx_k_sbatch_factor(p->kind, nb, count, order, n, (float *const *)A, lda, ipiv,
+                               p->nrhs, (float *const *)B, B ? ldb : NULL, info, p->d_counter,
+                               p->maxn, grid, s);
+}
+
+/* upload ints */
+static int *to_dev(const int *h, size_t cnt, const char *rout)
+{
+  int *d = NULL;
+  RT_CHECK(rout, hipMalloc((void **)&d, sizeof(int) * (cnt ? cnt : 1)));
+  if (cnt) RT_CHECK(rout, hipMemcpy(d, h, sizeof(int) * cnt, hipMemcpyHostToDevice));
+  return d;
+}
+
+static unsigned long long lcg(unsigned long long *s)
+{
+  *s = *s * 6364136223846793005ULL + 1442695040888963407ULL;
+  return *s >> 11;
+}
+
+/* Time every feasible configuration on a replica of a sample of the batch.
+ * Replica instances: the sampled sizes, matrices made diagonally dominant
+ * (so both Cholesky and LU run to completion), fresh for every timing. */
+static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const int *h_m,
+                     const int *h_k, const char *rout)
+{
+  const int esz = p->dbl ? 8 : 4;
+  const int cus = cu_count();
+  const int ns = p->count < AT_SAMPLE ? p->count : AT_SAMPLE;
+  int *sn = (int *)malloc(sizeof(int) * ns * 8), *meta;
+  size_t *offA = (size_t *)malloc(sizeof(size_t) * ns * 3), total = 0;
+  void **hA = (void **)malloc(sizeof(void *) * ns * 3);
+  unsigned char *pool = NULL, *pool0 = NULL;
+  void *const *dA, *const *dB, *const *dC;
+  int **hipiv = NULL, *const *dipiv = NULL, *dinfo = NULL, *ipool = NULL;
+  double mean = 0.0, best = 1e30;
+  int i, c, nbs[2] = {16, 32}, reps, b_max, b_mean;
+  hipEvent_t e0, e1;
+  hipStream_t s = 0;
+  unsigned long long seed = 0x5DEECE66DULL;
+  if (!sn || !offA || !hA) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
+
+  for (i = 0; i < p->count; ++i)
+    mean += (p->kind == BK_GEMM) ? cbrt((double)h_m[i] * h_n[i] * h_k[i]) : h_n[i];
+  mean /= p->count;
+  b_max = bucket(p->maxn);
+  b_mean = bucket(mean);
+  for (i = 0; i < at_n; ++i) {
+    const at_entry *a = &at_cache[i];
+    if (a->kind == p->kind && a->dbl == p->dbl && a->nrhs == p->nrhs && a->b_max == b_max &&
+        a->b_mean == b_mean) {
+      p->nb = a->nb;
+      p->grid = a->grid;
+      free(sn); free(offA); free(hA);
+      return;
+    }
+  }
+
+  /* sample evenly from the ordered batch; sizes and leading dimensions */
+  for (i = 0; i < ns; ++i) {
+    const int src = h_order[(int)((long)i * p->count / ns)];
+    const int n = h_n[src], m = h_m ? h_m[src] : n, k = h_k ? h_k[src] : n;
+    sn[i] = i;                      /* order */
+    sn[ns + i] = n;                 /* n */
+    if (p->kind == BK_GEMM) {
+      sn[2 * ns + i] = p->ta == KX_NOTRANS ? m : k;    /* lda */
+      sn[3 * ns + i] = p->tb == KX_NOTRANS ? k : n;    /* ldb */
+      sn[4 * ns + i] = m;
+      sn[5 * ns + i] = k;
+      sn[6 * ns + i] = m;                               /* ldc */
+      offA[3 * i] = total; total += (size_t)m * k * esz;
+      offA[3 * i + 1] = total; total += (size_t)k * n * esz;
+      offA[3 * i + 2] = total; total += (size_t)m * n * esz;
+    } else {
+      sn[2 * ns + i] = n;
+      sn[3 * ns + i] = n;
+      offA[3 * i] = total; total += (size_t)n * n * esz;
+      offA[3 * i + 1] = total; total += (size_t)n * (p->nrhs > 0 ? p->nrhs : 1) * esz;
+      offA[3 * i + 2] = total;
+    }
+    total = (total + 255) & ~(size_t)255;
+  }
+  if (p->kind == BK_GEMM) {       /* tile prefix sums */
+    int t = 0;
+    for (i = 0; i < ns; ++i) {
+      sn[7 * ns + i] = t;
+      t += ((sn[4 * ns + i] + ATLMX_BATCH_TILE - 1) / ATLMX_BATCH_TILE) *
+           ((sn[ns + i] + ATLMX_BATCH_TILE - 1) / ATLMX_BATCH_TILE);
+    }
+    sn = (int *)realloc(sn, sizeof(int) * (ns * 8 + 1));
+    sn[8 * ns] = t;
+  }
+  meta = to_dev(sn, (size_t)ns * 8 + (p->kind == BK_GEMM ? 1 : 0), rout);
+
+  /* the replica's arrays, filled on the host: diagonally dominant */
+  {
+    unsigned char *h = (unsigned char *)malloc(total ? total : 1);
+    size_t q;
+    if (!h) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
+    for (q = 0; q < total / esz; ++q) {
+      const double v = (double)(lcg(&seed) % 2000001) * 1e-6 - 1.0;
+      if (p->dbl) ((double *)h)[q] = v; else ((float *)h)[q] = (float)v;
+    }
+    if (p->kind != BK_GEMM)
+      for (i = 0; i < ns; ++i) {
+        const int n = sn[ns + i];
+        int d;
+        for (d = 0; d < n; ++d) {
+          const size_t o = offA[3 * i] / esz + (size_t)d * n + d;
+          if (p->dbl) ((double *)h)[o] = 2.0 * n; else ((float *)h)[o] = 2.0f * n;
+        }
+      }
+    RT_CHECK(rout, hipMalloc((void **)&pool, total ? total : 1));
+    RT_CHECK(rout, hipMalloc((void **)&pool0, total ? total : 1));
+    RT_CHECK(rout, hipMemcpy(pool0, h, total, hipMemcpyHostToDevice));
+    free(h);
+  }
+  for (i = 0; i < ns; ++i) {
+    hA[i] = pool + offA[3 * i];
+    hA[ns + i] = pool + offA[3 * i + 1];
+    hA[2 * ns + i] = pool + offA[3 * i + 2];
+  }
+  {
+    void **d;
+    RT_CHECK(rout, hipMalloc((void **)&d, sizeof(void *) * ns * 3));
+    RT_CHECK(rout, hipMemcpy(d, hA, sizeof(void *) * ns * 3, hipMemcpyHostToDevice));
+    dA = d; dB = d + ns; dC = d + 2 * ns;
+  }
+  if (p->kind == BK_GESV) {
+    int **hp = (int **)malloc(sizeof(int *) * ns);
+    RT_CHECK(rout, hipMalloc((void **)&ipool, sizeof(int) * (size_t)ns * (p->maxn + 1)));
+    for (i = 0; i < ns; ++i) hp[i] = ipool + (size_t)i * (p->maxn + 1);
+    RT_CHECK(rout, hipMalloc((void **)&hipiv, sizeof(int *) * ns));
+    RT_CHECK(rout, hipMemcpy(hipiv, hp, sizeof(int *) * ns, hipMemcpyHostToDevice));
+    dipiv = hipiv;
+    free(hp);
+  }
+  RT_CHECK(rout, hipMalloc((void **)&dinfo, sizeof(int) * ns));
+  RT_CHECK(rout, hipEventCreate(&e0));
+  RT_CHECK(rout, hipEventCreate(&e1));
+
+  p->nb = 32;
+  p->grid = cus;
+  for (c = 0; c < (p->kind == BK_GEMM ? 1 : 2); ++c)
+    for (reps = 1; reps <= (p->kind == BK_GEMM ? 4 : 3); ++reps) {
+      const int nb = nbs[c], grid = reps * cus;
+      float ms = 0.0f;
+      hipError_t e;
+      RT_CHECK(rout, hipMemcpy(pool, pool0, total, hipMemcpyDeviceToDevice));
+      e = launch(p, ns, meta, dA, p->nrhs > 0 || p->kind == BK_GEMM ? dB : NULL, dC, dipiv,
+                 dinfo, nb, grid, s);
+      if (e == hipErrorInvalidValue) { (void)hipGetLastError(); continue; }   /* infeasible */
+      RT_CHECK(rout, e);
+      RT_CHECK(rout, hipMemcpy(pool, pool0, total, hipMemcpyDeviceToDevice));
+      RT_CHECK(rout, hipEventRecord(e0, s));
+      RT_CHECK(rout, launch(p, ns, meta, dA, p->nrhs > 0 || p->kind == BK_GEMM ? dB : NULL, dC,
+                            dipiv, dinfo, nb, grid, s));
+      RT_CHECK(rout, hipEventRecord(e1, s));
+      RT_CHECK(rout, hipEventSynchronize(e1));
+      RT_CHECK(rout, hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; p->nb = nb; p->grid = grid; }
+    }
+  RT_CHECK(rout, hipDeviceSynchronize());
+  RT_CHECK(rout, hipEventDestroy(e0));
+  RT_CHECK(rout, hipEventDestroy(e1));
+  RT_CHECK(rout, hipFree(pool));
+  RT_CHECK(rout, hipFree(pool0));
+  RT_CHECK(rout, hipFree((void *)dA));
+  RT_CHECK(rout, hipFree(meta));
+  RT_CHECK(rout, hipFree(dinfo));
+  if (ipool) RT_CHECK(rout, hipFree(ipool));
+  if (hipiv) RT_CHECK(rout, hipFree(hipiv));
+  free(sn); free(offA); free(hA);
+  if (at_n < AT_CACHE) {
+    at_entry a = {p->kind, p->dbl, p->nrhs, b_max, b_mean, p->nb, p->grid};
+    at_cache[at_n++] = a;
+  }
+}
+
+static int cmp_cost_desc(const void *a, const void *b)
+{
+  const double x = ((const double *)a)[0], y = ((const double *)b)[0];
+  return (x < y) - (x > y);
+}
+
+static atlmx_batch *plan_create(int kind, int dbl, int count, int ta, int tb, const int *h_m,
+                                const int *h_n, const int *h_k, double alpha, void *const *A,
+                                const int *h_lda, void *const *B, const int *h_ldb, double beta,
+                                void *const *C, const int *h_ldc, int *const *ipiv, int nrhs,
+                                int *info, const char *rout)
+{
+  atlmx_batch *p = (atlmx_batch *)calloc(1, sizeof(*p));
+  double *cost;
+  int *h, i, t;
+  const size_t nmeta = (size_t)count * (kind == BK_GEMM ? 8 : 4) + 1;
+  if (!p) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
+  p->kind = kind; p->dbl = dbl; p->count = count; p->nrhs = nrhs;
+  p->A = A; p->B = B; p->C = C; p->ipiv = ipiv; p->info = info;
+  p->ta = ta; p->tb = tb; p->alpha = alpha; p->beta = beta;
+  cost = (double *)malloc(sizeof(double) * 2 * count);
+  h = (int *)malloc(sizeof(int) * nmeta);
+  if (!cost || !h) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
+  p->maxn = 1;
+  for (i = 0; i < count; ++i) {
+    const double n = h_n[i];
+    cost[2 * i] = (kind == BK_GEMM) ? (double)h_m[i] * h_n[i] * h_k[i] : n * n * n;
+    cost[2 * i + 1] = i;
+    if (h_n[i] > p->maxn) p->maxn = h_n[i];
+  }
+  qsort(cost, count, 2 * sizeof(double), cmp_cost_desc);
+  for (i = 0; i < count; ++i) h[i] = (int)cost[2 * i + 1];
+  /* n, lda, ldb are indexed by instance (the kernel reads n[order[it]]) */
+  for (i = 0; i < count; ++i) {
+    h[count + i] = h_n[i];
+    h[2 * count + i] = h_lda[i];
+    h[3 * count + i] = h_ldb ? h_ldb[i] : 1;
+  }
+  if (kind == BK_GEMM) {
+    for (i = 0; i < count; ++i) {
+      h[4 * count + i] = h_m[i];
+      h[5 * count + i] = h_k[i];
+      h[6 * count + i] = h_ldc[i];
+    }
+    for (t = 0, i = 0; i < count; ++i) {        /* tiles, in processing order */
+      const int src = h[i];
+      h[7 * count + i] = t;
+      t += ((h_m[src] + ATLMX_BATCH_TILE - 1) / ATLMX_BATCH_TILE) *
+           ((h_n[src] + ATLMX_BATCH_TILE - 1) / ATLMX_BATCH_TILE);
+    }
+    h[8 * count] = t;
+  }
+  p->d_meta = to_dev(h, nmeta, rout);
+  RT_CHECK(rout, hipMalloc((void **)&p->d_counter, 64));
+  {
+    int *ho = (int *)malloc(sizeof(int) * count);
+    for (i = 0; i < count; ++i) ho[i] = h[i];
+    autotune(p, ho, h_n, h_m, h_k, rout);
+    free(ho);
+  }
+  free(cost);
+  free(h);
+  return p;
+}
+
+int atlmx_batch_run(atlmx_batch *p)
+{
+  static const char *rout = "atlmx_batch_run";
+  atlmx_ctx *c = rt_ctx();
+  if (!p || p->count <= 0) return 0;
+  if (!p->captured) {
+    /* capture the counter reset and the persistent kernel once */
+    hipStream_t s;
+    RT_CHECK(rout, hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    RT_CHECK(rout, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    RT_CHECK(rout, launch(p, p->count, p->d_meta, p->A, p->B, p->C, p->ipiv, p->info, p->nb,
+                          p->grid, s));
+    RT_CHECK(rout, hipStreamEndCapture(s, &p->graph));
+    RT_CHECK(rout, hipGraphInstantiate(&p->exec, p->graph, NULL, NULL, 0));
+    RT_CHECK(rout, hipStreamDestroy(s));
+    p->captured = 1;
+  }
+  RT_CHECK(rout, hipGraphLaunch(p->exec, c->stream));
+  return 0;
+}
+
+void atlmx_batch_destroy(atlmx_batch *p)
+{
+  static const char *rout = "atlmx_batch_destroy";
+  if (!p) return;
+  RT_CHECK(rout, hipDeviceSynchronize());
+  if (p->captured) {
+    RT_CHECK(rout, hipGraphExecDestroy(p->exec));
+    RT_CHECK(rout, hipGraphDestroy(p->graph));
+  }
+  RT_CHECK(rout, hipFree(p->d_meta));
+  RT_CHECK(rout, hipFree(p->d_counter));
+  free(p);
+}
+
+void atlmx_batch_config(const atlmx_batch *p, int *nb, int *grid)
+{
+  if (nb) *nb = p ? p->nb : 0;
+  if (grid) *grid = p ? p->grid : 0;
+}
+
+/* ---- plans ---- */
+atlmx_batch *atlmx_dposv_batch_plan(int count, const int *h_n, double *const *d_A,
+                                    const int *h_lda, int nrhs, double *const *d_B,
+                                    const int *h_ldb, int *d_info)
+{
+  static const char *rout = "atlmx_dposv_batch_plan";
+  if (count < 0 || nrhs < 0 || nrhs > ATLMX_BATCH_MAXRHS) cblas_xerbla(5, rout, "");
+  return plan_create(BK_POSV, 1, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_A, h_lda,
+                     (void *const *)d_B, h_ldb, 0.0, NULL, NULL, NULL, nrhs, d_info, rout);
+}
+
+atlmx_batch *atlmx_sposv_batch_plan(int count, const int *h_n, float *const *d_A,
+                                    const int *h_lda, int nrhs, float *const *d_B,
+                                    const int *h_ldb, int *d_info)
+{
+  static const char *rout = "atlmx_sposv_batch_plan";
+  if (count < 0 || nrhs < 0 || nrhs > ATLMX_BATCH_MAXRHS) cblas_xerbla(5, rout, "");
+  return plan_create(BK_POSV, 0, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_A, h_lda,
+                     (void *const *)d_B, h_ldb, 0.0, NULL, NULL, NULL, nrhs, d_info, rout);
+}
+
+atlmx_batch *atlmx_dgesv_batch_plan(int count, const int *h_n, double *const *d_A,
+                                    const int *h_lda, int *const *d_ipiv, int nrhs,
+                                    double *const *d_B, const int *h_ldb, int *d_info)
+{
+  static const char *rout = "atlmx_dgesv_batch_plan";
+  if (count < 0 || nrhs < 0 || nrhs > ATLMX_BATCH_MAXRHS) cblas_xerbla(6, rout, "");
+  return plan_create(BK_GESV, 1, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_A, h_lda,
+                     (void *const *)d_B, h_ldb, 0.0, NULL, NULL, d_ipiv, nrhs, d_info, rout);
+}
+
+atlmx_batch *atlmx_sgesv_batch_plan(int count, const int *h_n, float *const *d_A,
+                                    const int *h_lda, int *const *d_ipiv, int nrhs,
+                                    float *const *d_B, const int *h_ldb, int *d_info)
+{
+  static const char *rout = "atlmx_sgesv_batch_plan";
+  if (count < 0 || nrhs < 0 || nrhs > ATLMX_BATCH_MAXRHS) cblas_xerbla(6, rout, "");
+  return plan_create(BK_GESV, 0, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_A, h_lda,
+                     (void *const *)d_B, h_ldb, 0.0, NULL, NULL, d_ipiv, nrhs, d_info, rout);
+}
+
+atlmx_batch *atlmx_dgemm_batch_plan(int TA, int TB, int count, const int *h_m, const int *h_n,
+                                    const int *h_k, double alpha, const double *const *d_A,
+                                    const int *h_lda, const double *const *d_B,
+                                    const int *h_ldb, double beta, double *const *d_C,
+                                    const int *h_ldc)
+{
+  static const char *rout = "atlmx_dgemm_batch_plan";
+  if (TA != CblasNoTrans && TA != CblasTrans) cblas_xerbla(1, rout, "");
+  if (TB != CblasNoTrans && TB != CblasTrans) cblas_xerbla(2, rout, "");
+  return plan_create(BK_GEMM, 1, count, TA == CblasNoTrans ? KX_NOTRANS : KX_TRANS,
+                     TB == CblasNoTrans ? KX_NOTRANS : KX_TRANS, h_m, h_n, h_k, alpha,
+                     (void *const *)d_A, h_lda, (void *const *)d_B, h_ldb, beta,
+                     (void *const *)d_C, h_ldc, NULL, 0, NULL, rout);
+}
+
+atlmx_batch *atlmx_sgemm_batch_plan(int TA, int TB, int count, const int *h_m, const int *h_n,
+                                    const int *h_k, float alpha, const float *const *d_A,
+                                    const int *h_lda, const float *const *d_B, const int *h_ldb,
+                                    float beta, float *const *d_C, const int *h_ldc)
+{
+  static const char *rout = "atlmx_sgemm_batch_plan";
+  if (TA != CblasNoTrans && TA != CblasTrans) cblas_xerbla(1, rout, "");
+  if (TB != CblasNoTrans && TB != CblasTrans) cblas_xerbla(2, rout, "");
+  return plan_create(BK_GEMM, 0, count, TA == CblasNoTrans ? KX_NOTRANS : KX_TRANS,
+                     TB == CblasNoTrans ? KX_NOTRANS : KX_TRANS, h_m, h_n, h_k, alpha,
+                     (void *const *)d_A, h_lda, (void *const *)d_B, h_ldb, beta,
+                     (void *const *)d_C, h_ldc, NULL, 0, NULL, rout);
+}

@@ -283,7 +283,7 @@ __device__ void wg_gesv(int n, T *A, long lda, int *ipiv, int nrhs, T *B, long l
       __syncthreads();
       const T piv = s_p[j * mp + j];
       if (piv == T(0)) {
-        if (tid == 0 && !first0) first0 = k + j + 1;
+        if (!first0) first0 = k + j + 1;          /* every thread: uniform control flow */
       } else {
         for (int r = j + 1 + tid; r < mp; r += NT) {
           const T l = s_p[j * mp + r] / piv;
@@ -431,12 +431,13 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(BatchArgs<T> g, int ma
 template <typename T>
 struct GemmBatchArgs {
   int count, ta, tb;
+  const int *order;              /* instance order of the tile numbering */
   const int *m, *n, *k;
   T alpha, beta;
   const T *const *A; const int *lda;
   const T *const *B; const int *ldb;
   T *const *C; const int *ldc;
-  const int *tile_start;         /* prefix sums of tiles per instance (count + 1) */
+  const int *tile_start;         /* prefix sums of tiles per position in order (count + 1) */
   int *counter;
 };
 
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(NT) void gemm_batched_kernel(GemmBatchArgs<T> g) {
       const int mid = (lo + hi + 1) >> 1;
       if (g.tile_start[mid] <= it) lo = mid; else hi = mid - 1;
     }
-    const int i = lo, t = it - g.tile_start[i];
+    const int i = g.order[lo], t = it - g.tile_start[lo];
     const int M = g.m[i], N = g.n[i], K = g.k[i];
     const int tm = (M + TS - 1) / TS;
     const int r0 = (t % tm) * TS, c0 = (t / tm) * TS;
@@ -558,7 +559,8 @@ hipError_t batch_factor(int kind, int nb, int count, const int *order, const int
 }
 
 template <typename T>
-hipError_t batch_gemm(int count, int ta, int tb, const int *m, const int *n, const int *k,
+hipError_t batch_gemm(int count, int ta, int tb, const int *order, const int *m, const int *n,
+                      const int *k,
                       T alpha, const T *const *A, const int *lda, const T *const *B,
                       const int *ldb, T beta, T *const *C, const int *ldc,
                       const int *tile_start, int *counter, int grid, hipStream_t s) {
@@ -567,7 +569,8 @@ hipError_t batch_gemm(int count, int ta, int tb, const int *m, const int *n, con
       grid <= 0)
     return hipErrorInvalidValue;
   GemmBatchArgs<T> g;
-  g.count = count; g.ta = ta; g.tb = tb; g.m = m; g.n = n; g.k = k; g.alpha = alpha;
+  g.count = count; g.ta = ta; g.tb = tb; g.order = order; g.m = m; g.n = n; g.k = k;
+  g.alpha = alpha;
   g.beta = beta; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
   g.tile_start = tile_start; g.counter = counter;
   hipLaunchKernelGGL(gemm_batched_kernel<T>, dim3(grid), dim3(NT), 0, s, g);
@@ -592,21 +595,23 @@ extern "C" hipError_t atlmx_k_sbatch_factor(int kind, int nb, int count, const i
   return batch_factor<float>(kind, nb, count, order, n, A, lda, ipiv, nrhs, B, ldb, info,
                              counter, maxn, grid, s);
 }
-extern "C" hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+extern "C" hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *order,
+                                          const int *m, const int *n,
                                           const int *k, double alpha, const double *const *A,
                                           const int *lda, const double *const *B,
                                           const int *ldb, double beta, double *const *C,
                                           const int *ldc, const int *tile_start, int *counter,
                                           int grid, hipStream_t s) {
-  return batch_gemm<double>(count, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+  return batch_gemm<double>(count, ta, tb, order, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                             tile_start, counter, grid, s);
 }
-extern "C" hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+extern "C" hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *order,
+                                          const int *m, const int *n,
                                           const int *k, float alpha, const float *const *A,
                                           const int *lda, const float *const *B, const int *ldb,
                                           float beta, float *const *C, const int *ldc,
                                           const int *tile_start, int *counter, int grid,
                                           hipStream_t s) {
-  return batch_gemm<float>(count, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
+  return batch_gemm<float>(count, ta, tb, order, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                            tile_start, counter, grid, s);
 }

@@ -105,13 +105,17 @@ hipError_t atlmx_k_sbatch_factor(int kind, int nb, int count, const int *order, 
                                  float *const *B, const int *ldb, int *info, int *counter,
                                  int maxn, int grid, hipStream_t s);
 /* C[i] = alpha op(A[i]) op(B[i]) + beta C[i] for count instances of sizes
- * m[i] x n[i] x k[i]; tile_start: prefix sums of 64 x 64 tiles (count+1). */
-hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+ * m[i] x n[i] x k[i]; tiles are numbered instance by instance in `order`,
+ * tile_start[p] = first tile of instance order[p] (count + 1 entries). */
+#define ATLMX_BATCH_TILE 64
+hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *order, const int *m,
+                               const int *n,
                                const int *k, double alpha, const double *const *A,
                                const int *lda, const double *const *B, const int *ldb,
                                double beta, double *const *C, const int *ldc,
                                const int *tile_start, int *counter, int grid, hipStream_t s);
-hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *m, const int *n,
+hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *order, const int *m,
+                               const int *n,
                                const int *k, float alpha, const float *const *A,
                                const int *lda, const float *const *B, const int *ldb,
                                float beta, float *const *C, const int *ldc,

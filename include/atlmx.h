@@ -62,6 +62,49 @@ int  atlmx_spotrf(int Uplo, int N, float *A, int lda);               /* synchron
 void atlmx_dpotrs(int Uplo, int N, int NRHS, const double *A, int lda, double *B, int ldb);
 void atlmx_spotrs(int Uplo, int N, int NRHS, const float *A, int lda, float *B, int ldb);
 
+/* ---- batched: thousands of small, uneven-size problems ------------------
+ * A plan is made once per batch shape: sizes and leading dimensions come as
+ * HOST arrays; matrices as DEVICE arrays of device pointers (column-major).
+ * The plan orders the instances largest first, autotunes the launch (panel
+ * width, persistent workgroups) on a synthetic replica of a sample of the
+ * batch (cached per shape class for the process), and captures one
+ * persistent launch into a hipGraph.  atlmx_batch_run replays it on this
+ * thread's stream (asynchronous); the pointer arrays and the data they
+ * point to are read at every run.
+ *   posv: A[i] = L L^T (lower triangle, in place; the upper is not
+ *         touched), then B[i] <- A[i]^{-1} B[i]; info[i] as clapack_?potrf
+ *         (the instance stops there).
+ *   gesv: A[i] = P L U (partial pivoting, ipiv[i] 0-based as clapack_?getrf),
+ *         then B[i] <- A[i]^{-1} B[i]; info[i] as clapack_?getrf (no solve
+ *         when nonzero).
+ *   gemm: C[i] = alpha op(A[i]) op(B[i]) + beta C[i].
+ * Orders up to 512 (posv) / 256..512 (gesv, by precision), nrhs <= 8. */
+typedef struct atlmx_batch atlmx_batch;
+atlmx_batch *atlmx_dposv_batch_plan(int count, const int *h_n, double *const *d_A,
+                                    const int *h_lda, int nrhs, double *const *d_B,
+                                    const int *h_ldb, int *d_info);
+atlmx_batch *atlmx_sposv_batch_plan(int count, const int *h_n, float *const *d_A,
+                                    const int *h_lda, int nrhs, float *const *d_B,
+                                    const int *h_ldb, int *d_info);
+atlmx_batch *atlmx_dgesv_batch_plan(int count, const int *h_n, double *const *d_A,
+                                    const int *h_lda, int *const *d_ipiv, int nrhs,
+                                    double *const *d_B, const int *h_ldb, int *d_info);
+atlmx_batch *atlmx_sgesv_batch_plan(int count, const int *h_n, float *const *d_A,
+                                    const int *h_lda, int *const *d_ipiv, int nrhs,
+                                    float *const *d_B, const int *h_ldb, int *d_info);
+atlmx_batch *atlmx_dgemm_batch_plan(int TA, int TB, int count, const int *h_m, const int *h_n,
+                                    const int *h_k, double alpha, const double *const *d_A,
+                                    const int *h_lda, const double *const *d_B,
+                                    const int *h_ldb, double beta, double *const *d_C,
+                                    const int *h_ldc);
+atlmx_batch *atlmx_sgemm_batch_plan(int TA, int TB, int count, const int *h_m, const int *h_n,
+                                    const int *h_k, float alpha, const float *const *d_A,
+                                    const int *h_lda, const float *const *d_B, const int *h_ldb,
+                                    float beta, float *const *d_C, const int *h_ldc);
+int  atlmx_batch_run(atlmx_batch *plan);
+void atlmx_batch_destroy(atlmx_batch *plan);                       /* synchronizes */
+void atlmx_batch_config(const atlmx_batch *plan, int *nb, int *grid);  /* the tuned launch */
+
 /* ---- FP64-accurate solve from a low-precision LU + FP64 refinement ------
  * A (N x N), B (N x NRHS), X (N x NRHS) are FP64 device arrays; A and B are
  * not modified.  The LU is done in FP32; the residual, the update and the
