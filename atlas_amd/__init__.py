@@ -350,3 +350,47 @@ def gemm_batch(ta, tb, alpha, As, Bs, beta, Cs):
         ta, tb, len(Cs), keep[3], keep[4], keep[5], alpha, dptr(pa), keep[6], dptr(pb),
         keep[7], beta, dptr(pc), keep[8])
     return Batch(h, keep)
+
+
+def _setup_adjoint(L):
+    if getattr(L, "_adj_ready", False):
+        return
+    _setup_batch(L)
+    P = C.c_void_p
+    for pre in ("d", "s"):
+        f = getattr(L, "atlmx_%sgesv_t_batch_plan" % pre)
+        f.argtypes = [C.c_int, P, P, P, P, C.c_int, P, P, P]
+        f.restype = P
+        f = getattr(L, "atlmx_%sposv_adjoint_batch_plan" % pre)
+        f.argtypes = [C.c_int, P, P, P, C.c_int, P, P, P, P, P, P]
+        f.restype = P
+    L._adj_ready = True
+
+
+def gesv_t_batch(As, ipivs, Bs, info):
+    """Plan LU factor + transposed solve A_i^T X_i = B_i."""
+    L = lib()
+    _setup_adjoint(L)
+    pre = "d" if As[0].dtype.itemsize == 8 else "s"
+    n = [a.shape[0] for a in As]
+    pa, pp, pb = _ptr_array(As), _ptr_array(ipivs), _ptr_array(Bs)
+    keep = (pa, pp, pb, _ints(n))
+    h = getattr(L, "atlmx_%sgesv_t_batch_plan" % pre)(
+        len(As), keep[3], dptr(pa), keep[3], dptr(pp), Bs[0].shape[0], dptr(pb), keep[3],
+        dptr(info))
+    return Batch(h, keep + (info,))
+
+
+def posv_adjoint_batch(Ls, Xs, Bbs, Gs, Ws):
+    """Plan the Cholesky adjoint: G_i = d<Xb, X>/d tril(A_i) from L_i
+    (factored A_i), X_i and Bb_i = A_i^{-1} Xb_i."""
+    L = lib()
+    _setup_adjoint(L)
+    pre = "d" if Ls[0].dtype.itemsize == 8 else "s"
+    n = [a.shape[0] for a in Ls]
+    arrs = [_ptr_array(t) for t in (Ls, Xs, Bbs, Gs, Ws)]
+    keep = tuple(arrs) + (_ints(n),)
+    h = getattr(L, "atlmx_%sposv_adjoint_batch_plan" % pre)(
+        len(Ls), keep[5], dptr(arrs[0]), keep[5], Xs[0].shape[0], dptr(arrs[1]), dptr(arrs[2]),
+        keep[5], dptr(arrs[3]), keep[5], dptr(arrs[4]))
+    return Batch(h, keep)

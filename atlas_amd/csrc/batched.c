@@ -31,7 +31,7 @@
 #define AT_SAMPLE 512
 #define AT_CACHE 64
 
-enum { BK_POSV = 0, BK_GESV = 1, BK_GEMM = 2 };
+enum { BK_POSV = 0, BK_GESV = 1, BK_GEMM = 2, BK_GESVT = 3, BK_CHOLADJ = 4 };
 
 struct atlmx_batch {
   int kind, dbl, count, nrhs, maxn;
@@ -39,6 +39,7 @@ struct atlmx_batch {
   int *d_meta;                    /* order | n | lda | ldb  (| m | k | ldc | tile_start) */
   int *d_counter;
   void *const *A, *const *B, *const *C;
+  void *const *D, *const *E;      /* Cholesky adjoint: gradient G, scratch W */
   int *const *ipiv;
   int *info;
   int ta, tb;
@@ -94,11 +95,26 @@ static hipError_t launch(const atlmx_batch *p, int count, const int *meta, void 
                                (float)p->beta, (float *const *)C, ldc, ts, p->d_counter, grid,
                                s);
   }
+  if (p->kind == BK_CHOLADJ) {
+    /* meta: order | n | lda | ldx | ldg; A = L, B = X, C = Bb */
+    const int *ldg = meta + 4 * count;
+    if (p->dbl)
+      return atlmx_k_dbatch_chol_adjoint(count, order, n, (double *const *)A, lda, p->nrhs,
+                                         (double *const *)B, (double *const *)C, ldb,
+                                         (double *const *)p->D, ldg, (double *const *)p->E,
+                                         p->d_counter, grid, s);
+    return atlmx_k_sbatch_chol_adjoint(count, order, n, (float *const *)A, lda, p->nrhs,
+                                       (float *const *)B, (float *const *)C, ldb,
+                                       (float *const *)p->D, ldg, (float *const *)p->E,
+                                       p->d_counter, grid, s);
+  }
   if (p->dbl)
-    return atlmx_k_dbatch_factor(p->kind, nb, count, order, n, (double *const *)A, lda, ipiv,
+    return atlmx_k_dbatch_factor(p->kind == BK_GESVT ? 2 : p->kind, nb, count, order, n,
+                                 (double *const *)A, lda, ipiv,
                                  p->nrhs, (double *const *)B, B ? ldb : NULL, info,
                                  p->d_counter, p->maxn, grid, s);
-  return atlmx_k_sbatch_factor(p->kind, nb, count, order, n, (float *const *)A, lda, ipiv,
+  return atlmx_k_sbatch_factor(p->kind == BK_GESVT ? 2 : p->kind, nb, count, order, n,
+                               (float *const *)A, lda, ipiv,
                                p->nrhs, (float *const *)B, B ? ldb : NULL, info, p->d_counter,
                                p->maxn, grid, s);
 }
@@ -139,6 +155,12 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   hipStream_t s = 0;
   unsigned long long seed = 0x5DEECE66DULL;
   if (!sn || !offA || !hA) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
+  if (p->kind == BK_CHOLADJ) {    /* one configuration: a workgroup per CU */
+    p->nb = 0;
+    p->grid = cus;
+    free(sn); free(offA); free(hA);
+    return;
+  }
 
   for (i = 0; i < p->count; ++i)
     mean += (p->kind == BK_GEMM) ? cbrt((double)h_m[i] * h_n[i] * h_k[i]) : h_n[i];
@@ -226,7 +248,7 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
     RT_CHECK(rout, hipMemcpy(d, hA, sizeof(void *) * ns * 3, hipMemcpyHostToDevice));
     dA = d; dB = d + ns; dC = d + 2 * ns;
   }
-  if (p->kind == BK_GESV) {
+  if (p->kind == BK_GESV || p->kind == BK_GESVT) {
     int **hp = (int **)malloc(sizeof(int *) * ns);
     RT_CHECK(rout, hipMalloc((void **)&ipool, sizeof(int) * (size_t)ns * (p->maxn + 1)));
     for (i = 0; i < ns; ++i) hp[i] = ipool + (size_t)i * (p->maxn + 1);
@@ -277,6 +299,12 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   }
 }
 
+static atlmx_batch *plan_create2(int kind, int dbl, int count, int ta, int tb, const int *h_m,
+                                 const int *h_n, const int *h_k, double alpha, void *const *A,
+                                 const int *h_lda, void *const *B, const int *h_ldb, double beta,
+                                 void *const *C, const int *h_ldc, int *const *ipiv, int nrhs,
+                                 int *info, void *const *D, void *const *E, const char *rout);
+
 static int cmp_cost_desc(const void *a, const void *b)
 {
   const double x = ((const double *)a)[0], y = ((const double *)b)[0];
@@ -289,13 +317,23 @@ static atlmx_batch *plan_create(int kind, int dbl, int count, int ta, int tb, co
                                 void *const *C, const int *h_ldc, int *const *ipiv, int nrhs,
                                 int *info, const char *rout)
 {
+  return plan_create2(kind, dbl, count, ta, tb, h_m, h_n, h_k, alpha, A, h_lda, B, h_ldb, beta,
+                      C, h_ldc, ipiv, nrhs, info, NULL, NULL, rout);
+}
+
+static atlmx_batch *plan_create2(int kind, int dbl, int count, int ta, int tb, const int *h_m,
+                                 const int *h_n, const int *h_k, double alpha, void *const *A,
+                                 const int *h_lda, void *const *B, const int *h_ldb, double beta,
+                                 void *const *C, const int *h_ldc, int *const *ipiv, int nrhs,
+                                 int *info, void *const *D, void *const *E, const char *rout)
+{
   atlmx_batch *p = (atlmx_batch *)calloc(1, sizeof(*p));
   double *cost;
   int *h, i, t;
-  const size_t nmeta = (size_t)count * (kind == BK_GEMM ? 8 : 4) + 1;
+  const size_t nmeta = (size_t)count * (kind == BK_GEMM ? 8 : 5) + 1;
   if (!p) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
   p->kind = kind; p->dbl = dbl; p->count = count; p->nrhs = nrhs;
-  p->A = A; p->B = B; p->C = C; p->ipiv = ipiv; p->info = info;
+  p->A = A; p->B = B; p->C = C; p->ipiv = ipiv; p->info = info; p->D = D; p->E = E;
   p->ta = ta; p->tb = tb; p->alpha = alpha; p->beta = beta;
   cost = (double *)malloc(sizeof(double) * 2 * count);
   h = (int *)malloc(sizeof(int) * nmeta);
@@ -315,6 +353,8 @@ static atlmx_batch *plan_create(int kind, int dbl, int count, int ta, int tb, co
     h[2 * count + i] = h_lda[i];
     h[3 * count + i] = h_ldb ? h_ldb[i] : 1;
   }
+  if (kind == BK_CHOLADJ)
+    for (i = 0; i < count; ++i) h[4 * count + i] = h_ldc[i];     /* ldg */
   if (kind == BK_GEMM) {
     for (i = 0; i < count; ++i) {
       h[4 * count + i] = h_m[i];
@@ -451,4 +491,51 @@ atlmx_batch *atlmx_sgemm_batch_plan(int TA, int TB, int count, const int *h_m, c
                      TB == CblasNoTrans ? KX_NOTRANS : KX_TRANS, h_m, h_n, h_k, alpha,
                      (void *const *)d_A, h_lda, (void *const *)d_B, h_ldb, beta,
                      (void *const *)d_C, h_ldc, NULL, 0, NULL, rout);
+}
+
+/* ---- adjoints ---- */
+atlmx_batch *atlmx_dgesv_t_batch_plan(int count, const int *h_n, double *const *d_A,
+                                      const int *h_lda, int *const *d_ipiv, int nrhs,
+                                      double *const *d_B, const int *h_ldb, int *d_info)
+{
+  static const char *rout = "atlmx_dgesv_t_batch_plan";
+  if (count < 0 || nrhs < 0 || nrhs > ATLMX_BATCH_MAXRHS) cblas_xerbla(6, rout, "");
+  return plan_create(BK_GESVT, 1, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_A, h_lda,
+                     (void *const *)d_B, h_ldb, 0.0, NULL, NULL, d_ipiv, nrhs, d_info, rout);
+}
+
+atlmx_batch *atlmx_sgesv_t_batch_plan(int count, const int *h_n, float *const *d_A,
+                                      const int *h_lda, int *const *d_ipiv, int nrhs,
+                                      float *const *d_B, const int *h_ldb, int *d_info)
+{
+  static const char *rout = "atlmx_sgesv_t_batch_plan";
+  if (count < 0 || nrhs < 0 || nrhs > ATLMX_BATCH_MAXRHS) cblas_xerbla(6, rout, "");
+  return plan_create(BK_GESVT, 0, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_A, h_lda,
+                     (void *const *)d_B, h_ldb, 0.0, NULL, NULL, d_ipiv, nrhs, d_info, rout);
+}
+
+atlmx_batch *atlmx_dposv_adjoint_batch_plan(int count, const int *h_n, double *const *d_L,
+                                            const int *h_lda, int nrhs, double *const *d_X,
+                                            double *const *d_Bb, const int *h_ldx,
+                                            double *const *d_G, const int *h_ldg,
+                                            double *const *d_W)
+{
+  static const char *rout = "atlmx_dposv_adjoint_batch_plan";
+  if (count < 0 || nrhs < 0) cblas_xerbla(5, rout, "");
+  return plan_create2(BK_CHOLADJ, 1, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_L,
+                      h_lda, (void *const *)d_X, h_ldx, 0.0, (void *const *)d_Bb, h_ldg, NULL,
+                      nrhs, NULL, (void *const *)d_G, (void *const *)d_W, rout);
+}
+
+atlmx_batch *atlmx_sposv_adjoint_batch_plan(int count, const int *h_n, float *const *d_L,
+                                            const int *h_lda, int nrhs, float *const *d_X,
+                                            float *const *d_Bb, const int *h_ldx,
+                                            float *const *d_G, const int *h_ldg,
+                                            float *const *d_W)
+{
+  static const char *rout = "atlmx_sposv_adjoint_batch_plan";
+  if (count < 0 || nrhs < 0) cblas_xerbla(5, rout, "");
+  return plan_create2(BK_CHOLADJ, 0, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_L,
+                      h_lda, (void *const *)d_X, h_ldx, 0.0, (void *const *)d_Bb, h_ldg, NULL,
+                      nrhs, NULL, (void *const *)d_G, (void *const *)d_W, rout);
 }

@@ -237,7 +237,7 @@ __device__ void wg_posv(int n, T *A, long lda, int nrhs, T *B, long ldb, int *in
 /* ---------------------------------------------------------------------- */
 template <typename T, int NB>
 __device__ void wg_gesv(int n, T *A, long lda, int *ipiv, int nrhs, T *B, long ldb, int *info,
-                        T *s_p, T *s_a, T *s_b, T *s_x, T *s_v, int *s_i) {
+                        T *s_p, T *s_a, T *s_b, T *s_x, T *s_v, int *s_i, bool trans) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   int first0 = 0;                          /* first exactly-zero pivot, 1-based */
   for (int k = 0; k < n; k += NB) {
@@ -339,35 +339,196 @@ __device__ void wg_gesv(int n, T *A, long lda, int *ipiv, int nrhs, T *B, long l
   }
   if (tid == 0) *info = first0;
   if (nrhs <= 0 || first0) return;
-  /* ---- getrs: P b, L y = b (unit), U x = y, in LDS ---- */
+  /* ---- getrs in LDS (ATL_getrs.c:8-89): NoTrans: P b, L y = b (unit),
+   * U x = y; Trans (A^T = U^T L^T P): U^T z = b, L^T w = z (unit), x = P^T w ---- */
   __syncthreads();
   for (int e = tid; e < n * nrhs; e += NT) s_x[e] = B[(size_t)(e / n) * ldb + e % n];
   __syncthreads();
-  if (tid < nrhs) {
+  if (!trans) {
+    if (tid < nrhs) {
+      for (int j = 0; j < n; ++j) {
+        const int p = ipiv[j];
+        if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
+      }
+    }
+    __syncthreads();
     for (int j = 0; j < n; ++j) {
-      const int p = ipiv[j];
-      if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
+      for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
+        const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
+        s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
+      }
+      __syncthreads();
     }
-  }
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {
-    for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
-      const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
-      s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
+    for (int j = n - 1; j >= 0; --j) {
+      const T d = A[(size_t)j * lda + j];
+      if (tid < nrhs) s_x[tid * n + j] /= d;
+      __syncthreads();
+      for (int e = tid; e < j * nrhs; e += NT) {
+        const int i = e % j, r = e / j;
+        s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  for (int j = n - 1; j >= 0; --j) {
-    const T d = A[(size_t)j * lda + j];
-    if (tid < nrhs) s_x[tid * n + j] /= d;
-    __syncthreads();
-    for (int e = tid; e < j * nrhs; e += NT) {
-      const int i = e % j, r = e / j;
-      s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
+  } else {
+    for (int j = 0; j < n; ++j) {            /* U^T z = b: U^T(i, j) = A(j, i) */
+      const T d = A[(size_t)j * lda + j];
+      if (tid < nrhs) s_x[tid * n + j] /= d;
+      __syncthreads();
+      for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
+        const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
+        s_x[r * n + i] -= A[(size_t)i * lda + j] * s_x[r * n + j];
+      }
+      __syncthreads();
+    }
+    for (int j = n - 1; j >= 0; --j) {       /* L^T w = z: L^T(i, j) = A(j, i), i < j */
+      for (int e = tid; e < j * nrhs; e += NT) {
+        const int i = e % j, r = e / j;
+        s_x[r * n + i] -= A[(size_t)i * lda + j] * s_x[r * n + j];
+      }
+      __syncthreads();
+    }
+    if (tid < nrhs) {                        /* x = P^T w: the swaps in reverse */
+      for (int j = n - 1; j >= 0; --j) {
+        const int p = ipiv[j];
+        if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
+      }
     }
     __syncthreads();
   }
   for (int e = tid; e < n * nrhs; e += NT) B[(size_t)(e / n) * ldb + e % n] = s_x[e];
+}
+
+/* ---------------------------------------------------------------------- */
+/* Cholesky adjoint of one instance by one workgroup                       */
+/* ---------------------------------------------------------------------- */
+/* C(tile r0.., c0..) = sum_k Aop(r, k) Bop(k, c) over k in [k0, k1), where
+ * Aop(r, k) = A[r*ar + k*ak] and Bop(k, c) = B[k*bk + c*bc] (strides give
+ * the transposes); only r >= c is written when lower_only. */
+template <typename T>
+__device__ void tile_gemm(T *C, long ldc, const T *A, long ar, long ak, const T *B, long bk,
+                          long bc, int k0, int k1, int mr, int mc, bool lower_only, int r0,
+                          int c0, T *s_a, T *s_b) {
+  const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  T acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = T(0);
+  for (int kb = k0; kb < k1; kb += 16) {
+    const int kc = min(16, k1 - kb);
+    __syncthreads();
+    for (int e = tid; e < 16 * TS; e += NT) {
+      const int kk = e / TS, rr = e % TS;
+      s_a[kk * TS + rr] = (kk < kc && rr < mr) ? A[(size_t)rr * ar + (size_t)(kb + kk) * ak] : T(0);
+      s_b[kk * TS + rr] = (kk < kc && rr < mc) ? B[(size_t)(kb + kk) * bk + (size_t)rr * bc] : T(0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int kk = 0; kk < 16; ++kk) {
+      T a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) a[i] = s_a[kk * TS + ty * 4 + i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[j] = s_b[kk * TS + tx * 4 + j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] += a[i] * b[j];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = ty * 4 + i, c = tx * 4 + j;
+      if (r < mr && c < mc && (!lower_only || r0 + r >= c0 + c)) C[(size_t)c * ldc + r] = acc[i][j];
+    }
+}
+
+/* Given L (lower triangle of A after potrf; its strict upper triangle is
+ * overwritten with zeros here), X = A^{-1} B and Bb = A^{-1} Xb (the solve's
+ * adjoint), write to G the gradient of <Xb, X> with respect to the lower
+ * triangle of A (the entries potrf reads; strict upper part zero):
+ *   Lb = tril(-(Bb X^T + X Bb^T) L)            (potrs adjoint: L's gradient)
+ *   P  = Phi(L^T Lb)   (lower triangle, diagonal halved)
+ *   S  = L^{-T} P L^{-1}                        (two triangular solves)
+ *   G  = tril(S + S^T) - diag(S)  (the symmetric gradient (S + S^T)/2, each
+ *        off-diagonal entry counted for both of its positions)
+ * (Murray, "Differentiation of the Cholesky decomposition", 2016).  W: an
+ * n x n scratch (leading dimension n). */
+template <typename T>
+__device__ void wg_chol_adjoint(int n, T *L, long lda, int nrhs, const T *Xs, const T *Bb,
+                                long ldx, T *G, long ldg, T *W, T *s_a, T *s_b) {
+  const int tid = threadIdx.x;
+  const int nt = (n + TS - 1) / TS;
+  /* W = -(Bb X^T + X Bb^T); G = 0; strict upper of L = 0 */
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e % n, j = e / n;
+    T v = T(0);
+    for (int r = 0; r < nrhs; ++r)
+      v += Bb[(size_t)r * ldx + i] * Xs[(size_t)r * ldx + j] +
+           Xs[(size_t)r * ldx + i] * Bb[(size_t)r * ldx + j];
+    W[(size_t)j * n + i] = -v;
+    G[(size_t)j * ldg + i] = T(0);
+    if (i < j) L[(size_t)j * lda + i] = T(0);
+  }
+  __syncthreads();
+  /* G = tril(W L): G(i, j) = sum_{k >= j} W(i, k) L(k, j) */
+  for (int tj = 0; tj < nt; ++tj)
+    for (int ti = tj; ti < nt; ++ti) {
+      const int r0 = ti * TS, c0 = tj * TS;
+      tile_gemm(G + (size_t)c0 * ldg + r0, ldg, W + r0, 1, n, L + (size_t)c0 * lda, 1, lda, c0,
+                n, min(TS, n - r0), min(TS, n - c0), ti == tj, r0, c0, s_a, s_b);
+    }
+  __syncthreads();
+  /* W = L^T G on the lower triangle: P(i, j) = sum_{k >= i} L(k, i) G(k, j) */
+  for (int tj = 0; tj < nt; ++tj)
+    for (int ti = tj; ti < nt; ++ti) {
+      const int r0 = ti * TS, c0 = tj * TS;
+      tile_gemm(W + (size_t)c0 * n + r0, n, L + (size_t)r0 * lda, lda, 1, G + (size_t)c0 * ldg,
+                1, ldg, r0, n, min(TS, n - r0), min(TS, n - c0), ti == tj, r0, c0, s_a, s_b);
+    }
+  __syncthreads();
+  /* Phi: halve the diagonal, zero the strict upper triangle */
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e % n, j = e / n;
+    if (i < j) W[(size_t)j * n + i] = T(0);
+    else if (i == j) W[(size_t)j * n + i] *= T(0.5);
+  }
+  __syncthreads();
+  /* Y = L^{-T} P in place: rows from the last; Y(i,:) /= L(i,i), then rows
+   * k < i lose L(i, k) Y(i,:) */
+  for (int i = n - 1; i >= 0; --i) {
+    const T d = L[(size_t)i * lda + i];
+    for (int c = tid; c < n; c += NT) W[(size_t)c * n + i] /= d;
+    __syncthreads();
+    for (int e = tid; e < i * n; e += NT) {
+      const int k = e % i, c = e / i;
+      W[(size_t)c * n + k] -= L[(size_t)k * lda + i] * W[(size_t)c * n + i];
+    }
+    __syncthreads();
+  }
+  /* S = Y L^{-1} in place: columns from the last; S(:,j) /= L(j,j), then
+   * columns j' < j lose S(:, j) L(j, j') */
+  for (int j = n - 1; j >= 0; --j) {
+    const T d = L[(size_t)j * lda + j];
+    for (int r = tid; r < n; r += NT) W[(size_t)j * n + r] /= d;
+    __syncthreads();
+    for (int e = tid; e < n * j; e += NT) {
+      const int r = e % n, jp = e / n;
+      W[(size_t)jp * n + r] -= W[(size_t)j * n + r] * L[(size_t)jp * lda + j];
+    }
+    __syncthreads();
+  }
+  /* G = tril(S + S^T) - diag(S) */
+  for (int e = tid; e < n * n; e += NT) {
+    const int i = e % n, j = e / n;
+    T v = T(0);
+    if (i > j) v = W[(size_t)j * n + i] + W[(size_t)i * n + j];
+    else if (i == j) v = W[(size_t)j * n + i];
+    G[(size_t)j * ldg + i] = v;
+  }
+  __syncthreads();
 }
 
 template <typename T>
@@ -404,7 +565,7 @@ __global__ __launch_bounds__(NT) void posv_batched_kernel(BatchArgs<T> g) {
 }
 
 template <typename T, int NB>
-__global__ __launch_bounds__(NT) void gesv_batched_kernel(BatchArgs<T> g, int maxn) {
+__global__ __launch_bounds__(NT) void gesv_batched_kernel(BatchArgs<T> g, int maxn, int trans) {
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   T *s_p = (T *)smem;                                  /* maxn x NB */
   T *s_a = s_p + (size_t)maxn * NB;                    /* 16 x 64 */
@@ -421,7 +582,7 @@ __global__ __launch_bounds__(NT) void gesv_batched_kernel(BatchArgs<T> g, int ma
     if (it >= g.count) return;
     const int i = g.order[it];
     wg_gesv<T, NB>(g.n[i], g.A[i], g.lda[i], g.ipiv[i], g.nrhs, g.B ? g.B[i] : nullptr,
-                   g.ldb ? g.ldb[i] : 1, &g.info[i], s_p, s_a, s_b, s_x, s_v, s_i);
+                   g.ldb ? g.ldb[i] : 1, &g.info[i], s_p, s_a, s_b, s_x, s_v, s_i, trans != 0);
   }
 }
 
@@ -517,6 +678,31 @@ __global__ __launch_bounds__(NT) void gemm_batched_kernel(GemmBatchArgs<T> g) {
   }
 }
 
+template <typename T>
+struct AdjArgs {
+  int count, nrhs;
+  const int *order, *n, *lda, *ldx, *ldg;
+  T *const *L; T *const *X; T *const *Bb; T *const *G; T *const *W;
+  /* L's strict upper triangle is zeroed */
+  int *counter;
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void chol_adjoint_kernel(AdjArgs<T> g) {
+  __shared__ T s_a[16 * TS], s_b[16 * TS];
+  __shared__ int s_item;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = atomicAdd(g.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= g.count) return;
+    const int i = g.order[it];
+    wg_chol_adjoint<T>(g.n[i], g.L[i], g.lda[i], g.nrhs, g.X[i], g.Bb[i], g.ldx[i], g.G[i],
+                       g.ldg[i], g.W[i], s_a, s_b);
+  }
+}
+
 template <typename T, int NB>
 hipError_t launch_posv(const BatchArgs<T> &g, int maxn, int grid, hipStream_t s) {
   const size_t lds = sizeof(T) * ((size_t)NB * (NB + 1) + 2 * 16 * TS + (size_t)maxn * MAXR);
@@ -529,13 +715,13 @@ hipError_t launch_posv(const BatchArgs<T> &g, int maxn, int grid, hipStream_t s)
 }
 
 template <typename T, int NB>
-hipError_t launch_gesv(const BatchArgs<T> &g, int maxn, int grid, hipStream_t s) {
+hipError_t launch_gesv(const BatchArgs<T> &g, int maxn, int grid, int trans, hipStream_t s) {
   const size_t lds = sizeof(T) * ((size_t)maxn * NB + 2 * 16 * TS + (size_t)maxn * MAXR);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipError_t e = hipFuncSetAttribute((const void *)gesv_batched_kernel<T, NB>,
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((gesv_batched_kernel<T, NB>), dim3(grid), dim3(NT), lds, s, g, maxn);
+  hipLaunchKernelGGL((gesv_batched_kernel<T, NB>), dim3(grid), dim3(NT), lds, s, g, maxn, trans);
   return hipGetLastError();
 }
 
@@ -551,9 +737,9 @@ hipError_t batch_factor(int kind, int nb, int count, const int *order, const int
   if (kind == 0) {
     if (nb == 16) return launch_posv<T, 16>(g, maxn, grid, s);
     if (nb == 32) return launch_posv<T, 32>(g, maxn, grid, s);
-  } else {
-    if (nb == 16) return launch_gesv<T, 16>(g, maxn, grid, s);
-    if (nb == 32) return launch_gesv<T, 32>(g, maxn, grid, s);
+  } else if (kind == 1 || kind == 2) {      /* 2: solve with A^T */
+    if (nb == 16) return launch_gesv<T, 16>(g, maxn, grid, kind == 2, s);
+    if (nb == 32) return launch_gesv<T, 32>(g, maxn, grid, kind == 2, s);
   }
   return hipErrorInvalidValue;
 }
@@ -577,7 +763,39 @@ hipError_t batch_gemm(int count, int ta, int tb, const int *order, const int *m,
   return hipGetLastError();
 }
 
+template <typename T>
+hipError_t chol_adjoint(int count, const int *order, const int *n, T *const *L, const int *lda,
+                        int nrhs, T *const *Xs, T *const *Bb, const int *ldx, T *const *G,
+                        const int *ldg, T *const *W, int *counter, int grid, hipStream_t s) {
+  if (count <= 0) return hipSuccess;
+  if (nrhs < 0 || grid <= 0) return hipErrorInvalidValue;
+  AdjArgs<T> g;
+  g.count = count; g.nrhs = nrhs; g.order = order; g.n = n; g.lda = lda; g.ldx = ldx;
+  g.ldg = ldg; g.L = L; g.X = Xs; g.Bb = Bb; g.G = G; g.W = W; g.counter = counter;
+  hipLaunchKernelGGL(chol_adjoint_kernel<T>, dim3(grid), dim3(NT), 0, s, g);
+  return hipGetLastError();
+}
+
 }  // namespace
+
+extern "C" hipError_t atlmx_k_dbatch_chol_adjoint(int count, const int *order, const int *n,
+                                                  double *const *L, const int *lda, int nrhs,
+                                                  double *const *X, double *const *Bb,
+                                                  const int *ldx, double *const *G,
+                                                  const int *ldg, double *const *W, int *counter,
+                                                  int grid, hipStream_t s) {
+  return chol_adjoint<double>(count, order, n, L, lda, nrhs, X, Bb, ldx, G, ldg, W, counter,
+                              grid, s);
+}
+extern "C" hipError_t atlmx_k_sbatch_chol_adjoint(int count, const int *order, const int *n,
+                                                  float *const *L, const int *lda, int nrhs,
+                                                  float *const *X, float *const *Bb,
+                                                  const int *ldx, float *const *G,
+                                                  const int *ldg, float *const *W, int *counter,
+                                                  int grid, hipStream_t s) {
+  return chol_adjoint<float>(count, order, n, L, lda, nrhs, X, Bb, ldx, G, ldg, W, counter,
+                             grid, s);
+}
 
 extern "C" hipError_t atlmx_k_dbatch_factor(int kind, int nb, int count, const int *order,
                                             const int *n, double *const *A, const int *lda,

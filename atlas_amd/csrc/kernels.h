@@ -104,6 +104,21 @@ hipError_t atlmx_k_sbatch_factor(int kind, int nb, int count, const int *order, 
                                  float *const *A, const int *lda, int *const *ipiv, int nrhs,
                                  float *const *B, const int *ldb, int *info, int *counter,
                                  int maxn, int grid, hipStream_t s);
+/* kind 2 of atlmx_k_?batch_factor: LU as kind 1, then solve A^T X = B. */
+/* Cholesky adjoint of the solve X = A^{-1} B per instance: L (potrf's lower
+ * triangle; its strict upper triangle is zeroed), X, Bb = A^{-1} Xb (n x
+ * nrhs, ld ldx); writes G (ld ldg) = gradient of <Xb, X> with respect to
+ * the lower triangle of A; W: n x n scratch (ld n). */
+hipError_t atlmx_k_dbatch_chol_adjoint(int count, const int *order, const int *n,
+                                       double *const *L, const int *lda, int nrhs,
+                                       double *const *X, double *const *Bb, const int *ldx,
+                                       double *const *G, const int *ldg, double *const *W,
+                                       int *counter, int grid, hipStream_t s);
+hipError_t atlmx_k_sbatch_chol_adjoint(int count, const int *order, const int *n,
+                                       float *const *L, const int *lda, int nrhs,
+                                       float *const *X, float *const *Bb, const int *ldx,
+                                       float *const *G, const int *ldg, float *const *W,
+                                       int *counter, int grid, hipStream_t s);
 /* C[i] = alpha op(A[i]) op(B[i]) + beta C[i] for count instances of sizes
  * m[i] x n[i] x k[i]; tiles are numbered instance by instance in `order`,
  * tile_start[p] = first tile of instance order[p] (count + 1 entries). */

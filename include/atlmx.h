@@ -101,6 +101,28 @@ atlmx_batch *atlmx_sgemm_batch_plan(int TA, int TB, int count, const int *h_m, c
                                     const int *h_k, float alpha, const float *const *d_A,
                                     const int *h_lda, const float *const *d_B, const int *h_ldb,
                                     float beta, float *const *d_C, const int *h_ldc);
+/* adjoints: gesv_t solves A[i]^T X = B[i] after the same LU (b-bar =
+ * A^{-T} x-bar); posv_adjoint takes L[i] (potrf's lower triangle, its strict
+ * upper triangle is zeroed), X[i] = A^{-1} B and Bb[i] = A^{-1} Xb and
+ * writes G[i] = the gradient of <Xb, X> with respect to the lower triangle
+ * of A[i], through Phi(L^T L-bar) and two triangular solves; W[i]: n x n
+ * scratch (leading dimension n). */
+atlmx_batch *atlmx_dgesv_t_batch_plan(int count, const int *h_n, double *const *d_A,
+                                      const int *h_lda, int *const *d_ipiv, int nrhs,
+                                      double *const *d_B, const int *h_ldb, int *d_info);
+atlmx_batch *atlmx_sgesv_t_batch_plan(int count, const int *h_n, float *const *d_A,
+                                      const int *h_lda, int *const *d_ipiv, int nrhs,
+                                      float *const *d_B, const int *h_ldb, int *d_info);
+atlmx_batch *atlmx_dposv_adjoint_batch_plan(int count, const int *h_n, double *const *d_L,
+                                            const int *h_lda, int nrhs, double *const *d_X,
+                                            double *const *d_Bb, const int *h_ldx,
+                                            double *const *d_G, const int *h_ldg,
+                                            double *const *d_W);
+atlmx_batch *atlmx_sposv_adjoint_batch_plan(int count, const int *h_n, float *const *d_L,
+                                            const int *h_lda, int nrhs, float *const *d_X,
+                                            float *const *d_Bb, const int *h_ldx,
+                                            float *const *d_G, const int *h_ldg,
+                                            float *const *d_W);
 int  atlmx_batch_run(atlmx_batch *plan);
 void atlmx_batch_destroy(atlmx_batch *plan);                       /* synchronizes */
 void atlmx_batch_config(const atlmx_batch *plan, int *nb, int *grid);  /* the tuned launch */
