@@ -191,7 +191,7 @@ class IRInfo(C.Structure):
     """atlmx_ir_info (include/atlmx.h)"""
     _fields_ = [("iterations", C.c_int), ("converged", C.c_int), ("fallback", C.c_int),
                 ("factor_info", C.c_int), ("backward_error", C.c_double),
-                ("method", C.c_int)]
+                ("method", C.c_int), ("gmres", C.c_int), ("inner_iterations", C.c_int)]
 
 
 MIXED_AUTO, MIXED_TSLU, MIXED_GEPP = 0, 1, 2
@@ -212,6 +212,18 @@ def dsgesv_dev(A, B, Xout, iter_max=30):
     st = IRInfo()
     info = lib().atlmx_dsgesv(n, nrhs, dptr(A), n, dptr(B), n, dptr(Xout), n, iter_max,
                               C.byref(st))
+    return info, st
+
+
+def dsposv_dev(A, B, Xout, iter_max=30):
+    """FP64-accurate SPD solve (FP32 Cholesky + FP64 refinement); A holds the
+    full symmetric matrix."""
+    n = A.shape[0]
+    nrhs = B.shape[0]
+    st = IRInfo()
+    L = lib()
+    L.atlmx_dsposv.argtypes = [_I, _I, _P, _I, _P, _I, _P, _I, _I, _P]
+    info = L.atlmx_dsposv(n, nrhs, dptr(A), n, dptr(B), n, dptr(Xout), n, iter_max, C.byref(st))
     return info, st
 
 

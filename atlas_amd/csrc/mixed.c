@@ -34,8 +34,17 @@ void sgemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, float alpha,
 void strsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
                float alpha, const float *A, long lda, float *B, long ldb,
                const int *abort_flag, const char *rout);
+int spotrf_dev(atlmx_ctx *c, int uplo, int N, float *A, long lda, const char *rout);
+void spotrs_dev(atlmx_ctx *c, int uplo, int N, int NRHS, const float *A, long lda, float *B,
+                long ldb, const char *rout);
+int dpotrf_dev(atlmx_ctx *c, int uplo, int N, double *A, long lda, const char *rout);
+void dpotrs_dev(atlmx_ctx *c, int uplo, int N, int NRHS, const double *A, long lda, double *B,
+                long ldb, const char *rout);
+void dgemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, double alpha,
+               const double *A, long lda, const double *B, long ldb, double beta, double *C,
+               long ldc, int tri, const int *abort_flag, const char *rout);
 
-enum { MP_AUTO = 0, MP_TSLU = 1, MP_GEPP = 2, MP_FP64 = 3 };
+enum { MP_AUTO = 0, MP_TSLU = 1, MP_GEPP = 2, MP_FP64 = 3, MP_CHOL = 4 };
 static int mp_method = MP_AUTO;
 
 /* 0: tournament-pivoting FP32 LU, then partial-pivoting FP32 LU, then
@@ -179,7 +188,7 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
 
 /* a low-precision factorization and how to apply its inverse */
 typedef struct {
-  int kind;                 /* MP_TSLU or MP_GEPP */
+  int kind;                 /* MP_TSLU, MP_GEPP or MP_CHOL */
   int n;
   float *As;                /* FP32 factor: S = A^T (MP_TSLU) or A, leading dim lda */
   long lda;
@@ -200,6 +209,13 @@ static void lp_solve(atlmx_ctx *c, const lp_fact *f, int nb, const double *R, lo
               f->Rs, f->n, NULL, rout);
     strsm_dev(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_NONUNIT, f->n, nb, 1.0f, f->As, f->lda,
               f->Rs, f->n, NULL, rout);
+    if (accumulate)
+      RT_CHECK(rout, atlmx_k_axpy_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
+    else
+      RT_CHECK(rout, atlmx_k_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
+  } else if (f->kind == MP_CHOL) {
+    RT_CHECK(rout, atlmx_k_d2s(f->n, nb, R, ldr, f->Rs, f->n, c->stream));
+    spotrs_dev(c, KX_LOWER, f->n, nb, f->As, f->lda, f->Rs, f->n, rout);
     if (accumulate)
       RT_CHECK(rout, atlmx_k_axpy_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
     else
@@ -265,6 +281,129 @@ static int refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const double 
   return 1;
 }
 
+/* <u, v> of two device vectors, via the GEMM kernel (synchronizes) */
+static double ddot_dev(atlmx_ctx *c, int n, const double *u, const double *v, double *tmp,
+                       const char *rout)
+{
+  double h = 0.0;
+  dgemm_dev(c, KX_TRANS, KX_NOTRANS, 1, 1, n, 1.0, u, n, v, n, 0.0, tmp, 1, 0, NULL, rout);
+  RT_CHECK(rout, hipMemcpyAsync(&h, tmp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  rt_sync(c, rout);
+  return h;
+}
+
+#define GM_RESTART 100
+
+/* GMRES-based refinement (Carson and Higham, "Accelerating the solution of
+ * linear systems by iterative refinement in three precisions", SIAM J. Sci.
+ * Comput. 40(2), 2018): each correction d of A d = r is found by GMRES in
+ * FP64 on the left-preconditioned system M^{-1} A d = M^{-1} r, M^{-1}
+ * applied with the low-precision factorization (classical Gram-Schmidt with
+ * one reorthogonalisation, Givens rotations on the host).  It converges for
+ * condition numbers well beyond plain refinement's kappa * u_factor < 1.
+ * Same stopping test as refine(); returns 1 when every RHS converged. */
+static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const double *A,
+                        int lda, const double *B, int ldb, double *X, int ldx, int outer_max,
+                        double anrm, double cte, double *R, double *nrm, void *ws,
+                        atlmx_ir_info *st, const char *rout)
+{
+  const int m = GM_RESTART;
+  double *V = (double *)rt_ws(c, WS_GMRES, sizeof(double) * ((size_t)N * (m + 3) + 4 * m + 64),
+                              rout);
+  double *T = V + (size_t)N * (m + 1), *Z = T + N, *hd = Z + N, *yd = hd + m + 2;
+  double H[GM_RESTART + 1][GM_RESTART], g[GM_RESTART + 1], cs[GM_RESTART], sn[GM_RESTART];
+  double y[GM_RESTART], h1[GM_RESTART + 1], h2[GM_RESTART + 1];
+  int j, it, k, i, conv = 1;
+  RT_CHECK(rout, atlmx_k_fill_i32((int *)Z, 2 * N, 0, c->stream));      /* Z = 0.0 */
+  st->backward_error = 0.0;
+  for (j = 0; j < NRHS; ++j) {
+    const double *b = B + (size_t)j * ldb;
+    double *x = X + (size_t)j * ldx;
+    double bn = 0.0, xn = 0.0, rn = 0.0;
+    int cj = 0;
+    RT_CHECK(rout, atlmx_k_dcolmax(N, 1, b, ldb, nrm + 64, c->stream));
+    RT_CHECK(rout, hipMemcpyAsync(&bn, nrm + 64, sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+    for (it = 0; it <= outer_max; ++it) {
+      double beta;
+      int kk = 0;
+      RT_CHECK(rout, atlmx_k_dresid(N, N, 1, A, lda, x, ldx, b, ldb, R, N, ws, c->stream));
+      RT_CHECK(rout, atlmx_k_dcolmax(N, 1, x, ldx, nrm, c->stream));
+      RT_CHECK(rout, atlmx_k_dcolmax(N, 1, R, N, nrm + 32, c->stream));
+      RT_CHECK(rout, hipMemcpyAsync(&xn, nrm, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      RT_CHECK(rout, hipMemcpyAsync(&rn, nrm + 32, sizeof(double), hipMemcpyDeviceToHost,
+                                    c->stream));
+      rt_sync(c, rout);
+      if (rn <= xn * cte) { cj = 1; break; }
+      if (it == outer_max) break;
+      /* GMRES on M^{-1} A d = M^{-1} r from d = 0 */
+      lp_solve(c, f, 1, R, N, V, N, 0, rout);
+      beta = sqrt(ddot_dev(c, N, V, V, hd, rout));
+      if (!(beta > 0.0) || !(beta < 1e300)) break;
+      RT_CHECK(rout, atlmx_k_dscal2d(N, 1, 1.0 / beta, V, N, c->stream));
+      memset(g, 0, sizeof(g));
+      g[0] = beta;
+      for (k = 0; k < m; ++k) {
+        double *vk = V + (size_t)k * N, *w = V + (size_t)(k + 1) * N, hn, t;
+        /* w = M^{-1} A v_k (T = 0 - A v_k, then negated after the solve) */
+        RT_CHECK(rout, atlmx_k_dresid(N, N, 1, A, lda, vk, N, Z, N, T, N, ws, c->stream));
+        lp_solve(c, f, 1, T, N, w, N, 0, rout);
+        RT_CHECK(rout, atlmx_k_dscal2d(N, 1, -1.0, w, N, c->stream));
+        /* classical Gram-Schmidt, twice */
+        dgemm_dev(c, KX_TRANS, KX_NOTRANS, k + 1, 1, N, 1.0, V, N, w, N, 0.0, hd, k + 1, 0, NULL,
+                  rout);
+        RT_CHECK(rout, hipMemcpyAsync(h1, hd, sizeof(double) * (k + 1), hipMemcpyDeviceToHost,
+                                      c->stream));
+        dgemm_dev(c, KX_NOTRANS, KX_NOTRANS, N, 1, k + 1, -1.0, V, N, hd, k + 1, 1.0, w, N, 0,
+                  NULL, rout);
+        dgemm_dev(c, KX_TRANS, KX_NOTRANS, k + 1, 1, N, 1.0, V, N, w, N, 0.0, hd, k + 1, 0, NULL,
+                  rout);
+        RT_CHECK(rout, hipMemcpyAsync(h2, hd, sizeof(double) * (k + 1), hipMemcpyDeviceToHost,
+                                      c->stream));
+        dgemm_dev(c, KX_NOTRANS, KX_NOTRANS, N, 1, k + 1, -1.0, V, N, hd, k + 1, 1.0, w, N, 0,
+                  NULL, rout);
+        hn = sqrt(ddot_dev(c, N, w, w, hd + m, rout));      /* synchronizes: h1, h2 ready */
+        for (i = 0; i <= k; ++i) H[i][k] = h1[i] + h2[i];
+        H[k + 1][k] = hn;
+        for (i = 0; i < k; ++i) {                           /* previous rotations */
+          t = cs[i] * H[i][k] + sn[i] * H[i + 1][k];
+          H[i + 1][k] = -sn[i] * H[i][k] + cs[i] * H[i + 1][k];
+          H[i][k] = t;
+        }
+        t = hypot(H[k][k], H[k + 1][k]);
+        cs[k] = (t > 0.0) ? H[k][k] / t : 1.0;
+        sn[k] = (t > 0.0) ? H[k + 1][k] / t : 0.0;
+        H[k][k] = t;
+        H[k + 1][k] = 0.0;
+        g[k + 1] = -sn[k] * g[k];
+        g[k] = cs[k] * g[k];
+        kk = k + 1;
+        if (hn > 0.0) RT_CHECK(rout, atlmx_k_dscal2d(N, 1, 1.0 / hn, w, N, c->stream));
+        if (!(fabs(g[k + 1]) > 1e-12 * beta) || !(hn > 0.0)) break;
+      }
+      /* d = V y with H y = g (upper triangular, kk x kk); x += d */
+      for (i = kk - 1; i >= 0; --i) {
+        double v = g[i];
+        for (k = i + 1; k < kk; ++k) v -= H[i][k] * y[k];
+        y[i] = v / H[i][i];
+      }
+      RT_CHECK(rout, hipMemcpyAsync(yd, y, sizeof(double) * kk, hipMemcpyHostToDevice,
+                                    c->stream));
+      dgemm_dev(c, KX_NOTRANS, KX_NOTRANS, N, 1, kk, 1.0, V, N, yd, kk, 1.0, x, ldx, 0, NULL,
+                rout);
+      rt_sync(c, rout);                                     /* y lives on the host stack */
+      st->inner_iterations += kk;
+    }
+    if (it > st->iterations) st->iterations = it;
+    {
+      const double be = rn / (anrm * xn + bn);
+      if (!(be <= st->backward_error)) st->backward_error = be;
+    }
+    if (!cj) { conv = 0; break; }
+  }
+  return conv;
+}
+
 int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
                  double *X, int ldx, int iter_max, atlmx_ir_info *st)
 {
@@ -306,6 +445,11 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
     if (!info)
       conv = refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm, ws,
                     st, rout);
+    if (!info && !conv) {
+      st->gmres = 1;
+      conv = gmres_refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm,
+                          ws, st, rout);
+    }
   }
   if (!conv && mp_method != MP_TSLU) {
     /* partial-pivoting FP32 LU of A itself */
@@ -319,9 +463,15 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
     info = sgetrf_dev(c, N, N, f.As, N, ipiv, rout);
     st->factor_info = info;
     st->method = MP_GEPP;
+    st->gmres = 0;
     if (!info)
       conv = refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm, ws,
                     st, rout);
+    if (!info && !conv) {
+      st->gmres = 1;
+      conv = gmres_refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm,
+                          ws, st, rout);
+    }
   }
   st->converged = conv;
   if (!conv) {
@@ -339,6 +489,91 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
     info = dgetrf_dev(c, N, N, Aw, N, ipiv, rout);
     if (info) return info;
     dgetrs_dev(c, KX_NOTRANS, N, NRHS, Aw, N, ipiv, X, ldx, rout);
+    st->backward_error = 0.0;
+    for (r0 = 0; r0 < NRHS; r0 += MP_MAXRHS) {
+      nb = (NRHS - r0 < MP_MAXRHS) ? NRHS - r0 : MP_MAXRHS;
+      RT_CHECK(rout, atlmx_k_dresid(N, N, nb, A, lda, X + (size_t)r0 * ldx, ldx,
+                                    B + (size_t)r0 * ldb, ldb, R, N, ws, c->stream));
+      RT_CHECK(rout, atlmx_k_dcolmax(N, nb, X + (size_t)r0 * ldx, ldx, nrm, c->stream));
+      RT_CHECK(rout, atlmx_k_dcolmax(N, nb, R, N, nrm + 32, c->stream));
+      RT_CHECK(rout, atlmx_k_dcolmax(N, nb, B + (size_t)r0 * ldb, ldb, nrm + 64, c->stream));
+      RT_CHECK(rout, hipMemcpyAsync(xn, nrm, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                    c->stream));
+      RT_CHECK(rout, hipMemcpyAsync(rn, nrm + 32, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                    c->stream));
+      RT_CHECK(rout, hipMemcpyAsync(bn, nrm + 64, sizeof(double) * nb, hipMemcpyDeviceToHost,
+                                    c->stream));
+      rt_sync(c, rout);
+      for (j = 0; j < nb; ++j) {
+        const double be = rn[j] / (anrm * xn[j] + bn[j]);
+        if (!(be <= st->backward_error)) st->backward_error = be;
+      }
+    }
+  }
+  rt_sync(c, rout);
+  return 0;
+}
+
+/* FP64-accurate SPD solve: FP32 Cholesky (lower) + FP64 refinement, the
+ * FP64 Cholesky if that does not converge.  A holds the full symmetric
+ * matrix (the residual uses all of it; the factorizations read the lower
+ * triangle). */
+int atlmx_dsposv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
+                 double *X, int ldx, int iter_max, atlmx_ir_info *st)
+{
+  static const char *rout = "atlmx_dsposv";
+  atlmx_ctx *c = rt_ctx();
+  const double eps = 1.1102230246251565e-16;   /* 2^-53, LAPACK dlamch('E') */
+  double anrm, cte;
+  double *R, *nrm, *ws;
+  int info = 0, conv = 0;
+  lp_fact f;
+  atlmx_ir_info loc;
+  if (!st) st = &loc;
+  memset(st, 0, sizeof(*st));
+  if (N <= 0 || NRHS <= 0) return 0;
+  if (lda < N || ldb < N || ldx < N) return -1;
+  ws = (double *)rt_ws(c, WS_MP3, atlmx_k_resid_ws_bytes(N, MP_MAXRHS), rout);
+  nrm = (double *)rt_ws(c, WS_MP4, sizeof(double) * ((size_t)N + 128), rout);
+  R = (double *)rt_ws(c, WS_MP2, sizeof(double) * (size_t)N * MP_MAXRHS, rout);
+  RT_CHECK(rout, atlmx_k_drowsum(N, N, A, lda, nrm, ws, c->stream));
+  anrm = host_max(c, nrm, N, rout);
+  cte = anrm * eps * sqrt((double)N);
+
+  memset(&f, 0, sizeof(f));
+  f.kind = MP_CHOL;
+  f.n = N;
+  f.lda = N;
+  f.As = (float *)rt_ws(c, WS_MP0, sizeof(float) * (size_t)N * N, rout);
+  f.Rs = (float *)rt_ws(c, WS_MP1, sizeof(float) * (size_t)N * MP_MAXRHS, rout);
+  RT_CHECK(rout, atlmx_k_d2s(N, N, A, lda, f.As, N, c->stream));
+  info = spotrf_dev(c, KX_LOWER, N, f.As, N, rout);
+  st->factor_info = info;
+  st->method = MP_CHOL;
+  if (!info)
+    conv = refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm, ws, st,
+                  rout);
+  if (!info && !conv) {
+    st->gmres = 1;
+    conv = gmres_refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm, ws,
+                        st, rout);
+  }
+  st->converged = conv;
+  if (!conv) {
+    double *Aw = (double *)rt_ws(c, WS_MP0, sizeof(double) * (size_t)N * N, rout);
+    double xn[MP_MAXRHS], rn[MP_MAXRHS], bn[MP_MAXRHS];
+    int r0, nb, j;
+    st->fallback = 1;
+    st->method = MP_FP64;
+    RT_CHECK(rout, hipMemcpy2DAsync(Aw, sizeof(double) * (size_t)N, A, sizeof(double) * lda,
+                                    sizeof(double) * (size_t)N, (size_t)N,
+                                    hipMemcpyDeviceToDevice, c->stream));
+    RT_CHECK(rout, hipMemcpy2DAsync(X, sizeof(double) * ldx, B, sizeof(double) * ldb,
+                                    sizeof(double) * (size_t)N, (size_t)NRHS,
+                                    hipMemcpyDeviceToDevice, c->stream));
+    info = dpotrf_dev(c, KX_LOWER, N, Aw, N, rout);
+    if (info) return info;
+    dpotrs_dev(c, KX_LOWER, N, NRHS, Aw, N, X, ldx, rout);
     st->backward_error = 0.0;
     for (r0 = 0; r0 < NRHS; r0 += MP_MAXRHS) {
       nb = (NRHS - r0 < MP_MAXRHS) ? NRHS - r0 : MP_MAXRHS;

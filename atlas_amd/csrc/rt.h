@@ -23,6 +23,7 @@ enum {
   WS_TRSV,          /* few-RHS triangular solve counter + flags */
   WS_NPINV,         /* no-pivot LU: inverted diagonal-block factors */
   WS_TSLU,          /* tournament pivoting: tree + move list    */
+  WS_GMRES,         /* GMRES refinement: Krylov basis + vectors */
   WS_MP0,           /* mixed-precision solver buffers          */
   WS_MP1,
   WS_MP2,

@@ -133,9 +133,11 @@ void atlmx_batch_config(const atlmx_batch *plan, int *nb, int *grid);  /* the tu
  * stopping test (LAPACK dsgesv's: ||r|| <= ||x|| ||A|| eps sqrt(N)) are
  * FP64.  First the FP32 LU with tournament pivoting (pivot rows of each
  * 64-column panel chosen by a reduction tree of partial-pivoting
- * eliminations); if its refinement stalls or a pivot is zero, the FP32 LU
- * with partial pivoting, then the FP64 LU.  Synchronizes; returns 0 or the
- * FP64 LU's info. */
+ * eliminations) with plain refinement; if that stalls, GMRES-IR (GMRES in
+ * FP64 preconditioned by the same FP32 factors) reaches condition numbers
+ * far beyond 1/eps32; if a pivot is zero or that fails too, the FP32 LU with
+ * partial pivoting (plain, then GMRES), then the FP64 LU.  Synchronizes;
+ * returns 0 or the FP64 LU's info. */
 typedef struct {
   int iterations;          /* refinement steps taken (max over RHS groups)  */
   int converged;           /* stopping test met without fallback            */
@@ -143,8 +145,16 @@ typedef struct {
   int factor_info;         /* info of the last low-precision factorization  */
   double backward_error;   /* max_j ||b-Ax||inf / (||A||inf ||x||inf + ||b||inf) */
   int method;              /* 1 tournament-pivoting FP32, 2 partial-pivoting
-                              FP32, 3 FP64 */
+                              FP32, 3 FP64, 4 FP32 Cholesky */
+  int gmres;               /* 1: the corrections came from GMRES-IR (plain
+                              refinement with these factors stalled) */
+  int inner_iterations;    /* GMRES iterations in total */
 } atlmx_ir_info;
+/* SPD twin: FP32 Cholesky (lower triangle) + FP64 refinement, then the FP64
+ * Cholesky.  A must hold the full symmetric matrix (the FP64 residual uses
+ * both triangles).  method: 4 FP32 Cholesky, 3 FP64. */
+int atlmx_dsposv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
+                 double *X, int ldx, int iter_max, atlmx_ir_info *info);
 /* 0 (default): tournament, then partial pivoting, then FP64; 1: tournament
  * then FP64; 2: partial pivoting then FP64.  Process-wide. */
 void atlmx_set_mixed_method(int method);

@@ -185,12 +185,74 @@ def test_dsgesv_fp64_accuracy(torch, method, want, n, nrhs, kappa):
     assert np.array_equal(_host(dA), A) and np.array_equal(_host(dB), b)   # inputs untouched
 
 
-def test_dsgesv_falls_back_to_fp64_when_fp32_cannot_converge(torch):
+def test_dsgesv_falls_back_to_fp64_when_fp32_factor_is_singular(torch):
+    """column 0 lies below FP32's range (it rounds to zero): both FP32
+    factorizations meet an exactly zero pivot; the FP64 LU solves it"""
     n = 300
-    A, x, b = _system(n, 1e12, 1, seed=3)     # kappa * eps32 >> 1
+    rng = np.random.default_rng(3)
+    A = rng.uniform(-1, 1, (n, n)) + 4 * np.eye(n)
+    A[:, 0] *= 1e-50
+    x = rng.uniform(-1, 1, (n, 1))
+    b = A @ x
     dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
     dX = torch.zeros_like(dB)
     info, st = X.dsgesv_dev(dA, dB, dX)
+    assert info == 0 and st.fallback == 1 and st.method == 3 and st.factor_info > 0
+    be = X.lib().atlmx_dbackward_error(n, 1, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64
+
+
+@pytest.mark.parametrize("method,want", [(X.MIXED_TSLU, 1), (X.MIXED_GEPP, 2)])
+@pytest.mark.parametrize("kappa", [1e8, 1e9])
+def test_dsgesv_gmres_ir_beyond_fp32(torch, method, want, kappa):
+    """kappa * eps32 >> 1: plain refinement with the FP32 factors stalls, GMRES
+    preconditioned by them reaches the FP64 backward-error bar"""
+    n = 500
+    A, x, b = _system(n, kappa, 2, seed=9)
+    dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    X.set_mixed_method(method)
+    try:
+        info, st = X.dsgesv_dev(dA, dB, dX)
+    finally:
+        X.set_mixed_method(X.MIXED_AUTO)
+    assert info == 0 and st.converged == 1 and st.fallback == 0
+    assert st.method == want and st.gmres == 1 and st.inner_iterations > 0
+    be = X.lib().atlmx_dbackward_error(n, 2, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64, be
+
+
+def _spd_system(n, kappa, nrhs, seed):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = kappa ** (-np.arange(n) / max(n - 1, 1))
+    A = (Q * s) @ Q.T
+    A = (A + A.T) / 2
+    x = rng.uniform(-1, 1, (n, nrhs))
+    return A, x, A @ x
+
+
+@pytest.mark.parametrize("n,nrhs,kappa", [(1, 1, 1.0), (130, 2, 1e4), (1001, 3, 1e6),
+                                          (700, 17, 1e5)])
+def test_dsposv_fp64_accuracy(torch, n, nrhs, kappa):
+    A, x, b = _spd_system(n, kappa, nrhs, seed=n)
+    dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    info, st = X.dsposv_dev(dA, dB, dX)
+    assert info == 0
+    assert st.converged == 1 and st.fallback == 0 and st.method == 4
+    be = X.lib().atlmx_dbackward_error(n, nrhs, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64, be
+    ferr = np.abs(_host(dX) - x).max() / np.abs(x).max()
+    assert ferr <= 10 * kappa * n * EPS64, ferr
+
+
+def test_dsposv_falls_back_to_fp64(torch):
+    n = 300
+    A, x, b = _spd_system(n, 1e12, 1, seed=4)
+    dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    info, st = X.dsposv_dev(dA, dB, dX)
     assert info == 0 and st.fallback == 1 and st.method == 3
     be = X.lib().atlmx_dbackward_error(n, 1, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
     assert be <= n * EPS64
