@@ -197,6 +197,15 @@ class IRInfo(C.Structure):
 MIXED_AUTO, MIXED_TSLU, MIXED_GEPP = 0, 1, 2
 
 
+def set_refine_stop(lapack):
+    """refinement stopping test: False -> backward error <= n*2^-52 (default),
+    True -> LAPACK dsgesv's."""
+    L = lib()
+    L.atlmx_set_refine_stop.argtypes = [_I]
+    L.atlmx_set_refine_stop.restype = None
+    L.atlmx_set_refine_stop(int(bool(lapack)))
+
+
 def set_mixed_method(m):
     """0 tournament-pivoting FP32 LU, then partial-pivoting FP32 LU, then
     FP64; 1 tournament then FP64; 2 partial pivoting then FP64."""

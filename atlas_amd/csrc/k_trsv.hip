@@ -1,7 +1,8 @@
 /*
  * k_trsv.hip -- triangular solve with a few right-hand sides, in ONE
  * launch: the HBM-bound case of TRSM (getrs/potrs with NRHS = 1..4 reads
- * the n^2/2 factor once).
+ * the n^2/2 factor once).  Both orientations keep one output row per lane,
+ * so a block's result needs no cross-lane reduction.
  *
  * ATLAS runs this case through the same recursive TRSM as the many-RHS
  * case (src/blas/ulevel3/ATL_trsm.c:13-228).  Here each 64-row block of the
@@ -91,58 +92,37 @@ __global__ __launch_bounds__(NT) void trsv_kernel(int n, int nrhs, int lower, in
 #pragma unroll
     for (int r = 0; r < MAXR; ++r) s_part[q][ln][r] = acc[r];
   } else {
-    /* thread (lane cc = ln of each tile, column group q): column segments of
-     * A are contiguous, so lanes read consecutive elements; the 64 lanes'
-     * partial dot products are added once at the end */
-    T acc[16][MAXR];
+    /* thread (row i = ln, column group q): op(A)[r0+i][c0+cc] = A[c0+cc][r0+i]
+     * is stored contiguously in cc, so each lane reads its own 16-element
+     * (64- or 128-byte) segment and keeps its row's sums -- no cross-lane
+     * reduction on the chain from one block to the next */
+    T acc[MAXR];
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
-#pragma unroll
-      for (int r = 0; r < MAXR; ++r) acc[t][r] = T(0);
+    for (int r = 0; r < MAXR; ++r) acc[r] = T(0);
     for (int s = 0; s < order; ++s) {
       const int j = fwd ? s : nblk - 1 - s;
       const int c0 = j * TB, cb = min(TB, n - c0);
-      /* op(A)[r0+i][c0+ln] = A[c0+ln][r0+i]; loaded before waiting */
       T a[16];
+      const T *arow = A + (size_t)(r0 + ln) * lda + c0 + q * 16;
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int i = q * 16 + t;
-        a[t] = (ln < cb && i < rb) ? A[(size_t)(r0 + i) * lda + (c0 + ln)] : T(0);
-      }
+      for (int t = 0; t < 16; ++t)
+        a[t] = (ln < rb && q * 16 + t < cb) ? arow[t] : T(0);
       if (tid == 0)
         while (__hip_atomic_load(&ctl[1 + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
           __builtin_amdgcn_s_sleep(1);
       __syncthreads();
-      T xv[MAXR];
 #pragma unroll
-      for (int r = 0; r < MAXR; ++r)
-        xv[r] = (ln < cb && r < nrhs) ? ldc(&X[(size_t)r * ldx + c0 + ln]) : T(0);
+      for (int t = 0; t < 16; ++t) {
+        const int cc = q * 16 + t;
+        if (cc < cb) {
 #pragma unroll
-      for (int t = 0; t < 16; ++t)
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) acc[t][r] += a[t] * xv[r];
-    }
-    /* sum each (column i, rhs r) over the 64 lanes */
-#pragma unroll
-    for (int t = 0; t < 16; ++t)
-#pragma unroll
-      for (int r = 0; r < MAXR; ++r) {
-        T v = acc[t][r];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        acc[t][r] = v;
+          for (int r = 0; r < MAXR; ++r)
+            if (r < nrhs) acc[r] += a[t] * ldc(&X[(size_t)r * ldx + c0 + cc]);
+        }
       }
-    if (ln == 0) {
-#pragma unroll
-      for (int t = 0; t < 16; ++t)
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) s_part[0][q * 16 + t][r] = acc[t][r];
     }
-    if (ln < 3) {
 #pragma unroll
-      for (int t = 0; t < 16; ++t)
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) s_part[1 + ln][q * 16 + t][r] = T(0);
-    }
+    for (int r = 0; r < MAXR; ++r) s_part[q][ln][r] = acc[r];
   }
   __syncthreads();
   if (q == 0) {

@@ -323,19 +323,20 @@ __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
   for (int i = tid; i <= loff; i += TR) a.cnt[i] = 0;
 }
 
-/* S columns dst <- src (rows of A) for every move, over all n entries of
- * each column: each workgroup moves a 64-entry slice of all listed columns
- * through LDS, so sources are read before any destination is written.  The
- * slice of the diagonal block (rows and entries k..k+w-1) then receives the
- * block's factor lu11. */
+/* S columns dst <- src (rows of A) for every move, over the entries
+ * [ebeg, nrows) of each column: each workgroup moves a 64-entry slice of all
+ * listed columns through LDS, so sources are read before any destination is
+ * written.  A slice inside the diagonal block (rows and entries k..k+w-1)
+ * then receives the block's factor lu11. */
 __global__ __launch_bounds__(256) void tslu_swap_kernel(float *S, long lds, int nrows,
                                                         const int *plan, int k, int w,
-                                                        const float *lu11, const int *info) {
+                                                        const float *lu11, const int *info,
+                                                        int ebeg) {
   __shared__ float buf[2 * TW][64];
   __shared__ int s_dst[2 * TW], s_src[2 * TW];
   if (*info) return;
   const int np = plan[0];
-  const int e0 = blockIdx.x * 64;
+  const int e0 = ebeg + blockIdx.x * 64;
   for (int t = threadIdx.x; t < np; t += 256) {
     s_dst[t] = plan[2 + 2 * t];
     s_src[t] = plan[3 + 2 * t];
@@ -420,13 +421,13 @@ extern "C" hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k
   return hipGetLastError();
 }
 
-extern "C" hipError_t atlmx_k_tslu_swap(float *S, long lds, int nrows, const int *plan, int k,
-                                        int w, const float *lu11, const int *info,
+extern "C" hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, const int *plan,
+                                        int k, int w, const float *lu11, const int *info,
                                         hipStream_t s) {
-  if (nrows <= 0) return hipSuccess;
-  if (w < 0 || w > TW || k + w > nrows || !info) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(tslu_swap_kernel, dim3((nrows + 63) / 64), dim3(256), 0, s, S, lds, nrows,
-                     plan, k, w, lu11, info);
+  if (eend <= ebeg) return hipSuccess;
+  if (ebeg < 0 || w < 0 || w > TW || !info) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tslu_swap_kernel, dim3((eend - ebeg + 63) / 64), dim3(256), 0, s, S, lds,
+                     eend, plan, k, w, lu11, info, ebeg);
   return hipGetLastError();
 }
 

@@ -83,11 +83,12 @@ size_t atlmx_k_tslu_ws_ints(int n);
 hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w, int *ws,
                                int *plan, int *perm, float *lu11, float *invl, float *invu,
                                int *info, hipStream_t s);
-/* columns dst <- src of S (rows of A) for the moves of plan, all nrows
- * entries; then rows/entries k..k+w-1 (the diagonal block) <- lu11.
+/* columns dst <- src of S (rows of A) for the moves of plan, entries
+ * [ebeg, eend) of each column (S rows, i.e. columns of A); the part of the
+ * diagonal block (rows and entries k..k+w-1) inside that range <- lu11.
  * Nothing is done when *info is nonzero. */
-hipError_t atlmx_k_tslu_swap(float *S, long lds, int nrows, const int *plan, int k, int w,
-                             const float *lu11, const int *info, hipStream_t s);
+hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, const int *plan, int k,
+                             int w, const float *lu11, const int *info, hipStream_t s);
 /* Batched factor + solve in one persistent launch (k_batched.hip).
  * kind 0: Cholesky (lower) + solve; kind 1: LU (partial pivoting) + solve.
  * Instance i = order[it] for it = 0.. (taken from *counter, zero on entry):

@@ -4,15 +4,16 @@
  *
  * The LU is done in FP32 on the MFMA GEMM; only the residual r = b - A*x,
  * the update x += d and the stopping test are in FP64.  The stopping test is
- * LAPACK dsgesv's: ||r||_inf <= ||x||_inf * ||A||_inf * eps * sqrt(n) for
- * every right-hand side.
+ * the normwise backward error ||r||_inf / (||A||_inf ||x||_inf + ||b||_inf)
+ * <= n * 2^-52 for every right-hand side (or, on request, LAPACK dsgesv's
+ * ||r||_inf <= ||x||_inf ||A||_inf 2^-53 sqrt(n)).
  *
  * First choice: the FP32 LU with tournament pivoting of S = A^T
  * (k_tslu.hip), whose panels need one reduction tree instead of one
  * device-wide exchange per column and whose row interchanges are contiguous
- * column moves of S.  If its refinement stalls or a pivot is zero, the FP32
- * LU with partial pivoting of A itself is tried, and failing that the FP64
- * LU.
+ * column moves of S.  If its plain refinement stalls, GMRES-IR with the same
+ * factors; if that fails or a pivot is zero, the FP32 LU with partial
+ * pivoting of A itself (plain, then GMRES), and failing that the FP64 LU.
  */
 #include <math.h>
 #include <stdio.h>
@@ -46,6 +47,23 @@ void dgemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, double alpha,
 
 enum { MP_AUTO = 0, MP_TSLU = 1, MP_GEPP = 2, MP_FP64 = 3, MP_CHOL = 4 };
 static int mp_method = MP_AUTO;
+static int mp_stop_lapack = 0;
+
+/* Stopping test of the refinement: 0 (default) the normwise backward error
+ * ||r|| / (||A|| ||x|| + ||b||) <= n * 2^-52 (the accuracy the solver
+ * promises); 1 LAPACK dsgesv's ||r|| <= ||x|| ||A|| 2^-53 sqrt(n). */
+void atlmx_set_refine_stop(int lapack)
+{
+  mp_stop_lapack = (lapack != 0);
+}
+
+/* converged: residual norm rn of a column with solution norm xn, right-hand
+ * side norm bn; cte = ||A|| 2^-53 sqrt(n) */
+static int ir_done(double rn, double xn, double bn, double anrm, double cte, int n)
+{
+  if (mp_stop_lapack) return rn <= xn * cte;
+  return rn <= (double)n * 2.220446049250313e-16 * (anrm * xn + bn);
+}
 
 /* 0: tournament-pivoting FP32 LU, then partial-pivoting FP32 LU, then
  * FP64; 1: tournament only (then FP64); 2: partial pivoting only (then FP64). */
@@ -90,7 +108,7 @@ static void tslu_panel_swap(atlmx_ctx *c, float *S, long lds, int n, int k, int 
                             const char *rout)
 {
   int pid = rt_prof_begin(c, PROF_LASWP, 0.0);
-  RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, n, plan, k, w, lu11, d_info, c->stream));
+  RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, 0, n, plan, k, w, lu11, d_info, c->stream));
   rt_prof_end(c, pid);
 }
 
@@ -231,7 +249,7 @@ static void lp_solve(atlmx_ctx *c, const lp_fact *f, int nb, const double *R, lo
 }
 
 /* FP64 iterative refinement of every right-hand side with one low-precision
- * factorization.  Stops a group when LAPACK dsgesv's test holds for all its
+ * factorization.  Stops a group when the stopping test holds for all its
  * columns, or when a step fails to halve some column's residual (the
  * factorization is then not good enough).  Returns 1 when all converged. */
 static int refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const double *A, int lda,
@@ -264,8 +282,9 @@ static int refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const double 
       rt_sync(c, rout);
       conv = 1;
       for (j = 0; j < nb; ++j) {
-        if (!(rn[j] <= xn[j] * cte)) conv = 0;
-        if (it > 0 && !(rn[j] <= 0.5 * rprev[j]) && !(rn[j] <= xn[j] * cte)) stalled = 1;
+        const int done = ir_done(rn[j], xn[j], bn[j], anrm, cte, N);
+        if (!done) conv = 0;
+        if (it > 0 && !(rn[j] <= 0.5 * rprev[j]) && !done) stalled = 1;
         rprev[j] = rn[j];
       }
       if (conv || stalled || it == iter_max) break;
@@ -334,7 +353,7 @@ static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const d
       RT_CHECK(rout, hipMemcpyAsync(&rn, nrm + 32, sizeof(double), hipMemcpyDeviceToHost,
                                     c->stream));
       rt_sync(c, rout);
-      if (rn <= xn * cte) { cj = 1; break; }
+      if (ir_done(rn, xn, bn, anrm, cte, N)) { cj = 1; break; }
       if (it == outer_max) break;
       /* GMRES on M^{-1} A d = M^{-1} r from d = 0 */
       lp_solve(c, f, 1, R, N, V, N, 0, rout);

@@ -130,8 +130,7 @@ void atlmx_batch_config(const atlmx_batch *plan, int *nb, int *grid);  /* the tu
 /* ---- FP64-accurate solve from a low-precision LU + FP64 refinement ------
  * A (N x N), B (N x NRHS), X (N x NRHS) are FP64 device arrays; A and B are
  * not modified.  The LU is done in FP32; the residual, the update and the
- * stopping test (LAPACK dsgesv's: ||r|| <= ||x|| ||A|| eps sqrt(N)) are
- * FP64.  First the FP32 LU with tournament pivoting (pivot rows of each
+ * stopping test (atlmx_set_refine_stop) are FP64.  First the FP32 LU with tournament pivoting (pivot rows of each
  * 64-column panel chosen by a reduction tree of partial-pivoting
  * eliminations) with plain refinement; if that stalls, GMRES-IR (GMRES in
  * FP64 preconditioned by the same FP32 factors) reaches condition numbers
@@ -155,6 +154,10 @@ typedef struct {
  * both triangles).  method: 4 FP32 Cholesky, 3 FP64. */
 int atlmx_dsposv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
                  double *X, int ldx, int iter_max, atlmx_ir_info *info);
+/* Refinement stopping test: 0 (default) normwise backward error
+ * ||b - Ax||inf / (||A||inf ||x||inf + ||b||inf) <= N * 2^-52; 1 LAPACK
+ * dsgesv's ||r||inf <= ||x||inf ||A||inf 2^-53 sqrt(N).  Process-wide. */
+void atlmx_set_refine_stop(int lapack);
 /* 0 (default): tournament, then partial pivoting, then FP64; 1: tournament
  * then FP64; 2: partial pivoting then FP64.  Process-wide. */
 void atlmx_set_mixed_method(int method);
