@@ -8,9 +8,9 @@
  * Grigori, Demmel, Xiang, "CALU: a communication optimal LU factorization
  * algorithm", SIAM J. Matrix Anal. Appl. 32(4), 2011) picks the W pivot
  * rows of a panel with a reduction tree: every leaf runs Gaussian
- * elimination with partial pivoting on 256 rows of the panel and keeps the
- * W rows it chose; every tree node does the same on the ORIGINAL values of
- * its children's chosen rows (up to 4 x W).  The root's choice, in its
+ * elimination with partial pivoting on 256 rows of the panel and keeps
+ * the W rows it chose; every tree node does the same on the ORIGINAL values
+ * of its children's chosen rows (up to 4 x W).  The root's choice, in its
  * pivot order, becomes the panel's pivot rows; the panel is then factored
  * without further interchanges.  One reduction tree per panel replaces one
  * device-wide exchange per column.
@@ -34,7 +34,8 @@ namespace {
 
 constexpr int TW = ATLMX_TSLU_W;   /* 64 */
 constexpr int TR = 256;            /* rows per elimination = threads */
-constexpr int FAN = 4;
+constexpr int NWV = TR / 64;       /* wavefronts per workgroup */
+constexpr int FAN = TR / TW;       /* children per tree node: 4 */
 
 __device__ inline int ldc_i(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -44,21 +45,20 @@ __device__ inline void stc_i(int *p, int v) {
 }
 
 template <int CTRL, int RM>
-__device__ __forceinline__ float dpp_max(float v) {
-  const int o = __builtin_amdgcn_update_dpp(__float_as_int(-2.0f), __float_as_int(v), CTRL, RM,
-                                            0xf, false);
-  return fmaxf(v, __int_as_float(o));
+__device__ __forceinline__ unsigned dpp_max(unsigned v) {
+  const unsigned o = (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, RM, 0xf, false);
+  return v > o ? v : o;
 }
 
-/* maximum over the wavefront, broadcast (values >= -2) */
-__device__ __forceinline__ float wave_max(float v) {
+/* maximum over the wavefront, broadcast */
+__device__ __forceinline__ unsigned wave_max(unsigned v) {
   v = dpp_max<0x111, 0xf>(v);
   v = dpp_max<0x112, 0xf>(v);
   v = dpp_max<0x114, 0xf>(v);
   v = dpp_max<0x118, 0xf>(v);
   v = dpp_max<0x142, 0xa>(v);
   v = dpp_max<0x143, 0xc>(v);
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+  return (unsigned)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -75,6 +75,8 @@ struct TsluArgs {
   float *lu11;      /* out: the diagonal block's L\U, TW x TW column-major */
   float *invl, *invu; /* out: inv(L11) (unit lower), inv(U11), TW x TW */
   int *info;        /* zero pivot: 1 + its column; nonzero on entry = skip */
+  const int *prev;  /* the previous panel's move list, or null */
+  int *plan2;       /* out (with prev): prev's moves followed by these, as one list */
 };
 
 /* row `row` of A, columns k..k+w-1 (zero beyond w, or for row < 0) */
@@ -84,53 +86,68 @@ __device__ inline void load_row(f4 (&r)[TW / 4], const TsluArgs &a, int row) {
 #pragma unroll
     for (int q = 0; q < TW / 4; ++q) r[q] = *reinterpret_cast<const f4 *>(src + 4 * q);
   } else {
+    /* the bound as a per-lane value, so that its 64 tests stay here as
+     * vector compares instead of being hoisted into scalar registers */
+    int ww;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(ww) : "v"(row >= 0 ? a.w : 0));
 #pragma unroll
     for (int q = 0; q < TW / 4; ++q)
 #pragma unroll
-      for (int e = 0; e < 4; ++e) r[q][e] = (row >= 0 && 4 * q + e < a.w) ? src[4 * q + e] : 0.0f;
+      for (int e = 0; e < 4; ++e) r[q][e] = (4 * q + e < ww) ? src[4 * q + e] : 0.0f;
   }
 }
 
 /* Gaussian elimination with partial pivoting on the rows held by the
  * workgroup (one per thread, in registers as float4), recording in s_sel the
- * rows chosen as pivots, in order.  Ties go to the lower thread.  The
- * multipliers are kept in place, so a row retired at column c holds its L
- * entries in r[0..c) and its U entries in r[c..TW); *mypos = c.  All TW
- * columns are eliminated: columns past the panel width hold zeros, so
- * their steps change nothing that is used (their choices are ignored), and
- * the loop body has no branch that would make the compiler copy r. */
-__device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[4][TW / 4],
-                            float (*s_val)[4], int (*s_thr)[4], int *s_sel, int *mypos) {
+ * rows chosen as pivots, in order.  Each step ranks a row by the integer
+ * bits(|entry|) + 1 (0 for a row already chosen, 1 for NaN: never better
+ * than a number), which orders like |entry|; every wavefront takes the
+ * maximum (ties: lowest lane), its owner stores the key
+ * rank << 32 | (TR - 1 - tid) and its row in the wavefront's slot, and
+ * after one barrier every thread takes the largest key -- the largest
+ * entry, ties to the lower thread -- and reads the winner's row.  Each
+ * thread notes the step that chose its row; s_sel is written from those
+ * after the loop (-1 where no row was left).  The multipliers are kept in
+ * place, so a row retired at column c holds its L entries in r[0..c) and its
+ * U entries in r[c..TW); *mypos = c.  All TW columns are eliminated:
+ * columns past the panel width hold zeros, so their steps change nothing
+ * that is used (their choices are ignored), and the loop body has no branch
+ * that would make the compiler copy r. */
+__device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[NWV][TW / 4],
+                            unsigned long long (*s_key)[NWV], int *s_sel, int *mypos) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool act = myrow >= 0;
   int pos = -1;
+  if (tid < TW) s_sel[tid] = -1;
 #pragma unroll
   for (int c = 0; c < TW; ++c) {
     const int par = c & 1;
     const float rc = r[c / 4][c % 4];
-    float v = act ? fabsf(rc) : -1.0f;
-    if (!(v == v)) v = 0.0f;                      /* NaN: never better than a number */
-    const float wm = wave_max(v);
-    const unsigned long long bal = __ballot(v == wm);
+    const float av = fabsf(rc);
+    const unsigned u = act ? ((av == av) ? __float_as_uint(av) + 1u : 1u) : 0u;
+    const unsigned wm = wave_max(u);
+    const unsigned long long bal = __ballot(u == wm);
     const int wl = __ffsll((long long)bal) - 1;
     if (lane == wl) {                             /* stores only */
-      s_val[par][wave] = wm;
-      s_thr[par][wave] = tid;
+      s_key[par][wave] = wm ? ((unsigned long long)wm << 32) | (unsigned)(TR - 1 - tid) : 0ull;
 #pragma unroll
       for (int q = c / 4; q < TW / 4; ++q) s_row[par][wave][q] = r[q];
     }
     __syncthreads();
-    float best = s_val[par][0];
-    int bw = 0;
+    unsigned long long kv[NWV];
 #pragma unroll
-    for (int q = 1; q < 4; ++q)
-      if (s_val[par][q] > best) { best = s_val[par][q]; bw = q; }
-    const bool any = best >= 0.0f;                /* some row was still active */
-    const bool me = any && tid == s_thr[par][bw];
-    if (me) s_sel[c] = myrow;
-    if (!any && tid == 0) s_sel[c] = -1;
+    for (int q = 0; q < NWV; ++q) kv[q] = s_key[par][q];
+#pragma unroll
+    for (int h = NWV / 2; h > 0; h /= 2)
+#pragma unroll
+      for (int q = 0; q < h; ++q) kv[q] = kv[q] > kv[q + h] ? kv[q] : kv[q + h];
+    const unsigned long long key = kv[0];
+    const bool any = key != 0ull;                 /* some row was still active */
+    const int wt = TR - 1 - (int)(unsigned)(key & 0xffffffffull);
+    const bool me = any && tid == wt;
     pos = me ? c : pos;
     act = act && !me;
+    const int bw = any ? (wt >> 6) : 0;
     const float piv = s_row[par][bw][c / 4][c % 4];
     const float l = (act && piv != 0.0f) ? rc / piv : 0.0f;
     r[c / 4][c % 4] = act ? l : rc;
@@ -142,15 +159,59 @@ __device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[4][TW / 4],
         if (4 * q + e > c) r[q][e] -= l * pr[e];
     }
   }
+  if (pos >= 0) s_sel[pos] = myrow;
   *mypos = pos;
   __syncthreads();
 }
 
-/* inverses of the unit lower and the upper triangle of the w x w factor in
- * s_lu (row-major, ld TW+1) by recursive doubling: for b = 1, 2, .., 32 each
- * 2b x 2b diagonal block gets its off-diagonal block of the inverse,
- * X21 = -X22 L21 X11 (lower) and X12 = -X11 U12 X22 (upper), from the
- * inverses of its two b x b halves.  Outside w the factor is the identity. */
+/* inverses of the unit lower and the upper triangle of the factor in s_lu
+ * (row-major, ld TW+1, the identity outside the leading w x w block) by
+ * recursive doubling: for b = 1, 2, .., 32 each 2b x 2b diagonal block gets
+ * its off-diagonal block of the inverse, X21 = -X22 L21 X11 (lower) and
+ * X12 = -X11 U12 X22 (upper), from the inverses of its two b x b halves.
+ * The sums run over all b terms (the triangles' zeros make up the rest), so
+ * every loop has a fixed trip count. */
+template <int B>
+__device__ __forceinline__ void tri_level(const float (*s_lu)[TW + 1], float (*s_xl)[TW + 1],
+                                          float (*s_xu)[TW + 1], float *s_t) {
+  constexpr int nblk = TW / (2 * B), per = B * B, tot = 2 * nblk * per;
+  const int tid = threadIdx.x;
+  /* T = L21 X11 (lower) into s_t[0..], T' = U12 X22 (upper) into s_t[TW*TW/2..] */
+  for (int e = tid; e < tot; e += TR) {
+    const bool up = e >= nblk * per;
+    const int f = e - (up ? nblk * per : 0);
+    const int q = f / per, i = (f % per) / B, j = f % B, o = q * 2 * B;
+    float acc = 0.0f;
+    if (!up) {
+#pragma unroll
+      for (int l = 0; l < B; ++l) acc += s_lu[o + B + i][o + l] * s_xl[o + l][o + j];
+      s_t[f] = acc;
+    } else {
+#pragma unroll
+      for (int l = 0; l < B; ++l) acc += s_lu[o + i][o + B + l] * s_xu[o + B + l][o + B + j];
+      s_t[TW * TW / 2 + f] = acc;
+    }
+  }
+  __syncthreads();
+  for (int e = tid; e < tot; e += TR) {
+    const bool up = e >= nblk * per;
+    const int f = e - (up ? nblk * per : 0);
+    const int q = f / per, i = (f % per) / B, j = f % B, o = q * 2 * B;
+    float acc = 0.0f;
+    if (!up) {
+#pragma unroll
+      for (int l = 0; l < B; ++l) acc += s_xl[o + B + i][o + B + l] * s_t[q * per + l * B + j];
+      s_xl[o + B + i][o + j] = -acc;
+    } else {
+#pragma unroll
+      for (int l = 0; l < B; ++l)
+        acc += s_xu[o + i][o + l] * s_t[TW * TW / 2 + q * per + l * B + j];
+      s_xu[o + i][o + B + j] = -acc;
+    }
+  }
+  __syncthreads();
+}
+
 __device__ void tri_inverses(const float (*s_lu)[TW + 1], int w, float (*s_xl)[TW + 1],
                              float (*s_xu)[TW + 1], float *s_t) {
   const int tid = threadIdx.x;
@@ -160,54 +221,22 @@ __device__ void tri_inverses(const float (*s_lu)[TW + 1], int w, float (*s_xl)[T
     s_xu[i][j] = (i == j) ? ((i < w) ? 1.0f / s_lu[i][i] : 1.0f) : 0.0f;
   }
   __syncthreads();
-  for (int b = 1; b < TW; b *= 2) {
-    const int nblk = TW / (2 * b), per = b * b;
-    /* T = L21 X11 (lower) into s_t[0..], T' = U12 X22 (upper) into s_t[TW*TW/2..] */
-    for (int e = tid; e < 2 * nblk * per; e += TR) {
-      const int up = e >= nblk * per, f = e - (up ? nblk * per : 0);
-      const int q = f / per, i = (f % per) / b, j = f % b, o = q * 2 * b;
-      float acc = 0.0f;
-      if (!up) {
-        for (int l = j; l < b; ++l) {
-          const int gr = o + b + i, gc = o + l;
-          const float lv = (gr < w && gc < w) ? s_lu[gr][gc] : 0.0f;
-          acc += lv * s_xl[o + l][o + j];
-        }
-        s_t[f] = acc;
-      } else {
-        for (int l = 0; l <= j; ++l) {
-          const int gr = o + i, gc = o + b + l;
-          const float uv = (gr < w && gc < w) ? s_lu[gr][gc] : 0.0f;
-          acc += uv * s_xu[o + b + l][o + b + j];
-        }
-        s_t[TW * TW / 2 + f] = acc;
-      }
-    }
-    __syncthreads();
-    for (int e = tid; e < 2 * nblk * per; e += TR) {
-      const int up = e >= nblk * per, f = e - (up ? nblk * per : 0);
-      const int q = f / per, i = (f % per) / b, j = f % b, o = q * 2 * b;
-      float acc = 0.0f;
-      if (!up) {
-        for (int l = 0; l <= i; ++l) acc += s_xl[o + b + i][o + b + l] * s_t[q * per + l * b + j];
-        s_xl[o + b + i][o + j] = -acc;
-      } else {
-        for (int l = i; l < b; ++l)
-          acc += s_xu[o + i][o + l] * s_t[TW * TW / 2 + q * per + l * b + j];
-        s_xu[o + i][o + b + j] = -acc;
-      }
-    }
-    __syncthreads();
-  }
+  tri_level<1>(s_lu, s_xl, s_xu, s_t);
+  tri_level<2>(s_lu, s_xl, s_xu, s_t);
+  tri_level<4>(s_lu, s_xl, s_xu, s_t);
+  tri_level<8>(s_lu, s_xl, s_xu, s_t);
+  tri_level<16>(s_lu, s_xl, s_xu, s_t);
+  tri_level<32>(s_lu, s_xl, s_xu, s_t);
 }
 
 __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
-  __shared__ __attribute__((aligned(16))) f4 s_row[2][4][TW / 4];
-  __shared__ float s_val[2][4];
-  __shared__ int s_thr[2][4];
+  __shared__ f4 s_row[2][NWV][TW / 4];
+  __shared__ unsigned long long s_key[2][NWV];
   __shared__ int s_sel[TW];
   __shared__ int s_go;
   __shared__ int s_aux[2 * TW];
+  __shared__ float s_lu[TW][TW + 1], s_xl[TW][TW + 1], s_xu[TW][TW + 1];
+  __shared__ float s_t[TW * TW];
   if (__hip_atomic_load(a.info, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
   const int tid = threadIdx.x;
   const int g = blockIdx.x;
@@ -224,7 +253,7 @@ __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
   load_row(r, a, myrow);
   int node = g, mypos = -1;
   for (;;) {
-    gepp_select(r, myrow, s_row, s_val, s_thr, s_sel, &mypos);
+    gepp_select(r, myrow, s_row, s_key, s_sel, &mypos);
     if (lsz == 1) break;                          /* root: s_sel is the answer */
     /* publish this node's choice, then count in at the parent */
     if (tid < TW) stc_i(&a.cand[(size_t)(loff + node) * TW + tid], tid < a.w ? s_sel[tid] : -1);
@@ -252,16 +281,18 @@ __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
 
   /* ---- root: the diagonal block's factor and inverses ---- */
   const int ws = min(a.w, m);                     /* pivots actually chosen */
-  __shared__ float s_lu[TW][TW + 1], s_xl[TW][TW + 1], s_xu[TW][TW + 1];
-  __shared__ float s_t[TW * TW];
-  for (int e = tid; e < TW * TW; e += TR) s_lu[e / TW][e % TW] = (e / TW == e % TW) ? 1.0f : 0.0f;
-  __syncthreads();
-  if (mypos >= 0 && mypos < ws) {
+  /* the chosen rows (all TW columns, staged in s_xl) -> s_lu, the identity
+   * outside the leading ws x ws block */
+  if (mypos >= 0) {
 #pragma unroll
     for (int q = 0; q < TW / 4; ++q)
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if (4 * q + e < ws) s_lu[mypos][4 * q + e] = r[q][e];
+      for (int e = 0; e < 4; ++e) s_xl[mypos][4 * q + e] = r[q][e];
+  }
+  __syncthreads();
+  for (int e = tid; e < TW * TW; e += TR) {
+    const int i = e / TW, j = e % TW;
+    s_lu[i][j] = (i < ws && j < ws) ? s_xl[i][j] : (i == j ? 1.0f : 0.0f);
   }
   if (tid == 0) s_aux[0] = 0x7fffffff;
   __syncthreads();
@@ -321,42 +352,217 @@ __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
   /* every child has counted in at every node by now: reset for the next panel
    * (loff is the root's number = the count of nodes below it) */
   for (int i = tid; i <= loff; i += TR) a.cnt[i] = 0;
+  if (a.prev) {
+    /* one list for the previous moves (P) followed by these (Q): a column Q
+     * writes takes Q's source as it stood after P, i.e. P's source where P
+     * wrote it; a column only P writes keeps P's move */
+    int *qd = reinterpret_cast<int *>(s_t), *qs = qd + 2 * TW;
+    int *pd = qs + 2 * TW, *ps = pd + 2 * TW;
+    const int npp = a.prev[0];
+    if (tid < np) { qd[tid] = a.plan[2 + 2 * tid]; qs[tid] = a.plan[3 + 2 * tid]; }
+    if (tid < npp) { pd[tid] = a.prev[2 + 2 * tid]; ps[tid] = a.prev[3 + 2 * tid]; }
+    if (tid == 0) s_aux[0] = np;
+    __syncthreads();
+    if (tid < np) {
+      const int s0 = qs[tid];
+      int src = s0;
+      for (int j = 0; j < npp; ++j) src = (pd[j] == s0) ? ps[j] : src;
+      a.plan2[2 + 2 * tid] = qd[tid];
+      a.plan2[3 + 2 * tid] = src;
+    }
+    if (tid < npp) {
+      bool inq = false;
+      for (int j = 0; j < np; ++j) inq = inq || (qd[j] == pd[tid]);
+      if (!inq) {
+        const int slot = atomicAdd(&s_aux[0], 1);
+        a.plan2[2 + 2 * slot] = pd[tid];
+        a.plan2[3 + 2 * slot] = ps[tid];
+      }
+    }
+    __syncthreads();
+    if (tid == 0) a.plan2[0] = s_aux[0];
+  }
 }
 
 /* S columns dst <- src (rows of A) for every move, over the entries
- * [ebeg, nrows) of each column: each workgroup moves a 64-entry slice of all
- * listed columns through LDS, so sources are read before any destination is
- * written.  A slice inside the diagonal block (rows and entries k..k+w-1)
- * then receives the block's factor lu11. */
-__global__ __launch_bounds__(256) void tslu_swap_kernel(float *S, long lds, int nrows,
-                                                        const int *plan, int k, int w,
-                                                        const float *lu11, const int *info,
-                                                        int ebeg) {
+ * [ebeg, eend) of each column except the gap [gbeg, gbeg + glen): each
+ * workgroup moves a 64-entry slice of all listed columns through LDS, so
+ * sources are read before any destination is written.  With lu11 given, the
+ * part of the diagonal block (rows and entries k..k+w-1) in the slice then
+ * receives the block's factor. */
+__global__ __launch_bounds__(256) void tslu_swap_kernel(float *S, long lds, int ebeg, int eend,
+                                                        int gbeg, int glen, const int *plan,
+                                                        int k, int w, const float *lu11,
+                                                        const int *info) {
   __shared__ float buf[2 * TW][64];
   __shared__ int s_dst[2 * TW], s_src[2 * TW];
   if (*info) return;
   const int np = plan[0];
-  const int e0 = ebeg + blockIdx.x * 64;
+  const int i0 = ebeg + blockIdx.x * 64;
+  const int j = threadIdx.x & 63;
+  /* entry of slice position j (positions past the gap shift over it) */
+  int e = i0 + j;
+  if (e >= gbeg) e += glen;
+  const bool live = e < eend;
   for (int t = threadIdx.x; t < np; t += 256) {
     s_dst[t] = plan[2 + 2 * t];
     s_src[t] = plan[3 + 2 * t];
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < np * 64; q += 256) {
-    const int mv = q >> 6, e = e0 + (q & 63);
-    if (e < nrows) buf[mv][q & 63] = S[(size_t)s_src[mv] * lds + e];
+  for (int mv = threadIdx.x >> 6; mv < np; mv += 4)
+    if (live) buf[mv][j] = S[(size_t)s_src[mv] * lds + e];
+  __syncthreads();
+  for (int mv = threadIdx.x >> 6; mv < np; mv += 4)
+    if (live) S[(size_t)s_dst[mv] * lds + e] = buf[mv][j];
+  if (lu11 && live && e >= k && e < k + w)
+    for (int r = threadIdx.x >> 6; r < w; r += 4)   /* row k+r of A, entry e */
+      S[(size_t)(k + r) * lds + e] = lu11[(size_t)(e - k) * TW + r];
+}
+
+/* The rest of a panel pair P (columns k..k+TW-1 of A) and Q (the next TW)
+ * once both were factored within the pair's own rows of S, for the S rows
+ * (columns of A) outside the pair, [0, k) and [k2, n), k2 = k + 2 TW, in
+ * blocks of 64: the combined row moves of P and Q (plan, at most 4 TW
+ * moves); then, for rows from k2, with U = those rows' entries in S
+ * columns k..k2-1 (A's U12 block of the pair, transposed):
+ *   U_P <- U_P inv(L_P)^T, U_Q <- U_Q - U_P B, U_Q <- U_Q inv(L_Q)^T
+ * with B = S[k.., kq..] (P's L21 restricted to Q's rows).  Everything is
+ * per S row, so the blocks are independent.  The moved values of columns
+ * k..k2-1 are taken from LDS, and those columns are written once, at the
+ * end. */
+struct TailArgs {
+  float *S;
+  long lds;
+  int n, k, nlo;           /* nlo = blocks below k */
+  const int *plan;
+  const float *invlp, *invlq;  /* TW x TW column-major */
+  const int *info;
+};
+
+__global__ __launch_bounds__(256) void tslu_tail_kernel(TailArgs t) {
+  constexpr int NB = 64;                          /* S rows per block */
+  __shared__ float buf[4 * TW][NB];               /* moved values, then operands */
+  __shared__ float U[NB][2 * TW + 1];
+  __shared__ int s_dst[4 * TW], s_src[4 * TW], s_map[2 * TW];
+  if (*t.info) return;
+  const int tid = threadIdx.x;
+  const int k = t.k, kq = k + TW, k2 = k + 2 * TW, n = t.n;
+  const bool hi = (int)blockIdx.x >= t.nlo;
+  const int r0 = hi ? k2 + ((int)blockIdx.x - t.nlo) * NB : (int)blockIdx.x * NB;
+  const int rend = hi ? n : k;
+  const int nr = min(NB, rend - r0);
+  float *S = t.S;
+  const long lds = t.lds;
+  const int np = t.plan[0];
+  for (int q = tid; q < np; q += 256) {
+    s_dst[q] = t.plan[2 + 2 * q];
+    s_src[q] = t.plan[3 + 2 * q];
+  }
+  if (tid < 2 * TW) s_map[tid] = -1;
+  __syncthreads();
+  if (hi)
+    for (int q = tid; q < np; q += 256)
+      if (s_dst[q] >= k && s_dst[q] < k2) s_map[s_dst[q] - k] = q;
+  /* every source is read before any destination is written */
+  for (int q = tid; q < np * NB; q += 256) {
+    const int mv = q / NB, i = q % NB;
+    if (i < nr) buf[mv][i] = S[(size_t)s_src[mv] * lds + r0 + i];
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < np * 64; q += 256) {
-    const int mv = q >> 6, e = e0 + (q & 63);
-    if (e < nrows) S[(size_t)s_dst[mv] * lds + e] = buf[mv][q & 63];
+  for (int q = tid; q < np * NB; q += 256) {
+    const int mv = q / NB, i = q % NB, d = s_dst[mv];
+    if (i < nr && !(hi && d >= k && d < k2)) S[(size_t)d * lds + r0 + i] = buf[mv][i];
   }
-  if (e0 + 64 > k && e0 < k + w) {
-    __syncthreads();
-    for (int q = threadIdx.x; q < w * 64; q += 256) {
-      const int r = q >> 6, e = e0 + (q & 63);       /* row k+r of A, entry e */
-      if (e >= k && e < k + w) S[(size_t)(k + r) * lds + e] = lu11[(size_t)(e - k) * TW + r];
-    }
+  if (!hi) return;
+  /* U: the pair's columns of these rows, after the moves */
+  for (int q = tid; q < 2 * TW * NB; q += 256) {
+    const int j = q / NB, i = q % NB, mv = s_map[j];
+    U[i][j] = (i >= nr) ? 0.0f : (mv >= 0 ? buf[mv][i] : S[(size_t)(k + j) * lds + r0 + i]);
+  }
+  __syncthreads();
+  /* operands: inv(L_P) as [j][m] = invlp(j, m), B as [m][j], inv(L_Q) */
+  float (*iLP)[TW + 1] = reinterpret_cast<float (*)[TW + 1]>(&buf[0][0]);
+  float (*Bm)[TW + 1] = iLP + TW;
+  float (*iLQ)[TW + 1] = Bm + TW;
+  for (int q = tid; q < TW * TW; q += 256) {
+    const int a = q % TW, b = q / TW;             /* column-major (a, b) */
+    iLP[a][b] = t.invlp[q];
+    iLQ[a][b] = t.invlq[q];
+    Bm[a][b] = S[(size_t)(kq + b) * lds + k + a];  /* B(a, b) = S[k+a, kq+b] */
+  }
+  __syncthreads();
+  /* thread: rows i0..i0+3, columns j0..j0+3 of each 64 x 64 product */
+  const int i0 = (tid & 15) * 4, j0 = (tid >> 4) * 4;
+  float acc[4][4];
+  /* U_P <- U_P inv(L_P)^T */
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+  for (int m = 0; m < TW; ++m) {
+    float x[4], y[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[a] = U[i0 + a][m];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) y[b] = iLP[j0 + b][m];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += x[a] * y[b];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) U[i0 + a][j0 + b] = acc[a][b];
+  __syncthreads();
+  /* U_Q <- U_Q - U_P B */
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = U[i0 + a][TW + j0 + b];
+  for (int m = 0; m < TW; ++m) {
+    float x[4], y[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[a] = U[i0 + a][m];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) y[b] = Bm[m][j0 + b];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] -= x[a] * y[b];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) U[i0 + a][TW + j0 + b] = acc[a][b];
+  __syncthreads();
+  /* U_Q <- U_Q inv(L_Q)^T */
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = 0.0f;
+  for (int m = 0; m < TW; ++m) {
+    float x[4], y[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) x[a] = U[i0 + a][TW + m];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) y[b] = iLQ[j0 + b][m];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) acc[a][b] += x[a] * y[b];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) U[i0 + a][TW + j0 + b] = acc[a][b];
+  __syncthreads();
+  for (int q = tid; q < 2 * TW * NB; q += 256) {
+    const int j = q / NB, i = q % NB;
+    if (i < nr) S[(size_t)(k + j) * lds + r0 + i] = U[i][j];
   }
 }
 
@@ -399,7 +605,8 @@ extern "C" size_t atlmx_k_tslu_ws_ints(int n) {
 
 extern "C" hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w,
                                           int *ws, int *plan, int *perm, float *lu11,
-                                          float *invl, float *invu, int *info, hipStream_t s) {
+                                          float *invl, float *invu, int *info,
+                                          const int *prev, int *plan2, hipStream_t s) {
   if (k >= n || w <= 0) return hipSuccess;
   if (w > TW || lds < n || (k & 3) || (lds & 3) || ((uintptr_t)S & 15) || !ws || !plan ||
       !perm || !lu11 || !invl || !invu || !info)
@@ -416,18 +623,38 @@ extern "C" hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k
     a.cnt = ws;
     a.cand = ws + nmax + 2;
   }
-  a.plan = plan; a.perm = perm;
+  if (prev && !plan2) return hipErrorInvalidValue;
+  a.plan = plan; a.perm = perm; a.prev = prev; a.plan2 = plan2;
   hipLaunchKernelGGL(tslu_select_kernel, dim3(a.nleaves), dim3(TR), 0, s, a);
   return hipGetLastError();
 }
 
-extern "C" hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, const int *plan,
-                                        int k, int w, const float *lu11, const int *info,
+extern "C" hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, int gbeg,
+                                        int gend, const int *plan, int k, int w,
+                                        const float *lu11, const int *info, hipStream_t s) {
+  if (gend <= gbeg) gbeg = gend = eend;
+  if (ebeg < 0 || gbeg < ebeg || gend > eend || w < 0 || w > TW || !info || !plan)
+    return hipErrorInvalidValue;
+  const int cnt = eend - ebeg - (gend - gbeg);
+  if (cnt <= 0) return hipSuccess;
+  hipLaunchKernelGGL(tslu_swap_kernel, dim3((cnt + 63) / 64), dim3(256), 0, s, S, lds, ebeg,
+                     eend, gbeg, gend - gbeg, plan, k, w, lu11, info);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t atlmx_k_tslu_tail(float *S, long lds, int n, int k, const int *plan,
+                                        const float *invlp, const float *invlq, const int *info,
                                         hipStream_t s) {
-  if (eend <= ebeg) return hipSuccess;
-  if (ebeg < 0 || w < 0 || w > TW || !info) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(tslu_swap_kernel, dim3((eend - ebeg + 63) / 64), dim3(256), 0, s, S, lds,
-                     eend, plan, k, w, lu11, info, ebeg);
+  const int k2 = k + 2 * TW;
+  if (k < 0 || k2 > n || !plan || !invlp || !invlq || !info) return hipErrorInvalidValue;
+  if ((k | (int)lds) < 0 || lds < n) return hipErrorInvalidValue;
+  TailArgs t;
+  t.S = S; t.lds = lds; t.n = n; t.k = k;
+  t.nlo = (k + 63) / 64;
+  t.plan = plan; t.invlp = invlp; t.invlq = invlq; t.info = info;
+  const int nb = t.nlo + (n - k2 + 63) / 64;
+  if (nb == 0) return hipSuccess;
+  hipLaunchKernelGGL(tslu_tail_kernel, dim3(nb), dim3(256), 0, s, t);
   return hipGetLastError();
 }
 

@@ -77,18 +77,31 @@ hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const float *A,
  * ATLMX_TSLU_W column-major, identity beyond w.  A zero or non-finite U11
  * pivot sets *info = k + j + 1 (if 0); with *info nonzero on entry nothing
  * is done.  ws: atlmx_k_tslu_ws_ints(n) ints, zero before the first call
- * (each call leaves its counters zero). */
+ * (each call leaves its counters zero).  With prev (the previous panel's
+ * plan) given, plan2 (2 + 8*ATLMX_TSLU_W ints) receives one move list equal
+ * to prev's moves followed by this panel's. */
 #define ATLMX_TSLU_W 64
 size_t atlmx_k_tslu_ws_ints(int n);
 hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w, int *ws,
                                int *plan, int *perm, float *lu11, float *invl, float *invu,
-                               int *info, hipStream_t s);
+                               int *info, const int *prev, int *plan2, hipStream_t s);
+/* the pair P (A columns k..k+63) and Q (the next 64, k + 128 <= n) after
+ * both were factored within S rows k..k+127: for the S rows [0, k) and
+ * [k+128, n), the row moves of plan (P's followed by Q's, as
+ * atlmx_k_tslu_select writes to plan2), then on the rows from k+128
+ * U12(P) = U12(P) inv(L_P)^T, U12(Q) -= U12(P) L21(P, Q rows),
+ * U12(Q) = U12(Q) inv(L_Q)^T (transposed in S). */
+hipError_t atlmx_k_tslu_tail(float *S, long lds, int n, int k, const int *plan,
+                             const float *invlp, const float *invlq, const int *info,
+                             hipStream_t s);
 /* columns dst <- src of S (rows of A) for the moves of plan, entries
- * [ebeg, eend) of each column (S rows, i.e. columns of A); the part of the
- * diagonal block (rows and entries k..k+w-1) inside that range <- lu11.
+ * [ebeg, eend) of each column (S rows, i.e. columns of A) except the gap
+ * [gbeg, gend) (none when gend <= gbeg); with lu11 non-NULL, the part of
+ * the diagonal block (rows and entries k..k+w-1) in that range <- lu11.
  * Nothing is done when *info is nonzero. */
-hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, const int *plan, int k,
-                             int w, const float *lu11, const int *info, hipStream_t s);
+hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, int gbeg, int gend,
+                             const int *plan, int k, int w, const float *lu11, const int *info,
+                             hipStream_t s);
 /* Batched factor + solve in one persistent launch (k_batched.hip).
  * kind 0: Cholesky (lower) + solve; kind 1: LU (partial pivoting) + solve.
  * Instance i = order[it] for it = 0.. (taken from *counter, zero on entry):

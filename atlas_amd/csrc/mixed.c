@@ -101,51 +101,125 @@ static void tslu_update(atlmx_ctx *c, float *S, long lds, int r0, int M, int c0,
   rt_prof_end(c, pid);
 }
 
-/* the panel at k (width w) after its pivots were chosen: row moves with the
- * factored diagonal block ... */
-static void tslu_panel_swap(atlmx_ctx *c, float *S, long lds, int n, int k, int w,
-                            const int *plan, const float *lu11, const int *d_info,
-                            const char *rout)
+/* what a panel's tournament leaves for the later steps: the row moves, the
+ * factored diagonal block and the inverses of its triangles */
+typedef struct {
+  int *plan;
+  float *lu11, *invL, *invU;
+} tslu_panel;
+
+/* choose the pivot rows of the panel at k (width w); with prev (the
+ * previous panel), plan2 receives prev's moves followed by these */
+static void tslu_select(atlmx_ctx *c, float *S, long lds, int n, int k, int w, int *tws,
+                        const tslu_panel *p, int *perm, const tslu_panel *prev, int *plan2,
+                        const int *d_info, const char *rout)
 {
-  int pid = rt_prof_begin(c, PROF_LASWP, 0.0);
-  RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, 0, n, plan, k, w, lu11, d_info, c->stream));
+  int pid = rt_prof_begin(c, PROF_PANEL, (double)(n - k) * w * sizeof(float));
+  RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k, w, tws, p->plan, perm, p->lu11, p->invL,
+                                     p->invU, (int *)d_info, prev ? prev->plan : NULL, plan2,
+                                     c->stream));
   rt_prof_end(c, pid);
 }
 
-/* ... then L21 = A21 inv(U11) and U12 = inv(L11) A12 over the m = n-k-w
- * rows/columns beyond the block (in-place GEMMs on S, each a single tile
- * row/column of the GEMM kernel, so every workgroup reads its operand in
- * full before writing it) */
-static void tslu_panel_solve(atlmx_ctx *c, float *S, long lds, int n, int k, int w,
-                             const float *invL, const float *invU, const int *d_info,
-                             const char *rout)
+/* the panel's row moves over the S rows [e0, e1) (columns of A) except
+ * [g0, g1), with the factored diagonal block when overlay is set */
+static void tslu_swap(atlmx_ctx *c, float *S, long lds, int e0, int e1, int g0, int g1, int k,
+                      int w, const tslu_panel *p, int overlay, const int *d_info,
+                      const char *rout)
 {
-  const int W = ATLMX_TSLU_W, m = n - k - w;
-  float *Skk = S + k + (size_t)k * lds;
-  if (m > 0) {
-    float *Lt = Skk + (size_t)w * lds;     /* L21^T: rows k..k+w-1, columns k+w.. of S */
-    float *Ut = Skk + w;                   /* U12^T: rows k+w.., columns k..k+w-1 of S */
-    sgemm_dev(c, KX_TRANS, KX_NOTRANS, w, m, w, 1.0f, invU, W, Lt, lds, 0.0f, Lt, lds, 0,
-              d_info, rout);
-    sgemm_dev(c, KX_NOTRANS, KX_TRANS, m, w, w, 1.0f, Ut, lds, invL, W, 0.0f, Ut, lds, 0,
-              d_info, rout);
+  int pid = rt_prof_begin(c, PROF_LASWP, 0.0);
+  RT_CHECK(rout, atlmx_k_tslu_swap(S, lds, e0, e1, g0, g1, p->plan, k, w,
+                                   overlay ? p->lu11 : NULL, d_info, c->stream));
+  rt_prof_end(c, pid);
+}
+
+/* L21 = A21 inv(U11) for the panel at k: in S, the rows k..k+w-1 right of
+ * the block, Lt = inv(U11)^T Lt (an in-place GEMM covering all w rows in
+ * one tile row, so every workgroup reads its operand in full before writing
+ * it) */
+static void tslu_l21(atlmx_ctx *c, float *S, long lds, int n, int k, int w,
+                     const tslu_panel *p, const int *d_info, const char *rout)
+{
+  const int m = n - k - w;
+  float *Lt = S + k + (size_t)(k + w) * lds;
+  if (m > 0)
+    sgemm_dev(c, KX_TRANS, KX_NOTRANS, w, m, w, 1.0f, p->invU, ATLMX_TSLU_W, Lt, lds, 0.0f,
+              Lt, lds, 0, d_info, rout);
+}
+
+/* U12 = inv(L11) A12 for the panel at k, restricted to A's columns (S rows)
+ * [r0, r1): Ut = Ut inv(L11)^T, in place as above */
+static void tslu_u12(atlmx_ctx *c, float *S, long lds, int k, int w, int r0, int r1,
+                     const tslu_panel *p, const int *d_info, const char *rout)
+{
+  float *Ut = S + r0 + (size_t)k * lds;
+  if (r1 > r0)
+    sgemm_dev(c, KX_NOTRANS, KX_TRANS, r1 - r0, w, w, 1.0f, Ut, lds, p->invL, ATLMX_TSLU_W,
+              0.0f, Ut, lds, 0, d_info, rout);
+}
+
+/* the pair of panels P at k and Q after it, confined to the pair's own S
+ * rows [k, k2) (A's columns): P's pivots, its moves and factored block on
+ * those rows, its L21 and its U12 on Q's rows; Q's columns get P's update;
+ * then Q's pivots, moves, block and L21.  Nothing outside rows [k, k2) of S
+ * is read or written, so this runs beside the trailing update of the rows
+ * below. */
+static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *tws,
+                           const tslu_panel *P, const tslu_panel *Q, int *planPQ, int *perm,
+                           const int *d_info, const char *rout)
+{
+  const int W = ATLMX_TSLU_W;
+  const int wp = (n - k < W) ? n - k : W, kq = k + wp;
+  const int wq = (n - kq < W) ? n - kq : W, k2 = kq + wq;
+  tslu_select(c, S, lds, n, k, wp, tws, P, perm, NULL, NULL, d_info, rout);
+  tslu_swap(c, S, lds, k, k2, 0, 0, k, wp, P, 1, d_info, rout);
+  tslu_l21(c, S, lds, n, k, wp, P, d_info, rout);
+  if (wq == 0) return;
+  tslu_u12(c, S, lds, k, wp, kq, k2, P, d_info, rout);
+  tslu_update(c, S, lds, kq, wq, kq, n - kq, k, wp, d_info, c->stream, rout);
+  tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, d_info, rout);
+  tslu_swap(c, S, lds, k, k2, 0, 0, kq, wq, Q, 1, d_info, rout);
+  tslu_l21(c, S, lds, n, kq, wq, Q, d_info, rout);
+}
+
+/* the rest of the pair at k, for the S rows outside [k, k2): P's and then
+ * Q's moves (one combined list), P's U12 on the rows from k2, those rows' Q
+ * columns updated by P (after Q's moves, so each moved row carries the
+ * updates of the row it replaces), then Q's U12 -- one launch, as every
+ * step is per S row.  With no Q (the last panel) there are no rows past
+ * the pair and only P's moves remain. */
+static void tslu_pair_tail(atlmx_ctx *c, float *S, long lds, int n, int k,
+                           const tslu_panel *P, const tslu_panel *Q, const int *planPQ,
+                           const int *d_info, const char *rout)
+{
+  const int W = ATLMX_TSLU_W;
+  const int wp = (n - k < W) ? n - k : W, kq = k + wp;
+  const int wq = (n - kq < W) ? n - kq : W, k2 = kq + wq;
+  int pid;
+  if (wq < W) {                                   /* the pair ends the matrix */
+    tslu_swap(c, S, lds, 0, k, 0, 0, k, wp, P, 0, d_info, rout);
+    if (wq > 0) tslu_swap(c, S, lds, 0, k, 0, 0, kq, wq, Q, 0, d_info, rout);
+    return;
   }
+  pid = rt_prof_begin2(c, PROF_LASWP, 2.0 * (n - k2) * 2 * W * 2 * W,
+                       ((double)n * 4 * W + 2.0 * (n - k2) * 2 * W) * 2 * sizeof(float));
+  RT_CHECK(rout, atlmx_k_tslu_tail(S, lds, n, k, planPQ, P->invL, Q->invL, d_info, c->stream));
+  rt_prof_end(c, pid);
+  (void)k2;
 }
 
 /* FP32 LU with tournament pivoting of A, held as S = A^T (column-major,
- * leading dimension lds), in pairs of W = ATLMX_TSLU_W column panels P, Q:
- *   1. P's pivot rows were chosen (tournament: move list + perm; the root
- *      also factors the chosen W x W block in pivot order and inverts L11
- *      and U11); move them over the whole matrix with the factored block,
- *      then L21 = A21 inv(U11), U12 = inv(L11) A12;
- *   2. Q's columns get P's update (K = W); choose Q's pivots and move its
- *      rows; only then do the rows of Q's block right of Q get P's update,
- *      so that every row moved by Q's interchanges carries the same updates
- *      as the row it replaces; finish Q like P;
- *   3. the next pair's first panel gets P's and Q's update together
- *      (K = 2W); its pivots are chosen on a second stream while the rest of
- *      the trailing matrix gets the same K = 2W update.
- * In S, A22 -= L21 U12 reads S22 -= U12^T L21^T.  On return PA = LU with row
+ * leading dimension lds), in pairs of W = ATLMX_TSLU_W column panels P, Q
+ * whose row interchanges reach the rest of the matrix late:
+ *   head (tslu_pair_head): both panels factored within the pair's own rows
+ *     of S (A's columns k..k2-1): tournament pivots, moves and factored
+ *     block, L21, U12 where Q needs it, P's update of Q's columns;
+ *   tail (tslu_pair_tail): the moves on the other rows, U12 there, Q's
+ *     columns of those rows updated by P.
+ * The next pair's rows then get this pair's update (K = 2W), and its head
+ * runs on a second stream while the rest of the trailing matrix gets the
+ * same update on the first: the tournaments leave the critical path.  In
+ * S, A22 -= L21 U12 reads S22 -= U12^T L21^T.  On return PA = LU with row
  * i of PA = row perm[i] of A; L (unit) is the strict upper triangle of S
  * transposed, U the lower triangle of S transposed.  Returns 0 or 1 + the
  * first column with a zero/non-finite pivot (every launch after that
@@ -155,49 +229,33 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
 {
   const int W = ATLMX_TSLU_W;
   const size_t wsi = atlmx_k_tslu_ws_ints(n);
-  float *lu11 = (float *)rt_ws(c, WS_NPINV, sizeof(float) * 3 * W * W + 256, rout);
-  float *invL = lu11 + (size_t)W * W, *invU = invL + (size_t)W * W;
-  int *d_info = (int *)(invU + (size_t)W * W);
-  int *tws = (int *)rt_ws(c, WS_TSLU, sizeof(int) * (wsi + 2 + 4 * W), rout);
-  int *plan = tws + wsi;
+  float *blk = (float *)rt_ws(c, WS_NPINV, sizeof(float) * 6 * W * W + 256, rout);
+  int *d_info = (int *)(blk + (size_t)6 * W * W);
+  int *tws = (int *)rt_ws(c, WS_TSLU, sizeof(int) * (wsi + 2 * (2 + 4 * W) + 2 + 8 * W), rout);
+  tslu_panel P = {tws + wsi, blk, blk + (size_t)W * W, blk + (size_t)2 * W * W};
+  tslu_panel Q = {tws + wsi + 2 + 4 * W, blk + (size_t)3 * W * W, blk + (size_t)4 * W * W,
+                  blk + (size_t)5 * W * W};
+  int *planPQ = tws + wsi + 2 * (2 + 4 * W);
   hipStream_t main = c->stream, aux = rt_aux(c, rout);
-  int info = 0, k, pid;
+  int info = 0, k = 0;
   RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, main));
   RT_CHECK(rout, atlmx_k_fill_i32(tws, (int)wsi, 0, main));
-  pid = rt_prof_begin(c, PROF_PANEL, (double)n * W * sizeof(float));
-  RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, 0, W < n ? W : n, tws, plan, perm, lu11, invL,
-                                     invU, d_info, main));
-  rt_prof_end(c, pid);
-  for (k = 0; k < n; k += 2 * W) {
-    const int wp = (n - k < W) ? n - k : W;          /* panel P at k */
-    const int mp = n - k - wp;
-    const int kq = k + wp, wq = (mp < W) ? mp : W;   /* panel Q at kq */
-    const int mq = mp - wq;
-    tslu_panel_swap(c, S, lds, n, k, wp, plan, lu11, d_info, rout);
-    tslu_panel_solve(c, S, lds, n, k, wp, invL, invU, d_info, rout);
-    if (mp == 0) break;
-    /* 2. */
-    tslu_update(c, S, lds, kq, wq, kq, mp, k, wp, d_info, main, rout);
-    pid = rt_prof_begin(c, PROF_PANEL, (double)mp * wq * sizeof(float));
-    RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, kq, wq, tws, plan, perm, lu11, invL, invU,
-                                       d_info, main));
-    rt_prof_end(c, pid);
-    tslu_panel_swap(c, S, lds, n, kq, wq, plan, lu11, d_info, rout);
-    tslu_update(c, S, lds, kq + wq, mq, kq, wq, k, wp, d_info, main, rout);
-    tslu_panel_solve(c, S, lds, n, kq, wq, invL, invU, d_info, rout);
-    if (mq == 0) break;
-    /* 3. */
-    {
-      const int k2 = kq + wq, w2 = (mq < W) ? mq : W;
-      tslu_update(c, S, lds, k2, w2, k2, mq, k, wp + wq, d_info, main, rout);
-      RT_CHECK(rout, hipEventRecord(c->ev[0], main));
-      RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
-      RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k2, w2, tws, plan, perm, lu11, invL, invU,
-                                         d_info, aux));
-      RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
-      tslu_update(c, S, lds, k2 + w2, mq - w2, k2, mq, k, wp + wq, d_info, main, rout);
-      RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
-    }
+  tslu_pair_head(c, S, lds, n, 0, tws, &P, &Q, planPQ, perm, d_info, rout);
+  for (;;) {
+    const int k2 = (n - k <= 2 * W) ? n : k + 2 * W;
+    const int R = (n - k2 < 2 * W) ? n - k2 : 2 * W;
+    tslu_pair_tail(c, S, lds, n, k, &P, &Q, planPQ, d_info, rout);
+    if (k2 >= n) break;
+    tslu_update(c, S, lds, k2, R, k2, n - k2, k, k2 - k, d_info, main, rout);
+    RT_CHECK(rout, hipEventRecord(c->ev[0], main));
+    RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
+    c->stream = aux;
+    tslu_pair_head(c, S, lds, n, k2, tws, &P, &Q, planPQ, perm, d_info, rout);
+    c->stream = main;
+    RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
+    tslu_update(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, d_info, main, rout);
+    RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
+    k = k2;
   }
   RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, main));
   rt_sync(c, rout);

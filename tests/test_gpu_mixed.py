@@ -81,7 +81,8 @@ def _tslu_select(torch, A, k, w):
     blk = torch.zeros(3, 64, 64, dtype=torch.float32, device="cuda")   # lu11, invL, invU
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
     _k("atlmx_k_tslu_select", X.dptr(S), C.c_long(lds), n, k, w, X.dptr(ws), X.dptr(plan),
-       X.dptr(perm), X.dptr(blk[0]), X.dptr(blk[1]), X.dptr(blk[2]), X.dptr(info), None)
+       X.dptr(perm), X.dptr(blk[0]), X.dptr(blk[1]), X.dptr(blk[2]), X.dptr(info), None, None,
+       None)
     torch.cuda.synchronize()
     return S, ws, plan, perm, blk, info
 
@@ -124,11 +125,13 @@ def test_tournament_choice_and_moves(torch, n, k, w):
     assert np.abs(iU[:ws_, :ws_] @ U11 - np.eye(ws_)).max() < 1e-4 * np.linalg.cond(U11)
     assert np.array_equal(iL[ws_:, ws_:], np.eye(64 - ws_))
     # applying the moves to S moves whole rows of A and writes the factored block
-    # in two pieces, as the lookahead applies them: the panel rows' range first
-    _k("atlmx_k_tslu_swap", X.dptr(S), C.c_long(S.shape[1]), k, n, X.dptr(dplan), k, ws_,
+    # in two pieces, as the lookahead applies them: the pair's rows with the
+    # factored block first, then the other rows around that gap
+    k2 = min(n, k + 2 * 64)
+    _k("atlmx_k_tslu_swap", X.dptr(S), C.c_long(S.shape[1]), k, k2, 0, 0, X.dptr(dplan), k, ws_,
        X.dptr(blk[0]), X.dptr(info), None)
-    _k("atlmx_k_tslu_swap", X.dptr(S), C.c_long(S.shape[1]), 0, k, X.dptr(dplan), k, ws_,
-       X.dptr(blk[0]), X.dptr(info), None)
+    _k("atlmx_k_tslu_swap", X.dptr(S), C.c_long(S.shape[1]), 0, n, k, k2, X.dptr(dplan), k, ws_,
+       None, X.dptr(info), None)
     torch.cuda.synchronize()
     Sh = S[:, :n].cpu().numpy()
     want_rows = A[perm].astype(np.float32)
@@ -146,10 +149,68 @@ def test_tournament_zero_pivot_sets_info(torch):
     before = dperm.clone()
     _k("atlmx_k_tslu_select", X.dptr(S), C.c_long(S.shape[1]), n, 64, 64, X.dptr(ws),
        X.dptr(dplan), X.dptr(dperm), X.dptr(blk[0]), X.dptr(blk[1]), X.dptr(blk[2]),
-       X.dptr(info), None)
+       X.dptr(info), None, None, None)
     torch.cuda.synchronize()
     assert int(ws[:3].abs().sum().item()) == 0 and int(info.item()) == 6   # 2 leaves + root
     assert torch.equal(before, dperm)
+
+
+def _apply_moves(cols, plan):
+    """column moves dst <- src of a move list, every source read first"""
+    mv = plan[2:2 + 2 * plan[0]].reshape(-1, 2)
+    out = cols.copy()
+    out[..., mv[:, 0]] = cols[..., mv[:, 1]]
+    return out
+
+
+@pytest.mark.parametrize("n", [300, 700, 1100])
+def test_pair_tail_vs_numpy(torch, n):
+    """Two tournaments (P at 0, Q at 64, the second composing both move
+    lists), then the pair's tail kernel on a random S: the composed list
+    equals P's moves followed by Q's, and the kernel's result equals the
+    numpy restatement -- both lists' moves on the S rows outside 0..127,
+    then U_P <- U_P inv(L_P)^T, U_Q <- U_Q - U_P B, U_Q <- U_Q inv(L_Q)^T on
+    the rows from 128 (B = S rows 0..63, columns 64..127)."""
+    rng = np.random.default_rng(n)
+    A = rng.uniform(-1, 1, (n, n))
+    S, ws, planP, perm, blkP, info = _tslu_select(torch, A, 0, 64)
+    planQ = torch.zeros(2 + 4 * 64, dtype=torch.int32, device="cuda")
+    planPQ = torch.full((2 + 8 * 64,), -5, dtype=torch.int32, device="cuda")
+    blkQ = torch.zeros(3, 64, 64, dtype=torch.float32, device="cuda")
+    _k("atlmx_k_tslu_select", X.dptr(S), C.c_long(S.shape[1]), n, 64, 64, X.dptr(ws),
+       X.dptr(planQ), X.dptr(perm), X.dptr(blkQ[0]), X.dptr(blkQ[1]), X.dptr(blkQ[2]),
+       X.dptr(info), X.dptr(planP), X.dptr(planPQ), None)
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    pP, pQ, pPQ = planP.cpu().numpy(), planQ.cpu().numpy(), planPQ.cpu().numpy()
+    assert pPQ[0] <= pP[0] + pQ[0]
+    d = pPQ[2:2 + 2 * pPQ[0]:2]
+    assert len(set(d.tolist())) == len(d)            # each column written once
+    iota = np.arange(n)
+    assert np.array_equal(_apply_moves(_apply_moves(iota, pP), pQ), _apply_moves(iota, pPQ))
+    # the tail on fresh random data (S rows = entries, torch rows = S columns)
+    St = torch.tensor(rng.uniform(-1, 1, S.shape).astype(np.float32), device="cuda")
+    iLP = np.tril(rng.uniform(-1, 1, (64, 64)), -1) + np.eye(64)
+    iLQ = np.tril(rng.uniform(-1, 1, (64, 64)), -1) + np.eye(64)
+    dLP = torch.tensor(iLP.T.astype(np.float32).copy(), device="cuda")   # column-major
+    dLQ = torch.tensor(iLQ.T.astype(np.float32).copy(), device="cuda")
+    S0 = St.cpu().numpy().astype(np.float64)          # [S column, entry]
+    _k("atlmx_k_tslu_tail", X.dptr(St), C.c_long(St.shape[1]), n, 0, X.dptr(planPQ),
+       X.dptr(dLP), X.dptr(dLQ), X.dptr(info), None)
+    torch.cuda.synchronize()
+    got = St.cpu().numpy().astype(np.float64)
+    M = S0[:n, :n].T.copy()                           # M[entry, column] = S(entry, column)
+    want = M.copy()
+    rows = np.r_[0:0, 128:n]
+    want[rows] = _apply_moves(M[rows], pPQ)
+    UP = want[128:, 0:64] @ iLP.astype(np.float32).T
+    UQ = want[128:, 64:128] - UP @ M[0:64, 64:128]
+    UQ = UQ @ iLQ.astype(np.float32).T
+    want[128:, 0:64], want[128:, 64:128] = UP, UQ
+    gotM = got[:n, :n].T
+    assert np.array_equal(gotM[:128], M[:128])        # the pair's own rows untouched
+    err = np.abs(gotM - want).max() / np.abs(want).max()
+    assert err < 1e-5, err
 
 
 def _system(n, kappa, nrhs, seed):
