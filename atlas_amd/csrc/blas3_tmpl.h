@@ -73,8 +73,9 @@ void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, 
   if (side == KX_LEFT && N <= ATLMX_TRSV_MAXRHS && !abort_flag) {
     /* few right-hand sides: the solve is bound by reading the factor once,
      * so one flag-chained launch replaces 2*nblk small GEMMs */
-    int *ctl = (int *)rt_ws(c, WS_TRSV, (size_t)(nblk + 1) * sizeof(int), rout);
-    RT_CHECK(rout, atlmx_k_fill_i32(ctl, nblk + 1, 0, c->stream));
+    const size_t nctl = atlmx_k_trsv_ctl_ints(na, sizeof(T) == sizeof(double));
+    int *ctl = (int *)rt_ws(c, WS_TRSV, nctl * sizeof(int), rout);
+    RT_CHECK(rout, atlmx_k_fill_i32(ctl, (int)nctl, 0, c->stream));
     pid = rt_prof_begin(c, PROF_TRSV, (double)na * na / 2 * sizeof(T));
     RT_CHECK(rout, K_TRSV(uplo, tinv, M, N, A, lda, inv, B, ldb, ctl, c->stream));
     rt_prof_end(c, pid);

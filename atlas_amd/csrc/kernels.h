@@ -58,7 +58,9 @@ hipError_t atlmx_k_stri_inv(int uplo, int diag, int n, const float *A,
 /* op(A) X = B for a few right-hand sides (nrhs <= ATLMX_TRSV_MAXRHS) in
  * one launch; X holds B on entry.  inv: the inverted 64x64 diagonal blocks
  * of A from atlmx_k_?tri_inv (with the same uplo/diag).  ctl: device ints,
- * 1 + ceil(n/64) of them, zero on entry. */
+ * atlmx_k_trsv_ctl_ints(n, is_double) of them (8-byte aligned), zero on
+ * entry. */
+size_t atlmx_k_trsv_ctl_ints(int n, int dbl);
 #define ATLMX_TRSV_MAXRHS 4
 hipError_t atlmx_k_dtrsv(int uplo, int trans, int n, int nrhs, const double *A,
                          long lda, const double *inv, double *X, long ldx,

@@ -124,7 +124,7 @@ def test_trsm_few_rhs_vs_oracle(dt):
     for uplo, ta, diag in itertools.product([X.Upper, X.Lower], [X.NoTrans, X.Trans],
                                             [X.NonUnit, X.Unit]):
         for M, N in ((1, 1), (63, 2), (64, 4), (65, 3), (700, 1), (20000, 2)):
-            if M == 20000 and (diag == X.Unit or dt == np.float32):
+            if M == 20000 and diag == X.Unit:
                 continue
             # off-diagonal entries O(1/M) keep the systems well conditioned
             # at every size; a unit diagonal holds a value that must not be read
@@ -137,7 +137,11 @@ def test_trsm_few_rhs_vs_oracle(dt):
             Xg = X.trsm(X.Left, uplo, ta, diag, 0.75, A, Bbuf[:M])
             assert np.all(Bbuf[M:] == 7.0)
             r = O.trsm_resid(X.Left, uplo, ta, diag, 0.75, A, Xg, B0)
-            assert r < 1.0, (uplo, ta, diag, M, N, r)
+            # each block's solution is the inverted diagonal block times the
+            # right-hand side: two roundings even for a 1 x 1 system, so the
+            # tester's normalised residual can reach 2 where a division
+            # (one rounding) stays below 1
+            assert r < 2.0, (uplo, ta, diag, M, N, r)
             if M <= 700:
                 Xo = O.trsm(X.Left, uplo, ta, diag, 0.75, A, B0.copy(order="F"))
                 tol = 1e-10 if dt == np.float64 else 2e-3
