@@ -44,6 +44,7 @@ template <> struct Mfma<double> {
   static constexpr int VEC = 2;
   static constexpr int BK = 16;     /* 2 x 36 KB LDS: two workgroups per CU */
   static constexpr int PADK = 2;    /* LDS row = 18 doubles = 144 B */
+  static constexpr int PADR = 2;    /* [k][row] tiles: 130 doubles per k */
   __device__ static acc_t mma(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
@@ -56,6 +57,7 @@ template <> struct Mfma<float> {
   static constexpr int VEC = 4;
   static constexpr int BK = 16;     /* 2 x 20 KB LDS: three workgroups per CU */
   static constexpr int PADK = 4;    /* LDS row = 20 floats = 80 B */
+  static constexpr int PADR = 4;    /* [k][row] tiles: 132 floats per k */
   __device__ static acc_t mma(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
@@ -118,9 +120,13 @@ struct SliceLoader {
     }
   }
 
-  /* S is the [ROWS][BK+PADK] LDS tile */
+  /* S is the LDS tile: [ROWS][BK + PADK] (k contiguous) when KCONT, else
+   * [BK][ROWS + PADR] (rows contiguous), so each loaded vector is stored
+   * whole */
+  static constexpr int LDK = BK + MF::PADK;
+  static constexpr int LDR = ROWS + MF::PADR;
+  static constexpr int SIZE = KCONT ? ROWS * LDK : BK * LDR;
   __device__ void store(T *S, int tid) const {
-    constexpr int LDK = BK + MF::PADK;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
       const int lin = tid + NT * i;
@@ -129,11 +135,12 @@ struct SliceLoader {
         *reinterpret_cast<typename MF::vec_t *>(S + r * LDK + k) = v[i];
       } else {
         const int k = lin / (ROWS / VEC), r = (lin % (ROWS / VEC)) * VEC;
-#pragma unroll
-        for (int e = 0; e < VEC; ++e) S[(r + e) * LDK + k] = v[i][e];
+        *reinterpret_cast<typename MF::vec_t *>(S + k * LDR + r) = v[i];
       }
     }
   }
+  /* element (r, k) of the tile */
+  __device__ static int at(int r, int k) { return KCONT ? r * LDK + k : k * LDR + r; }
 };
 
 /* Two workgroups per CU (at most 256 registers per lane, 2 x 72 KB LDS):
@@ -167,8 +174,10 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
   if (g.tri == KX_LOWER && m0 + BM - 1 < n0) return;   /* tile above diagonal */
   if (g.tri == KX_UPPER && n0 + BN - 1 < m0) return;   /* tile below diagonal */
 
-  __shared__ __attribute__((aligned(16))) T As[2][BM * LDK];
-  __shared__ __attribute__((aligned(16))) T Bs[2][BN * LDK];
+  typedef SliceLoader<T, A_KCONT, BM, BK> LA;
+  typedef SliceLoader<T, B_KCONT, BN, BK> LB;
+  __shared__ __attribute__((aligned(16))) T As[2][LA::SIZE];
+  __shared__ __attribute__((aligned(16))) T Bs[2][LB::SIZE];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
@@ -181,8 +190,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  SliceLoader<T, A_KCONT, BM, BK> la;
-  SliceLoader<T, B_KCONT, BN, BK> lb;
+  LA la;
+  LB lb;
   const int nk = (g.K + BK - 1) / BK;
 
   if (nk > 0) {
@@ -207,20 +216,19 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
     }
 #pragma unroll
     for (int kg = 0; kg < BK / 16; ++kg) {
-      T a[4][4], b[4][4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const T *pa = &As[cur][(wm + i * 16 + lr) * LDK + kg * 16 + kq];
-        const T *pb = &Bs[cur][(wn + i * 16 + lr) * LDK + kg * 16 + kq];
+      for (int s = 0; s < 4; ++s) {
+        T a[4], b[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) { a[i][s] = pa[s]; b[i][s] = pb[s]; }
-      }
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
+        for (int i = 0; i < 4; ++i) {
+          a[i] = As[cur][LA::at(wm + i * 16 + lr, kg * 16 + kq + s)];
+          b[i] = Bs[cur][LB::at(wn + i * 16 + lr, kg * 16 + kq + s)];
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j][s], a[i][s], acc[i][j]);
+          for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j], a[i], acc[i][j]);
+      }
     }
     if (kt + 1 < nk) {
       la.store(As[cur ^ 1], tid);
@@ -260,7 +268,8 @@ template <typename T, int KB>
 __global__ __launch_bounds__(NT, 2) void update_kernel(GemmArgs<T> g) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
-  constexpr int LDK = KB + MF::PADK;
+  typedef SliceLoader<T, false, BM, KB> LA;
+  typedef SliceLoader<T, true, BN, KB> LB;
   if (g.abort_flag && *g.abort_flag) return;
   const int bid = blockIdx.x;
   const int tile = (bid & 7) * g.per_xcd + (bid >> 3);
@@ -274,8 +283,8 @@ __global__ __launch_bounds__(NT, 2) void update_kernel(GemmArgs<T> g) {
   const int tn = (tile % per_group) / gsz;
   const int m0 = tm * BM, n0 = tn * BN;
 
-  __shared__ __attribute__((aligned(16))) T As[BM * LDK];
-  __shared__ __attribute__((aligned(16))) T Bs[BN * LDK];
+  __shared__ __attribute__((aligned(16))) T As[LA::SIZE];
+  __shared__ __attribute__((aligned(16))) T Bs[LB::SIZE];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int kq = (lane >> 4) * 4, lr = lane & 15;
@@ -293,8 +302,8 @@ __global__ __launch_bounds__(NT, 2) void update_kernel(GemmArgs<T> g) {
       }
 
   for (int k0 = 0; k0 < g.K; k0 += KB) {
-    SliceLoader<T, false, BM, KB> la;
-    SliceLoader<T, true, BN, KB> lb;
+    LA la;
+    LB lb;
     la.load(g.A, g.lda, m0, k0, g.M, g.K, g.vec_a, tid);
     lb.load(g.B, g.ldb, n0, k0, g.N, g.K, g.vec_b, tid);
 #pragma unroll
@@ -304,22 +313,20 @@ __global__ __launch_bounds__(NT, 2) void update_kernel(GemmArgs<T> g) {
     lb.store(Bs, tid);
     __syncthreads();
 #pragma unroll
-    for (int kg = 0; kg < KB / 16; ++kg) {
-      T a[4][4], b[4][4];
+    for (int kg = 0; kg < KB / 16; ++kg)
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const T *pa = &As[(wm + i * 16 + lr) * LDK + kg * 16 + kq];
-        const T *pb = &Bs[(wn + i * 16 + lr) * LDK + kg * 16 + kq];
+      for (int s = 0; s < 4; ++s) {
+        T a[4], b[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) { a[i][s] = pa[s]; b[i][s] = pb[s]; }
-      }
-#pragma unroll
-      for (int s = 0; s < 4; ++s)
+        for (int i = 0; i < 4; ++i) {
+          a[i] = As[LA::at(wm + i * 16 + lr, kg * 16 + kq + s)];
+          b[i] = Bs[LB::at(wn + i * 16 + lr, kg * 16 + kq + s)];
+        }
 #pragma unroll
         for (int i = 0; i < 4; ++i)
 #pragma unroll
-          for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j][s], a[i][s], acc[i][j]);
-    }
+          for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j], a[i], acc[i][j]);
+      }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)

@@ -25,7 +25,7 @@ def _gen(shape, dt):
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("ta,tb", list(itertools.product([X.NoTrans, X.Trans], repeat=2)))
 @pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 9, 5), (130, 129, 33), (255, 64, 257),
-                                   (64, 300, 16), (3, 200, 1000), (128, 128, 0)])
+                                   (64, 300, 16), (3, 200, 1000), (257, 130, 530), (128, 128, 0)])
 def test_gemm_shapes_vs_oracle(dt, ta, tb, M, N, K):
     A = _gen((M, K) if ta == X.NoTrans else (K, M), dt)
     B = _gen((K, N) if tb == X.NoTrans else (N, K), dt)
