@@ -195,6 +195,7 @@ class IRInfo(C.Structure):
 
 
 MIXED_AUTO, MIXED_TSLU, MIXED_GEPP = 0, 1, 2
+MIXED_BF16, MIXED_FP16 = 5, 6
 
 
 def set_refine_stop(lapack):
@@ -208,7 +209,9 @@ def set_refine_stop(lapack):
 
 def set_mixed_method(m):
     """0 tournament-pivoting FP32 LU, then partial-pivoting FP32 LU, then
-    FP64; 1 tournament then FP64; 2 partial pivoting then FP64."""
+    FP64; 1 tournament then FP64; 2 partial pivoting then FP64; 5 / 6 the
+    tournament LU with its trailing updates in bfloat16 / half (columns
+    scaled by powers of two), then as 0."""
     lib().atlmx_set_mixed_method(int(m))
 
 

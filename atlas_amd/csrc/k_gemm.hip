@@ -340,6 +340,128 @@ __global__ __launch_bounds__(NT, 2) void update_kernel(GemmArgs<T> g) {
       }
 }
 
+/* The same update with its operands rounded to 16 bits: FP32 A and B are
+ * converted (round to nearest even) as they are staged into LDS, the
+ * products run on v_mfma_f32_16x16x16_{bf16,f16} and accumulate in FP32
+ * onto the FP32 C tile.  LP = 1: bfloat16, LP = 2: IEEE half.  LDS holds
+ * both operands [row][k] with k contiguous, so a lane's four K values of an
+ * MFMA operand are one 8-byte load. */
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+
+__device__ inline unsigned short to_bf16(float x) {
+  const unsigned u = __float_as_uint(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40u);   /* NaN */
+  return (unsigned short)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ inline unsigned short to_f16(float x) {
+  const _Float16 h = (_Float16)x;
+  return __builtin_bit_cast(unsigned short, h);
+}
+
+template <int LP> struct Low;
+template <> struct Low<1> {
+  __device__ static unsigned short cvt(float x) { return to_bf16(x); }
+  __device__ static f4 mma(uint2 a, uint2 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(__builtin_bit_cast(s4, a),
+                                                     __builtin_bit_cast(s4, b), c, 0, 0, 0);
+  }
+};
+template <> struct Low<2> {
+  __device__ static unsigned short cvt(float x) { return to_f16(x); }
+  __device__ static f4 mma(uint2 a, uint2 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x16f16(__builtin_bit_cast(h4, a),
+                                                 __builtin_bit_cast(h4, b), c, 0, 0, 0);
+  }
+};
+
+template <int LP>
+__global__ __launch_bounds__(NT, 2) void update_lp_kernel(GemmArgs<float> g) {
+  typedef Mfma<float> MF;
+  constexpr int KB = 64;
+  constexpr int LDH = KB + 8;                     /* 16-bit elements per LDS row */
+  typedef SliceLoader<float, false, BM, KB> LA;   /* A: rows contiguous */
+  typedef SliceLoader<float, true, BN, KB> LB;    /* B: k contiguous */
+  if (g.abort_flag && *g.abort_flag) return;
+  const int bid = blockIdx.x;
+  const int tile = (bid & 7) * g.per_xcd + (bid >> 3);
+  const int ntiles = g.tiles_m * g.tiles_n;
+  if (tile >= ntiles) return;
+  const int per_group = GROUP_M * g.tiles_n;
+  const int group = tile / per_group;
+  const int first_m = group * GROUP_M;
+  const int gsz = min(g.tiles_m - first_m, GROUP_M);
+  const int tm = first_m + (tile % per_group) % gsz;
+  const int tn = (tile % per_group) / gsz;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  __shared__ __attribute__((aligned(16))) unsigned short As[BM * LDH];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[BN * LDH];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int kq = (lane >> 4) * 4, lr = lane & 15;
+
+  f4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lr;
+        const int col = n0 + wn + j * 16 + MF::drow(lane, r);
+        acc[i][j][r] = (row < g.M && col < g.N) ? g.C[(size_t)col * g.ldc + row] : 0.0f;
+      }
+
+  for (int k0 = 0; k0 < g.K; k0 += KB) {
+    LA la;
+    LB lb;
+    la.load(g.A, g.lda, m0, k0, g.M, g.K, g.vec_a, tid);
+    lb.load(g.B, g.ldb, n0, k0, g.N, g.K, g.vec_b, tid);
+    if (k0 > 0) __syncthreads();
+#pragma unroll
+    for (int i = 0; i < LA::NV; ++i) {            /* 4 rows at one k */
+      const int lin = tid + NT * i;
+      const int k = lin / (BM / 4), r = (lin % (BM / 4)) * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) As[(r + e) * LDH + k] = Low<LP>::cvt(g.alpha * la.v[i][e]);
+    }
+#pragma unroll
+    for (int i = 0; i < LB::NV; ++i) {            /* 4 k at one row */
+      const int lin = tid + NT * i;
+      const int r = lin / (KB / 4), k = (lin % (KB / 4)) * 4;
+      uint2 w;
+      w.x = (unsigned)Low<LP>::cvt(lb.v[i][0]) | ((unsigned)Low<LP>::cvt(lb.v[i][1]) << 16);
+      w.y = (unsigned)Low<LP>::cvt(lb.v[i][2]) | ((unsigned)Low<LP>::cvt(lb.v[i][3]) << 16);
+      *reinterpret_cast<uint2 *>(&Bs[r * LDH + k]) = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < KB / 16; ++ks) {
+      uint2 a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = *reinterpret_cast<const uint2 *>(&As[(wm + i * 16 + lr) * LDH + ks * 16 + kq]);
+        b[i] = *reinterpret_cast<const uint2 *>(&Bs[(wn + i * 16 + lr) * LDH + ks * 16 + kq]);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = Low<LP>::mma(b[j], a[i], acc[i][j]);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lr;
+        const int col = n0 + wn + j * 16 + MF::drow(lane, r);
+        if (row < g.M && col < g.N) g.C[(size_t)col * g.ldc + row] = acc[i][j][r];
+      }
+}
+
 template <typename T>
 hipError_t launch_gemm(int ta, int tb, int M, int N, int K, T alpha, const T *A,
                        long lda, const T *B, long ldb, T beta, T *C, long ldc,
@@ -396,6 +518,33 @@ extern "C" hipError_t atlmx_k_supdate(int M, int N, int K, float alpha, const fl
   g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % 4 == 0);
   hipLaunchKernelGGL((update_kernel<float, 64>), dim3((unsigned)(8 * g.per_xcd)), dim3(NT), 0, s,
                      g);
+  return hipGetLastError();
+}
+
+/* the same with the operands rounded to bfloat16 (lp = 1) or half (lp = 2) */
+extern "C" hipError_t atlmx_k_supdate_lp(int lp, int M, int N, int K, float alpha,
+                                         const float *A, long lda, const float *B, long ldb,
+                                         float *C, long ldc, const int *abort_flag,
+                                         hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+  if ((lp != 1 && lp != 2) || lda < M || ldb < K || ldc < M || !A || !B || !C)
+    return hipErrorInvalidValue;
+  GemmArgs<float> g;
+  g.M = M; g.N = N; g.K = K; g.alpha = alpha; g.beta = 1.0f;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.tri = 0; g.abort_flag = abort_flag;
+  g.tiles_m = (M + BM - 1) / BM;
+  g.tiles_n = (N + BN - 1) / BN;
+  const long ntiles = (long)g.tiles_m * g.tiles_n;
+  if (ntiles > (1L << 30)) return hipErrorInvalidValue;
+  g.per_xcd = (int)((ntiles + 7) / 8);
+  g.vec_a = (((uintptr_t)A % 16) == 0) && (lda % 4 == 0);
+  g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % 4 == 0);
+  const dim3 grid((unsigned)(8 * g.per_xcd)), block(NT);
+  if (lp == 1)
+    hipLaunchKernelGGL(update_lp_kernel<1>, grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL(update_lp_kernel<2>, grid, block, 0, s, g);
   return hipGetLastError();
 }
 

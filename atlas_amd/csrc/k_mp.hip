@@ -139,9 +139,48 @@ unsigned grid_for(long total) {
   return (unsigned)b;
 }
 
+/* scale[j] = 2^-e with mx[j] = f 2^e, f in [0.5, 1): mx[j] scale[j] in
+ * [0.5, 1), exactly (1 where mx[j] is zero or not finite) */
+__global__ void pow2_scale_kernel(int n, const double *mx, double *scale) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= n) return;
+  const double m = mx[j];
+  int e = 0;
+  if (m > 0.0 && m <= 1.7976931348623157e308) frexp(m, &e);
+  scale[j] = ldexp(1.0, -e);
+}
+
+/* X(i, :) (+)= scale[i] * D(i, :), D in FP32 */
+__global__ void scale_conv_kernel(int M, int N, const float *D, long ldd, const double *scale,
+                                  double *X, long ldx, int accumulate) {
+  const long total = (long)M * N;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total;
+       t += (long)gridDim.x * blockDim.x) {
+    const long j = t / M, i = t - j * M;
+    const double v = scale[i] * (double)D[(size_t)j * ldd + i];
+    double *x = X + (size_t)j * ldx + i;
+    *x = accumulate ? *x + v : v;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+hipError_t atlmx_k_pow2_scale(int n, const double *mx, double *scale, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(pow2_scale_kernel, dim3((n + 255) / 256), dim3(256), 0, s, n, mx, scale);
+  return hipGetLastError();
+}
+
+hipError_t atlmx_k_scale_s2d(int M, int N, const float *D, long ldd, const double *scale,
+                             double *X, long ldx, int accumulate, hipStream_t s) {
+  if (M <= 0 || N <= 0) return hipSuccess;
+  hipLaunchKernelGGL(scale_conv_kernel, dim3(grid_for((long)M * N)), dim3(256), 0, s, M, N, D,
+                     ldd, scale, X, ldx, accumulate);
+  return hipGetLastError();
+}
+
 
 hipError_t atlmx_k_d2s(int M, int N, const double *A, long lda, float *S, long lds,
                        hipStream_t s) {

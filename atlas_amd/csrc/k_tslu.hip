@@ -566,16 +566,19 @@ __global__ __launch_bounds__(256) void tslu_tail_kernel(TailArgs t) {
   }
 }
 
-/* S (n x n, FP32) = A^T for the n x n FP64 A, through 64 x 64 LDS tiles;
- * also perm = 0..n-1 */
+/* S (n x n, FP32) = A^T for the n x n FP64 A (column j of A times
+ * colscale[j] when given), through 64 x 64 LDS tiles; also perm = 0..n-1 */
 __global__ __launch_bounds__(256) void tslu_convert_kernel(int n, const double *A, long lda,
-                                                           float *S, long lds, int *perm) {
+                                                           float *S, long lds, int *perm,
+                                                           const double *colscale) {
   __shared__ float t[64][65];
   const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64;
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int jj = ty; jj < 64; jj += 4) {
     const int i = i0 + tx, j = j0 + jj;
-    t[jj][tx] = (i < n && j < n) ? (float)A[(size_t)j * lda + i] : 0.0f;
+    t[jj][tx] = (i < n && j < n)
+                    ? (float)(colscale ? A[(size_t)j * lda + i] * colscale[j] : A[(size_t)j * lda + i])
+                    : 0.0f;
   }
   __syncthreads();
   for (int ii = ty; ii < 64; ii += 4) {
@@ -659,12 +662,12 @@ extern "C" hipError_t atlmx_k_tslu_tail(float *S, long lds, int n, int k, const 
 }
 
 extern "C" hipError_t atlmx_k_tslu_convert(int n, const double *A, long lda, float *S, long lds,
-                                           int *perm, hipStream_t s) {
+                                           int *perm, const double *colscale, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (lda < n || lds < n) return hipErrorInvalidValue;
   const int nb = (n + 63) / 64;
   hipLaunchKernelGGL(tslu_convert_kernel, dim3(nb, nb), dim3(256), 0, s, n, A, lda, S, lds,
-                     perm);
+                     perm, colscale);
   return hipGetLastError();
 }
 

@@ -152,9 +152,14 @@ hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *order, cons
                                float beta, float *const *C, const int *ldc,
                                const int *tile_start, int *counter, int grid, hipStream_t s);
 
-/* S = (float) A^T, perm = 0..n-1 */
+/* S = (float) (A diag(colscale))^T (colscale may be NULL), perm = 0..n-1 */
 hipError_t atlmx_k_tslu_convert(int n, const double *A, long lda, float *S, long lds, int *perm,
-                                hipStream_t s);
+                                const double *colscale, hipStream_t s);
+/* scale[j] = the power of two with mx[j] * scale[j] in [0.5, 1) (1 for 0) */
+hipError_t atlmx_k_pow2_scale(int n, const double *mx, double *scale, hipStream_t s);
+/* X(i, :) = (or +=) scale[i] * D(i, :), D FP32, X FP64 */
+hipError_t atlmx_k_scale_s2d(int M, int N, const float *D, long ldd, const double *scale,
+                             double *X, long ldx, int accumulate, hipStream_t s);
 /* Y(i, :) = (float) X(perm[i], :) */
 hipError_t atlmx_k_tslu_gather(int n, int nrhs, const double *X, long ldx, const int *perm,
                                float *Y, long ldy, hipStream_t s);
@@ -163,6 +168,11 @@ hipError_t atlmx_k_tslu_gather(int n, int nrhs, const double *X, long ldx, const
 hipError_t atlmx_k_supdate(int M, int N, int K, float alpha, const float *A, long lda,
                            const float *B, long ldb, float *C, long ldc, const int *abort_flag,
                            hipStream_t s);
+/* the same with A and B rounded to bfloat16 (lp = 1) or IEEE half (lp = 2),
+ * round to nearest even, products accumulated in FP32 onto the FP32 C */
+hipError_t atlmx_k_supdate_lp(int lp, int M, int N, int K, float alpha, const float *A,
+                              long lda, const float *B, long ldb, float *C, long ldc,
+                              const int *abort_flag, hipStream_t s);
 
 /* B(M x N) *= alpha */
 hipError_t atlmx_k_dscal2d(int M, int N, double alpha, double *B, long ldb,

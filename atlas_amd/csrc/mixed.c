@@ -45,7 +45,7 @@ void dgemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, double alpha,
                const double *A, long lda, const double *B, long ldb, double beta, double *C,
                long ldc, int tri, const int *abort_flag, const char *rout);
 
-enum { MP_AUTO = 0, MP_TSLU = 1, MP_GEPP = 2, MP_FP64 = 3, MP_CHOL = 4 };
+enum { MP_AUTO = 0, MP_TSLU = 1, MP_GEPP = 2, MP_FP64 = 3, MP_CHOL = 4, MP_BF16 = 5, MP_FP16 = 6 };
 static int mp_method = MP_AUTO;
 static int mp_stop_lapack = 0;
 
@@ -69,7 +69,7 @@ static int ir_done(double rn, double xn, double bn, double anrm, double cte, int
  * FP64; 1: tournament only (then FP64); 2: partial pivoting only (then FP64). */
 void atlmx_set_mixed_method(int m)
 {
-  mp_method = (m == MP_TSLU || m == MP_GEPP) ? m : MP_AUTO;
+  mp_method = (m == MP_TSLU || m == MP_GEPP || m == MP_BF16 || m == MP_FP16) ? m : MP_AUTO;
 }
 
 static double host_max(atlmx_ctx *c, const double *d, int n, const char *rout)
@@ -89,15 +89,21 @@ static double host_max(atlmx_ctx *c, const double *d, int n, const char *rout)
  * K = columns k0.. of Ut (S rows, A's U12^T) and rows k0.. of Lt (A's
  * L21^T): the operands are S's own blocks, so this is one short-K update */
 static void tslu_update(atlmx_ctx *c, float *S, long lds, int r0, int M, int c0, int N,
-                        int k0, int K, const int *d_info, hipStream_t st, const char *rout)
+                        int k0, int K, int lp, const int *d_info, hipStream_t st,
+                        const char *rout)
 {
   int pid;
   if (M <= 0 || N <= 0 || K <= 0) return;
   pid = rt_prof_begin2(c, PROF_UPDATE, 2.0 * M * N * K,
                        ((double)M * K + (double)K * N + 2.0 * M * N) * sizeof(float));
-  RT_CHECK(rout, atlmx_k_supdate(M, N, K, -1.0f, S + r0 + (size_t)k0 * lds, lds,
-                                 S + k0 + (size_t)c0 * lds, lds, S + r0 + (size_t)c0 * lds, lds,
-                                 d_info, st));
+  if (lp)
+    RT_CHECK(rout, atlmx_k_supdate_lp(lp, M, N, K, -1.0f, S + r0 + (size_t)k0 * lds, lds,
+                                      S + k0 + (size_t)c0 * lds, lds,
+                                      S + r0 + (size_t)c0 * lds, lds, d_info, st));
+  else
+    RT_CHECK(rout, atlmx_k_supdate(M, N, K, -1.0f, S + r0 + (size_t)k0 * lds, lds,
+                                   S + k0 + (size_t)c0 * lds, lds, S + r0 + (size_t)c0 * lds,
+                                   lds, d_info, st));
   rt_prof_end(c, pid);
 }
 
@@ -166,7 +172,7 @@ static void tslu_u12(atlmx_ctx *c, float *S, long lds, int k, int w, int r0, int
  * below. */
 static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *tws,
                            const tslu_panel *P, const tslu_panel *Q, int *planPQ, int *perm,
-                           const int *d_info, const char *rout)
+                           int lp, const int *d_info, const char *rout)
 {
   const int W = ATLMX_TSLU_W;
   const int wp = (n - k < W) ? n - k : W, kq = k + wp;
@@ -176,7 +182,7 @@ static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *
   tslu_l21(c, S, lds, n, k, wp, P, d_info, rout);
   if (wq == 0) return;
   tslu_u12(c, S, lds, k, wp, kq, k2, P, d_info, rout);
-  tslu_update(c, S, lds, kq, wq, kq, n - kq, k, wp, d_info, c->stream, rout);
+  tslu_update(c, S, lds, kq, wq, kq, n - kq, k, wp, lp, d_info, c->stream, rout);
   tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, d_info, rout);
   tslu_swap(c, S, lds, k, k2, 0, 0, kq, wq, Q, 1, d_info, rout);
   tslu_l21(c, S, lds, n, kq, wq, Q, d_info, rout);
@@ -224,7 +230,7 @@ static void tslu_pair_tail(atlmx_ctx *c, float *S, long lds, int n, int k,
  * transposed, U the lower triangle of S transposed.  Returns 0 or 1 + the
  * first column with a zero/non-finite pivot (every launch after that
  * returns at once). */
-static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
+static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm, int lp,
                            const char *rout)
 {
   const int W = ATLMX_TSLU_W;
@@ -240,20 +246,20 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm,
   int info = 0, k = 0;
   RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, main));
   RT_CHECK(rout, atlmx_k_fill_i32(tws, (int)wsi, 0, main));
-  tslu_pair_head(c, S, lds, n, 0, tws, &P, &Q, planPQ, perm, d_info, rout);
+  tslu_pair_head(c, S, lds, n, 0, tws, &P, &Q, planPQ, perm, lp, d_info, rout);
   for (;;) {
     const int k2 = (n - k <= 2 * W) ? n : k + 2 * W;
     const int R = (n - k2 < 2 * W) ? n - k2 : 2 * W;
     tslu_pair_tail(c, S, lds, n, k, &P, &Q, planPQ, d_info, rout);
     if (k2 >= n) break;
-    tslu_update(c, S, lds, k2, R, k2, n - k2, k, k2 - k, d_info, main, rout);
+    tslu_update(c, S, lds, k2, R, k2, n - k2, k, k2 - k, lp, d_info, main, rout);
     RT_CHECK(rout, hipEventRecord(c->ev[0], main));
     RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
     c->stream = aux;
-    tslu_pair_head(c, S, lds, n, k2, tws, &P, &Q, planPQ, perm, d_info, rout);
+    tslu_pair_head(c, S, lds, n, k2, tws, &P, &Q, planPQ, perm, lp, d_info, rout);
     c->stream = main;
     RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
-    tslu_update(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, d_info, main, rout);
+    tslu_update(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, lp, d_info, main, rout);
     RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
     k = k2;
   }
@@ -271,6 +277,7 @@ typedef struct {
   const int *ipiv;          /* MP_GEPP: LAPACK interchanges */
   const int *perm;          /* MP_TSLU: row permutation */
   float *Rs;                /* FP32 right-hand-side staging, n x MP_MAXRHS */
+  const double *colscale;   /* MP_TSLU: the factor is of A diag(colscale), or NULL */
 } lp_fact;
 
 /* X (=, or += when accumulate) M^-1 R for nb <= MP_MAXRHS columns, with the
@@ -285,7 +292,10 @@ static void lp_solve(atlmx_ctx *c, const lp_fact *f, int nb, const double *R, lo
               f->Rs, f->n, NULL, rout);
     strsm_dev(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_NONUNIT, f->n, nb, 1.0f, f->As, f->lda,
               f->Rs, f->n, NULL, rout);
-    if (accumulate)
+    if (f->colscale)   /* (A D)^-1 = D^-1 A^-1: A^-1 r = D (A D)^-1 r */
+      RT_CHECK(rout, atlmx_k_scale_s2d(f->n, nb, f->Rs, f->n, f->colscale, X, ldx, accumulate,
+                                       c->stream));
+    else if (accumulate)
       RT_CHECK(rout, atlmx_k_axpy_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
     else
       RT_CHECK(rout, atlmx_k_s2d(f->n, nb, f->Rs, f->n, X, ldx, c->stream));
@@ -397,7 +407,7 @@ static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const d
     const double *b = B + (size_t)j * ldb;
     double *x = X + (size_t)j * ldx;
     double bn = 0.0, xn = 0.0, rn = 0.0;
-    int cj = 0;
+    int cj = 0, stalled = 0;
     RT_CHECK(rout, atlmx_k_dcolmax(N, 1, b, ldb, nrm + 64, c->stream));
     RT_CHECK(rout, hipMemcpyAsync(&bn, nrm + 64, sizeof(double), hipMemcpyDeviceToHost,
                                   c->stream));
@@ -412,7 +422,7 @@ static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const d
                                     c->stream));
       rt_sync(c, rout);
       if (ir_done(rn, xn, bn, anrm, cte, N)) { cj = 1; break; }
-      if (it == outer_max) break;
+      if (it == outer_max || stalled) break;
       /* GMRES on M^{-1} A d = M^{-1} r from d = 0 */
       lp_solve(c, f, 1, R, N, V, N, 0, rout);
       beta = sqrt(ddot_dev(c, N, V, V, hd, rout));
@@ -470,6 +480,9 @@ static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const d
                 rout);
       rt_sync(c, rout);                                     /* y lives on the host stack */
       st->inner_iterations += kk;
+      /* a whole restart cycle that did not halve the preconditioned
+       * residual: the factors are too weak a preconditioner to go on */
+      if (kk == m && !(fabs(g[kk]) <= 0.5 * beta)) stalled = 1;
     }
     if (it > st->iterations) st->iterations = it;
     {
@@ -507,7 +520,35 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
 
   memset(&f, 0, sizeof(f));
   f.n = N;
-  if (mp_method != MP_GEPP) {
+  if (mp_method == MP_BF16 || mp_method == MP_FP16) {
+    /* tournament-pivoting LU of S = (A D)^T, D the power-of-two column
+     * scaling that brings every column's largest entry into [0.5, 1), with
+     * the trailing updates' operands rounded to 16 bits */
+    int *perm = (int *)rt_ws(c, WS_IPIV, sizeof(int) * (size_t)N, rout);
+    double *csc = (double *)rt_ws(c, WS_MPSCALE, sizeof(double) * (size_t)N, rout);
+    f.kind = MP_TSLU;
+    f.lda = (N + 3) & ~3;
+    f.perm = perm;
+    f.colscale = csc;
+    f.As = (float *)rt_ws(c, WS_MP0, sizeof(float) * (size_t)f.lda * N, rout);
+    f.Rs = (float *)rt_ws(c, WS_MP1, sizeof(float) * (size_t)N * MP_MAXRHS, rout);
+    RT_CHECK(rout, atlmx_k_dcolmax(N, N, A, lda, csc, c->stream));
+    RT_CHECK(rout, atlmx_k_pow2_scale(N, csc, csc, c->stream));
+    RT_CHECK(rout, atlmx_k_tslu_convert(N, A, lda, f.As, f.lda, perm, csc, c->stream));
+    info = sgetrf_tslu_dev(c, N, f.As, f.lda, perm, mp_method == MP_BF16 ? 1 : 2, rout);
+    st->factor_info = info;
+    st->method = mp_method;
+    if (!info)
+      conv = refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm, ws,
+                    st, rout);
+    if (!info && !conv) {
+      st->gmres = 1;
+      conv = gmres_refine(c, &f, N, NRHS, A, lda, B, ldb, X, ldx, iter_max, anrm, cte, R, nrm,
+                          ws, st, rout);
+    }
+    f.colscale = NULL;
+  }
+  if (!conv && mp_method != MP_GEPP) {
     /* tournament-pivoting LU of S = A^T */
     int *perm = (int *)rt_ws(c, WS_IPIV, sizeof(int) * (size_t)N, rout);
     f.kind = MP_TSLU;
@@ -515,8 +556,9 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
     f.perm = perm;
     f.As = (float *)rt_ws(c, WS_MP0, sizeof(float) * (size_t)f.lda * N, rout);
     f.Rs = (float *)rt_ws(c, WS_MP1, sizeof(float) * (size_t)N * MP_MAXRHS, rout);
-    RT_CHECK(rout, atlmx_k_tslu_convert(N, A, lda, f.As, f.lda, perm, c->stream));
-    info = sgetrf_tslu_dev(c, N, f.As, f.lda, perm, rout);
+    st->gmres = 0;
+    RT_CHECK(rout, atlmx_k_tslu_convert(N, A, lda, f.As, f.lda, perm, NULL, c->stream));
+    info = sgetrf_tslu_dev(c, N, f.As, f.lda, perm, 0, rout);
     st->factor_info = info;
     st->method = MP_TSLU;
     if (!info)

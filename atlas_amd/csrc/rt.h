@@ -29,6 +29,7 @@ enum {
   WS_MP2,
   WS_MP3,
   WS_MP4,
+  WS_MPSCALE,       /* mixed precision: column scale factors    */
   WS_NSLOTS
 };
 

@@ -144,7 +144,8 @@ typedef struct {
   int factor_info;         /* info of the last low-precision factorization  */
   double backward_error;   /* max_j ||b-Ax||inf / (||A||inf ||x||inf + ||b||inf) */
   int method;              /* 1 tournament-pivoting FP32, 2 partial-pivoting
-                              FP32, 3 FP64, 4 FP32 Cholesky */
+                              FP32, 3 FP64, 4 FP32 Cholesky, 5 (6) tournament
+                              with bfloat16 (half) trailing updates */
   int gmres;               /* 1: the corrections came from GMRES-IR (plain
                               refinement with these factors stalled) */
   int inner_iterations;    /* GMRES iterations in total */
@@ -159,7 +160,10 @@ int atlmx_dsposv(int N, int NRHS, const double *A, int lda, const double *B, int
  * dsgesv's ||r||inf <= ||x||inf ||A||inf 2^-53 sqrt(N).  Process-wide. */
 void atlmx_set_refine_stop(int lapack);
 /* 0 (default): tournament, then partial pivoting, then FP64; 1: tournament
- * then FP64; 2: partial pivoting then FP64.  Process-wide. */
+ * then FP64; 2: partial pivoting then FP64; 5 (6): tournament with the
+ * trailing updates' operands rounded to bfloat16 (IEEE half), A's columns
+ * scaled by powers of two, then as 0.  The method that produced the result
+ * is reported in atlmx_ir_info.method.  Process-wide. */
 void atlmx_set_mixed_method(int method);
 int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int ldb,
                  double *X, int ldx, int iter_max, atlmx_ir_info *info);

@@ -224,3 +224,54 @@ def test_short_k_update_kernel_vs_numpy(M, N, K):
                                                       B.T[:K].astype(np.float64))
         assert np.abs(got.T[:M] - want).max() < 1e-5 * K
         assert np.array_equal(got.T[M:], Cm.T[M:])       # rows past M untouched
+
+
+def _round16(x, lp, mode="rne"):
+    """float32 -> bfloat16 (lp 1) or IEEE half (lp 2), back as float64; the
+    bfloat16 rounding restated on the bit pattern (nearest even, or
+    truncation for the contrast check)"""
+    x = np.asarray(x, np.float32)
+    if lp == 2:
+        return x.astype(np.float16).astype(np.float64)
+    u = x.view(np.uint32).astype(np.uint64)
+    if mode == "rne":
+        u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16
+    else:
+        u = u >> 16
+    return (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("lp", [1, 2])
+@pytest.mark.parametrize("M,N,K", [(300, 200, 128), (257, 190, 70), (128, 128, 64)])
+def test_low_precision_update_kernel_vs_numpy(lp, M, N, K):
+    """atlmx_k_supdate_lp: C += alpha round(A) round(B) with the operands
+    rounded to bfloat16 / half (nearest even) and FP32 accumulation, against
+    the numpy restatement (the same rounding, float64 products); the
+    kernel's result must be clearly closer to it than to the truncated
+    rounding of the operands (which pins the rounding and the operand
+    layout of the MFMA)."""
+    import ctypes as C
+    import torch
+    X.use_torch_stream()
+    rng = np.random.default_rng(7 * M + N + K + lp)
+    lda, ldb, ldc = M + 3, K + 1, M + 5
+    A = rng.uniform(-1, 1, (K, lda)).astype(np.float32)
+    B = rng.uniform(-1, 1, (N, ldb)).astype(np.float32)
+    Cm = rng.uniform(-1, 1, (N, ldc)).astype(np.float32)
+    dA, dB, dC = (torch.tensor(x, device="cuda") for x in (A, B, Cm))
+    fn = X.lib().atlmx_k_supdate_lp
+    fn.restype = C.c_int
+    r = fn(lp, M, N, K, C.c_float(-1.0), X.dptr(dA), C.c_long(lda), X.dptr(dB), C.c_long(ldb),
+           X.dptr(dC), C.c_long(ldc), None, None)
+    assert r == 0
+    torch.cuda.synchronize()
+    got = dC.cpu().numpy().T[:M].astype(np.float64)
+    Am, Bm = A.T[:M], B.T[:K]
+    want = Cm.T[:M] - _round16(Am, lp) @ _round16(Bm, lp)
+    scale = 1.0 + np.abs(Am) @ np.abs(Bm)
+    err = np.abs(got - want) / scale
+    assert err.max() < 4 * K * 2.0 ** -24, err.max()
+    if lp == 1:
+        trunc = Cm.T[:M] - _round16(Am, 1, "trunc") @ _round16(Bm, 1, "trunc")
+        assert (np.abs(got - trunc) / scale).max() > 10 * err.max()
+    assert np.array_equal(dC.cpu().numpy().T[M:], Cm.T[M:])

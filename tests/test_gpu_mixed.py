@@ -249,6 +249,68 @@ def test_dsgesv_fp64_accuracy(torch, method, want, n, nrhs, kappa):
     assert np.array_equal(_host(dA), A) and np.array_equal(_host(dB), b)   # inputs untouched
 
 
+@pytest.mark.parametrize("method,n,kappa", [(X.MIXED_BF16, 1024, 1e2), (X.MIXED_FP16, 1024, 1e2),
+                                            (X.MIXED_FP16, 1024, 1e4), (X.MIXED_FP16, 4096, 1e2)])
+def test_dsgesv_low_precision_updates(torch, method, n, kappa):
+    """the tournament LU with its trailing updates in bfloat16 / half: FP64
+    accuracy from its factors alone (plain refinement or GMRES-IR), no
+    fallback"""
+    A, x, b = _system(n, kappa, 1, seed=n)
+    dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    X.set_mixed_method(method)
+    try:
+        info, st = X.dsgesv_dev(dA, dB, dX)
+    finally:
+        X.set_mixed_method(X.MIXED_AUTO)
+    assert info == 0 and st.converged == 1 and st.fallback == 0 and st.method == method
+    be = X.lib().atlmx_dbackward_error(n, 1, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64, be
+    ferr = np.abs(_host(dX) - x).max() / np.abs(x).max()
+    assert ferr <= 10 * kappa * n * EPS64, ferr
+
+
+def test_dsgesv_half_updates_with_badly_scaled_columns(torch):
+    """columns scaled by 10^-6 .. 10^6: without the power-of-two column
+    scaling the half-precision operands would overflow (or vanish); with it
+    the half-update factorization still serves as the refinement's factors"""
+    n = 1024
+    A0, x0, _ = _system(n, 1e2, 1, seed=n)
+    cs = 10.0 ** np.random.default_rng(9).uniform(-6, 6, n)
+    A = A0 * cs                                   # column j times cs[j]
+    x = x0 / cs[:, None]
+    b = A @ x
+    dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    X.set_mixed_method(X.MIXED_FP16)
+    try:
+        info, st = X.dsgesv_dev(dA, dB, dX)
+    finally:
+        X.set_mixed_method(X.MIXED_AUTO)
+    assert info == 0 and st.converged == 1 and st.fallback == 0 and st.method == X.MIXED_FP16
+    be = X.lib().atlmx_dbackward_error(n, 1, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64, be
+
+
+def test_dsgesv_bfloat16_too_weak_falls_back_to_fp32(torch):
+    """at kappa 1e6 the bfloat16-update factors cannot drive refinement (or
+    GMRES-IR, which stops when a restart cycle fails to halve the backward
+    error); the FP32 tournament LU then solves it"""
+    n = 1024
+    A, x, b = _system(n, 1e6, 1, seed=n)
+    dA, dB = _dev(torch, A, torch.float64), _dev(torch, b, torch.float64)
+    dX = torch.zeros_like(dB)
+    X.set_mixed_method(X.MIXED_BF16)
+    try:
+        info, st = X.dsgesv_dev(dA, dB, dX)
+    finally:
+        X.set_mixed_method(X.MIXED_AUTO)
+    assert info == 0 and st.converged == 1 and st.fallback == 0 and st.method == X.MIXED_TSLU
+    assert st.inner_iterations <= 300                # the stalled GMRES stopped early
+    be = X.lib().atlmx_dbackward_error(n, 1, X.dptr(dA), n, X.dptr(dB), n, X.dptr(dX), n)
+    assert be <= n * EPS64, be
+
+
 def test_dsgesv_falls_back_to_fp64_when_fp32_factor_is_singular(torch):
     """column 0 lies below FP32's range (it rounds to zero): both FP32
     factorizations meet an exactly zero pivot; the FP64 LU solves it"""
