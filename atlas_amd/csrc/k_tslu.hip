@@ -409,8 +409,17 @@ __global__ __launch_bounds__(256) void tslu_swap_kernel(float *S, long lds, int 
     s_src[t] = plan[3 + 2 * t];
   }
   __syncthreads();
-  for (int mv = threadIdx.x >> 6; mv < np; mv += 4)
-    if (live) buf[mv][j] = S[(size_t)s_src[mv] * lds + e];
+  for (int m0 = threadIdx.x >> 6; m0 < np; m0 += 4 * 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int mv = m0 + 4 * u;
+      v[u] = (live && mv < np) ? S[(size_t)s_src[mv] * lds + e] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+      if (m0 + 4 * u < np) buf[m0 + 4 * u][j] = v[u];
+  }
   __syncthreads();
   for (int mv = threadIdx.x >> 6; mv < np; mv += 4)
     if (live) S[(size_t)s_dst[mv] * lds + e] = buf[mv][j];
@@ -463,10 +472,20 @@ __global__ __launch_bounds__(256) void tslu_tail_kernel(TailArgs t) {
   if (hi)
     for (int q = tid; q < np; q += 256)
       if (s_dst[q] >= k && s_dst[q] < k2) s_map[s_dst[q] - k] = q;
-  /* every source is read before any destination is written */
-  for (int q = tid; q < np * NB; q += 256) {
-    const int mv = q / NB, i = q % NB;
-    if (i < nr) buf[mv][i] = S[(size_t)s_src[mv] * lds + r0 + i];
+  /* every source is read before any destination is written; eight loads
+   * per thread are in flight before their LDS stores */
+  for (int q0 = tid; q0 < np * NB; q0 += 256 * 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + 256 * u, mv = q / NB, i = q % NB;
+      v[u] = (q < np * NB && i < nr) ? S[(size_t)s_src[mv] * lds + r0 + i] : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + 256 * u;
+      if (q < np * NB) buf[q / NB][q % NB] = v[u];
+    }
   }
   __syncthreads();
   for (int q = tid; q < np * NB; q += 256) {
@@ -475,20 +494,40 @@ __global__ __launch_bounds__(256) void tslu_tail_kernel(TailArgs t) {
   }
   if (!hi) return;
   /* U: the pair's columns of these rows, after the moves */
-  for (int q = tid; q < 2 * TW * NB; q += 256) {
-    const int j = q / NB, i = q % NB, mv = s_map[j];
-    U[i][j] = (i >= nr) ? 0.0f : (mv >= 0 ? buf[mv][i] : S[(size_t)(k + j) * lds + r0 + i]);
+  for (int q0 = tid; q0 < 2 * TW * NB; q0 += 256 * 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + 256 * u, j = q / NB, i = q % NB, mv = s_map[j];
+      v[u] = (i >= nr) ? 0.0f : (mv >= 0 ? buf[mv][i] : S[(size_t)(k + j) * lds + r0 + i]);
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int q = q0 + 256 * u;
+      U[q % NB][q / NB] = v[u];
+    }
   }
   __syncthreads();
   /* operands: inv(L_P) as [j][m] = invlp(j, m), B as [m][j], inv(L_Q) */
   float (*iLP)[TW + 1] = reinterpret_cast<float (*)[TW + 1]>(&buf[0][0]);
   float (*Bm)[TW + 1] = iLP + TW;
   float (*iLQ)[TW + 1] = Bm + TW;
-  for (int q = tid; q < TW * TW; q += 256) {
-    const int a = q % TW, b = q / TW;             /* column-major (a, b) */
-    iLP[a][b] = t.invlp[q];
-    iLQ[a][b] = t.invlq[q];
-    Bm[a][b] = S[(size_t)(kq + b) * lds + k + a];  /* B(a, b) = S[k+a, kq+b] */
+  for (int q0 = tid; q0 < TW * TW; q0 += 256 * 4) {
+    float x[4], y[4], z[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + 256 * u, a = q % TW, b = q / TW;   /* column-major (a, b) */
+      x[u] = t.invlp[q];
+      y[u] = t.invlq[q];
+      z[u] = S[(size_t)(kq + b) * lds + k + a];             /* B(a, b) = S[k+a, kq+b] */
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int q = q0 + 256 * u, a = q % TW, b = q / TW;
+      iLP[a][b] = x[u];
+      iLQ[a][b] = y[u];
+      Bm[a][b] = z[u];
+    }
   }
   __syncthreads();
   /* thread: rows i0..i0+3, columns j0..j0+3 of each 64 x 64 product */
