@@ -246,10 +246,10 @@ def main():
     gemm_avg_ms = gemm_ms / gemm_cnt
     gemm_flops_per_launch = gemm_work / gemm_cnt
     achieved = gemm_flops_per_launch / (gemm_avg_ms * 1e-3)
-    roofline = {"bound": "mfma", "kernel": "update_kernel<float,64> (FP32 LU trailing updates)",
+    roofline = {"bound": "mfma", "kernel": "update_kernel<float,32> (FP32 LU trailing updates)",
                 "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32_MFMA / 1e12,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_MFMA, 4),
-                "traffic": measured_traffic("update_kernel<float,64>", "tslu_n%d" % n),
+                "traffic": measured_traffic("update_kernel<float,32>", "tslu_n%d" % n),
                 "launches": gemm_cnt,
                 "avg_launch_ms": round(gemm_avg_ms, 4),
                 "algorithmic_bytes_per_launch": round(gemm_bytes / gemm_cnt)}
