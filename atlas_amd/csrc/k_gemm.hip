@@ -402,9 +402,9 @@ template <> struct Low<2> {
 };
 
 template <int LP>
-__global__ __launch_bounds__(NT, 2) void update_lp_kernel(GemmArgs<float> g) {
+__global__ __launch_bounds__(NT, 1) void update_lp_kernel(GemmArgs<float> g) {
   typedef Mfma<float> MF;
-  constexpr int KB = 64;
+  constexpr int KB = 32;
   constexpr int LDH = KB + 8;                     /* 16-bit elements per LDS row */
   typedef SliceLoader<float, false, BM, KB> LA;   /* A: rows contiguous */
   typedef SliceLoader<float, true, BN, KB> LB;    /* B: k contiguous */
@@ -426,24 +426,37 @@ __global__ __launch_bounds__(NT, 2) void update_lp_kernel(GemmArgs<float> g) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
   const int kq = (lane >> 4) * 4, lr = lane & 15;
+  /* the untransposed product: element r of block (i, j) is row rb(i) + r,
+   * column cb(j) of C, so the C tile moves as 16-byte vectors, loaded behind
+   * the first slice and added after the last MFMA */
+  const int rb0 = m0 + wm + 4 * (lane >> 4), cb0 = n0 + wn + lr;
+  const bool cvec = (g.ldc % 4 == 0) && (((uintptr_t)g.C % 16) == 0);
 
+  LA la;
+  LB lb;
+  la.load(g.A, g.lda, m0, 0, g.M, g.K, g.vec_a, tid);
+  lb.load(g.B, g.ldb, n0, 0, g.N, g.K, g.vec_b, tid);
+  f4 cv[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int row = rb0 + i * 16, col = cb0 + j * 16;
+      const float *cp = g.C + (size_t)col * g.ldc + row;
+      if (col < g.N && row + 3 < g.M && cvec) {
+        cv[i][j] = *reinterpret_cast<const f4 *>(cp);
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) cv[i][j][r] = (col < g.N && row + r < g.M) ? cp[r] : 0.0f;
+      }
+    }
   f4 acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + lr;
-        const int col = n0 + wn + j * 16 + MF::drow(lane, r);
-        acc[i][j][r] = (row < g.M && col < g.N) ? g.C[(size_t)col * g.ldc + row] : 0.0f;
-      }
+    for (int j = 0; j < 4; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
   for (int k0 = 0; k0 < g.K; k0 += KB) {
-    LA la;
-    LB lb;
-    la.load(g.A, g.lda, m0, k0, g.M, g.K, g.vec_a, tid);
-    lb.load(g.B, g.ldb, n0, k0, g.N, g.K, g.vec_b, tid);
     if (k0 > 0) __syncthreads();
 #pragma unroll
     for (int i = 0; i < LA::NV; ++i) {            /* 4 rows at one k */
@@ -462,6 +475,10 @@ __global__ __launch_bounds__(NT, 2) void update_lp_kernel(GemmArgs<float> g) {
       *reinterpret_cast<uint2 *>(&Bs[r * LDH + k]) = w;
     }
     __syncthreads();
+    if (k0 + KB < g.K) {
+      la.load(g.A, g.lda, m0, k0 + KB, g.M, g.K, g.vec_a, tid);
+      lb.load(g.B, g.ldb, n0, k0 + KB, g.N, g.K, g.vec_b, tid);
+    }
 #pragma unroll
     for (int ks = 0; ks < KB / 16; ++ks) {
       uint2 a[4], b[4];
@@ -473,19 +490,24 @@ __global__ __launch_bounds__(NT, 2) void update_lp_kernel(GemmArgs<float> g) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = Low<LP>::mma(b[j], a[i], acc[i][j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = Low<LP>::mma(a[i], b[j], acc[i][j]);
     }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j)
+    for (int j = 0; j < 4; ++j) {
+      const int row = rb0 + i * 16, col = cb0 + j * 16;
+      float *cp = g.C + (size_t)col * g.ldc + row;
+      const f4 v = acc[i][j] + cv[i][j];
+      if (col < g.N && row + 3 < g.M && cvec) {
+        *reinterpret_cast<f4 *>(cp) = v;
+      } else {
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = m0 + wm + i * 16 + lr;
-        const int col = n0 + wn + j * 16 + MF::drow(lane, r);
-        if (row < g.M && col < g.N) g.C[(size_t)col * g.ldc + row] = acc[i][j][r];
+        for (int r = 0; r < 4; ++r)
+          if (col < g.N && row + r < g.M) cp[r] = v[r];
       }
+    }
 }
 
 template <typename T>
