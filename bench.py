@@ -123,6 +123,191 @@ def cpu_baseline(n_cpu, seed):
                       "%dx%d system, 1 RHS, %.1f s, info=%d" % (n_cpu, n_cpu, t, info)}
 
 
+def _views(big, offs, shapes):
+    return [big[int(offs[i]):int(offs[i + 1])].view(*shapes[i]) for i in range(len(shapes))]
+
+
+def bench_batched(count, seed, torch, X, log, reps=3):
+    """The batched configurations, each as one graph-replayed persistent
+    launch: `count` SPD systems of order U{8..512} (Cholesky factor + solve,
+    one right-hand side), `count` general systems of order U{8..256} (LU with
+    partial pivoting + solve), `count` GEMMs with M, N, K = 16 U{1..24}; and
+    the differentiated configuration: 512 SPD and 512 general systems of
+    order 256 with 8 right-hand sides, forward solve + backward (adjoints)
+    through atlas_amd.autograd.  Inputs are synthetic and diagonally
+    dominant (SPD through the lower triangle the Cholesky reads).  Every
+    timed replay starts from the pristine inputs (copied back outside the
+    timed region); plan creation (the autotuner) is not timed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    res = {}
+
+    def timed(plan, restore):
+        ts = []
+        for _ in range(reps):
+            restore()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            plan.run()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        return min(ts)
+
+    def matrices(sizes, dt, dominant):
+        """column-major matrices in one buffer: U[-1,1), with n+1 on the
+        diagonal when dominant"""
+        sz = np.asarray(sizes, np.int64)
+        offs = np.concatenate([[0], np.cumsum(sz * sz)])
+        big = torch.rand(int(offs[-1]), dtype=dt, device="cuda") * 2 - 1
+        if dominant:
+            diag = np.concatenate([o + np.arange(n) * (n + 1) for o, n in zip(offs[:-1], sz)])
+            dval = np.repeat(sz + 1.0, sz)
+            big[torch.tensor(diag, device="cuda")] = torch.tensor(dval, dtype=dt, device="cuda")
+        return big, offs
+
+    for dt, tag in ((torch.float64, ""), (torch.float32, "_fp32")):
+        for kind, lo, hi in (("posv", 8, 512), ("gesv", 8, 256)):
+            sizes = rng.integers(lo, hi + 1, count)
+            big, offs = matrices(sizes, dt, kind == "posv")
+            pristine = big.clone()
+            As = _views(big, offs, [(n, n) for n in sizes])
+            boffs = np.concatenate([[0], np.cumsum(sizes)])
+            bbig = torch.rand(int(boffs[-1]), dtype=dt, device="cuda") - 0.5
+            bpr = bbig.clone()
+            Bs = _views(bbig, boffs, [(1, n) for n in sizes])
+            info = torch.zeros(count, dtype=torch.int32, device="cuda")
+            if kind == "posv":
+                plan = X.posv_batch(As, Bs, info)
+                flops = float(np.sum(sizes ** 3 / 3.0 + 2.0 * sizes ** 2))
+            else:
+                piv = _views(torch.zeros(int(boffs[-1]), dtype=torch.int32, device="cuda"), boffs,
+                             [(n,) for n in sizes])
+                plan = X.gesv_batch(As, piv, Bs, info)
+                flops = float(np.sum(2.0 * sizes ** 3 / 3.0 + 2.0 * sizes ** 2))
+
+            def restore():
+                big.copy_(pristine)
+                bbig.copy_(bpr)
+            with plan:
+                ms = timed(plan, restore)
+                nb, grid = plan.config()
+            bad = int((info != 0).sum().item())
+            key = kind + tag
+            res[key] = {"count": count, "n_range": [lo, hi], "nrhs": 1, "ms": round(ms, 3),
+                        "gflops": round(flops / (ms * 1e-3) / 1e9, 1), "panel_width": nb,
+                        "grid": grid, "failed_instances": bad,
+                        "inputs": "U[-1,1) + (n+1)I, lower triangle read (SPD)" if kind == "posv"
+                        else "U[-1,1)"}
+            log("batched %s x%d: %.2f ms, %.1f GFLOP/s" % (key, count, ms, res[key]["gflops"]))
+            del big, pristine, bbig, bpr, As, Bs
+
+        M, N, K = (16 * rng.integers(1, 25, count) for _ in range(3))
+        offa = np.concatenate([[0], np.cumsum(M * K)])
+        offb = np.concatenate([[0], np.cumsum(K * N)])
+        offc = np.concatenate([[0], np.cumsum(M * N)])
+        ga = torch.rand(int(offa[-1]), dtype=dt, device="cuda") - 0.5
+        gb = torch.rand(int(offb[-1]), dtype=dt, device="cuda") - 0.5
+        gc = torch.rand(int(offc[-1]), dtype=dt, device="cuda") - 0.5
+        gcp = gc.clone()
+        As = _views(ga, offa, [(k, m) for m, k in zip(M, K)])
+        Bs = _views(gb, offb, [(n, k) for k, n in zip(K, N)])
+        Cs = _views(gc, offc, [(n, m) for m, n in zip(M, N)])
+        with X.gemm_batch(X.NoTrans, X.NoTrans, 1.0, As, Bs, 0.5, Cs) as plan:
+            ms = timed(plan, lambda: gc.copy_(gcp))
+        flops = float(np.sum(2.0 * M * N * K))
+        res["gemm" + tag] = {"count": count, "mnk": "16*U{1..24}", "ms": round(ms, 3),
+                             "gflops": round(flops / (ms * 1e-3) / 1e9, 1)}
+        log("batched gemm%s x%d: %.2f ms, %.1f GFLOP/s" % (tag, count, ms,
+                                                          res["gemm" + tag]["gflops"]))
+        del ga, gb, gc, gcp, As, Bs, Cs
+
+    from atlas_amd.autograd import CholeskySolve, LUSolve
+    b, n, r = 512, 256, 8
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    Bm = torch.randn(b, n, n, dtype=torch.float64, device="cuda", generator=g)
+    Aspd = Bm @ Bm.transpose(1, 2).contiguous() / n + torch.eye(n, dtype=torch.float64, device="cuda")
+    Agen = torch.rand(b, n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1 + \
+        4 * torch.eye(n, dtype=torch.float64, device="cuda")
+    Bv = torch.randn(b, n, r, dtype=torch.float64, device="cuda", generator=g)
+    W = torch.randn(b, n, r, dtype=torch.float64, device="cuda", generator=g)
+    for name, f, A0 in (("adjoint_posv", CholeskySolve.apply, Aspd), ("adjoint_gesv", LUSolve.apply, Agen)):
+        ts = []
+        for _ in range(reps + 1):
+            A1 = A0.clone().requires_grad_(True)
+            B1 = Bv.clone().requires_grad_(True)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            (f(A1, B1) * W).sum().backward()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        res[name] = {"batch": b, "n": n, "nrhs": r, "ms_forward_backward": round(min(ts[1:]) * 1e3, 3)}
+        log("%s %dx%d: %.2f ms forward+backward" % (name, b, n, res[name]["ms_forward_backward"]))
+    return res
+
+
+def bench_sweep(n, seed, A6, b6, torch, X, log):
+    """The dense-solve configuration's other cases, one solve each: kappa
+    1e2 and 1e4 (and the headline's 1e6) with the FP32 tournament LU and
+    with half-precision trailing updates (bfloat16 at kappa 1e2 only: beyond
+    it its factors stall and the solver falls back), and 16 right-hand sides
+    at kappa 1e6.  Backward error, forward error against the FP64 LU's
+    solution of the same system, refinement and GMRES steps, and the method
+    that produced the result."""
+    L = X.lib()
+    out = []
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+
+    def fp64_solution(A, b):
+        Aw, x = A.clone(), b.clone()
+        L.atlmx_dgetrf(n, n, X.dptr(Aw), n, X.dptr(ipiv))
+        L.atlmx_dgetrs(X.NoTrans, n, b.shape[0], X.dptr(Aw), n, X.dptr(ipiv), X.dptr(x), n)
+        torch.cuda.synchronize()
+        return x
+
+    def one(A, b, x64, kappa, method, label):
+        x = torch.empty_like(b)
+        X.set_mixed_method(method)
+        try:
+            torch.cuda.synchronize()              # workspaces exist from the headline runs
+            t0 = time.perf_counter()
+            info, st = X.dsgesv_dev(A, b, x)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+        finally:
+            X.set_mixed_method(X.MIXED_AUTO)
+        nr = b.shape[0]
+        be = L.atlmx_dbackward_error(n, nr, X.dptr(A), n, X.dptr(b), n, X.dptr(x), n)
+        fe = float(((x - x64).abs().max() / x64.abs().max()).item())
+        used = {1: "fp32", 2: "fp32 gepp", 3: "fp64", 5: "bf16 updates", 6: "fp16 updates"}
+        row = {"kappa": kappa, "nrhs": nr, "requested": label, "used": used.get(st.method, "?"),
+               "ms": round(ms, 2), "backward_error": be, "forward_error_vs_fp64": fe,
+               "iterations": st.iterations, "gmres": st.gmres,
+               "gmres_steps": st.inner_iterations, "fallback_to_fp64": st.fallback}
+        log("sweep kappa=%g nrhs=%d %s -> %s: %.1f ms, be %.2e, fe %.2e, iters %d, gmres %d" %
+            (kappa, nr, label, row["used"], ms, be, fe, st.iterations, st.inner_iterations))
+        out.append(row)
+
+    for kappa in (1e2, 1e4):
+        A, b = make_system(n, kappa, 1, seed + int(kappa), torch, X)
+        x64 = fp64_solution(A, b)
+        one(A, b, x64, kappa, X.MIXED_AUTO, "fp32")
+        one(A, b, x64, kappa, X.MIXED_FP16, "fp16 updates")
+        if kappa == 1e2:
+            one(A, b, x64, kappa, X.MIXED_BF16, "bf16 updates")
+        del A, b, x64
+    x64 = fp64_solution(A6, b6)
+    one(A6, b6, x64, 1e6, X.MIXED_FP16, "fp16 updates")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed + 16)
+    b16 = torch.rand(16, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    x64 = fp64_solution(A6, b16)
+    one(A6, b16, x64, 1e6, X.MIXED_AUTO, "fp32")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,6 +320,11 @@ def main():
     ap.add_argument("--gemm-n", type=int, default=8192)
     ap.add_argument("--cpu-n", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true",
+                    help="skip the batched and adjoint timings")
+    ap.add_argument("--batch-count", type=int, default=20000)
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the other kappa / precision / 16-RHS cases")
     ap.add_argument("--profile-run", action="store_true",
                     help="one warm mixed solve plus a 1 GiB device copy (the byte-counter "
                          "calibration) only: the run rocprofv3 --pmc passes wrap")
@@ -254,6 +444,14 @@ def main():
                 "avg_launch_ms": round(gemm_avg_ms, 4),
                 "algorithmic_bytes_per_launch": round(gemm_bytes / gemm_cnt)}
 
+    sweep = None
+    if not args.no_sweep:
+        sweep = bench_sweep(n, args.seed, A, b, torch, X, log)
+
+    batched = None
+    if not args.no_batched:
+        batched = bench_batched(args.batch_count, args.seed, torch, X, log)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_n, args.seed)
@@ -278,7 +476,7 @@ def main():
         "dgemm_n": gn, "dgemm_tflops": round(dgemm_tflops, 3),
         "dgemm_pct_peak": round(100 * dgemm_tflops * 1e12 / PEAK_FP64_MFMA, 2),
         "kernel_time_per_step": classes, "profiled_ms_per_step": round(tmp_prof * 1e3, 3),
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "sweep": sweep, "batched": batched,
     }
     print(json.dumps(out), flush=True)
 
