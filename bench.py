@@ -248,6 +248,58 @@ def bench_batched(count, seed, torch, X, log, reps=3):
     return res
 
 
+def bench_lapack(n, seed, torch, X, log, reps=3):
+    """The plain FP64 factorizations at order n: getrf (partial pivoting,
+    A ~ U[-1,1)) with getrs on 1 and 64 right-hand sides, potrf (Lower) with
+    potrs on 1 and 64, A = B B^T / n + I with B standard normal.  Device
+    operands; best of reps, each on a fresh copy of A."""
+    L = X.lib()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    A0 = torch.rand(n, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1
+    Bm = torch.randn(n, n, dtype=torch.float64, device="cuda", generator=g)
+    S0 = torch.empty(n, n, dtype=torch.float64, device="cuda")
+    L.atlmx_dgemm(X.NoTrans, X.Trans, n, n, n, 1.0 / n, X.dptr(Bm), n, X.dptr(Bm), n, 0.0,
+                  X.dptr(S0), n)
+    S0 += torch.eye(n, dtype=torch.float64, device="cuda")
+    del Bm
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    R = {1: torch.rand(1, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1,
+         64: torch.rand(64, n, dtype=torch.float64, device="cuda", generator=g) * 2 - 1}
+    Aw = torch.empty_like(A0)
+
+    def best(fn, prep):
+        ts = []
+        for _ in range(reps):
+            prep()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    out = {"n": n}
+    t = best(lambda: L.atlmx_dgetrf(n, n, X.dptr(Aw), n, X.dptr(ipiv)), lambda: Aw.copy_(A0))
+    fl = 2.0 * n ** 3 / 3 - n ** 2 / 2.0 - n / 6.0
+    out["getrf"] = {"ms": round(t * 1e3, 2), "gflops": round(fl / t / 1e9, 1)}
+    for nr, rhs in R.items():
+        Xw = torch.empty_like(rhs)
+        t = best(lambda: L.atlmx_dgetrs(X.NoTrans, n, nr, X.dptr(Aw), n, X.dptr(ipiv), X.dptr(Xw), n),
+                 lambda: Xw.copy_(rhs))
+        out["getrs_%d" % nr] = {"ms": round(t * 1e3, 3), "gflops": round(2.0 * n * n * nr / t / 1e9, 1)}
+    t = best(lambda: L.atlmx_dpotrf(X.Lower, n, X.dptr(Aw), n), lambda: Aw.copy_(S0))
+    fl = n ** 3 / 3.0 + n ** 2 / 2.0 + n / 6.0
+    out["potrf"] = {"ms": round(t * 1e3, 2), "gflops": round(fl / t / 1e9, 1)}
+    for nr, rhs in R.items():
+        Xw = torch.empty_like(rhs)
+        t = best(lambda: L.atlmx_dpotrs(X.Lower, n, nr, X.dptr(Aw), n, X.dptr(Xw), n),
+                 lambda: Xw.copy_(rhs))
+        out["potrs_%d" % nr] = {"ms": round(t * 1e3, 3), "gflops": round(2.0 * n * n * nr / t / 1e9, 1)}
+    log("lapack n=%d: %s" % (n, json.dumps(out)))
+    return out
+
+
 def bench_sweep(n, seed, A6, b6, torch, X, log):
     """The dense-solve configuration's other cases, one solve each: kappa
     1e2 and 1e4 (and the headline's 1e6) with the FP32 tournament LU and
@@ -444,6 +496,8 @@ def main():
                 "avg_launch_ms": round(gemm_avg_ms, 4),
                 "algorithmic_bytes_per_launch": round(gemm_bytes / gemm_cnt)}
 
+    lapack = bench_lapack(8192, args.seed + 8192, torch, X, log)
+
     sweep = None
     if not args.no_sweep:
         sweep = bench_sweep(n, args.seed, A, b, torch, X, log)
@@ -476,7 +530,8 @@ def main():
         "dgemm_n": gn, "dgemm_tflops": round(dgemm_tflops, 3),
         "dgemm_pct_peak": round(100 * dgemm_tflops * 1e12 / PEAK_FP64_MFMA, 2),
         "kernel_time_per_step": classes, "profiled_ms_per_step": round(tmp_prof * 1e3, 3),
-        "roofline": roofline, "cpu_baseline": cpu, "sweep": sweep, "batched": batched,
+        "roofline": roofline, "cpu_baseline": cpu, "lapack_n8192": lapack, "sweep": sweep,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 
