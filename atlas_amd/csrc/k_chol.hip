@@ -3,10 +3,10 @@
  *
  * Replaces the leaves of ATLAS's recursive Cholesky: the closed-form 4x4,
  * 3x3, 2x2 kernels (src/lapack/ATL_potrfL.c:104-202, ATL_potrfU.c:98-197)
- * and the small-N recursion above them.  The whole block is held in LDS by
- * one 256-thread workgroup and factored right-looking, one column at a
- * time: sqrt of the pivot, scale the column by its reciprocal, rank-1
- * update of the trailing triangle.  A non-positive (or NaN) pivot stops the
+ * and the small-N recursion above them.  The whole block is held in the
+ * registers of one 256-thread workgroup and factored right-looking, one
+ * column at a time: sqrt of the pivot, scale the column by its reciprocal,
+ * rank-1 update of the trailing triangle.  A non-positive (or NaN) pivot stops the
  * factorization and records the 1-based global column in *info, as
  * ATL_potrfL.c:238-240 does; later kernels of the same Cholesky see the
  * nonzero info and do nothing.  Only the referenced triangle is read or
@@ -27,57 +27,103 @@ __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
 template <>
 __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
 
+/* The block lives in registers: thread (ti, tj) of a 16 x 16 grid holds the
+ * 8 x 8 tile of rows ti*8.., columns tj*8.. (tiles above the diagonal stay
+ * idle).  Step k: the owner of the pivot takes its square root, the owners
+ * of column k scale it and broadcast it through LDS (two buffers, by the
+ * parity of k, so a step's writes never meet the previous step's reads),
+ * and every tile right of column k subtracts the rank-1 term in registers.
+ * The eight steps of a tile column are unrolled, so every register index is
+ * fixed at compile time. */
 template <typename T>
 __global__ __launch_bounds__(NT) void potf2_kernel(int lower, int n, T *A, long lda,
                                                    int info_base, int *info) {
   if (*info != 0) return;
-  /* Lt[i][j] holds the lower-triangular view: L = A (lower) or U^T (upper) */
-  __shared__ T Lt[CNB][CNB + 1];
+  __shared__ T col[2][CNB];
   __shared__ int s_stop;
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < n * n; idx += NT) {
-    const int j = idx / n, i = idx - j * n;
-    if (i >= j) Lt[i][j] = lower ? A[(size_t)j * lda + i] : A[(size_t)i * lda + j];
-  }
+  const int ti = tid >> 4, tj = tid & 15;
+  const bool own = ti >= tj;
+  T a[8][8];
+#pragma unroll
+  for (int r = 0; r < 8; ++r)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const int i = ti * 8 + r, j = tj * 8 + c;
+      a[r][c] = (own && i < n && j < n && i >= j)
+                    ? (lower ? A[(size_t)j * lda + i] : A[(size_t)i * lda + j])
+                    : T(0);
+    }
   if (tid == 0) s_stop = 0;
   __syncthreads();
   int done = n;
-  for (int k = 0; k < n; ++k) {
-    if (tid == 0) {
-      const T d = Lt[k][k];
-      if (!(d > T(0))) {
-        s_stop = 1;
-        atomicCAS(info, 0, info_base + k + 1);
-      } else {
-        Lt[k][k] = dsqrt(d);
+  const int nkt = (n + 7) / 8;
+  for (int kt = 0; kt < nkt && done == n; ++kt) {
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) {
+      const int k = kt * 8 + kc;
+      if (k >= n || done != n) break;
+      T *ck = col[k & 1];
+      /* the pivot */
+      if (ti == kt && tj == kt) {
+        const T d = a[kc][kc];
+        if (!(d > T(0))) {
+          s_stop = 1;
+          atomicCAS(info, 0, info_base + k + 1);
+        } else {
+          a[kc][kc] = dsqrt(d);
+          ck[k] = a[kc][kc];
+        }
+      }
+      __syncthreads();
+      if (s_stop) { done = k; break; }
+      /* column k below the pivot, scaled, to LDS */
+      if (tj == kt && own) {
+        const T rd = T(1) / ck[k];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const int i = ti * 8 + r;
+          if (i > k && i < n) {
+            a[r][kc] *= rd;
+            ck[i] = a[r][kc];
+          }
+        }
+      }
+      __syncthreads();
+      /* rank-1 update of the tiles right of column k */
+      if (own && tj >= kt) {
+        T x[8], y[8];
+#pragma unroll
+        for (int r = 0; r < 8; ++r) {
+          const int i = ti * 8 + r;
+          x[r] = (i > k && i < n) ? ck[i] : T(0);
+        }
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          const int j = tj * 8 + c;
+          y[c] = (j > k && j < n) ? ck[j] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+          for (int c = 0; c < 8; ++c) a[r][c] -= x[r] * y[c];
       }
     }
-    __syncthreads();
-    if (s_stop) { done = k; break; }
-    const T rd = T(1) / Lt[k][k];
-    for (int i = k + 1 + tid; i < n; i += NT) Lt[i][k] *= rd;
-    __syncthreads();
-    /* trailing update of the lower triangle, columns k+1..n-1 */
-    const int m = n - k - 1;
-    for (int idx = tid; idx < m * m; idx += NT) {
-      const int jj = idx / m, ii = idx - jj * m;
-      if (ii >= jj) {
-        const int i = k + 1 + ii, j = k + 1 + jj;
-        Lt[i][j] -= Lt[i][k] * Lt[j][k];
-      }
-    }
-    __syncthreads();
   }
   /* write back the factored columns (the whole block when it succeeded; on
    * failure the columns before the failing one, as the reference leaves
    * them factored) */
-  const int wcols = done;
-  for (int idx = tid; idx < n * n; idx += NT) {
-    const int j = idx / n, i = idx - j * n;
-    if (i >= j && j < wcols) {
-      if (lower) A[(size_t)j * lda + i] = Lt[i][j];
-      else A[(size_t)i * lda + j] = Lt[i][j];
-    }
+  if (own) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const int i = ti * 8 + r, j = tj * 8 + c;
+        if (i < n && j < n && i >= j && j < done) {
+          if (lower) A[(size_t)j * lda + i] = a[r][c];
+          else A[(size_t)i * lda + j] = a[r][c];
+        }
+      }
   }
 }
 

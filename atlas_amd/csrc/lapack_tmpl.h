@@ -9,6 +9,7 @@
 #define POTRF_DEV NM(PFX, potrf_dev)
 #define POTRS_DEV NM(PFX, potrs_dev)
 #define PANEL_REC NM(PFX, panel_rec)
+#define GETRF_UPDATE NM(PFX, getrf_update)
 #define LASWP_ROWS NM(PFX, laswp_rows)
 
 /* Width of the outer LU block (the K of the trailing GEMM) and of the
@@ -77,17 +78,36 @@ static void PANEL_REC(atlmx_ctx *c, int M, T *A, long lda, int r0, int w, int *i
   }
 }
 
-/* Blocked right-looking LU, A = P*L*U in place (column-major M x N).
+/* Blocked right-looking LU, A = P*L*U in place (column-major M x N), with
+ * one panel of lookahead: after panel j's interchanges, the next panel's
+ * columns get panel j's update first and that panel is factored on a second
+ * stream (its own workspace bank) while the rest of the trailing matrix gets
+ * the update on the thread's stream.  The panels still run one after
+ * another, so the pivots are those of the plain right-looking order.
  * d_ipiv: device, min(M,N) absolute 0-based pivot rows (clapack
  * convention, src/lapack/ATL_getrfC.c:299,325).  Returns info (1-based
  * first exactly-zero pivot; the factorization still completes,
  * interfaces/lapack/C/src/clapack_dgetrf.c:35-45).  Synchronizes. */
+static void GETRF_UPDATE(atlmx_ctx *c, int M, T *A, long lda, int j, int jb, int c0, int nc,
+                         const char *rout)
+{
+  const int jn = j + jb;
+  if (nc <= 0) return;
+  NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, jb, nc, (T)1,
+                    A + j + (size_t)j * lda, lda, A + j + (size_t)c0 * lda, lda, NULL, rout);
+  if (jn < M)
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - jn, nc, jb, (T)-1,
+                      A + jn + (size_t)j * lda, lda, A + j + (size_t)c0 * lda, lda, (T)1,
+                      A + jn + (size_t)c0 * lda, lda, 0, NULL, rout);
+}
+
 int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const char *rout)
 {
   const int MN = M < N ? M : N;
-  int j, info = 0, pid;
+  int j, info = 0;
   int *d_info;
   void *scratch;
+  hipStream_t main = c->stream, aux;
   if (MN <= 0) return 0;
   d_info = (int *)rt_ws(c, WS_INFO, 256, rout);
   scratch = rt_ws(c, WS_PANEL, atlmx_k_panel_scratch_bytes(M, LU_PW), rout);
@@ -96,22 +116,26 @@ int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const cha
     c->panel_buf = scratch;
     c->panel_set = 0;
   }
+  aux = rt_aux(c, rout);
   RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
-  (void)pid;
+  PANEL_REC(c, M, A, lda, 0, (MN < LU_NB) ? MN : LU_NB, d_ipiv, d_info, scratch, rout);
   for (j = 0; j < MN; j += LU_NB) {
     const int jb = (MN - j < LU_NB) ? MN - j : LU_NB;
-    PANEL_REC(c, M, A, lda, j, jb, d_ipiv, d_info, scratch, rout);
-    LASWP_ROWS(c, A, lda, M, j, j + jb, d_ipiv, 0, N, j, j + jb, rout);
-    if (j + jb < N) {
-      NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, jb, N - j - jb, (T)1,
-                        A + j + (size_t)j * lda, lda, A + j + (size_t)(j + jb) * lda, lda,
-                        NULL, rout);
-      if (j + jb < M)
-        NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - j - jb, N - j - jb, jb, (T)-1,
-                          A + (j + jb) + (size_t)j * lda, lda,
-                          A + j + (size_t)(j + jb) * lda, lda, (T)1,
-                          A + (j + jb) + (size_t)(j + jb) * lda, lda, 0, NULL, rout);
+    const int jn = j + jb;
+    const int jnb = (jn < MN) ? ((MN - jn < LU_NB) ? MN - jn : LU_NB) : 0;
+    LASWP_ROWS(c, A, lda, M, j, jn, d_ipiv, 0, N, j, jn, rout);
+    if (jn >= N) break;
+    if (jnb > 0) {
+      GETRF_UPDATE(c, M, A, lda, j, jb, jn, jnb, rout);
+      RT_CHECK(rout, hipEventRecord(c->ev[0], main));
+      RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
+      rt_on_aux(c, main, 1);
+      PANEL_REC(c, M, A, lda, jn, jnb, d_ipiv, d_info, scratch, rout);
+      rt_on_aux(c, main, 0);
+      RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
     }
+    GETRF_UPDATE(c, M, A, lda, j, jb, jn + jnb, N - jn - jnb, rout);
+    if (jnb > 0) RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
   }
   RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost,
                                 c->stream));
@@ -537,4 +561,5 @@ int NM3(clapack_, PFX, posv)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO
 #undef POTRF_DEV
 #undef POTRS_DEV
 #undef PANEL_REC
+#undef GETRF_UPDATE
 #undef LASWP_ROWS

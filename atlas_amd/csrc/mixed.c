@@ -255,9 +255,9 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm, i
     tslu_update(c, S, lds, k2, R, k2, n - k2, k, k2 - k, lp, d_info, main, rout);
     RT_CHECK(rout, hipEventRecord(c->ev[0], main));
     RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
-    c->stream = aux;
+    rt_on_aux(c, main, 1);
     tslu_pair_head(c, S, lds, n, k2, tws, &P, &Q, planPQ, perm, lp, d_info, rout);
-    c->stream = main;
+    rt_on_aux(c, main, 0);
     RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
     tslu_update(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, lp, d_info, main, rout);
     RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));

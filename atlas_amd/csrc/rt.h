@@ -37,8 +37,11 @@ typedef struct atlmx_ctx {
   int dev;
   hipStream_t stream;       /* stream every launch of this thread uses */
   hipStream_t own;          /* default stream of this thread (0: legacy default) */
-  void *ws[WS_NSLOTS];
-  size_t ws_bytes[WS_NSLOTS];
+  /* workspaces, one bank per stream: bank 1 serves work issued on aux, so
+   * helpers running on the two streams at once never share a buffer */
+  void *ws[2][WS_NSLOTS];
+  size_t ws_bytes[2][WS_NSLOTS];
+  int bank;
   void *panel_buf;          /* LU panel exchange buffer last initialised */
   int panel_set;            /* header set the next panel launch uses     */
   hipStream_t aux;          /* second stream for lookahead work (lazily created) */
@@ -60,6 +63,9 @@ int rt_is_device(const void *p);
 void *rt_ws(atlmx_ctx *c, int slot, size_t bytes, const char *rout);
 /* the context's second stream and its two events (created on first use) */
 hipStream_t rt_aux(atlmx_ctx *c, const char *rout);
+/* issue the following work on aux (on = 1, with its workspace bank) or
+ * back on the thread's stream (on = 0, returning that stream) */
+void rt_on_aux(atlmx_ctx *c, hipStream_t main, int on);
 void rt_sync(atlmx_ctx *c, const char *rout);
 
 /* Stage a rows x cols column-major operand.  A device pointer is returned

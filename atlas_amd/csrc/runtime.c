@@ -72,21 +72,28 @@ void rt_sync(atlmx_ctx *c, const char *rout)
 
 void *rt_ws(atlmx_ctx *c, int slot, size_t bytes, const char *rout)
 {
+  const int b = c->bank;
   if (bytes == 0) bytes = 256;
-  if (c->ws_bytes[slot] < bytes) {
+  if (c->ws_bytes[b][slot] < bytes) {
     size_t nb = bytes + bytes / 8;
     nb = (nb + 4095) & ~(size_t)4095;
-    if (c->ws[slot]) {
+    if (c->ws[b][slot]) {
       /* the old buffer may still be in use by queued work */
       RT_CHECK(rout, hipStreamSynchronize(c->stream));
-      RT_CHECK(rout, hipFree(c->ws[slot]));
-      c->ws[slot] = NULL;
-      c->ws_bytes[slot] = 0;
+      RT_CHECK(rout, hipFree(c->ws[b][slot]));
+      c->ws[b][slot] = NULL;
+      c->ws_bytes[b][slot] = 0;
     }
-    RT_CHECK(rout, hipMalloc(&c->ws[slot], nb));
-    c->ws_bytes[slot] = nb;
+    RT_CHECK(rout, hipMalloc(&c->ws[b][slot], nb));
+    c->ws_bytes[b][slot] = nb;
   }
-  return c->ws[slot];
+  return c->ws[b][slot];
+}
+
+void rt_on_aux(atlmx_ctx *c, hipStream_t main, int on)
+{
+  c->stream = on ? c->aux : main;
+  c->bank = on ? 1 : 0;
 }
 
 void *rt_stage_in(atlmx_ctx *c, int slot, const void *h, int rows, int cols, long ld,
