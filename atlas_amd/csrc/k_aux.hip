@@ -18,60 +18,97 @@ constexpr int TNB = ATLMX_TRI_NB;
  * block is inverted once, so the TRSM becomes a chain of MFMA GEMMs
  * (X_k = inv(A_kk) * B_k, then the trailing update).
  *
- * One wavefront per 64x64 block, four blocks per workgroup.  Lane j builds
- * column j of the inverse by substitution with the column held in
- * registers (the loops are fully unrolled); the triangle is read from LDS
- * as a broadcast.  Lanes j > i compute x_i = 0 automatically, so all lanes
- * run the same instruction stream.
+ * One 256-thread workgroup per 64x64 block.  An upper block is inverted as
+ * the transpose of the inverse of its (lower) transpose.  The four 16x16
+ * diagonal sub-blocks are inverted by substitution (lane = column), then
+ * two doubling levels fill in the off-diagonal parts,
+ *     inv([A 0; C B]) = [inv(A) 0; -inv(B) C inv(A) inv(B)],
+ * as two small products each, spread over all threads.  LDS holds the
+ * factor's strict lower triangle, the inverse's strict lower triangle
+ * transposed into the strict upper half of the same array, the inverse's
+ * diagonal, and the product's intermediate; every product loop runs over
+ * the nonzero (triangular) range only.
  * ---------------------------------------------------------------------- */
 template <typename T>
 __global__ __launch_bounds__(256) void tri_inv_kernel(int lower, int unit, int n,
                                                       const T *__restrict__ A, long lda,
                                                       T *__restrict__ inv,
                                                       const int *abort_flag) {
+  static_assert(TNB == 64, "tri_inv_kernel is written for 64x64 blocks");
   if (abort_flag && *abort_flag) return;
-  __shared__ T L[4][TNB][TNB + 1];
-  const int wv = threadIdx.x >> 6, j = threadIdx.x & 63;
-  const int nblk = (n + TNB - 1) / TNB;
-  const int blk = blockIdx.x * 4 + wv;
+  __shared__ T Ls[TNB][TNB + 1];   /* (i > j): L(i,j) at [i][j], X(i,j) at [j][i] */
+  __shared__ T dL[TNB], dX[TNB];
+  __shared__ T Tb[32][33];
+  const int tid = threadIdx.x;
+  const int blk = blockIdx.x;
   const int b0 = blk * TNB;
-  const int nb = (blk < nblk) ? min(TNB, n - b0) : 0;
-  /* coalesced load: lane = row */
-  for (int c = 0; c < TNB; ++c) {
-    const int i = j;
-    T v = T(0);
-    const bool in = lower ? (i >= c) : (i <= c);
-    if (i < nb && c < nb) {
-      if (in) v = A[(size_t)(b0 + c) * lda + (b0 + i)];
-      if (i == c && unit) v = T(1);
-    } else if (i == c) {
-      v = T(1);
-    }
-    L[wv][i][c] = in ? v : T(0);
+  const int nb = min(TNB, n - b0);
+  /* load: the block as the lower triangle L (L = U^T for an upper block);
+   * outside the n x n matrix the block is the identity */
+  for (int e = tid; e < TNB * TNB; e += 256) {
+    const int r = e & (TNB - 1), c = e >> 6;           /* element (r, c) of the block */
+    const int i = lower ? r : c, j = lower ? c : r;    /* its place in L */
+    if (i < j) continue;
+    T v;
+    if (r < nb && c < nb) v = A[(size_t)(b0 + c) * lda + (b0 + r)];
+    else v = (i == j) ? T(1) : T(0);
+    if (i == j) dL[i] = (unit && i < nb) ? T(1) : v;
+    else Ls[i][j] = v;
   }
   __syncthreads();
-  if (blk >= nblk) return;
-  T x[TNB];
-  if (lower) {
+  /* the 16x16 diagonal sub-blocks, by substitution: lane (q, c) builds
+   * column c of inv(L_qq) */
+  if (tid < 64) {
+    const int q = tid >> 4, c = tid & 15, o = 16 * q;
+    T x[16];
 #pragma unroll
-    for (int i = 0; i < TNB; ++i) {
-      T s = (i == j) ? T(1) : T(0);
+    for (int i = 0; i < 16; ++i) {
+      T s = (i == c) ? T(1) : T(0);
 #pragma unroll
-      for (int k = 0; k < i; ++k) s -= L[wv][i][k] * x[k];
-      x[i] = s / L[wv][i][i];
+      for (int k = 0; k < i; ++k) s -= Ls[o + i][o + k] * x[k];   /* x[k] = 0 for k < c */
+      x[i] = (i >= c) ? s / dL[o + i] : T(0);
     }
-  } else {
 #pragma unroll
-    for (int i = TNB - 1; i >= 0; --i) {
-      T s = (i == j) ? T(1) : T(0);
-#pragma unroll
-      for (int k = i + 1; k < TNB; ++k) s -= L[wv][i][k] * x[k];
-      x[i] = s / L[wv][i][i];
+    for (int i = 0; i < 16; ++i) {
+      if (i == c) dX[o + c] = x[i];
+      else if (i > c) Ls[o + c][o + i] = x[i];
     }
   }
-  T *out = inv + (size_t)blk * TNB * TNB + (size_t)j * TNB;
+  __syncthreads();
+  /* doubling: blocks of 2h from two blocks of h */
 #pragma unroll
-  for (int i = 0; i < TNB; ++i) out[i] = x[i];
+  for (int h = 16; h < TNB; h *= 2) {
+    const int nout = (TNB / (2 * h)) * h * h;   /* 512, then 1024 */
+    /* T = L21 * X11, L21 = L(r0.., c0..), X11 = X(c0.., c0..) lower */
+    for (int e = tid; e < nout; e += 256) {
+      const int p = e / (h * h), i = (e / h) % h, j = e % h;
+      const int c0 = 2 * h * p, r0 = c0 + h;
+      T s = Ls[r0 + i][c0 + j] * dX[c0 + j];
+      for (int k = j + 1; k < h; ++k) s += Ls[r0 + i][c0 + k] * Ls[c0 + j][c0 + k];
+      Tb[p * h + i][j] = s;
+    }
+    __syncthreads();
+    /* X21 = -X22 * T, X22 = X(r0.., r0..) lower */
+    for (int e = tid; e < nout; e += 256) {
+      const int p = e / (h * h), i = (e / h) % h, j = e % h;
+      const int c0 = 2 * h * p, r0 = c0 + h;
+      T s = dX[r0 + i] * Tb[p * h + i][j];
+      for (int k = 0; k < i; ++k) s += Ls[r0 + k][r0 + i] * Tb[p * h + k][j];
+      Ls[c0 + j][r0 + i] = -s;
+    }
+    __syncthreads();
+  }
+  /* store inv(A) column-major with leading dimension 64, zeros outside
+   * the triangle */
+  T *out = inv + (size_t)blk * TNB * TNB;
+  for (int e = tid; e < TNB * TNB; e += 256) {
+    const int r = e & (TNB - 1), c = e >> 6;
+    const int i = lower ? r : c, j = lower ? c : r;    /* inv(U) = inv(L)^T */
+    T v = T(0);
+    if (i == j) v = dX[i];
+    else if (i > j) v = Ls[j][i];
+    out[(size_t)c * TNB + r] = v;
+  }
 }
 
 template <typename T>
@@ -131,7 +168,7 @@ hipError_t launch_tri_inv(int uplo, int diag, int n, const T *A, long lda, T *in
   if (n <= 0) return hipSuccess;
   if (lda < n || !A || !inv) return hipErrorInvalidValue;
   const int nblk = (n + TNB - 1) / TNB;
-  hipLaunchKernelGGL(tri_inv_kernel<T>, dim3((nblk + 3) / 4), dim3(256), 0, s,
+  hipLaunchKernelGGL(tri_inv_kernel<T>, dim3(nblk), dim3(256), 0, s,
                      uplo == KX_LOWER, diag == KX_UNIT, n, A, lda, inv, abort_flag);
   return hipGetLastError();
 }

@@ -105,7 +105,10 @@ __device__ inline void load_row(f4 (&r)[TW / 4], const TsluArgs &a, int row) {
  * maximum (ties: lowest lane), its owner stores the key
  * rank << 32 | (TR - 1 - tid) and its row in the wavefront's slot, and
  * after one barrier every thread takes the largest key -- the largest
- * entry, ties to the lower thread -- and reads the winner's row.  Each
+ * entry, ties to the lower thread -- and reads the winner's row.  The
+ * multiplier is the entry times the hardware reciprocal of the pivot
+ * (within an ulp of the quotient), which keeps the IEEE division's
+ * dependent chain off every step.  Each
  * thread notes the step that chose its row; s_sel is written from those
  * after the loop (-1 where no row was left).  The multipliers are kept in
  * place, so a row retired at column c holds its L entries in r[0..c) and its
@@ -149,7 +152,7 @@ __device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[NWV][TW / 4]
     act = act && !me;
     const int bw = any ? (wt >> 6) : 0;
     const float piv = s_row[par][bw][c / 4][c % 4];
-    const float l = (act && piv != 0.0f) ? rc / piv : 0.0f;
+    const float l = (act && piv != 0.0f) ? rc * __builtin_amdgcn_rcpf(piv) : 0.0f;
     r[c / 4][c % 4] = act ? l : rc;
 #pragma unroll
     for (int q = c / 4; q < TW / 4; ++q) {

@@ -166,39 +166,76 @@ void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
 
 /* Blocked right-looking Cholesky (Lower: A = L*L^T, Upper: A = U^T*U).
  * Replaces the recursion of src/lapack/ATL_potrfL.c:204-262 /
- * ATL_potrfU.c:199-257: diagonal block in LDS (k_chol.hip), TRSM of the
- * panel, SYRK trailing update on the GEMM kernel restricted to one
- * triangle.  A non-positive pivot stops the factorization: the device info
- * flag makes every queued kernel after it return at once.  Synchronizes;
+ * ATL_potrfU.c:199-257: diagonal block in registers (k_chol.hip), TRSM of
+ * the panel, SYRK trailing update on the GEMM kernel restricted to one
+ * triangle.  One block of lookahead: step j first updates only the next
+ * block column (block row for Upper) from panel j, then factors that block
+ * and solves its panel on the second stream while the main stream applies
+ * panel j to the rest of the trailing matrix.  Every block sees the same
+ * updates in the same order as without lookahead, so the result is the
+ * same.  A non-positive pivot stops the factorization: the device info flag
+ * makes every queued kernel after it return at once.  Synchronizes;
  * returns info. */
+static void NM(PFX, potrf_panel)(atlmx_ctx *c, int uplo, int N, T *A, long lda, int j,
+                                 int jb, int *d_info, const char *rout)
+{
+  const int nr = N - j - jb;
+  T *Ad = A + j + (size_t)j * lda;
+  int pid = rt_prof_begin(c, PROF_POTF2, 0.0);
+  RT_CHECK(rout, K_POTF2(uplo, jb, Ad, lda, j, d_info, c->stream));
+  rt_prof_end(c, pid);
+  if (nr <= 0) return;
+  if (uplo == KX_LOWER)
+    NM(PFX, trsm_dev)(c, KX_RIGHT, KX_LOWER, KX_TRANS, KX_NONUNIT, nr, jb, (T)1, Ad, lda,
+                      A + (j + jb) + (size_t)j * lda, lda, d_info, rout);
+  else
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, jb, nr, (T)1, Ad, lda,
+                      A + j + (size_t)(j + jb) * lda, lda, d_info, rout);
+}
+
+/* Trailing update from panel j (columns/rows j..j+jb) of the part of the
+ * trailing matrix that starts at diagonal index r0 and is nc wide (Lower:
+ * rows r0..N-1 of columns r0..r0+nc-1; Upper: the transposed block row).
+ * The GEMM's triangle mask is in the block's own coordinates, which start
+ * on the diagonal, so only the referenced triangle is written. */
+static void NM(PFX, potrf_update)(atlmx_ctx *c, int uplo, int N, T *A, long lda, int j,
+                                  int jb, int r0, int nc, const int *d_info, const char *rout)
+{
+  if (nc <= 0) return;
+  if (uplo == KX_LOWER)
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_TRANS, N - r0, nc, jb, (T)-1,
+                      A + r0 + (size_t)j * lda, lda, A + r0 + (size_t)j * lda, lda, (T)1,
+                      A + r0 + (size_t)r0 * lda, lda, KX_LOWER, d_info, rout);
+  else
+    NM(PFX, gemm_dev)(c, KX_TRANS, KX_NOTRANS, nc, N - r0, jb, (T)-1,
+                      A + j + (size_t)r0 * lda, lda, A + j + (size_t)r0 * lda, lda, (T)1,
+                      A + r0 + (size_t)r0 * lda, lda, KX_UPPER, d_info, rout);
+}
+
 int POTRF_DEV(atlmx_ctx *c, int uplo, int N, T *A, long lda, const char *rout)
 {
-  int j, info = 0, pid;
+  int j, info = 0;
   int *d_info;
+  hipStream_t main = c->stream, aux;
   if (N <= 0) return 0;
   d_info = (int *)rt_ws(c, WS_INFO, 256, rout);
+  aux = rt_aux(c, rout);
   RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
+  NM(PFX, potrf_panel)(c, uplo, N, A, lda, 0, (N < CHOL_NB) ? N : CHOL_NB, d_info, rout);
   for (j = 0; j < N; j += CHOL_NB) {
     const int jb = (N - j < CHOL_NB) ? N - j : CHOL_NB;
-    const int nr = N - j - jb;
-    T *Ad = A + j + (size_t)j * lda;
-    pid = rt_prof_begin(c, PROF_POTF2, 0.0);
-    RT_CHECK(rout, K_POTF2(uplo, jb, Ad, lda, j, d_info, c->stream));
-    rt_prof_end(c, pid);
-    if (nr <= 0) break;
-    if (uplo == KX_LOWER) {
-      T *Ar = A + (j + jb) + (size_t)j * lda;
-      NM(PFX, trsm_dev)(c, KX_RIGHT, KX_LOWER, KX_TRANS, KX_NONUNIT, nr, jb, (T)1, Ad, lda,
-                        Ar, lda, d_info, rout);
-      NM(PFX, syrk_dev)(c, KX_LOWER, KX_NOTRANS, nr, jb, (T)-1, Ar, lda, (T)1,
-                        A + (j + jb) + (size_t)(j + jb) * lda, lda, d_info, rout);
-    } else {
-      T *Ac = A + j + (size_t)(j + jb) * lda;
-      NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, jb, nr, (T)1, Ad, lda,
-                        Ac, lda, d_info, rout);
-      NM(PFX, syrk_dev)(c, KX_UPPER, KX_TRANS, nr, jb, (T)-1, Ac, lda, (T)1,
-                        A + (j + jb) + (size_t)(j + jb) * lda, lda, d_info, rout);
-    }
+    const int jn = j + jb;
+    const int jnb = (jn < N) ? ((N - jn < CHOL_NB) ? N - jn : CHOL_NB) : 0;
+    if (jnb <= 0) break;
+    NM(PFX, potrf_update)(c, uplo, N, A, lda, j, jb, jn, jnb, d_info, rout);
+    RT_CHECK(rout, hipEventRecord(c->ev[0], main));
+    RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
+    rt_on_aux(c, main, 1);
+    NM(PFX, potrf_panel)(c, uplo, N, A, lda, jn, jnb, d_info, rout);
+    rt_on_aux(c, main, 0);
+    RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
+    NM(PFX, potrf_update)(c, uplo, N, A, lda, j, jb, jn + jnb, N - jn - jnb, d_info, rout);
+    RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
   }
   RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost,
                                 c->stream));

@@ -103,13 +103,15 @@ def test_potrf_llttst(uplo, N, lda):
     assert O.llt_resid(uplo, A[:N, :N], A0[:N, :N]) < 100
 
 
-def test_potrf_not_spd_info():
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+@pytest.mark.parametrize("bad", [0, 127, 128, 250, 399])
+def test_potrf_not_spd_info(uplo, bad):
     n = 400
-    S = O.lltgen(X.Lower, n)
-    S[250, 250] = -1.0
+    S = O.lltgen(uplo, n)
+    S[bad, bad] = -1.0
     A = S.copy(order="F")
-    assert X.potrf(A, X.Lower) == 251
-    assert O.potrf(S.copy(order="F"), O.LOWER) == 251
+    assert X.potrf(A, uplo) == bad + 1
+    assert O.potrf(S.copy(order="F"), O.LOWER if uplo == X.Lower else O.UPPER) == bad + 1
 
 
 @pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
