@@ -77,6 +77,8 @@ struct TsluArgs {
   int *info;        /* zero pivot: 1 + its column; nonzero on entry = skip */
   const int *prev;  /* the previous panel's move list, or null */
   int *plan2;       /* out (with prev): prev's moves followed by these, as one list */
+  float *Sw;        /* with sw_e1 > sw_e0: apply the moves to S's entries */
+  int sw_e0, sw_e1; /*   [sw_e0, sw_e1) and write the factored block, at the root */
 };
 
 /* row `row` of A, columns k..k+w-1 (zero beyond w, or for row < 0) */
@@ -169,7 +171,7 @@ __device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[NWV][TW / 4]
 
 /* inverses of the unit lower and the upper triangle of the factor in s_lu
  * (row-major, ld TW+1, the identity outside the leading w x w block) by
- * recursive doubling: for b = 1, 2, .., 32 each 2b x 2b diagonal block gets
+ * recursive doubling: for b = 16, 32 each 2b x 2b diagonal block gets
  * its off-diagonal block of the inverse, X21 = -X22 L21 X11 (lower) and
  * X12 = -X11 U12 X22 (upper), from the inverses of its two b x b halves.
  * The sums run over all b terms (the triangles' zeros make up the rest), so
@@ -215,19 +217,46 @@ __device__ __forceinline__ void tri_level(const float (*s_lu)[TW + 1], float (*s
   __syncthreads();
 }
 
+/* The 16 x 16 diagonal blocks first, by substitution (lane = one column of
+ * one block: threads 0..63 the unit lower blocks, 64..127 the upper ones,
+ * the upper diagonal applied as its reciprocal), then the doubling levels
+ * 16 and 32 above them. */
 __device__ void tri_inverses(const float (*s_lu)[TW + 1], int w, float (*s_xl)[TW + 1],
                              float (*s_xu)[TW + 1], float *s_t) {
   const int tid = threadIdx.x;
   for (int e = tid; e < TW * TW; e += TR) {
     const int i = e / TW, j = e % TW;
-    s_xl[i][j] = (i == j) ? 1.0f : 0.0f;
-    s_xu[i][j] = (i == j) ? ((i < w) ? 1.0f / s_lu[i][i] : 1.0f) : 0.0f;
+    s_xl[i][j] = 0.0f;
+    s_xu[i][j] = 0.0f;
   }
   __syncthreads();
-  tri_level<1>(s_lu, s_xl, s_xu, s_t);
-  tri_level<2>(s_lu, s_xl, s_xu, s_t);
-  tri_level<4>(s_lu, s_xl, s_xu, s_t);
-  tri_level<8>(s_lu, s_xl, s_xu, s_t);
+  if (tid < 128) {
+    const int q = (tid >> 4) & 3, cc = tid & 15, o = 16 * q;
+    float x[16];
+    if (tid < 64) {
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        float acc = (i == cc) ? 1.0f : 0.0f;
+#pragma unroll
+        for (int kk = 0; kk < i; ++kk) acc -= s_lu[o + i][o + kk] * x[kk];
+        x[i] = (i >= cc) ? acc : 0.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) s_xl[o + i][o + cc] = x[i];
+    } else {
+#pragma unroll
+      for (int i = 15; i >= 0; --i) {
+        const float d = (o + i < w) ? 1.0f / s_lu[o + i][o + i] : 1.0f;
+        float acc = (i == cc) ? 1.0f : 0.0f;
+#pragma unroll
+        for (int kk = i + 1; kk < 16; ++kk) acc -= s_lu[o + i][o + kk] * x[kk];
+        x[i] = (i <= cc) ? acc * d : 0.0f;
+      }
+#pragma unroll
+      for (int i = 0; i < 16; ++i) s_xu[o + i][o + cc] = x[i];
+    }
+  }
+  __syncthreads();
   tri_level<16>(s_lu, s_xl, s_xu, s_t);
   tri_level<32>(s_lu, s_xl, s_xu, s_t);
 }
@@ -385,6 +414,42 @@ __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
     __syncthreads();
     if (tid == 0) a.plan2[0] = s_aux[0];
   }
+  if (a.sw_e1 > a.sw_e0) {
+    /* the moves on S's entries [sw_e0, sw_e1) (at most 2 TW), in two halves
+     * of 64 entries (entries are independent): each thread holds 32
+     * (move, entry) values in registers, all loads of a half are issued
+     * together and complete before the barrier, then every destination is
+     * written.  The factored block then overwrites rows k..k+ws-1 at
+     * entries k..k+ws-1 (as tslu_swap_kernel with lu11). */
+    int *s_mv = reinterpret_cast<int *>(&s_xu[0][0]);   /* inverses already stored */
+    __syncthreads();
+    for (int t = tid; t < np; t += TR) {
+      s_mv[2 * t] = a.plan[2 + 2 * t];
+      s_mv[2 * t + 1] = a.plan[3 + 2 * t];
+    }
+    __syncthreads();
+    const int ne = a.sw_e1 - a.sw_e0;
+    for (int h = 0; h < ne; h += 64) {
+      float v[32];
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        const int idx = tid + TR * i, mv = idx >> 6, e = h + (idx & 63);
+        v[i] = (mv < np && e < ne) ? a.Sw[(size_t)s_mv[2 * mv + 1] * a.lds + a.sw_e0 + e]
+                                   : 0.0f;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < 32; ++i) {
+        const int idx = tid + TR * i, mv = idx >> 6, e = h + (idx & 63);
+        if (mv < np && e < ne) a.Sw[(size_t)s_mv[2 * mv] * a.lds + a.sw_e0 + e] = v[i];
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < ws * TW; t += TR) {
+      const int r = t / TW, e = t % TW;            /* row k+r of A, entry k+e */
+      if (e < ws) a.Sw[(size_t)(a.k + r) * a.lds + a.k + e] = s_lu[r][e];
+    }
+  }
 }
 
 /* S columns dst <- src (rows of A) for every move, over the entries
@@ -426,6 +491,8 @@ __global__ __launch_bounds__(256) void tslu_swap_kernel(float *S, long lds, int 
   __syncthreads();
   for (int mv = threadIdx.x >> 6; mv < np; mv += 4)
     if (live) S[(size_t)s_dst[mv] * lds + e] = buf[mv][j];
+  /* the block's factor replaces what the moves put into rows k..k+w-1 */
+  __syncthreads();
   if (lu11 && live && e >= k && e < k + w)
     for (int r = threadIdx.x >> 6; r < w; r += 4)   /* row k+r of A, entry e */
       S[(size_t)(k + r) * lds + e] = lu11[(size_t)(e - k) * TW + r];
@@ -648,11 +715,15 @@ extern "C" size_t atlmx_k_tslu_ws_ints(int n) {
   return (size_t)nodes * TW + (size_t)nodes + 2 + 4 * TW + 64;
 }
 
-extern "C" hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w,
-                                          int *ws, int *plan, int *perm, float *lu11,
-                                          float *invl, float *invu, int *info,
-                                          const int *prev, int *plan2, hipStream_t s) {
+extern "C" hipError_t atlmx_k_tslu_select_swap(float *S, long lds, int n, int k, int w,
+                                               int *ws, int *plan, int *perm, float *lu11,
+                                               float *invl, float *invu, int *info,
+                                               const int *prev, int *plan2, int sw_e0,
+                                               int sw_e1, hipStream_t s) {
   if (k >= n || w <= 0) return hipSuccess;
+  if (sw_e1 > sw_e0 && (sw_e0 < 0 || sw_e1 > n || sw_e0 > k || sw_e1 - sw_e0 > 2 * TW ||
+                        k + min(w, n - k) > sw_e1))
+    return hipErrorInvalidValue;
   if (w > TW || lds < n || (k & 3) || (lds & 3) || ((uintptr_t)S & 15) || !ws || !plan ||
       !perm || !lu11 || !invl || !invu || !info)
     return hipErrorInvalidValue;
@@ -670,8 +741,17 @@ extern "C" hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k
   }
   if (prev && !plan2) return hipErrorInvalidValue;
   a.plan = plan; a.perm = perm; a.prev = prev; a.plan2 = plan2;
+  a.Sw = S; a.sw_e0 = sw_e0; a.sw_e1 = sw_e1;
   hipLaunchKernelGGL(tslu_select_kernel, dim3(a.nleaves), dim3(TR), 0, s, a);
   return hipGetLastError();
+}
+
+extern "C" hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w,
+                                          int *ws, int *plan, int *perm, float *lu11,
+                                          float *invl, float *invu, int *info,
+                                          const int *prev, int *plan2, hipStream_t s) {
+  return atlmx_k_tslu_select_swap(const_cast<float *>(S), lds, n, k, w, ws, plan, perm, lu11,
+                                  invl, invu, info, prev, plan2, 0, 0, s);
 }
 
 extern "C" hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, int gbeg,

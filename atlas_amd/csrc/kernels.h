@@ -87,6 +87,15 @@ size_t atlmx_k_tslu_ws_ints(int n);
 hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w, int *ws,
                                int *plan, int *perm, float *lu11, float *invl, float *invu,
                                int *info, const int *prev, int *plan2, hipStream_t s);
+/* the same, and then, with sw_e1 > sw_e0 (sw_e0 <= k, k + w <= sw_e1,
+ * at most 2*ATLMX_TSLU_W entries), what atlmx_k_tslu_swap(S, lds, sw_e0,
+ * sw_e1, 0, 0, plan, k, w, lu11, ...) does: the moves on S's entries
+ * [sw_e0, sw_e1) and the factored block written over rows k..k+w-1 --
+ * done by the last workgroup, which holds the plan and the block. */
+hipError_t atlmx_k_tslu_select_swap(float *S, long lds, int n, int k, int w, int *ws,
+                                    int *plan, int *perm, float *lu11, float *invl,
+                                    float *invu, int *info, const int *prev, int *plan2,
+                                    int sw_e0, int sw_e1, hipStream_t s);
 /* the pair P (A columns k..k+63) and Q (the next 64, k + 128 <= n) after
  * both were factored within S rows k..k+127: for the S rows [0, k) and
  * [k+128, n), the row moves of plan (P's followed by Q's, as

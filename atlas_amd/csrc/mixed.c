@@ -115,15 +115,18 @@ typedef struct {
 } tslu_panel;
 
 /* choose the pivot rows of the panel at k (width w); with prev (the
- * previous panel), plan2 receives prev's moves followed by these */
+ * previous panel), plan2 receives prev's moves followed by these.  The
+ * moves on S's rows [e0, e1) (A's columns) and the factored block are
+ * applied by the same launch (e1 > e0). */
 static void tslu_select(atlmx_ctx *c, float *S, long lds, int n, int k, int w, int *tws,
                         const tslu_panel *p, int *perm, const tslu_panel *prev, int *plan2,
-                        const int *d_info, const char *rout)
+                        int e0, int e1, const int *d_info, const char *rout)
 {
   int pid = rt_prof_begin(c, PROF_PANEL, (double)(n - k) * w * sizeof(float));
-  RT_CHECK(rout, atlmx_k_tslu_select(S, lds, n, k, w, tws, p->plan, perm, p->lu11, p->invL,
-                                     p->invU, (int *)d_info, prev ? prev->plan : NULL, plan2,
-                                     c->stream));
+  RT_CHECK(rout, atlmx_k_tslu_select_swap(S, lds, n, k, w, tws, p->plan, perm, p->lu11,
+                                          p->invL, p->invU, (int *)d_info,
+                                          prev ? prev->plan : NULL, plan2, e0, e1,
+                                          c->stream));
   rt_prof_end(c, pid);
 }
 
@@ -177,14 +180,12 @@ static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *
   const int W = ATLMX_TSLU_W;
   const int wp = (n - k < W) ? n - k : W, kq = k + wp;
   const int wq = (n - kq < W) ? n - kq : W, k2 = kq + wq;
-  tslu_select(c, S, lds, n, k, wp, tws, P, perm, NULL, NULL, d_info, rout);
-  tslu_swap(c, S, lds, k, k2, 0, 0, k, wp, P, 1, d_info, rout);
+  tslu_select(c, S, lds, n, k, wp, tws, P, perm, NULL, NULL, k, k2, d_info, rout);
   tslu_l21(c, S, lds, n, k, wp, P, d_info, rout);
   if (wq == 0) return;
   tslu_u12(c, S, lds, k, wp, kq, k2, P, d_info, rout);
   tslu_update(c, S, lds, kq, wq, kq, n - kq, k, wp, lp, d_info, c->stream, rout);
-  tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, d_info, rout);
-  tslu_swap(c, S, lds, k, k2, 0, 0, kq, wq, Q, 1, d_info, rout);
+  tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, k, k2, d_info, rout);
   tslu_l21(c, S, lds, n, kq, wq, Q, d_info, rout);
 }
 

@@ -139,6 +139,42 @@ def test_tournament_choice_and_moves(torch, n, k, w):
     assert np.array_equal(Sh, want_rows)
 
 
+@pytest.mark.parametrize("n,k,e0,e1", [(700, 128, 128, 256), (700, 192, 128, 256),
+                                       (300, 256, 256, 300), (5000, 0, 0, 128)])
+def test_select_with_moves_matches_select_then_swap(torch, n, k, e0, e1):
+    """The tournament launch that also applies its moves to S's rows [e0, e1)
+    and writes the factored block leaves S, the plan, perm and the block
+    exactly as the tournament followed by atlmx_k_tslu_swap does."""
+    rng = np.random.default_rng(7 * n + k)
+    A = rng.uniform(-1, 1, (n, n))
+    w = min(64, n - k)
+    outs = []
+    for fused in (False, True):
+        lds = (n + 3) // 4 * 4
+        S = torch.zeros(n, lds, dtype=torch.float32, device="cuda")
+        S[:, :n] = torch.tensor(A.astype(np.float32), device="cuda")
+        L = X.lib()
+        L.atlmx_k_tslu_ws_ints.restype = C.c_size_t
+        ws = torch.zeros(L.atlmx_k_tslu_ws_ints(n), dtype=torch.int32, device="cuda")
+        plan = torch.zeros(2 + 4 * 64, dtype=torch.int32, device="cuda")
+        perm = torch.arange(n, dtype=torch.int32, device="cuda")
+        blk = torch.zeros(3, 64, 64, dtype=torch.float32, device="cuda")
+        info = torch.zeros(1, dtype=torch.int32, device="cuda")
+        args = (X.dptr(S), C.c_long(lds), n, k, w, X.dptr(ws), X.dptr(plan), X.dptr(perm),
+                X.dptr(blk[0]), X.dptr(blk[1]), X.dptr(blk[2]), X.dptr(info), None, None)
+        if fused:
+            _k("atlmx_k_tslu_select_swap", *args, e0, e1, None)
+        else:
+            _k("atlmx_k_tslu_select", *args, None)
+            _k("atlmx_k_tslu_swap", X.dptr(S), C.c_long(lds), e0, e1, 0, 0, X.dptr(plan), k, w,
+               X.dptr(blk[0]), X.dptr(info), None)
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy() for t in (S, plan, perm, blk, info)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert int(outs[0][4][0]) == 0
+
+
 def test_tournament_zero_pivot_sets_info(torch):
     n = 300
     A = np.random.default_rng(1).uniform(-1, 1, (n, n))
