@@ -25,42 +25,65 @@ __global__ void convert_kernel(int M, int N, const TI *__restrict__ src, long ld
 }
 
 /* R(:, 0:nrhs) = B - A*X, column-major, nrhs <= MP_MAXRHS.  Grid:
- * (ceil(M/256), ksplit); each block sums its K-slice for 256 rows into
- * part[ks][nrhs][M]; a second pass adds the slices in fixed order, so the
- * result is deterministic. */
+ * (ceil(M/512), ksplit); each thread takes two consecutive rows (one
+ * 16-byte load per column when A's columns are 16-byte aligned) over its
+ * block's K slice, with eight columns' loads in flight, and writes the two
+ * partial sums to part[ks][nrhs][M]; a second pass adds the slices in fixed
+ * order, so the result is deterministic.  NR = 1 is the one-right-hand-side
+ * case without per-column predicates. */
 constexpr int MAXRHS = MP_MAXRHS;
+template <int NR>
 __global__ __launch_bounds__(256) void gemv_part_kernel(int M, int K, int nrhs, int kchunk,
                                                         const double *__restrict__ A, long lda,
                                                         const double *__restrict__ X, long ldx,
                                                         double *__restrict__ part) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
+  const int i = (blockIdx.x * 256 + threadIdx.x) * 2;
   const int ks = blockIdx.y;
   const int k0 = ks * kchunk, k1 = min(K, k0 + kchunk);
-  __shared__ double xs[64][MAXRHS];
-  double acc[MAXRHS];
+  const bool vec = ((lda & 1) == 0) && (((uintptr_t)A & 15) == 0) && i + 1 < M;
+  __shared__ double xs[64][NR];
+  double acc0[NR], acc1[NR];
 #pragma unroll
-  for (int r = 0; r < MAXRHS; ++r) acc[r] = 0.0;
+  for (int r = 0; r < NR; ++r) acc0[r] = acc1[r] = 0.0;
   for (int kb = k0; kb < k1; kb += 64) {
     const int kn = min(64, k1 - kb);
     __syncthreads();
-    for (int t = threadIdx.x; t < 64 * MAXRHS; t += 256) {
-      const int kk = t / MAXRHS, r = t - kk * MAXRHS;
+    for (int t = threadIdx.x; t < 64 * NR; t += 256) {
+      const int kk = t / NR, r = t - kk * NR;
       xs[kk][r] = (kk < kn && r < nrhs) ? X[(size_t)r * ldx + kb + kk] : 0.0;
     }
     __syncthreads();
-    if (i < M) {
-      for (int kk = 0; kk < kn; ++kk) {
-        const double a = A[(size_t)(kb + kk) * lda + i];
+    if (i >= M) continue;
+    const double *p = A + (size_t)kb * lda + i;
+    if (vec && kn == 64) {
+#pragma unroll 8
+      for (int kk = 0; kk < 64; ++kk) {
+        const double2 a = *reinterpret_cast<const double2 *>(p + (size_t)kk * lda);
 #pragma unroll
-        for (int r = 0; r < MAXRHS; ++r)
-          if (r < nrhs) acc[r] += a * xs[kk][r];
+        for (int r = 0; r < NR; ++r) {
+          acc0[r] += a.x * xs[kk][r];
+          acc1[r] += a.y * xs[kk][r];
+        }
+      }
+    } else {
+      for (int kk = 0; kk < kn; ++kk) {
+        const double a0 = p[(size_t)kk * lda];
+        const double a1 = (i + 1 < M) ? p[(size_t)kk * lda + 1] : 0.0;
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          acc0[r] += a0 * xs[kk][r];
+          acc1[r] += a1 * xs[kk][r];
+        }
       }
     }
   }
   if (i < M)
 #pragma unroll
-    for (int r = 0; r < MAXRHS; ++r)
-      if (r < nrhs) part[((size_t)ks * nrhs + r) * M + i] = acc[r];
+    for (int r = 0; r < NR; ++r)
+      if (r < nrhs) {
+        part[((size_t)ks * nrhs + r) * M + i] = acc0[r];
+        if (i + 1 < M) part[((size_t)ks * nrhs + r) * M + i + 1] = acc1[r];
+      }
 }
 
 __global__ void gemv_reduce_kernel(int M, int nrhs, int ksplit, const double *__restrict__ part,
@@ -210,8 +233,13 @@ hipError_t atlmx_k_dresid(int M, int K, int nrhs, const double *A, long lda, con
   const int kchunk = (K + MP_KSPLIT - 1) / MP_KSPLIT;
   const int ksplit = K > 0 ? (K + kchunk - 1) / kchunk : 1;
   double *part = (double *)ws;
-  hipLaunchKernelGGL(gemv_part_kernel, dim3((M + 255) / 256, ksplit), dim3(256), 0, s, M, K,
-                     nrhs, kchunk > 0 ? kchunk : 1, A, lda, X, ldx, part);
+  const dim3 grid((M + 511) / 512, ksplit);
+  if (nrhs == 1)
+    hipLaunchKernelGGL(gemv_part_kernel<1>, grid, dim3(256), 0, s, M, K, nrhs,
+                       kchunk > 0 ? kchunk : 1, A, lda, X, ldx, part);
+  else
+    hipLaunchKernelGGL(gemv_part_kernel<MAXRHS>, grid, dim3(256), 0, s, M, K, nrhs,
+                       kchunk > 0 ? kchunk : 1, A, lda, X, ldx, part);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(gemv_reduce_kernel, dim3(grid_for((long)M * nrhs)), dim3(256), 0, s, M,

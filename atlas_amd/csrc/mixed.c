@@ -107,6 +107,28 @@ static void tslu_update(atlmx_ctx *c, float *S, long lds, int r0, int M, int c0,
   rt_prof_end(c, pid);
 }
 
+/* The trailing update that runs beside a pair head, as two launches when it
+ * is large: the first takes only as many 128 x 128 tiles as leave one
+ * workgroup slot free for each of the head's tournament leaves (the update
+ * kernel fits two workgroups per compute unit and then fills every
+ * register; a tournament workgroup fits beside one), so the tournament
+ * starts at once instead of waiting for update workgroups to retire; the
+ * second launch takes the remaining rows. */
+static void tslu_update_beside(atlmx_ctx *c, float *S, long lds, int r0, int M, int c0, int N,
+                               int k0, int K, int lp, int leaves, const int *d_info,
+                               hipStream_t st, const char *rout)
+{
+  const int tiles_n = (N + 127) / 128;
+  const int cap = 2 * c->ncu - leaves;
+  const int m1 = (tiles_n > 0 && cap > 0) ? (cap / tiles_n) * 128 : 0;
+  if (m1 > 0 && M > m1) {
+    tslu_update(c, S, lds, r0, m1, c0, N, k0, K, lp, d_info, st, rout);
+    tslu_update(c, S, lds, r0 + m1, M - m1, c0, N, k0, K, lp, d_info, st, rout);
+  } else {
+    tslu_update(c, S, lds, r0, M, c0, N, k0, K, lp, d_info, st, rout);
+  }
+}
+
 /* what a panel's tournament leaves for the later steps: the row moves, the
  * factored diagonal block and the inverses of its triangles */
 typedef struct {
@@ -260,7 +282,8 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm, i
     tslu_pair_head(c, S, lds, n, k2, tws, &P, &Q, planPQ, perm, lp, d_info, rout);
     rt_on_aux(c, main, 0);
     RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
-    tslu_update(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, lp, d_info, main, rout);
+    tslu_update_beside(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, lp,
+                       (n - k2 + 255) / 256, d_info, main, rout);
     RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
     k = k2;
   }

@@ -46,6 +46,7 @@ typedef struct atlmx_ctx {
   int panel_set;            /* header set the next panel launch uses     */
   hipStream_t aux;          /* second stream for lookahead work (lazily created) */
   hipEvent_t ev[2];         /* ordering events between stream and aux   */
+  int ncu;                  /* compute units of the device (set with aux) */
 } atlmx_ctx;
 
 atlmx_ctx *rt_ctx(void);
