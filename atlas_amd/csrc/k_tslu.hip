@@ -79,6 +79,7 @@ struct TsluArgs {
   int *plan2;       /* out (with prev): prev's moves followed by these, as one list */
   float *Sw;        /* with sw_e1 > sw_e0: apply the moves to S's entries */
   int sw_e0, sw_e1; /*   [sw_e0, sw_e1) and write the factored block, at the root */
+  int u12_r1;       /* > k + w: S rows [k + w, u12_r1) get U12 = Ut inv(L11)^T */
 };
 
 /* row `row` of A, columns k..k+w-1 (zero beyond w, or for row < 0) */
@@ -449,6 +450,43 @@ __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
       const int r = t / TW, e = t % TW;            /* row k+r of A, entry k+e */
       if (e < ws) a.Sw[(size_t)(a.k + r) * a.lds + a.k + e] = s_lu[r][e];
     }
+    if (a.u12_r1 > a.k + ws) {
+      /* U12 on the next panel's S rows [k + ws, u12_r1) (moved above):
+       * Ut(i, j) <- sum_t Ut(i, t) invL(j, t), a 4 x 4 tile per thread */
+      const int r0 = a.k + ws, nr = a.u12_r1 - r0;
+      float (*ut)[TW + 1] = s_lu;                  /* the factor is written out */
+      __syncthreads();
+      for (int t = tid; t < TW * TW; t += TR) {
+        const int i = t % TW, j = t / TW;          /* S row r0+i, column k+j */
+        ut[i][j] = (i < nr && j < ws) ? a.Sw[(size_t)(a.k + j) * a.lds + r0 + i] : 0.0f;
+      }
+      __syncthreads();
+      const int ti = tid >> 4, tj = tid & 15;
+      float acc[4][4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = 0.0f;
+#pragma unroll 4
+      for (int t = 0; t < TW; ++t) {
+        float u[4], v[4];
+#pragma unroll
+        for (int x = 0; x < 4; ++x) u[x] = ut[4 * ti + x][t];
+#pragma unroll
+        for (int y = 0; y < 4; ++y) v[y] = s_xl[4 * tj + y][t];
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+          for (int y = 0; y < 4; ++y) acc[x][y] += u[x] * v[y];
+      }
+#pragma unroll
+      for (int y = 0; y < 4; ++y)
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int i = 4 * ti + x, j = 4 * tj + y;
+          if (i < nr && j < ws) a.Sw[(size_t)(a.k + j) * a.lds + r0 + i] = acc[x][y];
+        }
+    }
   }
 }
 
@@ -719,10 +757,13 @@ extern "C" hipError_t atlmx_k_tslu_select_swap(float *S, long lds, int n, int k,
                                                int *ws, int *plan, int *perm, float *lu11,
                                                float *invl, float *invu, int *info,
                                                const int *prev, int *plan2, int sw_e0,
-                                               int sw_e1, hipStream_t s) {
+                                               int sw_e1, int u12_r1, hipStream_t s) {
   if (k >= n || w <= 0) return hipSuccess;
+  const int wk = min(w, n - k);
   if (sw_e1 > sw_e0 && (sw_e0 < 0 || sw_e1 > n || sw_e0 > k || sw_e1 - sw_e0 > 2 * TW ||
-                        k + min(w, n - k) > sw_e1))
+                        k + wk > sw_e1))
+    return hipErrorInvalidValue;
+  if (u12_r1 > k + wk && (sw_e1 <= sw_e0 || u12_r1 > sw_e1 || u12_r1 - (k + wk) > TW))
     return hipErrorInvalidValue;
   if (w > TW || lds < n || (k & 3) || (lds & 3) || ((uintptr_t)S & 15) || !ws || !plan ||
       !perm || !lu11 || !invl || !invu || !info)
@@ -741,7 +782,7 @@ extern "C" hipError_t atlmx_k_tslu_select_swap(float *S, long lds, int n, int k,
   }
   if (prev && !plan2) return hipErrorInvalidValue;
   a.plan = plan; a.perm = perm; a.prev = prev; a.plan2 = plan2;
-  a.Sw = S; a.sw_e0 = sw_e0; a.sw_e1 = sw_e1;
+  a.Sw = S; a.sw_e0 = sw_e0; a.sw_e1 = sw_e1; a.u12_r1 = u12_r1;
   hipLaunchKernelGGL(tslu_select_kernel, dim3(a.nleaves), dim3(TR), 0, s, a);
   return hipGetLastError();
 }
@@ -751,7 +792,7 @@ extern "C" hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k
                                           float *invl, float *invu, int *info,
                                           const int *prev, int *plan2, hipStream_t s) {
   return atlmx_k_tslu_select_swap(const_cast<float *>(S), lds, n, k, w, ws, plan, perm, lu11,
-                                  invl, invu, info, prev, plan2, 0, 0, s);
+                                  invl, invu, info, prev, plan2, 0, 0, 0, s);
 }
 
 extern "C" hipError_t atlmx_k_tslu_swap(float *S, long lds, int ebeg, int eend, int gbeg,

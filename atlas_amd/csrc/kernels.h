@@ -91,11 +91,14 @@ hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w, in
  * at most 2*ATLMX_TSLU_W entries), what atlmx_k_tslu_swap(S, lds, sw_e0,
  * sw_e1, 0, 0, plan, k, w, lu11, ...) does: the moves on S's entries
  * [sw_e0, sw_e1) and the factored block written over rows k..k+w-1 --
- * done by the last workgroup, which holds the plan and the block. */
+ * done by the last workgroup, which holds the plan and the block.  With
+ * u12_r1 > k + w (<= sw_e1, at most ATLMX_TSLU_W rows) it also forms U12
+ * on S rows [k + w, u12_r1): Ut = Ut inv(L11)^T over S columns k..k+w-1,
+ * as the in-place GEMM with invl would. */
 hipError_t atlmx_k_tslu_select_swap(float *S, long lds, int n, int k, int w, int *ws,
                                     int *plan, int *perm, float *lu11, float *invl,
                                     float *invu, int *info, const int *prev, int *plan2,
-                                    int sw_e0, int sw_e1, hipStream_t s);
+                                    int sw_e0, int sw_e1, int u12_r1, hipStream_t s);
 /* the pair P (A columns k..k+63) and Q (the next 64, k + 128 <= n) after
  * both were factored within S rows k..k+127: for the S rows [0, k) and
  * [k+128, n), the row moves of plan (P's followed by Q's, as

@@ -142,12 +142,12 @@ typedef struct {
  * applied by the same launch (e1 > e0). */
 static void tslu_select(atlmx_ctx *c, float *S, long lds, int n, int k, int w, int *tws,
                         const tslu_panel *p, int *perm, const tslu_panel *prev, int *plan2,
-                        int e0, int e1, const int *d_info, const char *rout)
+                        int e0, int e1, int u12_r1, const int *d_info, const char *rout)
 {
   int pid = rt_prof_begin(c, PROF_PANEL, (double)(n - k) * w * sizeof(float));
   RT_CHECK(rout, atlmx_k_tslu_select_swap(S, lds, n, k, w, tws, p->plan, perm, p->lu11,
                                           p->invL, p->invU, (int *)d_info,
-                                          prev ? prev->plan : NULL, plan2, e0, e1,
+                                          prev ? prev->plan : NULL, plan2, e0, e1, u12_r1,
                                           c->stream));
   rt_prof_end(c, pid);
 }
@@ -178,17 +178,6 @@ static void tslu_l21(atlmx_ctx *c, float *S, long lds, int n, int k, int w,
               Lt, lds, 0, d_info, rout);
 }
 
-/* U12 = inv(L11) A12 for the panel at k, restricted to A's columns (S rows)
- * [r0, r1): Ut = Ut inv(L11)^T, in place as above */
-static void tslu_u12(atlmx_ctx *c, float *S, long lds, int k, int w, int r0, int r1,
-                     const tslu_panel *p, const int *d_info, const char *rout)
-{
-  float *Ut = S + r0 + (size_t)k * lds;
-  if (r1 > r0)
-    sgemm_dev(c, KX_NOTRANS, KX_TRANS, r1 - r0, w, w, 1.0f, Ut, lds, p->invL, ATLMX_TSLU_W,
-              0.0f, Ut, lds, 0, d_info, rout);
-}
-
 /* the pair of panels P at k and Q after it, confined to the pair's own S
  * rows [k, k2) (A's columns): P's pivots, its moves and factored block on
  * those rows, its L21 and its U12 on Q's rows; Q's columns get P's update;
@@ -202,12 +191,12 @@ static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *
   const int W = ATLMX_TSLU_W;
   const int wp = (n - k < W) ? n - k : W, kq = k + wp;
   const int wq = (n - kq < W) ? n - kq : W, k2 = kq + wq;
-  tslu_select(c, S, lds, n, k, wp, tws, P, perm, NULL, NULL, k, k2, d_info, rout);
+  /* P's tournament also moves the pair's rows and forms U12 on Q's rows */
+  tslu_select(c, S, lds, n, k, wp, tws, P, perm, NULL, NULL, k, k2, k2, d_info, rout);
   tslu_l21(c, S, lds, n, k, wp, P, d_info, rout);
   if (wq == 0) return;
-  tslu_u12(c, S, lds, k, wp, kq, k2, P, d_info, rout);
   tslu_update(c, S, lds, kq, wq, kq, n - kq, k, wp, lp, d_info, c->stream, rout);
-  tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, k, k2, d_info, rout);
+  tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, k, k2, 0, d_info, rout);
   tslu_l21(c, S, lds, n, kq, wq, Q, d_info, rout);
 }
 

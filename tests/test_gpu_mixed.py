@@ -144,7 +144,9 @@ def test_tournament_choice_and_moves(torch, n, k, w):
 def test_select_with_moves_matches_select_then_swap(torch, n, k, e0, e1):
     """The tournament launch that also applies its moves to S's rows [e0, e1)
     and writes the factored block leaves S, the plan, perm and the block
-    exactly as the tournament followed by atlmx_k_tslu_swap does."""
+    exactly as the tournament followed by atlmx_k_tslu_swap does; where S
+    rows [k + w, e1) remain, it also forms U12 = Ut inv(L11)^T there (FP32
+    sums, checked against the product of the unfused path's Ut and inverse)."""
     rng = np.random.default_rng(7 * n + k)
     A = rng.uniform(-1, 1, (n, n))
     w = min(64, n - k)
@@ -163,14 +165,27 @@ def test_select_with_moves_matches_select_then_swap(torch, n, k, e0, e1):
         args = (X.dptr(S), C.c_long(lds), n, k, w, X.dptr(ws), X.dptr(plan), X.dptr(perm),
                 X.dptr(blk[0]), X.dptr(blk[1]), X.dptr(blk[2]), X.dptr(info), None, None)
         if fused:
-            _k("atlmx_k_tslu_select_swap", *args, e0, e1, None)
+            _k("atlmx_k_tslu_select_swap", *args, e0, e1, e1, None)
         else:
             _k("atlmx_k_tslu_select", *args, None)
             _k("atlmx_k_tslu_swap", X.dptr(S), C.c_long(lds), e0, e1, 0, 0, X.dptr(plan), k, w,
                X.dptr(blk[0]), X.dptr(info), None)
         torch.cuda.synchronize()
         outs.append([t.cpu().numpy() for t in (S, plan, perm, blk, info)])
-    for a, b in zip(*outs):
+    S0, S1 = outs[0][0], outs[1][0]
+    r0 = k + w
+    if e1 > r0:
+        # S rows [r0, e1) at S columns k..k+w-1 (A's rows k..k+w-1): U12
+        Ut = S0[k:k + w, r0:e1].T.astype(np.float64)          # (rows, t)
+        iL = outs[0][3][1].T.astype(np.float64)[:w, :w]       # inv(L11)(j, t)
+        want = Ut @ iL.T
+        got = S1[k:k + w, r0:e1].T.astype(np.float64)
+        scale = np.abs(Ut) @ np.abs(iL).T
+        assert np.all(np.abs(got - want) <= 64 * 2.0**-23 * scale + 1e-30)
+        S0 = S0.copy(); S1 = S1.copy()
+        S0[k:k + w, r0:e1] = 0; S1[k:k + w, r0:e1] = 0
+    assert np.array_equal(S0, S1)
+    for a, b in zip(outs[0][1:], outs[1][1:]):
         assert np.array_equal(a, b)
     assert int(outs[0][4][0]) == 0
 
