@@ -490,6 +490,100 @@ __global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
   }
 }
 
+/* A value as the 16-bit update operand it becomes (round to nearest even):
+ * lp = 1 bfloat16, lp = 2 IEEE half, else unchanged. */
+__device__ inline float as_lp(float x, int lp) {
+  if (lp == 1) {
+    const unsigned u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return __uint_as_float((u | 0x400000u) & 0xffff0000u);
+    return __uint_as_float(((u + 0x7fffu + ((u >> 16) & 1u)) >> 16) << 16);
+  }
+  if (lp == 2) return (float)(_Float16)x;
+  return x;
+}
+
+/* After the pair's first panel P (S rows k..k+63) is chosen and its block
+ * factored: for the S columns [kq, n) (A's rows below P's block), 64 per
+ * workgroup, L21 = inv(U11)^T Lt on S rows k..k+63, then the second panel's
+ * S rows [kq, kq + wq) -= U12 L21 (U12 already formed on those rows at S
+ * columns k..k+63), both 64 x 64 x 64 products in registers (a 4 x 4 tile
+ * per thread).  A workgroup touches only its own columns, so the two steps
+ * need no ordering between workgroups.  lp != 0 rounds the update's
+ * operands to 16 bits as the low-precision update kernel does. */
+__global__ __launch_bounds__(256) void tslu_l21up_kernel(float *S, long lds, int n, int k,
+                                                         int wq, const float *invu, int lp,
+                                                         const int *info) {
+  __shared__ float s_iu[TW][TW + 1];   /* inv(U11)(t, r) at [t][r] */
+  __shared__ float s_b[TW][TW + 1];    /* Lt, then L21: [t][column] */
+  __shared__ float s_u[TW][TW + 1];    /* U12 on the second panel's rows: [i][t] */
+  if (*info) return;
+  const int kq = k + TW;
+  const int j0 = kq + blockIdx.x * 64, nc = min(64, n - j0);
+  const int tid = threadIdx.x, ti = tid >> 4, tj = tid & 15;
+  for (int e = tid; e < TW * TW; e += 256) {
+    const int t = e % TW, c = e / TW;
+    s_iu[t][c] = invu[(size_t)c * TW + t];
+    s_b[t][c] = (c < nc) ? S[(size_t)(j0 + c) * lds + k + t] : 0.0f;
+    s_u[c][t] = (c < wq) ? as_lp(S[(size_t)(k + t) * lds + kq + c], lp) : 0.0f;
+  }
+  __syncthreads();
+  float acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = 0.0f;
+#pragma unroll 4
+  for (int t = 0; t < TW; ++t) {
+    float u[4], v[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) u[x] = s_iu[t][4 * ti + x];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) v[y] = s_b[t][4 * tj + y];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) acc[x][y] += u[x] * v[y];
+  }
+  __syncthreads();                       /* every Lt value read: s_b takes L21 */
+#pragma unroll
+  for (int y = 0; y < 4; ++y)
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int r = 4 * ti + x, c = 4 * tj + y;
+      if (c < nc) S[(size_t)(j0 + c) * lds + k + r] = acc[x][y];
+      s_b[r][c] = as_lp(acc[x][y], lp);
+    }
+  __syncthreads();
+  float cv[4][4];
+#pragma unroll
+  for (int y = 0; y < 4; ++y)
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int i = 4 * ti + x, c = 4 * tj + y;
+      cv[x][y] = (i < wq && c < nc) ? S[(size_t)(j0 + c) * lds + kq + i] : 0.0f;
+      acc[x][y] = 0.0f;
+    }
+#pragma unroll 4
+  for (int t = 0; t < TW; ++t) {
+    float u[4], v[4];
+#pragma unroll
+    for (int x = 0; x < 4; ++x) u[x] = s_u[4 * ti + x][t];
+#pragma unroll
+    for (int y = 0; y < 4; ++y) v[y] = s_b[t][4 * tj + y];
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+      for (int y = 0; y < 4; ++y) acc[x][y] += u[x] * v[y];
+  }
+#pragma unroll
+  for (int y = 0; y < 4; ++y)
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int i = 4 * ti + x, c = 4 * tj + y;
+      if (i < wq && c < nc) S[(size_t)(j0 + c) * lds + kq + i] = cv[x][y] - acc[x][y];
+    }
+}
+
 /* S columns dst <- src (rows of A) for every move, over the entries
  * [ebeg, eend) of each column except the gap [gbeg, gbeg + glen): each
  * workgroup moves a 64-entry slice of all listed columns through LDS, so
@@ -840,5 +934,16 @@ extern "C" hipError_t atlmx_k_tslu_gather(int n, int nrhs, const double *X, long
   if (ldx < n || ldy < n || nrhs > 65535) return hipErrorInvalidValue;
   hipLaunchKernelGGL(tslu_gather_kernel, dim3((n + 255) / 256, nrhs), dim3(256), 0, s, n, nrhs,
                      X, ldx, perm, Y, ldy);
+  return hipGetLastError();
+}
+
+extern "C" hipError_t atlmx_k_tslu_l21up(float *S, long lds, int n, int k, int wq,
+                                         const float *invu, int lp, const int *info,
+                                         hipStream_t s) {
+  const int kq = k + TW;
+  if (kq >= n || wq <= 0) return hipSuccess;
+  if (wq > TW || kq + wq > n || lds < n || !S || !invu || !info) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tslu_l21up_kernel, dim3((n - kq + 63) / 64), dim3(256), 0, s, S, lds, n, k,
+                     wq, invu, lp, info);
   return hipGetLastError();
 }

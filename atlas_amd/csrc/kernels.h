@@ -99,6 +99,15 @@ hipError_t atlmx_k_tslu_select_swap(float *S, long lds, int n, int k, int w, int
                                     int *plan, int *perm, float *lu11, float *invl,
                                     float *invu, int *info, const int *prev, int *plan2,
                                     int sw_e0, int sw_e1, int u12_r1, hipStream_t s);
+/* after the pair's first panel P at k (64 wide) is factored on its own S
+ * rows and U12 formed on the second panel's wq S rows [k+64, k+64+wq): for
+ * every S column in [k+64, n), L21 = inv(U11)^T Lt on S rows k..k+63 (invu:
+ * P's inv(U11), ATLMX_TSLU_W x ATLMX_TSLU_W column-major), then those wq
+ * rows -= U12 L21, with the update's operands rounded to bfloat16 (lp = 1)
+ * or half (lp = 2) as atlmx_k_supdate_lp does.  One launch in place of the
+ * in-place L21 GEMM and the narrow update. */
+hipError_t atlmx_k_tslu_l21up(float *S, long lds, int n, int k, int wq, const float *invu,
+                              int lp, const int *info, hipStream_t s);
 /* the pair P (A columns k..k+63) and Q (the next 64, k + 128 <= n) after
  * both were factored within S rows k..k+127: for the S rows [0, k) and
  * [k+128, n), the row moves of plan (P's followed by Q's, as

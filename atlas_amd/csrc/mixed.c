@@ -178,6 +178,18 @@ static void tslu_l21(atlmx_ctx *c, float *S, long lds, int n, int k, int w,
               Lt, lds, 0, d_info, rout);
 }
 
+/* P's L21 on S rows k..k+W-1 and P's update of Q's S rows [k+W, k+W+wq),
+ * over every S column from k+W, in one launch */
+static void tslu_l21up(atlmx_ctx *c, float *S, long lds, int n, int k, int wq,
+                       const tslu_panel *p, int lp, const int *d_info, const char *rout)
+{
+  const double m = (double)(n - k - ATLMX_TSLU_W);
+  int pid = rt_prof_begin2(c, PROF_GEMM, 2.0 * m * ATLMX_TSLU_W * (ATLMX_TSLU_W + wq),
+                           (m * (2.0 * ATLMX_TSLU_W + 2.0 * wq)) * sizeof(float));
+  RT_CHECK(rout, atlmx_k_tslu_l21up(S, lds, n, k, wq, p->invU, lp, d_info, c->stream));
+  rt_prof_end(c, pid);
+}
+
 /* the pair of panels P at k and Q after it, confined to the pair's own S
  * rows [k, k2) (A's columns): P's pivots, its moves and factored block on
  * those rows, its L21 and its U12 on Q's rows; Q's columns get P's update;
@@ -193,9 +205,11 @@ static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *
   const int wq = (n - kq < W) ? n - kq : W, k2 = kq + wq;
   /* P's tournament also moves the pair's rows and forms U12 on Q's rows */
   tslu_select(c, S, lds, n, k, wp, tws, P, perm, NULL, NULL, k, k2, k2, d_info, rout);
-  tslu_l21(c, S, lds, n, k, wp, P, d_info, rout);
-  if (wq == 0) return;
-  tslu_update(c, S, lds, kq, wq, kq, n - kq, k, wp, lp, d_info, c->stream, rout);
+  if (wq == 0) {
+    tslu_l21(c, S, lds, n, k, wp, P, d_info, rout);
+    return;
+  }
+  tslu_l21up(c, S, lds, n, k, wq, P, lp, d_info, rout);   /* L21(P), Q's rows updated */
   tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, k, k2, 0, d_info, rout);
   tslu_l21(c, S, lds, n, kq, wq, Q, d_info, rout);
 }

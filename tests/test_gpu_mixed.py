@@ -190,6 +190,57 @@ def test_select_with_moves_matches_select_then_swap(torch, n, k, e0, e1):
     assert int(outs[0][4][0]) == 0
 
 
+def _as_lp(x, lp):
+    """FP32 values as the 16-bit update operands they become (round to
+    nearest even): lp 1 bfloat16, lp 2 IEEE half."""
+    x = np.asarray(x, np.float32)
+    if lp == 1:
+        u = x.view(np.uint32).astype(np.uint64)
+        u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16) << 16
+        return (u & 0xFFFFFFFF).astype(np.uint32).view(np.float32)
+    if lp == 2:
+        return x.astype(np.float16).astype(np.float32)
+    return x
+
+
+@pytest.mark.parametrize("n,k,wq,lp", [(300, 64, 64, 0), (200, 64, 8, 0), (130, 0, 64, 0),
+                                       (300, 64, 64, 1), (300, 64, 40, 2)])
+def test_l21_and_second_panel_update_vs_numpy(torch, n, k, wq, lp):
+    """One launch forms P's L21 = inv(U11)^T Lt over every S column from
+    k + 64 and subtracts U12 L21 from the second panel's wq S rows, with the
+    update's operands rounded to 16 bits for lp = 1, 2 (checked against
+    numpy on the kernel's own L21, so a rounding boundary cannot flip)."""
+    rng = np.random.default_rng(n + k + wq + 10 * lp)
+    lds = (n + 3) // 4 * 4
+    S0 = np.zeros((n, lds), np.float32)              # S0[col][row]: column-major S
+    S0[:, :n] = rng.uniform(-1, 1, (n, n)).astype(np.float32)
+    iU = np.triu(rng.uniform(-1, 1, (64, 64))).astype(np.float32) + 2 * np.eye(64, dtype=np.float32)
+    S = torch.tensor(S0, device="cuda")
+    invu = torch.tensor(np.ascontiguousarray(iU.T), device="cuda")   # column-major iU
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    _k("atlmx_k_tslu_l21up", X.dptr(S), C.c_long(lds), n, k, wq, X.dptr(invu), lp,
+       X.dptr(info), None)
+    torch.cuda.synchronize()
+    S1 = S.cpu().numpy()
+    kq = k + 64
+    Lt = S0[kq:n, k:kq].T.astype(np.float64)           # (t, column)
+    want = iU.T.astype(np.float64) @ Lt
+    got = S1[kq:n, k:kq].T.astype(np.float64)
+    scale = np.abs(iU.T).astype(np.float64) @ np.abs(Lt)
+    assert np.all(np.abs(got - want) <= 64 * 2.0**-23 * scale + 1e-30)
+    U12 = _as_lp(S0[k:kq, kq:kq + wq].T, lp).astype(np.float64)     # (i, t)
+    L21 = _as_lp(S1[kq:n, k:kq].T, lp).astype(np.float64)           # (t, column)
+    Cw = S0[kq:n, kq:kq + wq].T.astype(np.float64) - U12 @ L21
+    Cg = S1[kq:n, kq:kq + wq].T.astype(np.float64)
+    sc = np.abs(U12) @ np.abs(L21) + np.abs(S0[kq:n, kq:kq + wq].T)
+    assert np.all(np.abs(Cg - Cw) <= 64 * 2.0**-23 * sc + 1e-30)
+    # nothing else moved
+    mask = np.ones_like(S0, bool)
+    mask[kq:n, k:kq] = False
+    mask[kq:n, kq:kq + wq] = False
+    assert np.array_equal(S1[mask], S0[mask])
+
+
 def test_tournament_zero_pivot_sets_info(torch):
     n = 300
     A = np.random.default_rng(1).uniform(-1, 1, (n, n))
