@@ -198,7 +198,7 @@ static void tslu_l21up(atlmx_ctx *c, float *S, long lds, int n, int k, int wq,
  * below. */
 static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *tws,
                            const tslu_panel *P, const tslu_panel *Q, int *planPQ, int *perm,
-                           int lp, const int *d_info, const char *rout)
+                           int lp, hipEvent_t evq, const int *d_info, const char *rout)
 {
   const int W = ATLMX_TSLU_W;
   const int wp = (n - k < W) ? n - k : W, kq = k + wp;
@@ -207,10 +207,14 @@ static void tslu_pair_head(atlmx_ctx *c, float *S, long lds, int n, int k, int *
   tslu_select(c, S, lds, n, k, wp, tws, P, perm, NULL, NULL, k, k2, k2, d_info, rout);
   if (wq == 0) {
     tslu_l21(c, S, lds, n, k, wp, P, d_info, rout);
+    if (evq) RT_CHECK(rout, hipEventRecord(evq, c->stream));
     return;
   }
   tslu_l21up(c, S, lds, n, k, wq, P, lp, d_info, rout);   /* L21(P), Q's rows updated */
   tslu_select(c, S, lds, n, kq, wq, tws, Q, perm, P, planPQ, k, k2, 0, d_info, rout);
+  /* what the pair's tail needs is ready here; Q's L21 is needed only by the
+   * next update */
+  if (evq) RT_CHECK(rout, hipEventRecord(evq, c->stream));
   tslu_l21(c, S, lds, n, kq, wq, Q, d_info, rout);
 }
 
@@ -248,6 +252,8 @@ static void tslu_pair_tail(atlmx_ctx *c, float *S, long lds, int n, int k,
  *     block, L21, U12 where Q needs it, P's update of Q's columns;
  *   tail (tslu_pair_tail): the moves on the other rows, U12 there, Q's
  *     columns of those rows updated by P.
+ * The pair's tail waits for the head only through Q's tournament (Q's
+ * L21, which only the next update reads, overlaps it).
  * The next pair's rows then get this pair's update (K = 2W), and its head
  * runs on a second stream while the rest of the trailing matrix gets the
  * same update on the first: the tournaments leave the critical path.  In
@@ -272,24 +278,28 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm, i
   int info = 0, k = 0;
   RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, main));
   RT_CHECK(rout, atlmx_k_fill_i32(tws, (int)wsi, 0, main));
-  tslu_pair_head(c, S, lds, n, 0, tws, &P, &Q, planPQ, perm, lp, d_info, rout);
+  tslu_pair_head(c, S, lds, n, 0, tws, &P, &Q, planPQ, perm, lp, NULL, d_info, rout);
   for (;;) {
     const int k2 = (n - k <= 2 * W) ? n : k + 2 * W;
     const int R = (n - k2 < 2 * W) ? n - k2 : 2 * W;
     tslu_pair_tail(c, S, lds, n, k, &P, &Q, planPQ, d_info, rout);
     if (k2 >= n) break;
+    /* the head of this pair ran on aux (but the first): its last launch,
+     * Q's L21, overlapped the tail above; the update below reads it */
+    if (k > 0) RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
     tslu_update(c, S, lds, k2, R, k2, n - k2, k, k2 - k, lp, d_info, main, rout);
     RT_CHECK(rout, hipEventRecord(c->ev[0], main));
     RT_CHECK(rout, hipStreamWaitEvent(aux, c->ev[0], 0));
     rt_on_aux(c, main, 1);
-    tslu_pair_head(c, S, lds, n, k2, tws, &P, &Q, planPQ, perm, lp, d_info, rout);
+    tslu_pair_head(c, S, lds, n, k2, tws, &P, &Q, planPQ, perm, lp, c->ev[2], d_info, rout);
     rt_on_aux(c, main, 0);
     RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
     tslu_update_beside(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, lp,
                        (n - k2 + 255) / 256, d_info, main, rout);
-    RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
+    RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[2], 0));   /* head through Q's choice */
     k = k2;
   }
+  if (k > 0) RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[1], 0));
   RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, main));
   rt_sync(c, rout);
   return info;

@@ -45,7 +45,7 @@ typedef struct atlmx_ctx {
   void *panel_buf;          /* LU panel exchange buffer last initialised */
   int panel_set;            /* header set the next panel launch uses     */
   hipStream_t aux;          /* second stream for lookahead work (lazily created) */
-  hipEvent_t ev[2];         /* ordering events between stream and aux   */
+  hipEvent_t ev[3];         /* ordering events between stream and aux   */
   int ncu;                  /* compute units of the device (set with aux) */
 } atlmx_ctx;
 

@@ -61,6 +61,7 @@ hipStream_t rt_aux(atlmx_ctx *c, const char *rout)
     RT_CHECK(rout, hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
     RT_CHECK(rout, hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming));
     RT_CHECK(rout, hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming));
+    RT_CHECK(rout, hipEventCreateWithFlags(&c->ev[2], hipEventDisableTiming));
     RT_CHECK(rout, hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount,
                                          c->dev));
   }
