@@ -445,6 +445,74 @@ __device__ void tile_gemm(T *C, long ldc, const T *A, long ar, long ak, const T 
     }
 }
 
+/* The adjoint's two triangular solves for n <= NT, in registers: the
+ * right-hand sides are independent, so a block of ADJ_CB of them is solved
+ * at a time with one unknown per thread.  Step i (from the last) has the
+ * owner of unknown i divide by L(i,i) and publish its ADJ_CB values through
+ * LDS (two buffers by the parity of i), then every thread t < i subtracts
+ * L(i, t) times them: one barrier per step.  L(i, t) is loaded ADJ_PF
+ * steps ahead into a ring of registers (the steps run in unrolled groups of
+ * ADJ_PF, so every ring index is fixed).
+ *   lt = 1: Y = L^{-T} W (W's columns are the right-hand sides; thread t
+ *           holds row t of a block of columns);
+ *   lt = 0: S = W L^{-1} (W's rows are; thread t holds column t of a block
+ *           of rows).
+ * Both read L(i, t) = L[t*lda + i] at step i. */
+constexpr int ADJ_CB = 16;
+constexpr int ADJ_PF = 8;
+
+template <typename T>
+__device__ void wg_adj_solve(int n, const T *L, long lda, T *W, int lt,
+                             T (*s_y)[ADJ_CB]) {
+  const int t = threadIdx.x;
+  for (int b0 = 0; b0 < n; b0 += ADJ_CB) {
+    const int bb = min(ADJ_CB, n - b0);
+    T y[ADJ_CB];
+#pragma unroll
+    for (int q = 0; q < ADJ_CB; ++q)
+      y[q] = (t < n && q < bb) ? (lt ? W[(size_t)(b0 + q) * n + t] : W[(size_t)t * n + b0 + q])
+                               : T(0);
+    T ring[ADJ_PF];
+#pragma unroll
+    for (int q = 0; q < ADJ_PF; ++q) {
+      const int i = n - 1 - q;
+      ring[q] = (i >= 0 && t < i) ? L[(size_t)t * lda + i] : T(0);
+    }
+    for (int i0 = n - 1; i0 >= 0; i0 -= ADJ_PF) {
+#pragma unroll
+      for (int q = 0; q < ADJ_PF; ++q) {
+        const int i = i0 - q;
+        if (i >= 0) {
+          const T li = ring[q];
+          const int ip = i - ADJ_PF;               /* the step this slot serves next */
+          ring[q] = (ip >= 0 && t < ip) ? L[(size_t)t * lda + ip] : T(0);
+          T *buf = s_y[i & 1];
+          if (t == i) {
+            const T rd = T(1) / L[(size_t)i * lda + i];
+#pragma unroll
+            for (int r = 0; r < ADJ_CB; ++r) {
+              y[r] *= rd;
+              buf[r] = y[r];
+            }
+          }
+          __syncthreads();
+          if (t < i) {
+#pragma unroll
+            for (int r = 0; r < ADJ_CB; ++r) y[r] -= li * buf[r];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < ADJ_CB; ++q)
+      if (t < n && q < bb) {
+        if (lt) W[(size_t)(b0 + q) * n + t] = y[q];
+        else W[(size_t)t * n + b0 + q] = y[q];
+      }
+    __syncthreads();
+  }
+}
+
 /* Given L (lower triangle of A after potrf; its strict upper triangle is
  * overwritten with zeros here), X = A^{-1} B and Bb = A^{-1} Xb (the solve's
  * adjoint), write to G the gradient of <Xb, X> with respect to the lower
@@ -458,7 +526,8 @@ __device__ void tile_gemm(T *C, long ldc, const T *A, long ar, long ak, const T 
  * n x n scratch (leading dimension n). */
 template <typename T>
 __device__ void wg_chol_adjoint(int n, T *L, long lda, int nrhs, const T *Xs, const T *Bb,
-                                long ldx, T *G, long ldg, T *W, T *s_a, T *s_b) {
+                                long ldx, T *G, long ldg, T *W, T *s_a, T *s_b,
+                                T (*s_y)[ADJ_CB]) {
   const int tid = threadIdx.x;
   const int nt = (n + TS - 1) / TS;
   /* W = -(Bb X^T + X Bb^T); G = 0; strict upper of L = 0 */
@@ -496,6 +565,10 @@ __device__ void wg_chol_adjoint(int n, T *L, long lda, int nrhs, const T *Xs, co
     else if (i == j) W[(size_t)j * n + i] *= T(0.5);
   }
   __syncthreads();
+  if (n <= NT) {
+    wg_adj_solve<T>(n, L, lda, W, 1, s_y);   /* Y = L^{-T} P */
+    wg_adj_solve<T>(n, L, lda, W, 0, s_y);   /* S = Y L^{-1} */
+  } else {
   /* Y = L^{-T} P in place: rows from the last; Y(i,:) /= L(i,i), then rows
    * k < i lose L(i, k) Y(i,:) */
   for (int i = n - 1; i >= 0; --i) {
@@ -519,6 +592,7 @@ __device__ void wg_chol_adjoint(int n, T *L, long lda, int nrhs, const T *Xs, co
       W[(size_t)jp * n + r] -= W[(size_t)j * n + r] * L[(size_t)jp * lda + j];
     }
     __syncthreads();
+  }
   }
   /* G = tril(S + S^T) - diag(S) */
   for (int e = tid; e < n * n; e += NT) {
@@ -690,6 +764,7 @@ struct AdjArgs {
 template <typename T>
 __global__ __launch_bounds__(NT) void chol_adjoint_kernel(AdjArgs<T> g) {
   __shared__ T s_a[16 * TS], s_b[16 * TS];
+  __shared__ T s_y[2][ADJ_CB];
   __shared__ int s_item;
   for (;;) {
     __syncthreads();
@@ -699,7 +774,7 @@ __global__ __launch_bounds__(NT) void chol_adjoint_kernel(AdjArgs<T> g) {
     if (it >= g.count) return;
     const int i = g.order[it];
     wg_chol_adjoint<T>(g.n[i], g.L[i], g.lda[i], g.nrhs, g.X[i], g.Bb[i], g.ldx[i], g.G[i],
-                       g.ldg[i], g.W[i], s_a, s_b);
+                       g.ldg[i], g.W[i], s_a, s_b, s_y);
   }
 }
 
