@@ -155,9 +155,12 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   hipStream_t s = 0;
   unsigned long long seed = 0x5DEECE66DULL;
   if (!sn || !offA || !hA) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
-  if (p->kind == BK_CHOLADJ) {    /* one configuration: a workgroup per CU */
+  if (p->kind == BK_CHOLADJ) {
+    /* one configuration: two workgroups per CU (184 registers and 16 KB of
+     * LDS each), so one's barrier-bound triangular solves overlap the
+     * other's */
     p->nb = 0;
-    p->grid = cus;
+    p->grid = 2 * cus;
     free(sn); free(offA); free(hA);
     return;
   }
