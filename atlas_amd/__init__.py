@@ -75,6 +75,28 @@ def _declare(L):
         f = getattr(L, "clapack_%sposv" % p)
         f.argtypes = [_I, _I, _I, _I, _P, _I, _P, _I]
         f.restype = _I
+        f = getattr(L, "cblas_%strmm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%ssymm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, t, _P, _I, _P, _I, t, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%ssyr2k" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, t, _P, _I, _P, _I, t, _P, _I]
+        f.restype = None
+        f = getattr(L, "clapack_%sgetri" % p)
+        f.argtypes = [_I, _I, _P, _I, _P]
+        f.restype = _I
+        for nm in ("potri", "lauum"):
+            f = getattr(L, "clapack_%s%s" % (p, nm))
+            f.argtypes = [_I, _I, _I, _P, _I]
+            f.restype = _I
+        f = getattr(L, "clapack_%sgeqrf" % p)
+        f.argtypes = [_I, _I, _I, _P, _I, _P]
+        f.restype = _I
+        f = getattr(L, "clapack_%strtri" % p)
+        f.argtypes = [_I, _I, _I, _I, _P, _I]
+        f.restype = _I
         # device-pointer API
         f = getattr(L, "atlmx_%sgemm" % p)
         f.argtypes = [_I, _I, _I, _I, _I, t, _P, _I, _P, _I, t, _P, _I]
@@ -176,6 +198,51 @@ def potrf(A, uplo=Lower):
 def potrs(L, B, uplo=Lower):
     return getattr(lib(), "clapack_%spotrs" % _pre(B.dtype))(
         ColMajor, uplo, L.shape[0], B.shape[1], _ptr(L), _ld(L), _ptr(B), _ld(B))
+
+
+def trmm(side, uplo, ta, diag, alpha, A, B):
+    M, N = B.shape
+    getattr(lib(), "cblas_%strmm" % _pre(B.dtype))(
+        ColMajor, side, uplo, ta, diag, M, N, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B))
+    return B
+
+
+def symm(side, uplo, alpha, A, B, beta, Cm):
+    M, N = Cm.shape
+    getattr(lib(), "cblas_%ssymm" % _pre(Cm.dtype))(
+        ColMajor, side, uplo, M, N, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(Cm),
+        _ld(Cm))
+    return Cm
+
+
+def syr2k(uplo, trans, alpha, A, B, beta, Cm):
+    N = Cm.shape[0]
+    K = A.shape[1] if trans == NoTrans else A.shape[0]
+    getattr(lib(), "cblas_%ssyr2k" % _pre(Cm.dtype))(
+        ColMajor, uplo, trans, N, K, alpha, _ptr(A), _ld(A), _ptr(B), _ld(B), beta, _ptr(Cm),
+        _ld(Cm))
+    return Cm
+
+
+def trtri(A, uplo=Lower, diag=NonUnit):
+    return getattr(lib(), "clapack_%strtri" % _pre(A.dtype))(ColMajor, uplo, diag, A.shape[0],
+                                                             _ptr(A), _ld(A))
+
+
+def lauum(A, uplo=Lower):
+    return getattr(lib(), "clapack_%slauum" % _pre(A.dtype))(ColMajor, uplo, A.shape[0],
+                                                             _ptr(A), _ld(A))
+
+
+def potri(A, uplo=Lower):
+    return getattr(lib(), "clapack_%spotri" % _pre(A.dtype))(ColMajor, uplo, A.shape[0],
+                                                             _ptr(A), _ld(A))
+
+
+def getri(LU, ipiv):
+    ipiv = np.ascontiguousarray(ipiv, dtype=np.int32)
+    return getattr(lib(), "clapack_%sgetri" % _pre(LU.dtype))(ColMajor, LU.shape[0], _ptr(LU),
+                                                              _ld(LU), _ptr(ipiv))
 
 
 # --------------------------------------------------------------------------

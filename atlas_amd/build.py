@@ -20,8 +20,10 @@ LIB = os.path.join(HERE, "libatlas_mi355x.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 
-C_SOURCES = ["runtime.c", "xerbla.c", "blas3.c", "lapack.c", "mixed.c", "batched.c"]
-HIP_SOURCES = ["k_gemm.hip", "k_aux.hip", "k_lu.hip", "k_chol.hip", "k_mp.hip", "k_trsv.hip", "k_tslu.hip", "k_batched.hip"]
+C_SOURCES = ["runtime.c", "xerbla.c", "blas3.c", "lapack.c", "mixed.c", "batched.c",
+             "f77.c"]
+HIP_SOURCES = ["k_gemm.hip", "k_aux.hip", "k_lu.hip", "k_chol.hip", "k_mp.hip", "k_trsv.hip", "k_tslu.hip", "k_batched.hip",
+               "k_qr.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-function",
           "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
@@ -57,7 +59,12 @@ def build(verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs +
+    # -Bsymbolic-functions: calls between this library's own entry points (the F77
+    # layer into cblas_/clapack_) bind here, whatever else the process has loaded;
+    # the error handlers in interposable.list stay replaceable by the program
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-Wl,-Bsymbolic-functions",
+          "-Wl,--dynamic-list=" + os.path.join(CSRC, "interposable.list"),
+          "-o", tmp] + objs +
          ["-lpthread", "-lm"])
     os.replace(tmp, LIB)
     return LIB

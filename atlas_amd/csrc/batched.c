@@ -21,6 +21,7 @@
  * buckets), so later plans of a similar shape skip the timing.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -52,6 +53,9 @@ struct atlmx_batch {
 typedef struct { int kind, dbl, nrhs, b_max, b_mean, nb, grid; } at_entry;
 static at_entry at_cache[AT_CACHE];
 static int at_n = 0;
+/* plans may be made from any host thread: the cache is read and extended
+ * under this lock (the timing itself runs outside it) */
+static pthread_mutex_t at_lock = PTHREAD_MUTEX_INITIALIZER;
 
 static int cu_count(void)
 {
@@ -170,16 +174,19 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   mean /= p->count;
   b_max = bucket(p->maxn);
   b_mean = bucket(mean);
+  pthread_mutex_lock(&at_lock);
   for (i = 0; i < at_n; ++i) {
     const at_entry *a = &at_cache[i];
     if (a->kind == p->kind && a->dbl == p->dbl && a->nrhs == p->nrhs && a->b_max == b_max &&
         a->b_mean == b_mean) {
       p->nb = a->nb;
       p->grid = a->grid;
+      pthread_mutex_unlock(&at_lock);
       free(sn); free(offA); free(hA);
       return;
     }
   }
+  pthread_mutex_unlock(&at_lock);
 
   /* sample evenly from the ordered batch; sizes and leading dimensions */
   for (i = 0; i < ns; ++i) {
@@ -296,10 +303,12 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   if (ipool) RT_CHECK(rout, hipFree(ipool));
   if (hipiv) RT_CHECK(rout, hipFree(hipiv));
   free(sn); free(offA); free(hA);
+  pthread_mutex_lock(&at_lock);
   if (at_n < AT_CACHE) {
     at_entry a = {p->kind, p->dbl, p->nrhs, b_max, b_mean, p->nb, p->grid};
     at_cache[at_n++] = a;
   }
+  pthread_mutex_unlock(&at_lock);
 }
 
 static atlmx_batch *plan_create2(int kind, int dbl, int count, int ta, int tb, const int *h_m,
@@ -330,10 +339,28 @@ static atlmx_batch *plan_create2(int kind, int dbl, int count, int ta, int tb, c
                                  void *const *C, const int *h_ldc, int *const *ipiv, int nrhs,
                                  int *info, void *const *D, void *const *E, const char *rout)
 {
-  atlmx_batch *p = (atlmx_batch *)calloc(1, sizeof(*p));
+  atlmx_batch *p;
   double *cost;
   int *h, i, t;
   const size_t nmeta = (size_t)count * (kind == BK_GEMM ? 8 : 5) + 1;
+  /* refuse orders the factor kernels cannot hold in LDS before touching
+   * the device: a NULL plan and a message, not a failed launch later */
+  if (kind == BK_POSV || kind == BK_GESV || kind == BK_GESVT) {
+    const int lim = atlmx_k_batch_max_order(kind == BK_POSV ? 0 : 1, dbl);
+    for (i = 0; i < count; ++i)
+      if (h_n[i] < 0 || h_n[i] > lim) {
+        fprintf(stderr, "%s: instance %d has order %d; this plan holds orders 0..%d\n",
+                rout, i, h_n[i], lim);
+        return NULL;
+      }
+  }
+  if (kind == BK_CHOLADJ)
+    for (i = 0; i < count; ++i)
+      if (h_n[i] < 0) {
+        fprintf(stderr, "%s: instance %d has order %d\n", rout, i, h_n[i]);
+        return NULL;
+      }
+  p = (atlmx_batch *)calloc(1, sizeof(*p));
   if (!p) { fprintf(stderr, "%s: out of host memory\n", rout); exit(-1); }
   p->kind = kind; p->dbl = dbl; p->count = count; p->nrhs = nrhs;
   p->A = A; p->B = B; p->C = C; p->ipiv = ipiv; p->info = info; p->D = D; p->E = E;
@@ -541,4 +568,9 @@ atlmx_batch *atlmx_sposv_adjoint_batch_plan(int count, const int *h_n, float *co
   return plan_create2(BK_CHOLADJ, 0, count, 0, 0, NULL, h_n, NULL, 0.0, (void *const *)d_L,
                       h_lda, (void *const *)d_X, h_ldx, 0.0, (void *const *)d_Bb, h_ldg, NULL,
                       nrhs, NULL, (void *const *)d_G, (void *const *)d_W, rout);
+}
+
+int atlmx_batch_max_order(int kind, int dbl)
+{
+  return atlmx_k_batch_max_order(kind == 0 ? 0 : 1, dbl ? 1 : 0);
 }

@@ -19,6 +19,7 @@
 #define K_SCAL atlmx_k_dscal2d
 #define K_IAMAX atlmx_k_idamax
 #define K_TRSV atlmx_k_dtrsv
+#define K_TRICOPY atlmx_k_dtricopy
 #include "blas3_tmpl.h"
 #undef T
 #undef PFX
@@ -27,6 +28,7 @@
 #undef K_SCAL
 #undef K_IAMAX
 #undef K_TRSV
+#undef K_TRICOPY
 
 #define T float
 #define PFX s
@@ -35,4 +37,5 @@
 #define K_SCAL atlmx_k_sscal2d
 #define K_IAMAX atlmx_k_isamax
 #define K_TRSV atlmx_k_strsv
+#define K_TRICOPY atlmx_k_stricopy
 #include "blas3_tmpl.h"

@@ -48,9 +48,12 @@ void GEMM_DEV(atlmx_ctx *c, int ta, int tb, int M, int N, int K, T alpha, const 
  * writing it) and one GEMM update of the rest of B.  Left-side solves with
  * at most ATLMX_TRSV_MAXRHS columns run as one flag-chained launch.
  * ---------------------------------------------------------------------- */
-void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, T alpha,
-              const T *A, long lda, T *B, long ldb, const int *abort_flag,
-              const char *rout)
+/* inv_ws / ctl_ws: NULL, or caller-provided workspaces of at least
+ * nblk*64*64 elements / atlmx_k_trsv_ctl_ints(na, T is double) ints (the
+ * allocation-free entry points pass slices of their static array). */
+void NM(PFX, trsm_dev_ws)(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
+                          T alpha, const T *A, long lda, T *B, long ldb,
+                          const int *abort_flag, T *inv_ws, int *ctl_ws, const char *rout)
 {
   const int NB = ATLMX_TRI_NB;
   const int na = (side == KX_LEFT) ? M : N;
@@ -65,7 +68,8 @@ void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, 
     RT_CHECK(rout, K_SCAL(M, N, alpha, B, ldb, c->stream));
     if (alpha == (T)0) return;
   }
-  inv = (T *)rt_ws(c, WS_TRI_INV, (size_t)nblk * NB * NB * sizeof(T), rout);
+  inv = inv_ws ? inv_ws
+               : (T *)rt_ws(c, WS_TRI_INV, (size_t)nblk * NB * NB * sizeof(T), rout);
   pid = rt_prof_begin(c, PROF_TRSM_INV, (double)na * NB * sizeof(T));
   RT_CHECK(rout, K_TRI_INV(uplo, diag, na, A, lda, inv, abort_flag, c->stream));
   rt_prof_end(c, pid);
@@ -74,7 +78,7 @@ void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, 
     /* few right-hand sides: the solve is bound by reading the factor once,
      * so one flag-chained launch replaces 2*nblk small GEMMs */
     const size_t nctl = atlmx_k_trsv_ctl_ints(na, sizeof(T) == sizeof(double));
-    int *ctl = (int *)rt_ws(c, WS_TRSV, nctl * sizeof(int), rout);
+    int *ctl = ctl_ws ? ctl_ws : (int *)rt_ws(c, WS_TRSV, nctl * sizeof(int), rout);
     RT_CHECK(rout, atlmx_k_fill_i32(ctl, (int)nctl, 0, c->stream));
     pid = rt_prof_begin(c, PROF_TRSV, (double)na * na / 2 * sizeof(T));
     RT_CHECK(rout, K_TRSV(uplo, tinv, M, N, A, lda, inv, B, ldb, ctl, c->stream));
@@ -119,6 +123,14 @@ void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, 
   }
 }
 
+void TRSM_DEV(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, T alpha,
+              const T *A, long lda, T *B, long ldb, const int *abort_flag,
+              const char *rout)
+{
+  NM(PFX, trsm_dev_ws)(c, side, uplo, ta, diag, M, N, alpha, A, lda, B, ldb, abort_flag, NULL,
+                       NULL, rout);
+}
+
 /* ------------------------------------------------------------------------
  * SYRK: C = alpha*op(A)*op(A)^T + beta*C on one triangle, as the GEMM
  * kernel with the triangle mask (tiles wholly outside the triangle exit at
@@ -133,6 +145,77 @@ void SYRK_DEV(atlmx_ctx *c, int uplo, int trans, int N, int K, T alpha, const T 
   else
     GEMM_DEV(c, KX_TRANS, KX_NOTRANS, N, N, K, alpha, A, lda, A, lda, beta, C, ldc, uplo,
              abort_flag, rout);
+}
+
+/* ------------------------------------------------------------------------
+ * TRMM: B = alpha*op(A)*B (Left) or alpha*B*op(A) (Right), A triangular, in
+ * place.  Replaces src/blas/ulevel3/ATL_trmm.c:149 (recursive halving on
+ * GEMM).  Here A's triangle is expanded into a dense workspace (zeros
+ * outside, ones on a unit diagonal) and B copied aside, so the product is
+ * one launch of the MFMA GEMM over the whole of B (twice the triangle's
+ * flops, at the GEMM's rate).
+ * ---------------------------------------------------------------------- */
+void NM(PFX, trmm_dev)(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
+                       T alpha, const T *A, long lda, T *B, long ldb, const char *rout)
+{
+  const int na = (side == KX_LEFT) ? M : N;
+  T *Tm, *W;
+  if (M <= 0 || N <= 0) return;
+  if (alpha == (T)0) {                  /* ATL_dreftrmm.c:182: B = 0 */
+    RT_CHECK(rout, K_SCAL(M, N, (T)0, B, ldb, c->stream));
+    return;
+  }
+  Tm = (T *)rt_ws(c, WS_L3A, (size_t)na * na * sizeof(T), rout);
+  W = (T *)rt_ws(c, WS_L3B, (size_t)M * N * sizeof(T), rout);
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_EXPAND_TRI, uplo, diag, na, na, (T *)A, lda, Tm, na,
+                           c->stream));
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_COPY, 0, 0, M, N, W, M, B, ldb, c->stream));
+  if (side == KX_LEFT)
+    GEMM_DEV(c, ta, KX_NOTRANS, M, N, M, alpha, Tm, na, W, M, (T)0, B, ldb, 0, NULL, rout);
+  else
+    GEMM_DEV(c, KX_NOTRANS, ta, M, N, N, alpha, W, M, Tm, na, (T)0, B, ldb, 0, NULL, rout);
+}
+
+/* SYMM: C = alpha*A*B + beta*C (Left) or alpha*B*A + beta*C (Right), A
+ * symmetric in one triangle.  Replaces src/blas/ulevel3/ATL_symmL.c /
+ * ATL_symmR.c: the symmetric matrix is expanded into a dense workspace and
+ * the product is one GEMM launch. */
+void NM(PFX, symm_dev)(atlmx_ctx *c, int side, int uplo, int M, int N, T alpha, const T *A,
+                       long lda, const T *B, long ldb, T beta, T *C, long ldc,
+                       const char *rout)
+{
+  const int na = (side == KX_LEFT) ? M : N;
+  T *S;
+  if (M <= 0 || N <= 0 || (alpha == (T)0 && beta == (T)1)) return;
+  if (alpha == (T)0) {
+    GEMM_DEV(c, KX_NOTRANS, KX_NOTRANS, M, N, 0, (T)0, B, ldb, B, ldb, beta, C, ldc, 0, NULL,
+             rout);
+    return;
+  }
+  S = (T *)rt_ws(c, WS_L3A, (size_t)na * na * sizeof(T), rout);
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_EXPAND_SYM, uplo, KX_NONUNIT, na, na, (T *)A, lda, S, na,
+                           c->stream));
+  if (side == KX_LEFT)
+    GEMM_DEV(c, KX_NOTRANS, KX_NOTRANS, M, N, M, alpha, S, na, B, ldb, beta, C, ldc, 0, NULL,
+             rout);
+  else
+    GEMM_DEV(c, KX_NOTRANS, KX_NOTRANS, M, N, N, alpha, B, ldb, S, na, beta, C, ldc, 0, NULL,
+             rout);
+}
+
+/* SYR2K: C = alpha*(op(A) op(B)^T + op(B) op(A)^T) + beta*C on one
+ * triangle.  Replaces src/blas/ulevel3/ATL_syr2k.c:255: two launches of the
+ * triangle-masked GEMM (tiles outside the triangle exit at once), the
+ * second accumulating onto the first. */
+void NM(PFX, syr2k_dev)(atlmx_ctx *c, int uplo, int trans, int N, int K, T alpha,
+                        const T *A, long lda, const T *B, long ldb, T beta, T *C, long ldc,
+                        const char *rout)
+{
+  const int t1 = (trans == KX_NOTRANS) ? KX_NOTRANS : KX_TRANS;
+  const int t2 = (trans == KX_NOTRANS) ? KX_TRANS : KX_NOTRANS;
+  if (N <= 0) return;
+  GEMM_DEV(c, t1, t2, N, N, K, alpha, A, lda, B, ldb, beta, C, ldc, uplo, NULL, rout);
+  GEMM_DEV(c, t1, t2, N, N, K, alpha, B, ldb, A, lda, (T)1, C, ldc, uplo, NULL, rout);
 }
 
 /* ====================================================================== */
@@ -252,19 +335,14 @@ void NM3(cblas_, PFX, gemm)(const enum CBLAS_ORDER Order, const enum CBLAS_TRANS
   rt_sync(c, rout);
 }
 
-/* interfaces/blas/C/src/cblas_dtrsm.c:13-106 */
-void NM3(cblas_, PFX, trsm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
-                            const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
-                            const enum CBLAS_DIAG Diag, const int M, const int N,
-                            const T alpha, const T *A, const int lda, T *B, const int ldb)
+/* Argument checks shared by cblas_?trsm and cblas_?trmm, whose checks are
+ * the same (interfaces/blas/C/src/cblas_dtrsm.c:23-75, cblas_dtrmm.c). */
+static int NM(PFX, trxm_check)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                               const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
+                               const enum CBLAS_DIAG Diag, const int M, const int N,
+                               const int lda, const int ldb)
 {
-  static const char *rout = "cblas_" STR(PFX) "trsm";
   int info = 2000;
-  int side, uplo, m, n, na;
-  atlmx_ctx *c;
-  long dla, dlb;
-  T *dA, *dB;
-
   if (Order == CblasColMajor) {
     if (Side == CblasLeft) {
       if (lda < M || lda < 1)
@@ -303,8 +381,23 @@ void NM3(cblas_, PFX, trsm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE 
                         CblasNonUnit, Diag);
   if (M < 0) info = cblas_errprn(6, info, "M cannot be less than zero; it is set to %d.", M);
   if (N < 0) info = cblas_errprn(7, info, "N cannot be less than zero; it is set to %d.", N);
-  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
+  return info;
+}
 
+/* interfaces/blas/C/src/cblas_dtrsm.c:13-106 */
+void NM3(cblas_, PFX, trsm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                            const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
+                            const enum CBLAS_DIAG Diag, const int M, const int N,
+                            const T alpha, const T *A, const int lda, T *B, const int ldb)
+{
+  static const char *rout = "cblas_" STR(PFX) "trsm";
+  const int info = NM(PFX, trxm_check)(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
+  int side, uplo, m, n, na;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+
+  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
   if (M == 0 || N == 0) return;
   /* RowMajor: flip Side and Uplo, swap M and N (:100-105) */
   if (Order == CblasColMajor) {
@@ -377,6 +470,158 @@ void NM3(cblas_, PFX, syrk)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO 
   dA = (T *)rt_stage_in(c, WS_STAGE_A, A, ar, ac, lda, sizeof(T), 1, &dla, rout);
   dC = (T *)rt_stage_in(c, WS_STAGE_C, C, N, N, ldc, sizeof(T), 1, &dlc, rout);
   SYRK_DEV(c, uplo, trans, N, K, alpha, dA, dla, beta, dC, dlc, NULL, rout);
+  rt_stage_out(c, C, dC, N, N, ldc, dlc, sizeof(T), rout);
+  rt_sync(c, rout);
+}
+
+/* interfaces/blas/C/src/cblas_dtrmm.c (RowMajor flips Side and Uplo and
+ * swaps M and N, as for TRSM) */
+void NM3(cblas_, PFX, trmm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                            const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
+                            const enum CBLAS_DIAG Diag, const int M, const int N,
+                            const T alpha, const T *A, const int lda, T *B, const int ldb)
+{
+  static const char *rout = "cblas_" STR(PFX) "trmm";
+  const int info = NM(PFX, trxm_check)(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
+  int side, uplo, m, n, na;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
+  if (M == 0 || N == 0) return;
+  if (Order == CblasColMajor) {
+    side = (Side == CblasLeft) ? KX_LEFT : KX_RIGHT;
+    uplo = (Uplo == CblasUpper) ? KX_UPPER : KX_LOWER;
+    m = M; n = N;
+  } else {
+    side = (Side == CblasLeft) ? KX_RIGHT : KX_LEFT;
+    uplo = (Uplo == CblasUpper) ? KX_LOWER : KX_UPPER;
+    m = N; n = M;
+  }
+  na = (side == KX_LEFT) ? m : n;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, na, na, lda, sizeof(T), 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, m, n, ldb, sizeof(T), alpha != (T)0, &dlb, rout);
+  NM(PFX, trmm_dev)(c, side, uplo, TA == CblasNoTrans ? KX_NOTRANS : KX_TRANS,
+                    Diag == CblasUnit ? KX_UNIT : KX_NONUNIT, m, n, alpha, dA, dla, dB, dlb,
+                    rout);
+  rt_stage_out(c, B, dB, m, n, ldb, dlb, sizeof(T), rout);
+  rt_sync(c, rout);
+}
+
+/* interfaces/blas/C/src/cblas_dsymm.c (RowMajor flips Side and Uplo and
+ * swaps M and N) */
+void NM3(cblas_, PFX, symm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                            const enum CBLAS_UPLO Uplo, const int M, const int N,
+                            const T alpha, const T *A, const int lda, const T *B,
+                            const int ldb, const T beta, T *C, const int ldc)
+{
+  static const char *rout = "cblas_" STR(PFX) "symm";
+  int info = 2000, side, uplo, m, n, na;
+  atlmx_ctx *c;
+  long dla, dlb, dlc;
+  T *dA, *dB, *dC;
+  if (Order == CblasColMajor || Order == CblasRowMajor) {
+    const int rows = (Order == CblasColMajor) ? M : N;
+    const char *rn = (Order == CblasColMajor) ? "M" : "N";
+    if (Side == CblasLeft) {
+      if (lda < M || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
+    } else if (Side == CblasRight) {
+      if (lda < N || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
+    } else
+      info = cblas_errprn(2, info, "SIDE must be %d or %d, but is set to %d", CblasRight,
+                          CblasLeft, Side);
+    if (ldb < rows || ldb < 1)
+      info = cblas_errprn(10, info, "ldb must be >= MAX(%s,1): ldb=%d %s=%d", rn, ldb, rn,
+                          rows);
+    if (ldc < rows || ldc < 1)
+      info = cblas_errprn(13, info, "ldc must be >= MAX(%s,1): ldc=%d %s=%d", rn, ldc, rn,
+                          rows);
+  } else
+    info = cblas_errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
+                        CblasColMajor, Order);
+  if (Uplo != CblasUpper && Uplo != CblasLower)
+    info = cblas_errprn(3, info, "UPLO must be %d or %d, but is set to %d", CblasUpper,
+                        CblasLower, Uplo);
+  if (M < 0) info = cblas_errprn(4, info, "M cannot be less than zero; it is set to %d.", M);
+  if (N < 0) info = cblas_errprn(5, info, "N cannot be less than zero; it is set to %d.", N);
+  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
+  if (M == 0 || N == 0 || (alpha == (T)0 && beta == (T)1)) return;
+  if (Order == CblasColMajor) {
+    side = (Side == CblasLeft) ? KX_LEFT : KX_RIGHT;
+    uplo = (Uplo == CblasUpper) ? KX_UPPER : KX_LOWER;
+    m = M; n = N;
+  } else {
+    side = (Side == CblasLeft) ? KX_RIGHT : KX_LEFT;
+    uplo = (Uplo == CblasUpper) ? KX_LOWER : KX_UPPER;
+    m = N; n = M;
+  }
+  na = (side == KX_LEFT) ? m : n;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, na, na, lda, sizeof(T), 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, m, n, ldb, sizeof(T), 1, &dlb, rout);
+  dC = (T *)rt_stage_in(c, WS_STAGE_C, C, m, n, ldc, sizeof(T), beta != (T)0, &dlc, rout);
+  NM(PFX, symm_dev)(c, side, uplo, m, n, alpha, dA, dla, dB, dlb, beta, dC, dlc, rout);
+  rt_stage_out(c, C, dC, m, n, ldc, dlc, sizeof(T), rout);
+  rt_sync(c, rout);
+}
+
+/* interfaces/blas/C/src/cblas_dsyr2k.c (RowMajor flips Uplo and Trans on
+ * the same memory, as for SYRK) */
+void NM3(cblas_, PFX, syr2k)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                             const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                             const T alpha, const T *A, const int lda, const T *B,
+                             const int ldb, const T beta, T *C, const int ldc)
+{
+  static const char *rout = "cblas_" STR(PFX) "syr2k";
+  int info = 2000, uplo, trans, ar, ac;
+  atlmx_ctx *c;
+  long dla, dlb, dlc;
+  T *dA, *dB, *dC;
+  if (Uplo != CblasUpper && Uplo != CblasLower)
+    info = cblas_errprn(2, info, "UPLO must be %d or %d, but is set to %d", CblasUpper,
+                        CblasLower, Uplo);
+  if (N < 0) info = cblas_errprn(4, info, "N cannot be less than zero; it is set to %d.", N);
+  if (K < 0) info = cblas_errprn(5, info, "K cannot be less than zero; it is set to %d.", K);
+  if (Order == CblasColMajor || Order == CblasRowMajor) {
+    const int nt = (Order == CblasColMajor) == (Trans == CblasNoTrans);
+    if (Trans != CblasNoTrans && Trans != CblasTrans && Trans != CblasConjTrans)
+      info = cblas_errprn(3, info, "Trans must be %d or %d, but is set to %d", CblasNoTrans,
+                          CblasTrans, Trans);
+    if (nt) {
+      if (lda < N || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
+      if (ldb < N || ldb < 1)
+        info = cblas_errprn(10, info, "ldb must be >= MAX(N,1): ldb=%d N=%d", ldb, N);
+    } else {
+      if (lda < K || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(K,1): lda=%d K=%d", lda, K);
+      if (ldb < K || ldb < 1)
+        info = cblas_errprn(10, info, "ldb must be >= MAX(K,1): ldb=%d K=%d", ldb, K);
+    }
+  } else
+    info = cblas_errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
+                        CblasColMajor, Order);
+  if (ldc < N || ldc < 1)
+    info = cblas_errprn(13, info, "ldc must be >= MAX(N,1): ldc=%d N=%d", ldc, N);
+  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
+  if (N == 0 || ((alpha == (T)0 || K == 0) && beta == (T)1)) return;
+  if (Order == CblasColMajor) {
+    uplo = (Uplo == CblasUpper) ? KX_UPPER : KX_LOWER;
+    trans = (Trans == CblasNoTrans) ? KX_NOTRANS : KX_TRANS;
+  } else {
+    uplo = (Uplo == CblasUpper) ? KX_LOWER : KX_UPPER;
+    trans = (Trans == CblasNoTrans) ? KX_TRANS : KX_NOTRANS;
+  }
+  ar = (trans == KX_NOTRANS) ? N : K;
+  ac = (trans == KX_NOTRANS) ? K : N;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, ar, ac, lda, sizeof(T), 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, ar, ac, ldb, sizeof(T), 1, &dlb, rout);
+  dC = (T *)rt_stage_in(c, WS_STAGE_C, C, N, N, ldc, sizeof(T), 1, &dlc, rout);
+  NM(PFX, syr2k_dev)(c, uplo, trans, N, K, alpha, dA, dla, dB, dlb, beta, dC, dlc, rout);
   rt_stage_out(c, C, dC, N, N, ldc, dlc, sizeof(T), rout);
   rt_sync(c, rout);
 }

@@ -1,0 +1,53 @@
+/*
+ * drivers.h -- stream-ordered drivers on device pointers shared between the
+ * host C files (defined in blas3.c and lapack.c from their precision
+ * templates).  Column-major, 64-bit leading dimensions, work ordered on the
+ * context's stream.
+ */
+#ifndef ATLMX_DRIVERS_H
+#define ATLMX_DRIVERS_H
+#include "rt.h"
+
+#define ATLMX_DRV_DECLARE(P, T)                                                      \
+void P##gemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, T alpha,          \
+                 const T *A, long lda, const T *B, long ldb, T beta, T *C, long ldc,  \
+                 int tri, const int *abort_flag, const char *rout);                   \
+void P##trsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,    \
+                 T alpha, const T *A, long lda, T *B, long ldb,                       \
+                 const int *abort_flag, const char *rout);                            \
+void P##trsm_dev_ws(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N, \
+                    T alpha, const T *A, long lda, T *B, long ldb,                    \
+                    const int *abort_flag, T *inv_ws, int *ctl_ws, const char *rout); \
+void P##syrk_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, T alpha,            \
+                 const T *A, long lda, T beta, T *C, long ldc,                        \
+                 const int *abort_flag, const char *rout);                            \
+int  P##getrf_dev(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv,            \
+                  const char *rout);                                                  \
+void P##getrs_dev(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,     \
+                  const int *d_ipiv, T *B, long ldb, const char *rout);               \
+void P##getrs_dev_ws(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,  \
+                     const int *d_ipiv, T *B, long ldb, T *inv, int *ctl,             \
+                     const char *rout);                                               \
+int  P##potrf_dev(atlmx_ctx *c, int uplo, int N, T *A, long lda, const char *rout);   \
+void P##potrs_dev(atlmx_ctx *c, int uplo, int N, int NRHS, const T *A, long lda,      \
+                  T *B, long ldb, const char *rout);                                  \
+void P##trmm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,    \
+                 T alpha, const T *A, long lda, T *B, long ldb, const char *rout);    \
+void P##symm_dev(atlmx_ctx *c, int side, int uplo, int M, int N, T alpha,             \
+                 const T *A, long lda, const T *B, long ldb, T beta, T *C, long ldc,  \
+                 const char *rout);                                                   \
+void P##syr2k_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, T alpha,           \
+                  const T *A, long lda, const T *B, long ldb, T beta, T *C, long ldc, \
+                  const char *rout);                                                  \
+int  P##trtri_dev(atlmx_ctx *c, int uplo, int diag, int N, T *A, long lda,            \
+                  const char *rout);                                                  \
+void P##lauum_dev(atlmx_ctx *c, int uplo, int N, T *A, long lda, const char *rout);   \
+int  P##getri_dev(atlmx_ctx *c, int N, T *A, long lda, const int *d_ipiv,             \
+                  const char *rout);                                                  \
+void P##geqrf_dev(atlmx_ctx *c, int M, int N, T *A, long lda, T *d_tau,               \
+                  const char *rout);
+
+ATLMX_DRV_DECLARE(d, double)
+ATLMX_DRV_DECLARE(s, float)
+
+#endif

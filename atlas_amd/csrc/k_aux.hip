@@ -162,6 +162,90 @@ __global__ void fill_i32_kernel(int *p, int n, int v) {
   if (i < n) p[i] = v;
 }
 
+/* ------------------------------------------------------------------------
+ * Triangle copies for the routines that run on the dense GEMM (TRMM, SYMM,
+ * LAUUM) or on a dense workspace (TRTRI, GETRI).  One thread per element of
+ * the n x n (or m x n) destination, 64-wide column strips per workgroup row
+ * so the column-major accesses coalesce.
+ *   mode TC_EXPAND_TRI: W = triangle of A, zero outside (unit diag -> 1)
+ *   mode TC_EXPAND_SYM: W = the symmetric matrix held in A's triangle
+ *   mode TC_STORE_TRI : A's triangle = W's (diagonal left alone if unit)
+ *   mode TC_IDENTITY  : W = I
+ *   mode TC_COPY      : A = W (m x n)
+ *   mode TC_TRANSPOSE : W (n x m) = A^T (A m x n)
+ * ---------------------------------------------------------------------- */
+template <typename T>
+__global__ __launch_bounds__(256) void tricopy_kernel(int mode, int lower, int unit, int m,
+                                                      int n, T *A, long lda, T *W, long ldw) {
+  const int i = blockIdx.y * 64 + (threadIdx.x & 63);
+  const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= m || j >= n) return;
+  const bool in = lower ? (i >= j) : (i <= j);
+  switch (mode) {
+    case ATLMX_TC_EXPAND_TRI: {
+      T v = in ? A[(size_t)j * lda + i] : T(0);
+      if (i == j && unit) v = T(1);
+      W[(size_t)j * ldw + i] = v;
+      break;
+    }
+    case ATLMX_TC_EXPAND_SYM:
+      W[(size_t)j * ldw + i] = in ? A[(size_t)j * lda + i] : A[(size_t)i * lda + j];
+      break;
+    case ATLMX_TC_STORE_TRI:
+      if (in && !(i == j && unit)) A[(size_t)j * lda + i] = W[(size_t)j * ldw + i];
+      break;
+    case ATLMX_TC_IDENTITY:
+      W[(size_t)j * ldw + i] = (i == j) ? T(1) : T(0);
+      break;
+    case ATLMX_TC_TRANSPOSE:
+      W[(size_t)i * ldw + j] = A[(size_t)j * lda + i];
+      break;
+    default:
+      A[(size_t)j * lda + i] = W[(size_t)j * ldw + i];
+      break;
+  }
+}
+
+/* *info = 1 + the first i < n with A(i,i) == 0, else 0 (info starts at
+ * INT_MAX; a second pass turns INT_MAX into 0) */
+template <typename T>
+__global__ __launch_bounds__(256) void diag_zero_kernel(int n, const T *A, long lda, int *info) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+    if (A[(size_t)i * lda + i] == T(0)) atomicMin(info, i + 1);
+}
+
+__global__ void info_fix_kernel(int *info) {
+  if (*info == 0x7fffffff) *info = 0;
+}
+
+template <typename T>
+hipError_t launch_tricopy(int mode, int uplo, int diag, int m, int n, T *A, long lda, T *W,
+                          long ldw, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (mode < ATLMX_TC_EXPAND_TRI || mode > ATLMX_TC_TRANSPOSE) return hipErrorInvalidValue;
+  if (mode <= ATLMX_TC_STORE_TRI && m != n) return hipErrorInvalidValue;
+  if ((mode != ATLMX_TC_IDENTITY && (!A || lda < m)) || !W ||
+      ldw < (mode == ATLMX_TC_TRANSPOSE ? n : m))
+    return hipErrorInvalidValue;
+  const int gy = (m + 63) / 64;            /* row strips (grid y), column quads (grid x) */
+  if (gy > 65535) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(tricopy_kernel<T>, dim3((n + 3) / 4, gy), dim3(256), 0, s,
+                     mode, uplo == KX_LOWER, diag == KX_UNIT, m, n, A, lda, W, ldw);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_diag_zero(int n, const T *A, long lda, int *info, hipStream_t s) {
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(64), 0, s, info, 1, 0x7fffffff);
+  if (n > 0) {
+    int g = (n + 255) / 256;
+    if (g > 1024) g = 1024;
+    hipLaunchKernelGGL(diag_zero_kernel<T>, dim3(g), dim3(256), 0, s, n, A, lda, info);
+  }
+  hipLaunchKernelGGL(info_fix_kernel, dim3(1), dim3(1), 0, s, info);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_tri_inv(int uplo, int diag, int n, const T *A, long lda, T *inv,
                           const int *abort_flag, hipStream_t s) {
@@ -218,6 +302,20 @@ hipError_t atlmx_k_idamax(int n, const double *x, int incx, int *out, hipStream_
 }
 hipError_t atlmx_k_isamax(int n, const float *x, int incx, int *out, hipStream_t s) {
   return launch_iamax<float>(n, x, incx, out, s);
+}
+hipError_t atlmx_k_dtricopy(int mode, int uplo, int diag, int m, int n, double *A, long lda,
+                            double *W, long ldw, hipStream_t s) {
+  return launch_tricopy<double>(mode, uplo, diag, m, n, A, lda, W, ldw, s);
+}
+hipError_t atlmx_k_stricopy(int mode, int uplo, int diag, int m, int n, float *A, long lda,
+                            float *W, long ldw, hipStream_t s) {
+  return launch_tricopy<float>(mode, uplo, diag, m, n, A, lda, W, ldw, s);
+}
+hipError_t atlmx_k_ddiag_zero(int n, const double *A, long lda, int *info, hipStream_t s) {
+  return launch_diag_zero<double>(n, A, lda, info, s);
+}
+hipError_t atlmx_k_sdiag_zero(int n, const float *A, long lda, int *info, hipStream_t s) {
+  return launch_diag_zero<float>(n, A, lda, info, s);
 }
 hipError_t atlmx_k_fill_i32(int *p, int n, int v, hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -819,6 +819,16 @@ hipError_t batch_factor(int kind, int nb, int count, const int *order, const int
   return hipErrorInvalidValue;
 }
 
+/* Largest instance order whose LDS working set fits (panel width 16, the
+ * smaller of the two the autotuner tries). */
+template <typename T>
+int max_order(int kind) {
+  const long cap = 160 * 1024 / (long)sizeof(T);
+  const long fixed = (kind == 0 ? 16L * 17 : 0L) + 2L * 16 * TS;
+  const long per_row = (kind == 0 ? 0L : 16L) + MAXR;
+  return (int)((cap - fixed) / per_row);
+}
+
 template <typename T>
 hipError_t batch_gemm(int count, int ta, int tb, const int *order, const int *m, const int *n,
                       const int *k,
@@ -907,4 +917,10 @@ extern "C" hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *
                                           hipStream_t s) {
   return batch_gemm<float>(count, ta, tb, order, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
                            tile_start, counter, grid, s);
+}
+
+/* kind 0 Cholesky, 1 LU (2, the transposed solve, as 1) */
+extern "C" int atlmx_k_batch_max_order(int kind, int dbl) {
+  const int k = kind == 0 ? 0 : 1;
+  return dbl ? max_order<double>(k) : max_order<float>(k);
 }

@@ -247,6 +247,117 @@ __global__ void slots_init_kernel(long long *h, size_t n) {
     h[i] = SENT;
 }
 
+/* ------------------------------------------------------------------------
+ * Panels taller than the cooperative kernel holds (more than 2*NT*MAXG
+ * rows): the same column steps as LU1, three launches per column on the
+ * panel in HBM -- a partial arg-max per workgroup, a one-wavefront kernel
+ * that reduces the partials (same tie and NaN rules), swaps the two panel
+ * rows and records the pivot, and the scaling + rank-1 update of the rows
+ * below.  The partials live in the candidate-row area of the scratch, which
+ * holds nothing between launches; the other header set is reset as the
+ * cooperative kernel would, so the two paths can alternate.
+ * ---------------------------------------------------------------------- */
+constexpr int TALL_G = 1024;
+
+__global__ void tall_reset_kernel(long long *hv, long long *hi, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    st_coh(hv + i, SENT);
+    st_coh(hi + i, SENT);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void tall_amax_kernel(int m, int c, const T *P, long ld,
+                                                       double *pv, int *pi) {
+  __shared__ double s_v[NT / 64];
+  __shared__ int s_i[NT / 64];
+  const T *col = P + (size_t)c * ld;
+  double bv = -2.0;
+  int bi = 0x7fffffff, bz = 0;
+  for (long r = c + (long)blockIdx.x * NT + threadIdx.x; r < m; r += (long)gridDim.x * NT) {
+    double v = (double)col[r];
+    v = v < 0.0 ? -v : v;
+    if (!(v == v)) v = -1.0;
+    atlmx::argmax_combine(bv, bi, bz, v, (int)r, 0);
+  }
+  atlmx::wave_argmax(bv, bi, bz);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) { s_v[wave] = bv; s_i[wave] = bi; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    bv = s_v[0];
+    bi = s_i[0];
+    for (int k = 1; k < NT / 64; ++k) atlmx::argmax_combine(bv, bi, bz, s_v[k], s_i[k], 0);
+    pv[blockIdx.x] = bv;
+    pi[blockIdx.x] = bi;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void tall_swap_kernel(int w, int c, T *P, long ld, int row0,
+                                                       int *ipiv, int *info, const double *pv,
+                                                       const int *pi, int G) {
+  const int lane = threadIdx.x;
+  double v = -2.0;
+  int i = 0x7fffffff, z = 0;
+  for (int k = lane; k < G; k += 64) atlmx::argmax_combine(v, i, z, pv[k], pi[k], 0);
+  atlmx::wave_argmax(v, i, z);
+  const int p = (v > 0.0) ? i : c;      /* all zero or NaN: iamax answers c */
+  const T pivval = P[(size_t)c * ld + p];
+  if (p != c)
+    for (int k = lane; k < w; k += 64) {
+      const T a = P[(size_t)k * ld + c], b = P[(size_t)k * ld + p];
+      P[(size_t)k * ld + c] = b;
+      P[(size_t)k * ld + p] = a;
+    }
+  if (lane == 0) {
+    ipiv[c] = row0 + p;
+    if (pivval == T(0)) atomicCAS(info, 0, row0 + c + 1);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(NT) void tall_update_kernel(int m, int w, int c, T *P, long ld) {
+  const T pv = P[(size_t)c * ld + c];
+  if (pv == T(0)) return;
+  const bool recip = (pv < T(0) ? -pv : pv) >= Lim<T>::safmin();
+  const T rcp = T(1) / pv;
+  for (long r = c + 1 + (long)blockIdx.x * NT + threadIdx.x; r < m; r += (long)gridDim.x * NT) {
+    const T x = P[(size_t)c * ld + r];
+    const T l = recip ? x * rcp : x / pv;
+    P[(size_t)c * ld + r] = l;
+    for (int k = c + 1; k < w; ++k) P[(size_t)k * ld + r] -= l * P[(size_t)k * ld + c];
+  }
+}
+
+template <typename T>
+hipError_t launch_tall(int m, int w, T *P, long ld, int row0, int *ipiv, int *info,
+                       void *scratch, int set, hipStream_t s) {
+  PanelSlots S;
+  {
+    char *b = (char *)scratch;
+    S.hval = (long long *)b;  b += (size_t)2 * MAXW * MAXG * 8;
+    S.hidx = (long long *)b;  b += (size_t)2 * MAXW * MAXG * 8;
+    S.crow = (double *)b;
+  }
+  double *pv = S.crow;                       /* TALL_G doubles, then TALL_G ints */
+  int *pi = (int *)(S.crow + TALL_G);
+  const size_t o = (size_t)(set ^ 1) * MAXW * MAXG;
+  hipLaunchKernelGGL(tall_reset_kernel, dim3(64), dim3(256), 0, s, S.hval + o, S.hidx + o,
+                     MAXW * MAXG);
+  for (int c = 0; c < w; ++c) {
+    long rows = (long)m - c;
+    int G = (int)((rows + NT - 1) / NT);
+    if (G > TALL_G) G = TALL_G;
+    hipLaunchKernelGGL(tall_amax_kernel<T>, dim3(G), dim3(NT), 0, s, m, c, (const T *)P, ld,
+                       pv, pi);
+    hipLaunchKernelGGL(tall_swap_kernel<T>, dim3(1), dim3(64), 0, s, w, c, P, ld, row0, ipiv,
+                       info, (const double *)pv, (const int *)pi, G);
+    hipLaunchKernelGGL(tall_update_kernel<T>, dim3(G), dim3(NT), 0, s, m, w, c, P, ld);
+  }
+  return hipGetLastError();
+}
+
 template <typename T, int W, int R>
 hipError_t launch_wr(int G, void **args, hipStream_t s) {
   static int attr_set = 0;
@@ -274,7 +385,8 @@ hipError_t launch_panel(int m, int w, T *P, long ld, int row0, int *ipiv, int *i
     const int G = (m + NT - 1) / NT;
     return launch_wr<T, MAXW, NT>(G, args, s);
   }
-  if ((long)m > (long)2 * NT * MAXG) return hipErrorInvalidValue;
+  if ((long)m > (long)2 * NT * MAXG) return launch_tall<T>(m, w, P, ld, row0, ipiv, info,
+                                                          scratch, set, s);
   if (w > MAXW / 2) return hipErrorInvalidValue;
   const int G = (m + 2 * NT - 1) / (2 * NT);
   return launch_wr<T, MAXW / 2, 2 * NT>(G, args, s);

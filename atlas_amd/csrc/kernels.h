@@ -142,6 +142,9 @@ hipError_t atlmx_k_sbatch_factor(int kind, int nb, int count, const int *order, 
                                  float *const *B, const int *ldb, int *info, int *counter,
                                  int maxn, int grid, hipStream_t s);
 /* kind 2 of atlmx_k_?batch_factor: LU as kind 1, then solve A^T X = B. */
+/* Largest order n[i] the factor kernels hold in LDS for kind (0 Cholesky,
+ * 1 or 2 LU) and precision (dbl 1: FP64).  Host-only arithmetic. */
+int atlmx_k_batch_max_order(int kind, int dbl);
 /* Cholesky adjoint of the solve X = A^{-1} B per instance: L (potrf's lower
  * triangle; its strict upper triangle is zeroed), X, Bb = A^{-1} Xb (n x
  * nrhs, ld ldx); writes G (ld ldg) = gradient of <Xb, X> with respect to
@@ -202,8 +205,9 @@ hipError_t atlmx_k_sscal2d(int M, int N, float alpha, float *B, long ldb,
                            hipStream_t s);
 
 /* LU panel: factor the m x w column panel P (ld), w <= ATLMX_PANEL_MAXW
- * (w <= ATLMX_PANEL_MAXW/2 when m > ATLMX_PANEL_TALL), m <= 2*65536, with
- * partial pivoting, in one cooperative launch.  ipiv[c] receives the
+ * (w <= ATLMX_PANEL_MAXW/2 when m > ATLMX_PANEL_TALL), with partial
+ * pivoting: in one cooperative launch up to 2*65536 rows, beyond that as
+ * three launches per column on the panel in HBM.  ipiv[c] receives the
  * absolute pivot row (row0 + panel row); info (device int) gets row0+c+1
  * for the first exactly-zero pivot if it is still 0.  scratch: device
  * buffer of atlmx_k_panel_scratch_bytes() bytes, initialised once with
@@ -258,6 +262,47 @@ hipError_t atlmx_k_idamax(int n, const double *x, int incx, int *out,
                           hipStream_t s);
 hipError_t atlmx_k_isamax(int n, const float *x, int incx, int *out,
                           hipStream_t s);
+
+/* Triangle copies (k_aux.hip), column-major, for the routines built on the
+ * dense GEMM or a dense workspace:
+ *   EXPAND_TRI: W (n x n) = A's uplo triangle, zero outside, 1 on the
+ *               diagonal when diag is KX_UNIT
+ *   EXPAND_SYM: W = the symmetric matrix stored in A's uplo triangle
+ *   STORE_TRI : A's uplo triangle = W's (diagonal not written when unit)
+ *   IDENTITY  : W (m x n) = I (A unused)
+ *   COPY      : A (m x n) = W
+ *   TRANSPOSE : W (n x m, ldw >= n) = A^T, A m x n */
+#define ATLMX_TC_EXPAND_TRI 0
+#define ATLMX_TC_EXPAND_SYM 1
+#define ATLMX_TC_STORE_TRI 2
+#define ATLMX_TC_IDENTITY 3
+#define ATLMX_TC_COPY 4
+#define ATLMX_TC_TRANSPOSE 5
+hipError_t atlmx_k_dtricopy(int mode, int uplo, int diag, int m, int n, double *A, long lda,
+                            double *W, long ldw, hipStream_t s);
+hipError_t atlmx_k_stricopy(int mode, int uplo, int diag, int m, int n, float *A, long lda,
+                            float *W, long ldw, hipStream_t s);
+/* *info (device) = 1 + the first i < n with A(i,i) == 0, else 0 */
+hipError_t atlmx_k_ddiag_zero(int n, const double *A, long lda, int *info, hipStream_t s);
+hipError_t atlmx_k_sdiag_zero(int n, const float *A, long lda, int *info, hipStream_t s);
+
+/* Householder QR (k_qr.hip).  geqr2_panel: factor the m x w panel P (w <=
+ * ATLMX_QR_NB) column by column as LAPACK's geqr2/larfg: R on and above
+ * the diagonal, the reflectors' vectors (unit leading entry implied) below,
+ * tau[0..min(m,w)); work: atlmx_k_qr_work_doubles() doubles of scratch.
+ * qr_vcopy: V (m x w, ld m) = the panel's unit lower trapezoid.  larft:
+ * the w x w upper triangular factor T of the block reflector from G =
+ * V^T V (ld w) and tau. */
+#define ATLMX_QR_NB 32
+size_t atlmx_k_qr_work_doubles(void);
+hipError_t atlmx_k_dgeqr2_panel(int m, int w, double *P, long ld, double *tau, double *work,
+                                hipStream_t s);
+hipError_t atlmx_k_sgeqr2_panel(int m, int w, float *P, long ld, float *tau, double *work,
+                                hipStream_t s);
+hipError_t atlmx_k_dqr_vcopy(int m, int w, const double *P, long ld, double *V, hipStream_t s);
+hipError_t atlmx_k_sqr_vcopy(int m, int w, const float *P, long ld, float *V, hipStream_t s);
+hipError_t atlmx_k_dlarft(int w, const double *G, const double *tau, double *Tm, hipStream_t s);
+hipError_t atlmx_k_slarft(int w, const float *G, const float *tau, float *Tm, hipStream_t s);
 
 /* Fill n ints with v (stream-ordered, no memset node). */
 hipError_t atlmx_k_fill_i32(int *p, int n, int v, hipStream_t s);

@@ -16,25 +16,7 @@
 #define NM3_(a, b, c) a##b##c
 #define NM3(a, b, c) NM3_(a, b, c)
 
-/* drivers defined in blas3.c */
-void dgemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, double alpha,
-               const double *A, long lda, const double *B, long ldb, double beta, double *C,
-               long ldc, int tri, const int *abort_flag, const char *rout);
-void sgemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, float alpha,
-               const float *A, long lda, const float *B, long ldb, float beta, float *C,
-               long ldc, int tri, const int *abort_flag, const char *rout);
-void dtrsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
-               double alpha, const double *A, long lda, double *B, long ldb,
-               const int *abort_flag, const char *rout);
-void strsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
-               float alpha, const float *A, long lda, float *B, long ldb,
-               const int *abort_flag, const char *rout);
-void dsyrk_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, double alpha,
-               const double *A, long lda, double beta, double *C, long ldc,
-               const int *abort_flag, const char *rout);
-void ssyrk_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, float alpha,
-               const float *A, long lda, float beta, float *C, long ldc,
-               const int *abort_flag, const char *rout);
+#include "drivers.h"
 
 #define T double
 #define PFX d
@@ -42,6 +24,11 @@ void ssyrk_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, float alpha,
 #define K_LASWP atlmx_k_dlaswp
 #define K_LASWP_APPLY atlmx_k_dlaswp_apply
 #define K_POTF2 atlmx_k_dpotf2
+#define K_TRICOPY atlmx_k_dtricopy
+#define K_DIAG_ZERO atlmx_k_ddiag_zero
+#define K_GEQR2 atlmx_k_dgeqr2_panel
+#define K_QR_VCOPY atlmx_k_dqr_vcopy
+#define K_LARFT atlmx_k_dlarft
 #include "lapack_tmpl.h"
 #undef T
 #undef PFX
@@ -49,6 +36,11 @@ void ssyrk_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, float alpha,
 #undef K_LASWP
 #undef K_LASWP_APPLY
 #undef K_POTF2
+#undef K_TRICOPY
+#undef K_DIAG_ZERO
+#undef K_GEQR2
+#undef K_QR_VCOPY
+#undef K_LARFT
 
 #define T float
 #define PFX s
@@ -56,4 +48,9 @@ void ssyrk_dev(atlmx_ctx *c, int uplo, int trans, int N, int K, float alpha,
 #define K_LASWP atlmx_k_slaswp
 #define K_LASWP_APPLY atlmx_k_slaswp_apply
 #define K_POTF2 atlmx_k_spotf2
+#define K_TRICOPY atlmx_k_stricopy
+#define K_DIAG_ZERO atlmx_k_sdiag_zero
+#define K_GEQR2 atlmx_k_sgeqr2_panel
+#define K_QR_VCOPY atlmx_k_sqr_vcopy
+#define K_LARFT atlmx_k_slarft
 #include "lapack_tmpl.h"

@@ -144,24 +144,33 @@ int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const cha
 }
 
 /* src/lapack/ATL_getrs.c:8-89 (ColMajor).  NoTrans: P*B, L-unit solve, U
- * solve; Trans: U^T solve, L^T-unit solve, P^T*B. */
-void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
-               const int *d_ipiv, T *B, long ldb, const char *rout)
+ * solve; Trans: U^T solve, L^T-unit solve, P^T*B.  inv / ctl: NULL, or the
+ * two TRSMs' workspaces (see trsm_dev_ws; the solves run one after the
+ * other on the stream, so they share them). */
+void NM(PFX, getrs_dev_ws)(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
+                           const int *d_ipiv, T *B, long ldb, T *inv, int *ctl,
+                           const char *rout)
 {
   if (N <= 0 || NRHS <= 0) return;
   if (trans == KX_NOTRANS) {
     RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, 1, N, c->stream));
-    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
-                      ldb, NULL, rout);
-    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda,
-                      B, ldb, NULL, rout);
+    NM(PFX, trsm_dev_ws)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
+                         ldb, NULL, inv, ctl, rout);
+    NM(PFX, trsm_dev_ws)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda,
+                         B, ldb, NULL, inv, ctl, rout);
   } else {
-    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda, B,
-                      ldb, NULL, rout);
-    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
-                      ldb, NULL, rout);
+    NM(PFX, trsm_dev_ws)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda, B,
+                         ldb, NULL, inv, ctl, rout);
+    NM(PFX, trsm_dev_ws)(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
+                         ldb, NULL, inv, ctl, rout);
     RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, -1, N, c->stream));
   }
+}
+
+void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
+               const int *d_ipiv, T *B, long ldb, const char *rout)
+{
+  NM(PFX, getrs_dev_ws)(c, trans, N, NRHS, A, lda, d_ipiv, B, ldb, NULL, NULL, rout);
 }
 
 /* Blocked right-looking Cholesky (Lower: A = L*L^T, Upper: A = U^T*U).
@@ -258,6 +267,118 @@ void POTRS_DEV(atlmx_ctx *c, int uplo, int N, int NRHS, const T *A, long lda, T 
                       B, ldb, NULL, rout);
     NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda, B,
                       ldb, NULL, rout);
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * Inverses.  TRTRI (src/lapack/ATL_trtri.c:8-42): info = first exactly-zero
+ * diagonal entry when non-unit (A untouched then), else A's triangle is
+ * replaced by its inverse -- computed as op(A) X = I on the TRSM (inverted
+ * 64x64 diagonal blocks + MFMA GEMM updates) into a dense workspace, whose
+ * triangle is stored back (the diagonal left alone when unit).  LAUUM
+ * (ATL_lauum.c:6-22): U U^T (upper) or L^T L (lower) into the same
+ * triangle, as one triangle-masked GEMM of the expanded triangle with
+ * itself.  POTRI = TRTRI + LAUUM (interfaces/lapack/C/src/clapack_dpotri.c).
+ * GETRI (ATL_getriC.c): info from U's diagonal, else A^{-1} = the solve of
+ * P L U X = I (getrs on the factors) written over the factors.
+ * Each synchronizes and returns info.
+ * ---------------------------------------------------------------------- */
+static int NM(PFX, diag_info)(atlmx_ctx *c, int N, const T *A, long lda, const char *rout)
+{
+  int info = 0;
+  int *d_info = (int *)rt_ws(c, WS_INFO, 256, rout);
+  RT_CHECK(rout, K_DIAG_ZERO(N, A, lda, d_info, c->stream));
+  RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost,
+                                c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+int NM(PFX, trtri_dev)(atlmx_ctx *c, int uplo, int diag, int N, T *A, long lda,
+                       const char *rout)
+{
+  T *W;
+  int info;
+  if (N <= 0) return 0;
+  if (diag == KX_NONUNIT && (info = NM(PFX, diag_info)(c, N, A, lda, rout)) != 0) return info;
+  W = (T *)rt_ws(c, WS_L3A, (size_t)N * N * sizeof(T), rout);
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_IDENTITY, uplo, diag, N, N, NULL, N, W, N, c->stream));
+  NM(PFX, trsm_dev)(c, KX_LEFT, uplo, KX_NOTRANS, diag, N, N, (T)1, A, lda, W, N, NULL, rout);
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_STORE_TRI, uplo, diag, N, N, A, lda, W, N, c->stream));
+  rt_sync(c, rout);
+  return 0;
+}
+
+void NM(PFX, lauum_dev)(atlmx_ctx *c, int uplo, int N, T *A, long lda, const char *rout)
+{
+  T *W;
+  if (N <= 0) return;
+  W = (T *)rt_ws(c, WS_L3B, (size_t)N * N * sizeof(T), rout);
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_EXPAND_TRI, uplo, KX_NONUNIT, N, N, A, lda, W, N,
+                           c->stream));
+  if (uplo == KX_LOWER)
+    NM(PFX, gemm_dev)(c, KX_TRANS, KX_NOTRANS, N, N, N, (T)1, W, N, W, N, (T)0, A, lda,
+                      KX_LOWER, NULL, rout);
+  else
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_TRANS, N, N, N, (T)1, W, N, W, N, (T)0, A, lda,
+                      KX_UPPER, NULL, rout);
+  rt_sync(c, rout);
+}
+
+int NM(PFX, getri_dev)(atlmx_ctx *c, int N, T *A, long lda, const int *d_ipiv,
+                       const char *rout)
+{
+  T *W;
+  int info;
+  if (N <= 0) return 0;
+  if ((info = NM(PFX, diag_info)(c, N, A, lda, rout)) != 0) return info;
+  W = (T *)rt_ws(c, WS_L3A, (size_t)N * N * sizeof(T), rout);
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_IDENTITY, 0, 0, N, N, NULL, N, W, N, c->stream));
+  GETRS_DEV(c, KX_NOTRANS, N, N, A, lda, d_ipiv, W, N, rout);
+  RT_CHECK(rout, K_TRICOPY(ATLMX_TC_COPY, 0, 0, N, N, A, lda, W, N, c->stream));
+  rt_sync(c, rout);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------
+ * QR (src/lapack/ATL_geqrf.c): blocked Householder QR with 32-wide panels.
+ * Each panel is factored column by column (k_qr.hip: larfg and the
+ * reflector's application within the panel), then the block reflector
+ * H = I - V T V^T (T from larft on V^T V) is applied to the trailing
+ * columns as C -= V (T^T (V^T C)) -- three MFMA GEMMs.  Same storage as
+ * LAPACK's geqrf: R on and above the diagonal, the reflectors below,
+ * tau[0..min(M,N)).
+ * ---------------------------------------------------------------------- */
+void NM(PFX, geqrf_dev)(atlmx_ctx *c, int M, int N, T *A, long lda, T *d_tau,
+                        const char *rout)
+{
+  const int K = M < N ? M : N, nb = ATLMX_QR_NB;
+  double *work;
+  T *V, *G, *Tm, *W, *W2;
+  int j;
+  if (K <= 0) return;
+  work = (double *)rt_ws(c, WS_QR, atlmx_k_qr_work_doubles() * sizeof(double), rout);
+  V = (T *)rt_ws(c, WS_QRV, ((size_t)M * nb + 2 * (size_t)nb * nb + 2 * (size_t)nb * N) *
+                            sizeof(T), rout);
+  G = V + (size_t)M * nb;
+  Tm = G + (size_t)nb * nb;
+  W = Tm + (size_t)nb * nb;
+  W2 = W + (size_t)nb * N;
+  for (j = 0; j < K; j += nb) {
+    const int jb = (K - j < nb) ? K - j : nb, mr = M - j, nc = N - j - jb;
+    T *P = A + j + (size_t)j * lda, *Cp = A + j + (size_t)(j + jb) * lda;
+    RT_CHECK(rout, K_GEQR2(mr, jb, P, lda, d_tau + j, work, c->stream));
+    if (nc <= 0) continue;
+    RT_CHECK(rout, K_QR_VCOPY(mr, jb, P, lda, V, c->stream));
+    NM(PFX, gemm_dev)(c, KX_TRANS, KX_NOTRANS, jb, jb, mr, (T)1, V, mr, V, mr, (T)0, G, jb, 0,
+                      NULL, rout);
+    RT_CHECK(rout, K_LARFT(jb, G, d_tau + j, Tm, c->stream));
+    NM(PFX, gemm_dev)(c, KX_TRANS, KX_NOTRANS, jb, nc, mr, (T)1, V, mr, Cp, lda, (T)0, W, jb,
+                      0, NULL, rout);
+    NM(PFX, gemm_dev)(c, KX_TRANS, KX_NOTRANS, jb, nc, jb, (T)1, Tm, jb, W, jb, (T)0, W2, jb, 0,
+                      NULL, rout);
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, mr, nc, jb, (T)-1, V, mr, W2, jb, (T)1, Cp,
+                      lda, 0, NULL, rout);
   }
 }
 
@@ -591,6 +712,200 @@ int NM3(clapack_, PFX, posv)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO
   rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
   rt_sync(c, rout);
   return info;
+}
+
+/* Order / Uplo / N / lda checks of the inverse routines' C wrappers
+ * (interfaces/lapack/C/src/clapack_dpotri.c, clapack_dlauum.c,
+ * clapack_dtrtri.c); the position of lda follows the Diag argument when
+ * there is one. */
+static int NM(PFX, inv_check)(const char *rout, const enum CBLAS_ORDER Order, int has_uplo,
+                              const enum CBLAS_UPLO Uplo, int has_diag,
+                              const enum CBLAS_DIAG Diag, const int N, const int lda)
+{
+  int ierr = 0, pos = 2;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (has_uplo) {
+    if (Uplo != CblasUpper && Uplo != CblasLower) {
+      ierr = -pos;
+      cblas_xerbla(pos, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
+                   CblasLower, Uplo);
+    }
+    ++pos;
+  }
+  if (has_diag) {
+    if (Diag != CblasUnit && Diag != CblasNonUnit) {
+      ierr = -pos;
+      cblas_xerbla(pos, rout, "Diag must be %d or %d, but is set to %d\n", CblasNonUnit,
+                   CblasUnit, Diag);
+    }
+    ++pos;
+  }
+  if (N < 0) {
+    ierr = -pos;
+    cblas_xerbla(pos, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  pos += 2;
+  if (lda < N || lda < 1) {
+    ierr = -pos;
+    cblas_xerbla(pos, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  return ierr;
+}
+
+/* interfaces/lapack/C/src/clapack_dtrtri.c.  Its wrapper only calls
+ * ATL_dtrtri when an argument was bad (`if (ierr)`), so it never inverts;
+ * here the inverse is computed, which is what ATL_trtri and LAPACK's dtrtri
+ * do.  RowMajor is the column-major inverse of the transposed view (Uplo
+ * flipped). */
+int NM3(clapack_, PFX, trtri)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                              const enum CBLAS_DIAG Diag, const int N, T *A, const int lda)
+{
+  static const char *rout = "clapack_" STR(PFX) "trtri";
+  int ierr = NM(PFX, inv_check)(rout, Order, 1, Uplo, 1, Diag, N, lda), low, info;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  info = NM(PFX, trtri_dev)(c, low ? KX_LOWER : KX_UPPER,
+                            Diag == CblasUnit ? KX_UNIT : KX_NONUNIT, N, dA, dla, rout);
+  if (!info) rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
+  rt_sync(c, rout);
+  return info;
+}
+
+/* interfaces/lapack/C/src/clapack_dlauum.c; RowMajor: the other triangle
+ * of the transposed view holds the same product */
+int NM3(clapack_, PFX, lauum)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                              const int N, T *A, const int lda)
+{
+  static const char *rout = "clapack_" STR(PFX) "lauum";
+  int ierr = NM(PFX, inv_check)(rout, Order, 1, Uplo, 0, CblasNonUnit, N, lda), low;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  NM(PFX, lauum_dev)(c, low ? KX_LOWER : KX_UPPER, N, dA, dla, rout);
+  rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
+  rt_sync(c, rout);
+  return 0;
+}
+
+/* interfaces/lapack/C/src/clapack_dpotri.c: TRTRI (non-unit) then LAUUM */
+int NM3(clapack_, PFX, potri)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                              const int N, T *A, const int lda)
+{
+  static const char *rout = "clapack_" STR(PFX) "potri";
+  int ierr = NM(PFX, inv_check)(rout, Order, 1, Uplo, 0, CblasNonUnit, N, lda), low, info;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  info = NM(PFX, trtri_dev)(c, low ? KX_LOWER : KX_UPPER, KX_NONUNIT, N, dA, dla, rout);
+  if (!info) {
+    NM(PFX, lauum_dev)(c, low ? KX_LOWER : KX_UPPER, N, dA, dla, rout);
+    rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
+  }
+  rt_sync(c, rout);
+  return info;
+}
+
+/* interfaces/lapack/C/src/clapack_dgetri.c -> src/lapack/ATL_getri.c.
+ * RowMajor holds the factors of A^T, whose inverse is the row-major
+ * inverse of A, so both orders are the same column-major computation. */
+int NM3(clapack_, PFX, getri)(const enum CBLAS_ORDER Order, const int N, T *A, const int lda,
+                              const int *ipiv)
+{
+  static const char *rout = "clapack_" STR(PFX) "getri";
+  int ierr = NM(PFX, inv_check)(rout, Order, 0, CblasUpper, 0, CblasNonUnit, N, lda), info;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, sizeof(T), 1, &dla, rout);
+  info = NM(PFX, getri_dev)(c, N, dA, dla, NM(PFX, stage_ipiv)(c, ipiv, N, rout), rout);
+  if (!info) rt_stage_out(c, A, dA, N, N, lda, dla, sizeof(T), rout);
+  rt_sync(c, rout);
+  return info;
+}
+
+/* interfaces/lapack/C/src/clapack_dgeqrf.c.  RowMajor there is the LQ
+ * factorization of the column-major view (ATL_gelqf), which for real types
+ * holds the same numbers as the QR of the matrix itself: the storage is
+ * transposed into a workspace, factored, and transposed back. */
+int NM3(clapack_, PFX, geqrf)(const enum CBLAS_ORDER Order, const int M, const int N, T *A,
+                              const int lda, T *TAU)
+{
+  static const char *rout = "clapack_" STR(PFX) "geqrf";
+  int ierr = 0, K;
+  atlmx_ctx *c;
+  long dla;
+  T *dA, *dt;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (M < 0) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "M cannot be less than zero 0,; is set to %d.\n", M);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (Order == CblasColMajor) {
+    if (lda < M || lda < 1) {
+      ierr = -5;
+      cblas_xerbla(5, rout, "lda must be >= MAX(M,1): lda=%d M=%d\n", lda, M);
+    }
+  } else if (lda < N || lda < 1) {
+    ierr = -5;
+    cblas_xerbla(5, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ierr) return ierr;
+  K = M < N ? M : N;
+  if (K == 0) return 0;
+  c = rt_ctx();
+  dt = rt_is_device(TAU) ? TAU : (T *)rt_ws(c, WS_TAU, (size_t)K * sizeof(T), rout);
+  if (Order == CblasColMajor) {
+    dA = (T *)rt_stage_in(c, WS_STAGE_A, A, M, N, lda, sizeof(T), 1, &dla, rout);
+    NM(PFX, geqrf_dev)(c, M, N, dA, dla, dt, rout);
+    rt_stage_out(c, A, dA, M, N, lda, dla, sizeof(T), rout);
+  } else {
+    /* the storage is the column-major N x M matrix A^T */
+    T *dB = (T *)rt_stage_in(c, WS_STAGE_A, A, N, M, lda, sizeof(T), 1, &dla, rout);
+    T *W = (T *)rt_ws(c, WS_STAGE_B, (size_t)M * N * sizeof(T), rout);
+    RT_CHECK(rout, K_TRICOPY(ATLMX_TC_TRANSPOSE, 0, 0, N, M, dB, dla, W, M, c->stream));
+    NM(PFX, geqrf_dev)(c, M, N, W, M, dt, rout);
+    RT_CHECK(rout, K_TRICOPY(ATLMX_TC_TRANSPOSE, 0, 0, M, N, W, M, dB, dla, c->stream));
+    rt_stage_out(c, A, dB, N, M, lda, dla, sizeof(T), rout);
+  }
+  if (dt != TAU)
+    RT_CHECK(rout, hipMemcpyAsync(TAU, dt, (size_t)K * sizeof(T), hipMemcpyDeviceToHost,
+                                  c->stream));
+  rt_sync(c, rout);
+  return 0;
 }
 
 #undef GETRF_DEV

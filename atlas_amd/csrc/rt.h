@@ -30,6 +30,12 @@ enum {
   WS_MP3,
   WS_MP4,
   WS_MPSCALE,       /* mixed precision: column scale factors    */
+  WS_L3A,           /* TRMM/SYMM/LAUUM: dense copy of the triangle or
+                       symmetric matrix; TRTRI/GETRI: the inverse      */
+  WS_L3B,           /* TRMM: copy of B                          */
+  WS_QR,            /* QR: panel partial sums, larfg coefficients */
+  WS_QRV,           /* QR: V, V^T V, T and the two W blocks     */
+  WS_TAU,           /* QR: device copy of TAU                   */
   WS_NSLOTS
 };
 
@@ -50,6 +56,8 @@ typedef struct atlmx_ctx {
 } atlmx_ctx;
 
 atlmx_ctx *rt_ctx(void);
+/* this thread's context if it exists, else NULL (creates nothing) */
+atlmx_ctx *rt_ctx_peek(void);
 
 /* Abort through cblas_xerbla's convention (message on stderr, exit(-1)) on
  * any HIP failure: a GPU error never returns garbage. */

@@ -169,6 +169,8 @@ int atlmx_init(void)
   return 0;
 }
 
+atlmx_ctx *rt_ctx_peek(void) { return tls_ctx; }
+
 /* NULL selects the legacy default stream. */
 void atlmx_set_stream(void *stream)
 {

@@ -36,36 +36,8 @@ def dgesv_flops(n, nrhs):
 
 def make_system(n, kappa, nrhs, seed, torch, X):
     """A = U diag(sigma) V^T with Haar U, V (QR of Gaussian) and b ~ U[-1,1)."""
-    dev = torch.device("cuda:0")
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
-
-    def haar():
-        G = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
-        Q, R = torch.linalg.qr(G)
-        Q *= torch.sign(torch.diagonal(R)).unsqueeze(0)
-        return Q
-
-    U = haar()
-    V = haar()
-    sigma = kappa ** (-torch.arange(n, dtype=torch.float64, device=dev) / (n - 1))
-    U *= sigma.unsqueeze(0)                       # U diag(sigma)
-    # A = (U diag(sigma)) V^T on the library's DGEMM.  torch tensors are
-    # row-major: the column-major view of a row-major M is M^T, so compute
-    # A^T = V (U diag(sigma))^T  <=>  col-major C = opA * opB with
-    # opA = V (col-major view of V^T storage transposed) ...
-    A = torch.empty(n, n, dtype=torch.float64, device=dev)
-    X.use_torch_stream()
-    # column-major views: Vc = V^T, Uc = (U diag s)^T, Ac = A^T.
-    # A^T = V diag(s) U^T = (Vc)^T * Uc  -> gemm(Trans, NoTrans, Vc, Uc)
-    X.lib().atlmx_dgemm(X.Trans, X.NoTrans, n, n, n, 1.0, X.dptr(V), n, X.dptr(U), n, 0.0,
-                        X.dptr(A), n)
-    torch.cuda.synchronize()
-    del U, V
-    b = torch.rand(nrhs, n, dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    # the column-major matrix the library sees is A.T (storage of A); its
-    # condition number is the same.
-    return A, b
+    from atlas_amd import synth
+    return synth.haar_system(n, kappa, nrhs, seed, torch, X)
 
 
 def measured_traffic(kernel, workload):
@@ -97,30 +69,58 @@ def time_steps(fn, steps, warmup, torch):
     return (time.perf_counter() - t0) / steps
 
 
-def cpu_baseline(n_cpu, seed):
-    """The oracle's restatement of ATLAS's CPU dgesv (recursive getrfC with a
-    threaded blocked GEMM, then getrs), timed on this host's cores on a
-    bounded sample of the workload (one n_cpu x n_cpu random system)."""
+def _cpu_has(*flags):
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    have = set(line.split(":", 1)[1].split())
+                    return all(fl in have for fl in flags)
+    except OSError:
+        pass
+    return False
+
+
+def cpu_baseline(n_cpu, seed, torch, X):
+    """The oracle's restatement of ATLAS's CPU dgesv -- recursive getrfC
+    (LU2 leaves, recursive TRSM, blocked GEMM with a 2-vector x 4-column
+    register kernel) then getrs -- timed on this host's cores on a bounded
+    sample of the workload: the same Haar construction at order n_cpu,
+    kappa 1e6, one right-hand side.  The AVX2/FMA build
+    (oracle/liboracle_fast.so) runs when the CPU has those instructions,
+    else the portable one; threads = the CPUs this process may use, at most
+    16 (the host's share for one GPU)."""
+    import ctypes as C
     import numpy as np
-    import oracle as O
+    from atlas_amd import synth
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
     cores = max(1, min(cores, 16))
-    O.set_threads(cores)
-    A = O.gegen(n_cpu, n_cpu, seed=seed)
-    B = O.gegen(n_cpu, 1, seed=seed + 1)
+    fast = _cpu_has("avx2", "fma")
+    lib = C.CDLL(os.path.join(ROOT, "oracle", "liboracle_fast.so" if fast else "liboracle.so"))
+    lib.orc_set_threads.argtypes = [C.c_int]
+    lib.orc_dgesv.argtypes = [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                              C.c_void_p, C.c_int]
+    lib.orc_dgesv.restype = C.c_int
+    lib.orc_set_threads(cores)
+    Ad, bd = synth.haar_system(n_cpu, 1e6, 1, seed + 7, torch, X)
+    A = np.asfortranarray(Ad.cpu().numpy().T)       # the column-major matrix itself
+    B = np.asfortranarray(bd.cpu().numpy().T)
+    del Ad, bd
     ipiv = np.zeros(n_cpu, dtype=np.int32)
     t0 = time.perf_counter()
-    info = O.lib().orc_dgesv(O.COL, n_cpu, 1, A.ctypes.data, n_cpu, ipiv.ctypes.data,
-                             B.ctypes.data, n_cpu)
+    info = lib.orc_dgesv(102, n_cpu, 1, A.ctypes.data, n_cpu, ipiv.ctypes.data, B.ctypes.data,
+                         n_cpu)
     t = time.perf_counter() - t0
-    O.set_threads(1)
+    lib.orc_set_threads(1)
     return {"value": round(dgesv_flops(n_cpu, 1) / t / 1e9, 3), "unit": "GFLOP/s",
             "cores": cores, "kind": "port",
-            "sample": "oracle dgesv (recursive ATL_getrfC restatement + threaded GEMM) on one "
-                      "%dx%d system, 1 RHS, %.1f s, info=%d" % (n_cpu, n_cpu, t, info)}
+            "sample": "oracle dgesv (recursive ATL_getrfC restatement, recursive TRSM, "
+                      "blocked GEMM; %s build) on the Haar system at n=%d, kappa 1e6, "
+                      "1 RHS: %.1f s, info=%d" % ("AVX2/FMA" if fast else "portable", n_cpu, t,
+                                                  info)}
 
 
 def _views(big, offs, shapes):
@@ -283,6 +283,31 @@ def bench_lapack(n, seed, torch, X, log, reps=3):
     t = best(lambda: L.atlmx_dgetrf(n, n, X.dptr(Aw), n, X.dptr(ipiv)), lambda: Aw.copy_(A0))
     fl = 2.0 * n ** 3 / 3 - n ** 2 / 2.0 - n / 6.0
     out["getrf"] = {"ms": round(t * 1e3, 2), "gflops": round(fl / t / 1e9, 1)}
+    # one more factorization with HIP events around the panel launches
+    # (getf2_panel_lds_kernel: bytes = the panel loaded and stored once) and
+    # the GEMM launches (trailing updates and the panel recursion's, flops)
+    Aw.copy_(A0)
+    torch.cuda.synchronize()
+    X.prof_enable((1 << X.PROF_PANEL) | (1 << X.PROF_GEMM))
+    L.atlmx_dgetrf(n, n, X.dptr(Aw), n, X.dptr(ipiv))
+    pms, pcnt, pwork = X.prof_query(X.PROF_PANEL)
+    gms, gcnt, gwork = X.prof_query(X.PROF_GEMM)
+    X.prof_enable(0)
+    if pcnt:
+        bw = pwork / pcnt / (pms / pcnt * 1e-3)
+        out["roofline_panel"] = {
+            "bound": "hbm", "kernel": "getf2_panel_lds_kernel (FP64 LU panel)",
+            "achieved": round(bw / 1e9, 2), "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+            "frac": round(bw / PEAK_HBM, 5), "launches": pcnt,
+            "avg_launch_ms": round(pms / pcnt, 4), "ms_total": round(pms, 2),
+            "algorithmic_bytes_per_launch": round(pwork / pcnt)}
+    if gcnt:
+        fr = gwork / (gms * 1e-3)
+        out["roofline_gemm"] = {
+            "bound": "mfma", "kernel": "gemm_kernel<double> (FP64 LU updates)",
+            "achieved": round(fr / 1e12, 3), "peak": PEAK_FP64_MFMA / 1e12, "unit": "TFLOP/s",
+            "frac": round(fr / PEAK_FP64_MFMA, 4), "launches": gcnt,
+            "ms_total": round(gms, 2)}
     for nr, rhs in R.items():
         Xw = torch.empty_like(rhs)
         t = best(lambda: L.atlmx_dgetrs(X.NoTrans, n, nr, X.dptr(Aw), n, X.dptr(ipiv), X.dptr(Xw), n),
@@ -370,7 +395,7 @@ def main():
     ap.add_argument("--kappa", type=float, default=1e6)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--gemm-n", type=int, default=8192)
-    ap.add_argument("--cpu-n", type=int, default=4096)
+    ap.add_argument("--cpu-n", type=int, default=10240)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched and adjoint timings")
@@ -454,47 +479,79 @@ def main():
 
     xmp = torch.empty_like(b)
     st = [None]
+    # the headline stops refinement with LAPACK dsgesv's test,
+    # ||r||_inf <= ||x||_inf ||A||_inf 2^-53 sqrt(n) (stricter than n eps64)
+    X.set_refine_stop(True)
 
     def mp_step():
         info[0], st[0] = X.dsgesv_dev(A, b, xmp)
 
     tmp = time_steps(mp_step, args.steps, args.warmup, torch)
     # the same steps again with a HIP event pair around every library launch
-    # (on the library's stream): per-kernel-class times and the roofline
+    # (on the library's stream): per-kernel-class times and the rooflines
     X.prof_enable(-1)
     tmp_prof = time_steps(mp_step, args.steps, 0, torch)
-    classes = {}
+    classes, raw = {}, {}
     for cls, name in enumerate(["gemm", "panel", "trsm_inv", "laswp", "potf2", "misc", "trsv",
                                 "update"]):
         ms, cnt, work, nbytes = X.prof_query(cls, with_bytes=True)
+        raw[name] = (ms, cnt, work, nbytes)
         if cnt:
             classes[name] = {"ms_per_step": round(ms / args.steps, 3),
                              "launches_per_step": cnt // args.steps}
-            if cls == X.PROF_UPDATE:
-                gemm_ms, gemm_cnt, gemm_work, gemm_bytes = ms, cnt, work, nbytes
     X.prof_enable(0)
+    X.set_refine_stop(False)
     bemp = L.atlmx_dbackward_error(n, nrhs, X.dptr(A), n, X.dptr(b), n, X.dptr(xmp), n)
+    # checked with torch (rocBLAS), independently of the library: LAPACK's
+    # stopping ratio ||r|| / (||x|| ||A|| 2^-53 sqrt(n)) (pass <= 1) and the
+    # forward error against the FP64 LU's solution
+    Ainf = float(A.abs().sum(dim=0).max().item())    # rows of the matrix = columns of A's storage
+    r = b.T - A.T @ xmp.T
+    lapack_ratio = float((r.abs().amax(dim=0) /
+                          (xmp.T.abs().amax(dim=0) * Ainf * 2.0 ** -53 * n ** 0.5)).max().item())
+    fwd = float(((xmp - x64).abs().amax(dim=1) / x64.abs().amax(dim=1)).max().item())
+    del r
     gmp = dgesv_flops(n, nrhs) / tmp / 1e9
     log("mixed dsgesv: %.1f ms, %.1f GFLOP/s, iters %d, converged %d, fallback %d, "
-        "backward error %.3g" % (tmp * 1e3, gmp, st[0].iterations, st[0].converged,
-                                 st[0].fallback, bemp))
+        "backward error %.3g, LAPACK stop ratio %.3g, forward error vs FP64 %.3g" %
+        (tmp * 1e3, gmp, st[0].iterations, st[0].converged, st[0].fallback, bemp,
+         lapack_ratio, fwd))
     ok_mp = (bemp <= n * EPS64) and not st[0].fallback
     value = gmp if ok_mp else g64
 
-    # roofline of the dominant compute kernel of the headline solve: the FP32
-    # MFMA trailing update of the LU (2*M*N*K flops per launch); the panels'
-    # pivot choice runs on a second stream, overlapped with these updates,
-    # so only the first panel's appears under "panel"
-    gemm_avg_ms = gemm_ms / gemm_cnt
-    gemm_flops_per_launch = gemm_work / gemm_cnt
-    achieved = gemm_flops_per_launch / (gemm_avg_ms * 1e-3)
-    roofline = {"bound": "mfma", "kernel": "update_kernel<float,32> (FP32 LU trailing updates)",
-                "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32_MFMA / 1e12,
-                "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_MFMA, 4),
-                "traffic": measured_traffic("update_kernel<float,32>", "tslu_n%d" % n),
-                "launches": gemm_cnt,
-                "avg_launch_ms": round(gemm_avg_ms, 4),
-                "algorithmic_bytes_per_launch": round(gemm_bytes / gemm_cnt)}
+    def kernel_roof(name, kernel, bound, peak, unit, traffic_key, per_step_ms):
+        ms, cnt, work, nbytes = raw[name]
+        if not cnt:
+            return None
+        avg = ms / cnt
+        if bound == "hbm":
+            per_launch = work / cnt                      # algorithmic bytes
+            achieved = per_launch / (avg * 1e-3) / 1e9
+        else:
+            per_launch = work / cnt                      # flops
+            achieved = per_launch / (avg * 1e-3) / 1e12
+        return {"bound": bound, "kernel": kernel, "achieved": round(achieved, 3),
+                "peak": peak, "unit": unit, "frac": round(achieved / peak, 5),
+                "traffic": measured_traffic(kernel.split(" ")[0], traffic_key),
+                "launches": cnt, "avg_launch_ms": round(avg, 4),
+                ("algorithmic_bytes_per_launch" if bound == "hbm" else "flops_per_launch"):
+                    round(per_launch),
+                "ms_per_step": round(per_step_ms, 3)}
+
+    # The kernel that dominates the headline step is the tournament pivot
+    # choice (tslu_select_kernel, class "panel"): its algorithmic bytes are
+    # the panel read once, (n - k) x 64 FP32 values per launch; it is bound
+    # by the latency of its 64 dependent elimination steps per tree level,
+    # so its HBM fraction is tiny.  The FP32 MFMA trailing update
+    # (update_kernel, class "update") is reported beside it.
+    roofline = kernel_roof("panel", "tslu_select_kernel (tournament pivoting, FP32 LU panel)",
+                           "hbm", PEAK_HBM / 1e9, "GB/s", "tslu_n%d" % n,
+                           raw["panel"][0] / args.steps)
+    roof_update = kernel_roof("update", "update_kernel<float,32> (FP32 LU trailing updates)",
+                              "mfma", PEAK_FP32_MFMA / 1e12, "TFLOP/s", "tslu_n%d" % n,
+                              raw["update"][0] / args.steps)
+    if raw["update"][1]:
+        roof_update["algorithmic_bytes_per_launch"] = round(raw["update"][3] / raw["update"][1])
 
     lapack = bench_lapack(8192, args.seed + 8192, torch, X, log)
 
@@ -508,7 +565,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(args.cpu_n, args.seed)
+        cpu = cpu_baseline(args.cpu_n, args.seed, torch, X)
         log("cpu baseline: %s" % cpu)
 
     out = {
@@ -522,6 +579,8 @@ def main():
         "fp64_dgesv_gflops": round(g64, 2), "fp64_ms_per_step": round(t64 * 1e3, 3),
         "fp64_backward_error": be64,
         "mixed_dgesv_gflops": round(gmp, 2), "mixed_backward_error": bemp,
+        "mixed_refine_stop": "LAPACK dsgesv: ||r|| <= ||x|| ||A|| 2^-53 sqrt(n)",
+        "mixed_lapack_stop_ratio": lapack_ratio, "mixed_forward_error_vs_fp64": fwd,
         "mixed_iterations": st[0].iterations, "mixed_fallback": st[0].fallback,
         "mixed_factorization": {1: "fp32 LU, tournament pivoting",
                                 2: "fp32 partial-pivoting LU",
@@ -530,7 +589,7 @@ def main():
         "dgemm_n": gn, "dgemm_tflops": round(dgemm_tflops, 3),
         "dgemm_pct_peak": round(100 * dgemm_tflops * 1e12 / PEAK_FP64_MFMA, 2),
         "kernel_time_per_step": classes, "profiled_ms_per_step": round(tmp_prof * 1e3, 3),
-        "roofline": roofline, "cpu_baseline": cpu, "lapack_n8192": lapack, "sweep": sweep,
+        "roofline": roofline, "roofline_update": roof_update, "cpu_baseline": cpu, "lapack_n8192": lapack, "sweep": sweep,
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

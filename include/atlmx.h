@@ -78,7 +78,11 @@ void atlmx_spotrs(int Uplo, int N, int NRHS, const float *A, int lda, float *B, 
  *         then B[i] <- A[i]^{-1} B[i]; info[i] as clapack_?getrf (no solve
  *         when nonzero).
  *   gemm: C[i] = alpha op(A[i]) op(B[i]) + beta C[i].
- * Orders up to 512 (posv) / 256..512 (gesv, by precision), nrhs <= 8. */
+ * nrhs <= 8.  Orders are bounded by what one workgroup holds in LDS:
+ * atlmx_batch_max_order(kind, dbl) (kind 0 posv, 1 gesv): 2270 / 768 in
+ * FP64, 4830 / 1621 in FP32.  A plan with a larger order is refused: a
+ * NULL plan and a message on stderr. */
+int atlmx_batch_max_order(int kind, int dbl);
 typedef struct atlmx_batch atlmx_batch;
 atlmx_batch *atlmx_dposv_batch_plan(int count, const int *h_n, double *const *d_A,
                                     const int *h_lda, int nrhs, double *const *d_B,

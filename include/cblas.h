@@ -77,6 +77,41 @@ void cblas_dtrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
                  const double alpha, const double *A, const int lda, double *B,
                  const int ldb);
 
+/* Replaces ATLAS include/cblas.h:460 / :490 (cblas_?trmm,
+ * interfaces/blas/C/src/cblas_dtrmm.c -> src/blas/ulevel3/ATL_trmm.c:149). */
+void cblas_strmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N,
+                 const float alpha, const float *A, const int lda, float *B,
+                 const int ldb);
+void cblas_dtrmm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N,
+                 const double alpha, const double *A, const int lda, double *B,
+                 const int ldb);
+
+/* Replaces ATLAS include/cblas.h:446 / :476 (cblas_?symm,
+ * interfaces/blas/C/src/cblas_dsymm.c -> src/blas/ulevel3/ATL_symmL.c). */
+void cblas_ssymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N,
+                 const float alpha, const float *A, const int lda, const float *B,
+                 const int ldb, const float beta, float *C, const int ldc);
+void cblas_dsymm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const int M, const int N,
+                 const double alpha, const double *A, const int lda, const double *B,
+                 const int ldb, const double beta, double *C, const int ldc);
+
+/* Replaces ATLAS include/cblas.h:455 / :485 (cblas_?syr2k,
+ * interfaces/blas/C/src/cblas_dsyr2k.c -> src/blas/ulevel3/ATL_syr2k.c:255). */
+void cblas_ssyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                  const float alpha, const float *A, const int lda, const float *B,
+                  const int ldb, const float beta, float *C, const int ldc);
+void cblas_dsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                  const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                  const double alpha, const double *A, const int lda, const double *B,
+                  const int ldb, const double beta, double *C, const int ldc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,38 @@ int clapack_dpotrs(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
                    const int N, const int NRHS, const double *A, const int lda,
                    double *B, const int ldb);
 
+/* The inverses: ATLAS include/clapack.h:30,40,42,44 (single) and
+ * :69,79,81,83 (double); interfaces/lapack/C/src/clapack_dgetri.c ->
+ * src/lapack/ATL_getri.c:8, clapack_dpotri.c, clapack_dlauum.c ->
+ * ATL_lauum.c:6, clapack_dtrtri.c -> ATL_trtri.c:8.  Positive info: the
+ * 1-based index of the first exactly-zero diagonal entry of U (getri) or of
+ * A (trtri, potri, non-unit); A is left unchanged then. */
+int clapack_sgetri(const enum CBLAS_ORDER Order, const int N, float *A, const int lda,
+                   const int *ipiv);
+int clapack_spotri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   float *A, const int lda);
+int clapack_slauum(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   float *A, const int lda);
+int clapack_strtri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                   const enum CBLAS_DIAG Diag, const int N, float *A, const int lda);
+int clapack_dgetri(const enum CBLAS_ORDER Order, const int N, double *A, const int lda,
+                   const int *ipiv);
+int clapack_dpotri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   double *A, const int lda);
+int clapack_dlauum(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   double *A, const int lda);
+int clapack_dtrtri(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                   const enum CBLAS_DIAG Diag, const int N, double *A, const int lda);
+
+/* QR: ATLAS include/clapack.h:57 / :96, interfaces/lapack/C/src/
+ * clapack_dgeqrf.c -> src/lapack/ATL_geqrf.c:29.  LAPACK's storage (R on
+ * and above the diagonal, Householder vectors below, TAU[min(M,N)]);
+ * returns 0. */
+int clapack_sgeqrf(const enum CBLAS_ORDER Order, const int M, const int N, float *A,
+                   const int lda, float *TAU);
+int clapack_dgeqrf(const enum CBLAS_ORDER Order, const int M, const int N, double *A,
+                   const int lda, double *TAU);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,12 @@ void P##syrk_ref(int uplo, int trans, int N, int K, T alpha, const T *A,      \
                  int lda, T beta, T *C, int ldc);                             \
 void P##trmm_ref(int side, int uplo, int ta, int diag, int M, int N, T alpha, \
                  const T *A, int lda, T *B, int ldb);                         \
+void P##trmm_fast(int side, int uplo, int ta, int diag, int M, int N,         \
+                  T alpha, const T *A, int lda, T *B, int ldb);               \
+void P##symm_ref(int side, int uplo, int M, int N, T alpha, const T *A,       \
+                 int lda, const T *B, int ldb, T beta, T *C, int ldc);        \
+void P##syr2k_ref(int uplo, int trans, int N, int K, T alpha, const T *A,     \
+                  int lda, const T *B, int ldb, T beta, T *C, int ldc);       \
 void P##laswp(int N, T *A, int lda, int K1, int K2, const int *piv, int inci);\
 int  P##getf2(int M, int N, T *A, int lda, int *ipiv);                        \
 int  P##getrf(int order, int M, int N, T *A, int lda, int *ipiv);             \
@@ -101,7 +107,19 @@ int  P##gemm_ferr_check(int M, int N, int K, T alpha, T beta, const T *C,     \
                         double *ferr);                                        \
 T    P##trsm_resid(int side, int uplo, int ta, int diag, int M, int N,        \
                    T alpha, const T *A, int lda, const T *X, int ldx,         \
-                   const T *B0, int ldb0);
+                   const T *B0, int ldb0);                                   \
+T    P##trinrm1(int uplo, int diag, int N, const T *A, int lda);              \
+void P##trigen(int uplo, int diag, int N, T *A, int lda, T padval, int seed); \
+T    P##trtri_resid(int uplo, int diag, int N, const T *Ainv, int ldi,        \
+                    const T *A0, int lda0);                                   \
+T    P##inv_resid(int N, const T *A, int lda, const T *AI, int ldi);          \
+T    P##poinv_resid(int uplo, int N, const T *A, int lda, const T *AI,        \
+                    int ldi);                                                 \
+T    P##lauum_resid(int uplo, int N, const T *R, int ldr, const T *T0,        \
+                    int ld0);                                                 \
+void P##geqr2(int M, int N, T *A, int lda, T *tau);                           \
+T    P##qr_resid(int M, int N, const T *QR, int lda, const T *tau,            \
+                 const T *A0, int lda0);
 
 ORC_DECLARE(orc_d, double)
 ORC_DECLARE(orc_s, float)

@@ -80,6 +80,9 @@ def _declare(L):
             g(n).argtypes = [_I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I]
         g("trsm").argtypes = [_I, _I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I]
         g("syrk_ref").argtypes = [_I, _I, _I, _I, t, _P, _I, t, _P, _I]
+        g("trmm_fast").argtypes = [_I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I]
+        g("symm_ref").argtypes = [_I, _I, _I, _I, t, _P, _I, _P, _I, t, _P, _I]
+        g("syr2k_ref").argtypes = [_I, _I, _I, _I, t, _P, _I, _P, _I, t, _P, _I]
         g("laswp").argtypes = [_I, _P, _I, _I, _I, _P, _I]
         g("getf2").argtypes = [_I, _I, _P, _I, _P]
         g("getf2").restype = _I
@@ -110,6 +113,20 @@ def _declare(L):
         g("gemm_ferr_check").restype = _I
         g("trsm_resid").argtypes = [_I, _I, _I, _I, _I, _I, t, _P, _I, _P, _I, _P, _I]
         g("trsm_resid").restype = t
+        g("trinrm1").argtypes = [_I, _I, _I, _P, _I]
+        g("trinrm1").restype = t
+        g("trigen").argtypes = [_I, _I, _I, _P, _I, t, _I]
+        g("trtri_resid").argtypes = [_I, _I, _I, _P, _I, _P, _I]
+        g("trtri_resid").restype = t
+        g("inv_resid").argtypes = [_I, _P, _I, _P, _I]
+        g("inv_resid").restype = t
+        g("poinv_resid").argtypes = [_I, _I, _P, _I, _P, _I]
+        g("poinv_resid").restype = t
+        g("lauum_resid").argtypes = [_I, _I, _P, _I, _P, _I]
+        g("lauum_resid").restype = t
+        g("geqr2").argtypes = [_I, _I, _P, _I, _P]
+        g("qr_resid").argtypes = [_I, _I, _P, _I, _P, _P, _I]
+        g("qr_resid").restype = t
 
 
 def _p(a):
@@ -208,6 +225,44 @@ def syrk(uplo, trans, alpha, A, beta, C):
     return C
 
 
+def trmm(side, uplo, ta, diag, alpha, A, B):
+    """B (Fortran, in place) = alpha op(A) B (Left) or alpha B op(A) (Right),
+    in the reference loop order (ATL_dreftrmm*.c)."""
+    M, N = B.shape
+    _fn(B.dtype, "trmm_ref")(side, uplo, ta, diag, M, N, alpha, _p(A), _ld(A), _p(B), _ld(B))
+    return B
+
+
+def symm(side, uplo, alpha, A, B, beta, Cm):
+    M, N = Cm.shape
+    _fn(Cm.dtype, "symm_ref")(side, uplo, M, N, alpha, _p(A), _ld(A), _p(B), _ld(B), beta,
+                              _p(Cm), _ld(Cm))
+    return Cm
+
+
+def syr2k(uplo, trans, alpha, A, B, beta, Cm):
+    N = Cm.shape[0]
+    K = A.shape[1] if trans == NOTRANS else A.shape[0]
+    _fn(Cm.dtype, "syr2k_ref")(uplo, trans, N, K, alpha, _p(A), _ld(A), _p(B), _ld(B), beta,
+                               _p(Cm), _ld(Cm))
+    return Cm
+
+
+def infnrm(x):
+    x = np.ascontiguousarray(x)
+    return float(_fn(x.dtype, "infnrm")(x.size, _p(x), 1))
+
+
+def gediffnrm1(A, B):
+    return float(_fn(A.dtype, "gediffnrm1")(A.shape[0], A.shape[1], _p(A), _ld(A), _p(B),
+                                            _ld(B)))
+
+
+def lltnrm1(uplo, A):
+    N = A.shape[1]
+    return float(_fn(A.dtype, "lltnrm1")(uplo, N, _p(A), _ld(A)))
+
+
 # ---------------------------------------------------------------- LAPACK
 def getrf(A, order=COL, gaps=None):
     """In-place recursive LU (ATL_getrfC).  Returns (info, ipiv)."""
@@ -303,3 +358,49 @@ def trsm_resid(side, uplo, ta, diag, alpha, A, X, B0):
     M, N = X.shape
     return float(_fn(X.dtype, "trsm_resid")(side, uplo, ta, diag, M, N, alpha, _p(A), _ld(A),
                                             _p(X), _ld(X), _p(B0), _ld(B0)))
+
+
+def trigen(uplo, diag, N, lda=None, padval=0.0, seed=None, dtype=np.float64):
+    """bin/trtritst.c trigen (column-major), seed N*1029+lda by default."""
+    lda = lda or max(N, 1)
+    seed = N * 1029 + lda if seed is None else seed
+    A = np.zeros((lda, N), dtype=dtype, order="F")
+    _fn(dtype, "trigen")(uplo, diag, N, _p(A), lda, padval, seed)
+    return A
+
+
+def trinrm1(uplo, diag, A):
+    return float(_fn(A.dtype, "trinrm1")(uplo, diag, A.shape[1], _p(A), _ld(A)))
+
+
+def trtri_resid(uplo, diag, Ainv, A0):
+    N = A0.shape[1]
+    return float(_fn(A0.dtype, "trtri_resid")(uplo, diag, N, _p(Ainv), _ld(Ainv), _p(A0),
+                                              _ld(A0)))
+
+
+def inv_resid(A, AI):
+    return float(_fn(A.dtype, "inv_resid")(A.shape[1], _p(A), _ld(A), _p(AI), _ld(AI)))
+
+
+def poinv_resid(uplo, A, AI):
+    return float(_fn(A.dtype, "poinv_resid")(uplo, A.shape[1], _p(A), _ld(A), _p(AI), _ld(AI)))
+
+
+def lauum_resid(uplo, R, T0):
+    return float(_fn(R.dtype, "lauum_resid")(uplo, T0.shape[1], _p(R), _ld(R), _p(T0),
+                                             _ld(T0)))
+
+
+def geqr2(A):
+    """In-place unblocked Householder QR (ATL_geqr2.c / ATL_larfg.c); tau."""
+    M, N = A.shape
+    tau = np.zeros(max(min(M, N), 1), dtype=A.dtype)
+    _fn(A.dtype, "geqr2")(M, N, _p(A), _ld(A), _p(tau))
+    return tau[:min(M, N)]
+
+
+def qr_resid(QR, tau, A0):
+    M, N = A0.shape
+    tau = np.ascontiguousarray(tau, dtype=A0.dtype)
+    return float(_fn(A0.dtype, "qr_resid")(M, N, _p(QR), _ld(QR), _p(tau), _p(A0), _ld(A0)))

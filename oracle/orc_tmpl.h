@@ -100,7 +100,8 @@ T F_(gediffnrm1)(int M, int N, const T *A, int lda, const T *B, int ldb)
    return mx;
 }
 
-/* src/testing/ATL_infnrm.c: max |x_i| */
+/* src/testing/ATL_infnrm.c:9-35: max |x_i|; a NaN entry is returned as
+ * soon as it is met (:30-31). */
 T F_(infnrm)(int N, const T *X, int incX)
 {
    int i;
@@ -108,6 +109,7 @@ T F_(infnrm)(int N, const T *X, int incX)
    for (i = 0; i < N; i++)
    {
       t = ORC_ABS(X[(size_t)i*incX]);
+      if (t != t) return t;
       if (t > mx) mx = t;
    }
    return mx;
@@ -260,57 +262,101 @@ typedef struct {
    int lda, ldb, ldc;
 } F_(gemm_job_t);
 
+/* C(mb x nb) += Ap Bp with Ap packed [l][i] (mb per l) and Bp packed
+ * [j][l] (kb per j): register tiles of 2 vectors x 4 columns, the vectors
+ * GCC generic vectors of 32 bytes (AVX2 where the build targets it). */
+typedef T F_(vec_t) __attribute__((vector_size(32), aligned(sizeof(T))));
+#define ORC_VL ((int)(32 / sizeof(T)))
+static void F_(gemm_micro)(int mb, int nb, int kb, const T *ap, const T *bp, T *C, int ldc)
+{
+   const int MR = 2 * ORC_VL, NR = 4;
+   int i0, j0, i, j, l;
+   for (j0 = 0; j0 + NR <= nb; j0 += NR)
+   {
+      const T *b0 = bp + (size_t)j0*kb, *b1 = b0 + kb, *b2 = b1 + kb, *b3 = b2 + kb;
+      for (i0 = 0; i0 + MR <= mb; i0 += MR)
+      {
+         F_(vec_t) c00 = {0}, c01 = {0}, c02 = {0}, c03 = {0};
+         F_(vec_t) c10 = {0}, c11 = {0}, c12 = {0}, c13 = {0};
+         for (l = 0; l < kb; l++)
+         {
+            /* the vector type is aligned to one element: unaligned loads */
+            const F_(vec_t) a0 = *(const F_(vec_t) *)(ap + (size_t)l*mb + i0);
+            const F_(vec_t) a1 = *(const F_(vec_t) *)(ap + (size_t)l*mb + i0 + ORC_VL);
+            c00 += a0 * b0[l]; c10 += a1 * b0[l];
+            c01 += a0 * b1[l]; c11 += a1 * b1[l];
+            c02 += a0 * b2[l]; c12 += a1 * b2[l];
+            c03 += a0 * b3[l]; c13 += a1 * b3[l];
+         }
+         for (i = 0; i < ORC_VL; i++)
+         {
+            T *c = C + (size_t)j0*ldc + i0 + i;
+            c[0] += c00[i]; c[ORC_VL] += c10[i];
+            c[ldc] += c01[i]; c[ldc + ORC_VL] += c11[i];
+            c[2 * (size_t)ldc] += c02[i]; c[2 * (size_t)ldc + ORC_VL] += c12[i];
+            c[3 * (size_t)ldc] += c03[i]; c[3 * (size_t)ldc + ORC_VL] += c13[i];
+         }
+      }
+      for (i = i0; i < mb; i++)                   /* leftover rows */
+         for (j = j0; j < j0 + NR; j++)
+         {
+            T acc = 0;
+            for (l = 0; l < kb; l++) acc += ap[(size_t)l*mb + i] * bp[(size_t)j*kb + l];
+            C[(size_t)j*ldc + i] += acc;
+         }
+   }
+   for (j = j0; j < nb; j++)                      /* leftover columns */
+      for (i = 0; i < mb; i++)
+      {
+         T acc = 0;
+         for (l = 0; l < kb; l++) acc += ap[(size_t)l*mb + i] * bp[(size_t)j*kb + l];
+         C[(size_t)j*ldc + i] += acc;
+      }
+}
+#undef ORC_VL
+
 static void F_(gemm_block)(const F_(gemm_job_t) *g)
 {
-   T *bp = (T *)malloc(sizeof(T) * ORC_KB * ORC_NB);
-   T *ap = (T *)malloc(sizeof(T) * ORC_KB * ORC_MB);
-   int j0, k0, i0, i, j, l;
-   for (j0 = g->j0; j0 < g->j1; j0 += ORC_NB)
+   const int ncol = g->j1 - g->j0;
+   T *bp, *ap;
+   int k0, i0, jb, i, j, l;
+   if (ncol <= 0) return;
+   for (j = g->j0; j < g->j1; j++)
    {
-      const int nb = (g->j1 - j0 < ORC_NB) ? g->j1 - j0 : ORC_NB;
-      for (j = j0; j < j0 + nb; j++)
-      {
-         T *c = g->C + (size_t)j*g->ldc;
-         if (g->beta == 0) for (i = 0; i < g->M; i++) c[i] = 0;
-         else if (g->beta != 1) for (i = 0; i < g->M; i++) c[i] *= g->beta;
-      }
-      for (k0 = 0; k0 < g->K; k0 += ORC_KB)
-      {
-         const int kb = (g->K - k0 < ORC_KB) ? g->K - k0 : ORC_KB;
-         for (j = 0; j < nb; j++)
-            for (l = 0; l < kb; l++)
-            {
-               const int gj = j0 + j, gl = k0 + l;
-               T b = (g->TB == ORC_NOTRANS) ? g->B[(size_t)gj*g->ldb + gl]
-                                            : g->B[(size_t)gl*g->ldb + gj];
-               bp[(size_t)j*kb + l] = g->alpha * b;
-            }
-         for (i0 = 0; i0 < g->M; i0 += ORC_MB)
+      T *c = g->C + (size_t)j*g->ldc;
+      if (g->beta == 0) for (i = 0; i < g->M; i++) c[i] = 0;
+      else if (g->beta != 1) for (i = 0; i < g->M; i++) c[i] *= g->beta;
+   }
+   bp = (T *)malloc(sizeof(T) * ORC_KB * (size_t)ncol);
+   ap = (T *)malloc(sizeof(T) * ORC_KB * ORC_MB);
+   for (k0 = 0; k0 < g->K; k0 += ORC_KB)
+   {
+      const int kb = (g->K - k0 < ORC_KB) ? g->K - k0 : ORC_KB;
+      /* alpha * op(B) for this thread's columns, [j][l] */
+      for (j = 0; j < ncol; j++)
+         for (l = 0; l < kb; l++)
          {
-            const int mb = (g->M - i0 < ORC_MB) ? g->M - i0 : ORC_MB;
-            /* pack op(A) block as [l][i] so the inner loop runs over i */
+            const int gj = g->j0 + j, gl = k0 + l;
+            T b = (g->TB == ORC_NOTRANS) ? g->B[(size_t)gj*g->ldb + gl]
+                                         : g->B[(size_t)gl*g->ldb + gj];
+            bp[(size_t)j*kb + l] = g->alpha * b;
+         }
+      for (i0 = 0; i0 < g->M; i0 += ORC_MB)
+      {
+         const int mb = (g->M - i0 < ORC_MB) ? g->M - i0 : ORC_MB;
+         /* op(A) block as [l][i], packed once per (k0, i0) */
+         if (g->TA == ORC_NOTRANS)
             for (l = 0; l < kb; l++)
-               for (i = 0; i < mb; i++)
-               {
-                  const int gi = i0 + i, gl = k0 + l;
-                  ap[(size_t)l*mb + i] = (g->TA == ORC_NOTRANS)
-                     ? g->A[(size_t)gl*g->lda + gi]
-                     : g->A[(size_t)gi*g->lda + gl];
-               }
-            for (j = 0; j < nb; j++)
-            {
-               T *c = g->C + (size_t)(j0 + j)*g->ldc + i0;
-               const T *b = bp + (size_t)j*kb;
-               T acc[ORC_MB];
-               for (i = 0; i < mb; i++) acc[i] = 0;
+               memcpy(ap + (size_t)l*mb, g->A + (size_t)(k0 + l)*g->lda + i0, sizeof(T) * mb);
+         else
+            for (i = 0; i < mb; i++)
                for (l = 0; l < kb; l++)
-               {
-                  const T bl = b[l];
-                  const T *a = ap + (size_t)l*mb;
-                  for (i = 0; i < mb; i++) acc[i] += a[i] * bl;
-               }
-               for (i = 0; i < mb; i++) c[i] += acc[i];
-            }
+                  ap[(size_t)l*mb + i] = g->A[(size_t)(i0 + i)*g->lda + k0 + l];
+         for (jb = 0; jb < ncol; jb += ORC_NB)
+         {
+            const int nb = (ncol - jb < ORC_NB) ? ncol - jb : ORC_NB;
+            F_(gemm_micro)(mb, nb, kb, ap, bp + (size_t)jb*kb,
+                           g->C + (size_t)(g->j0 + jb)*g->ldc + i0, g->ldc);
          }
       }
    }
@@ -356,6 +402,24 @@ void F_(gemm_fast)(int TA, int TB, int M, int N, int K, T alpha,
          pthread_create(&th[t], NULL, F_(gemm_thread), &jobs[t]);
    for (t = 0; t < nt; t++)
       if (jobs[t].j0 < jobs[t].j1) pthread_join(th[t], NULL);
+}
+
+/* op = left, lower, no-transpose, unit: the TRSM of the LU recursion,
+ * recursive halving with GEMM updates as src/blas/ulevel3/ATL_trsm.c does
+ * it, the reference loops at the 64-row leaves. */
+static void F_(trsm_llnu)(int M, int N, const T *A, int lda, T *B, int ldb)
+{
+   int m1;
+   if (M <= 64)
+   {
+      F_(trsm_ref)(ORC_LEFT, ORC_LOWER, ORC_NOTRANS, ORC_UNIT, M, N, (T)1, A, lda, B, ldb);
+      return;
+   }
+   m1 = M / 2;
+   F_(trsm_llnu)(m1, N, A, lda, B, ldb);
+   F_(gemm_fast)(ORC_NOTRANS, ORC_NOTRANS, M - m1, N, m1, (T)-1, A + m1, lda, B, ldb, (T)1,
+                 B + m1, ldb);
+   F_(trsm_llnu)(M - m1, N, A + m1 + (size_t)m1*lda, lda, B + m1, ldb);
 }
 
 /* interfaces/blas/C/src/cblas_dgemm.c:130-134: ColMajor direct, RowMajor by
@@ -494,11 +558,12 @@ void F_(trsm)(int order, int side, int uplo, int ta, int diag, int M, int N,
                    ta, diag, N, M, alpha, A, lda, B, ldb);
 }
 
-/* B := alpha*op(A)*B or alpha*B*op(A), A triangular.  Needed by the lutst
- * residual (bin/lutst.c:84-134 calls cblas_trmm).  Only the value matters,
- * so it is written as a dense product with the triangle expanded. */
-void F_(trmm_ref)(int side, int uplo, int ta, int diag, int M, int N, T alpha,
-                  const T *A, int lda, T *B, int ldb)
+/* B := alpha*op(A)*B or alpha*B*op(A), A triangular, as a dense product
+ * with the triangle expanded (threaded GEMM).  The lutst residual uses it
+ * (bin/lutst.c:84-134 calls cblas_trmm), where only the value matters;
+ * trmm_ref below keeps the reference loop orders. */
+void F_(trmm_fast)(int side, int uplo, int ta, int diag, int M, int N, T alpha,
+                   const T *A, int lda, T *B, int ldb)
 {
    const int na = (side == ORC_LEFT) ? M : N;
    int i, j;
@@ -525,6 +590,254 @@ void F_(trmm_ref)(int side, int uplo, int ta, int diag, int M, int N, T alpha,
    free(Tm);
    free(W);
 }
+
+/* B := alpha*op(A)*B (Left) or alpha*B*op(A) (Right), A triangular, in the
+ * loop order of src/blas/reference/level3/ATL_dreftrmm*.c: alpha == 0
+ * zeroes B (ATL_dreftrmm.c:180-183); each branch keeps where alpha and the
+ * diagonal enter:
+ *   LLN*: ATL_dreftrmmLLNN.c  LUN*: ATL_dreftrmmLUNN.c
+ *   LLT*: ATL_dreftrmmLLTN.c  LUT*: ATL_dreftrmmLUTN.c
+ *   RLN*: ATL_dreftrmmRLNN.c  RUN*: ATL_dreftrmmRUNN.c
+ *   RLT*: ATL_dreftrmmRLTN.c  RUT*: ATL_dreftrmmRUTN.c
+ * The *U (unit) files drop the diagonal product. */
+void F_(trmm_ref)(int side, int uplo, int ta, int diag, int M, int N, T alpha,
+                  const T *A, int lda, T *B, int ldb)
+{
+   int i, j, k;
+   T t0;
+   const int nu = (diag == ORC_NONUNIT);
+   const int tr = (ta != ORC_NOTRANS);
+   if (M == 0 || N == 0) return;
+   if (alpha == 0)
+   {
+      for (j = 0; j < N; j++) for (i = 0; i < M; i++) BE(i, j) = 0;
+      return;
+   }
+   if (side == ORC_LEFT)
+   {
+      if (uplo == ORC_LOWER && !tr)            /* rows bottom-up, axpy below k */
+         for (j = 0; j < N; j++)
+            for (k = M - 1; k >= 0; k--)
+            {
+               t0 = alpha * BE(k, j);
+               BE(k, j) = t0;
+               if (nu) BE(k, j) *= AE(k, k);
+               for (i = k + 1; i < M; i++) BE(i, j) += t0 * AE(i, k);
+            }
+      else if (uplo == ORC_UPPER && !tr)       /* rows top-down, axpy above k */
+         for (j = 0; j < N; j++)
+            for (k = 0; k < M; k++)
+            {
+               t0 = alpha * BE(k, j);
+               for (i = 0; i < k; i++) BE(i, j) += t0 * AE(i, k);
+               if (nu) t0 *= AE(k, k);
+               BE(k, j) = t0;
+            }
+      else if (uplo == ORC_LOWER)              /* L^T: rows top-down, dot */
+         for (j = 0; j < N; j++)
+            for (i = 0; i < M; i++)
+            {
+               t0 = BE(i, j);
+               if (nu) t0 *= AE(i, i);
+               for (k = i + 1; k < M; k++) t0 += AE(k, i) * BE(k, j);
+               BE(i, j) = alpha * t0;
+            }
+      else                                     /* U^T: rows bottom-up, dot */
+         for (j = 0; j < N; j++)
+            for (i = M - 1; i >= 0; i--)
+            {
+               t0 = BE(i, j);
+               if (nu) t0 *= AE(i, i);
+               for (k = 0; k < i; k++) t0 += AE(k, i) * BE(k, j);
+               BE(i, j) = alpha * t0;
+            }
+   }
+   else
+   {
+      if (uplo == ORC_LOWER && !tr)            /* columns left to right */
+         for (j = 0; j < N; j++)
+         {
+            t0 = alpha;
+            if (nu) t0 *= AE(j, j);
+            for (i = 0; i < M; i++) BE(i, j) *= t0;
+            for (k = j + 1; k < N; k++)
+            {
+               t0 = alpha * AE(k, j);
+               for (i = 0; i < M; i++) BE(i, j) += t0 * BE(i, k);
+            }
+         }
+      else if (uplo == ORC_UPPER && !tr)       /* columns right to left */
+         for (j = N - 1; j >= 0; j--)
+         {
+            t0 = alpha;
+            if (nu) t0 *= AE(j, j);
+            for (i = 0; i < M; i++) BE(i, j) *= t0;
+            for (k = 0; k < j; k++)
+            {
+               t0 = alpha * AE(k, j);
+               for (i = 0; i < M; i++) BE(i, j) += t0 * BE(i, k);
+            }
+         }
+      else if (uplo == ORC_LOWER)              /* B*L^T: k right to left */
+         for (k = N - 1; k >= 0; k--)
+         {
+            for (j = k + 1; j < N; j++)
+            {
+               t0 = alpha * AE(j, k);
+               for (i = 0; i < M; i++) BE(i, j) += t0 * BE(i, k);
+            }
+            t0 = alpha;
+            if (nu) t0 *= AE(k, k);
+            for (i = 0; i < M; i++) BE(i, k) = t0 * BE(i, k);
+         }
+      else                                     /* B*U^T: k left to right */
+         for (k = 0; k < N; k++)
+         {
+            for (j = 0; j < k; j++)
+            {
+               t0 = alpha * AE(j, k);
+               for (i = 0; i < M; i++) BE(i, j) += t0 * BE(i, k);
+            }
+            t0 = alpha;
+            if (nu) t0 *= AE(k, k);
+            for (i = 0; i < M; i++) BE(i, k) = t0 * BE(i, k);
+         }
+   }
+}
+
+/* beta scaling of one element as the reference BLAS does it
+ * (include/atlas_refmisc.h:194-198): 0 stores zero, 1 leaves it. */
+#define ORC_BSCAL(b_, x_) do { if ((b_) == 0) (x_) = 0; else if ((b_) != 1) (x_) *= (b_); } while (0)
+#define CE(i_, j_) Cm[(size_t)(j_)*ldc + (i_)]
+
+/* C := alpha*A*B + beta*C (Left) or alpha*B*A + beta*C (Right), A symmetric
+ * held in one triangle, in the loop order of
+ * src/blas/reference/level3/ATL_drefsymm{LL,LU,RL,RU}.c; quick returns and
+ * the alpha == 0 scaling of ATL_drefsymm.c. */
+void F_(symm_ref)(int side, int uplo, int M, int N, T alpha, const T *A, int lda,
+                  const T *B, int ldb, T beta, T *Cm, int ldc)
+{
+   int i, j, k;
+   T t0, t1;
+   if (M == 0 || N == 0 || (alpha == 0 && beta == 1)) return;
+   if (alpha == 0)
+   {
+      for (j = 0; j < N; j++) for (i = 0; i < M; i++) ORC_BSCAL(beta, CE(i, j));
+      return;
+   }
+   if (side == ORC_LEFT)
+   {
+      for (j = 0; j < N; j++)
+      {
+         if (uplo == ORC_LOWER)
+            for (i = M - 1; i >= 0; i--)
+            {
+               t0 = alpha * BE(i, j);
+               t1 = 0;
+               for (k = i + 1; k < M; k++)
+               {
+                  CE(k, j) += t0 * AE(k, i);
+                  t1 += BE(k, j) * AE(k, i);
+               }
+               ORC_BSCAL(beta, CE(i, j));
+               CE(i, j) += t0 * AE(i, i) + alpha * t1;
+            }
+         else
+            for (i = 0; i < M; i++)
+            {
+               t0 = alpha * BE(i, j);
+               t1 = 0;
+               for (k = 0; k < i; k++)
+               {
+                  CE(k, j) += t0 * AE(k, i);
+                  t1 += BE(k, j) * AE(k, i);
+               }
+               ORC_BSCAL(beta, CE(i, j));
+               CE(i, j) += t0 * AE(i, i) + alpha * t1;
+            }
+      }
+   }
+   else
+   {
+      for (j = 0; j < N; j++)
+      {
+         t0 = alpha * AE(j, j);
+         for (i = 0; i < M; i++)
+         {
+            ORC_BSCAL(beta, CE(i, j));
+            CE(i, j) += t0 * BE(i, j);
+         }
+         for (k = 0; k < j; k++)                /* A(k,j) for k < j */
+         {
+            t0 = alpha * (uplo == ORC_LOWER ? AE(j, k) : AE(k, j));
+            for (i = 0; i < M; i++) CE(i, j) += t0 * BE(i, k);
+         }
+         for (k = j + 1; k < N; k++)            /* A(k,j) for k > j */
+         {
+            t0 = alpha * (uplo == ORC_LOWER ? AE(k, j) : AE(j, k));
+            for (i = 0; i < M; i++) CE(i, j) += t0 * BE(i, k);
+         }
+      }
+   }
+}
+
+/* C := alpha*(op(A)*op(B)^T + op(B)*op(A)^T) + beta*C on one triangle, in
+ * the loop order of src/blas/reference/level3/ATL_drefsyr2k{LN,LT,UN,UT}.c
+ * with the quick returns and alpha == 0 scaling of ATL_drefsyr2k.c. */
+void F_(syr2k_ref)(int uplo, int trans, int N, int K, T alpha, const T *A, int lda,
+                   const T *B, int ldb, T beta, T *Cm, int ldc)
+{
+   int i, j, l, i0, i1;
+   T t0, t1;
+   if (N == 0 || ((alpha == 0 || K == 0) && beta == 1)) return;
+   if (alpha == 0)
+   {
+      for (j = 0; j < N; j++)
+      {
+         i0 = (uplo == ORC_UPPER) ? 0 : j;
+         i1 = (uplo == ORC_UPPER) ? j + 1 : N;
+         for (i = i0; i < i1; i++)
+         {
+            if (beta == 0) CE(i, j) = 0;
+            else if (beta != 1) CE(i, j) *= beta;
+         }
+      }
+      return;
+   }
+   for (j = 0; j < N; j++)
+   {
+      i0 = (uplo == ORC_UPPER) ? 0 : j;
+      i1 = (uplo == ORC_UPPER) ? j + 1 : N;
+      if (trans == ORC_NOTRANS)
+      {
+         for (i = i0; i < i1; i++)
+         {
+            if (beta == 0) CE(i, j) = 0;
+            else if (beta != 1) CE(i, j) *= beta;
+         }
+         for (l = 0; l < K; l++)
+         {
+            t0 = alpha * AE(j, l);
+            t1 = alpha * BE(j, l);
+            for (i = i0; i < i1; i++) CE(i, j) += t1 * AE(i, l) + t0 * BE(i, l);
+         }
+      }
+      else
+         for (i = i0; i < i1; i++)
+         {
+            t0 = 0;
+            t1 = 0;
+            for (l = 0; l < K; l++)
+            {
+               t0 += AE(l, i) * BE(l, j);
+               t1 += BE(l, i) * AE(l, j);
+            }
+            ORC_BSCAL(beta, CE(i, j));
+            CE(i, j) += alpha * t0 + alpha * t1;
+         }
+   }
+}
+#undef CE
 
 /* C := alpha*op(A)*op(A)^T + beta*C on one triangle (the operation of
  * src/blas/ulevel3/ATL_syrk.c:257, used by the Cholesky recursion
@@ -709,8 +1022,7 @@ static int F_(getrfC)(int M, int N, T *A, int lda, int *ipiv, int gcol)
       Ac = A + (size_t)Nleft * lda;
       An = Ac + Nleft;
       F_(laswp)(Nright, Ac, lda, 0, Nleft, ipiv, 1);
-      F_(trsm_ref)(ORC_LEFT, ORC_LOWER, ORC_NOTRANS, ORC_UNIT, Nleft, Nright,
-                   (T)1, A, lda, Ac, lda);
+      F_(trsm_llnu)(Nleft, Nright, A, lda, Ac, lda);
       F_(gemm_fast)(ORC_NOTRANS, ORC_NOTRANS, M - Nleft, Nright, Nleft, (T)-1,
                     A + Nleft, lda, Ac, lda, (T)1, An, lda);
       i = F_(getrfC)(M - Nleft, Nright, An, lda, ipiv + Nleft, gcol + Nleft);
@@ -1013,16 +1325,16 @@ T F_(lu_resid)(int M, int N, const T *LU, int lda, const int *ipiv,
          C[(size_t)j*M + j] = 1;
          for (i = j + 1; i < M; i++) C[(size_t)j*M + i] = LU[(size_t)j*lda + i];
       }
-      F_(trmm_ref)(ORC_RIGHT, ORC_UPPER, ORC_NOTRANS, ORC_NONUNIT, M, N, (T)1,
-                   LU, lda, C, M);
+      F_(trmm_fast)(ORC_RIGHT, ORC_UPPER, ORC_NOTRANS, ORC_NONUNIT, M, N, (T)1,
+                    LU, lda, C, M);
    }
    else
    {
       for (j = 0; j < N; j++)
          for (i = 0; i <= (j < M - 1 ? j : M - 1); i++)
             C[(size_t)j*M + i] = LU[(size_t)j*lda + i];
-      F_(trmm_ref)(ORC_LEFT, ORC_LOWER, ORC_NOTRANS, ORC_UNIT, M, N, (T)1,
-                   LU, lda, C, M);
+      F_(trmm_fast)(ORC_LEFT, ORC_LOWER, ORC_NOTRANS, ORC_UNIT, M, N, (T)1,
+                    LU, lda, C, M);
    }
    for (j = 0; j < N; j++)
       for (i = 0; i < M; i++) PA[(size_t)j*M + i] = A0[(size_t)j*lda0 + i];
@@ -1243,6 +1555,265 @@ T F_(trsm_resid)(int side, int uplo, int ta, int diag, int M, int N, T alpha,
    eps = F_(epsilon)();
    free(R); free(Tm);
    return nR / (nA * nX * ORC_ABS(alpha) * eps * (T)(M > N ? M : N));
+}
+
+
+/* ======================================================================= */
+/* Testers of the inverses (bin/trtritst.c, bin/invtst.c, bin/uumtst.c)    */
+/* ======================================================================= */
+
+/* bin/trtritst.c:129-160 (trinrm1): max column abs-sum over the triangle,
+ * the diagonal left out when unit (column-major). */
+T F_(trinrm1)(int uplo, int diag, int N, const T *A, int lda)
+{
+   const int skip = (diag == ORC_UNIT);
+   int i, j;
+   T mx = 0, t;
+   for (j = 0; j < N; j++)
+   {
+      t = 0;
+      if (uplo == ORC_UPPER)
+         for (i = 0; i < j + 1 - skip; i++) t += ORC_ABS(A[(size_t)j*lda + i]);
+      else
+         for (i = j + skip; i < N; i++) t += ORC_ABS(A[(size_t)j*lda + i]);
+      if (t > mx) mx = t;
+   }
+   return mx;
+}
+
+/* bin/trtritst.c:162-258 (trddom + trigen, column-major): gegen, the
+ * unreferenced part (and the diagonal when unit) set to padval, then the
+ * off-diagonal triangle scaled by 0.9/N (times the smallest |diagonal|
+ * when non-unit) so the matrix is diagonally dominant. */
+void F_(trigen)(int uplo, int diag, int N, T *A, int lda, T padval, int seed)
+{
+   const int skipdiag = (diag == ORC_NONUNIT);
+   int i, j;
+   T alpha, smallest;
+   F_(gegen)(N, N, A, lda, seed);
+   for (j = 0; j < N; j++)
+   {
+      if (uplo == ORC_UPPER)
+         for (i = j + skipdiag; i < N; i++) A[(size_t)j*lda + i] = padval;
+      else
+         for (i = 0; i < j + 1 - skipdiag; i++) A[(size_t)j*lda + i] = padval;
+   }
+   if (N <= 0) return;
+   alpha = (T)0.9 / (T)N;
+   if (diag == ORC_NONUNIT)
+   {
+      smallest = ORC_ABS(A[0]);
+      for (i = 1; i < N; i++)
+         if (ORC_ABS(A[(size_t)i*lda + i]) < smallest) smallest = ORC_ABS(A[(size_t)i*lda + i]);
+      alpha *= smallest;
+   }
+   for (j = 0; j < N; j++)
+   {
+      if (uplo == ORC_UPPER)
+         for (i = 0; i < j; i++) A[(size_t)j*lda + i] *= alpha;
+      else
+         for (i = j + 1; i < N; i++) A[(size_t)j*lda + i] *= alpha;
+   }
+}
+
+/* bin/trtritst.c:300-397 (trtritest's residual): ||inv(A)*A - I||_1 /
+ * (trinrm1(A) * eps * N), the product by the reference-order TRMM of the
+ * computed inverse (uplo/diag triangle of Ainv) times the original
+ * triangle expanded with zeros (and a unit diagonal when unit). */
+T F_(trtri_resid)(int uplo, int diag, int N, const T *Ainv, int ldi, const T *A0, int lda0)
+{
+   T *C = (T *)calloc((size_t)N * N > 0 ? (size_t)N * N : 1, sizeof(T));
+   T nrm, r;
+   int i, j;
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+      {
+         const int in = (uplo == ORC_LOWER) ? (i >= j) : (i <= j);
+         C[(size_t)j*N + i] = in ? A0[(size_t)j*lda0 + i] : 0;
+         if (i == j && diag == ORC_UNIT) C[(size_t)j*N + i] = 1;
+      }
+   nrm = F_(trinrm1)(uplo, diag, N, A0, lda0);
+   F_(trmm_ref)(ORC_LEFT, uplo, ORC_NOTRANS, diag, N, N, (T)1, Ainv, ldi, C, N);
+   for (i = 0; i < N; i++) C[(size_t)i*N + i] -= 1;
+   r = F_(genrm1)(N, N, C, N);
+   free(C);
+   return r / (nrm * F_(epsilon)() * (T)N);
+}
+
+/* bin/invtst.c:388-418 (geresid): ||A*AI - I||_1 / (N eps ||A||_1 ||AI||_1) */
+T F_(inv_resid)(int N, const T *A, int lda, const T *AI, int ldi)
+{
+   T *C = (T *)calloc((size_t)N * N > 0 ? (size_t)N * N : 1, sizeof(T));
+   T num, den;
+   int i;
+   F_(gemm_fast)(ORC_NOTRANS, ORC_NOTRANS, N, N, N, (T)1, A, lda, AI, ldi, (T)0, C, N);
+   for (i = 0; i < N; i++) C[(size_t)i*N + i] -= 1;
+   num = F_(genrm1)(N, N, C, N);
+   den = F_(genrm1)(N, N, A, lda) * F_(genrm1)(N, N, AI, ldi) * (T)N * F_(epsilon)();
+   free(C);
+   return num / den;
+}
+
+/* bin/invtst.c:420-463 (poresid): both symmetric matrices held in their
+ * uplo triangle, ||A*AI - I||_1 / (N eps synrm(A) synrm(AI)) with synrm
+ * the symmetric 1-norm (the same sum as lltnrm1). */
+T F_(poinv_resid)(int uplo, int N, const T *A, int lda, const T *AI, int ldi)
+{
+   T *Sa = (T *)calloc((size_t)N * N > 0 ? (size_t)N * N : 1, sizeof(T));
+   T *Si = (T *)calloc((size_t)N * N > 0 ? (size_t)N * N : 1, sizeof(T));
+   T *C = (T *)calloc((size_t)N * N > 0 ? (size_t)N * N : 1, sizeof(T));
+   T num, den;
+   int i, j;
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+      {
+         const int in = (uplo == ORC_LOWER) ? (i >= j) : (i <= j);
+         Sa[(size_t)j*N + i] = in ? A[(size_t)j*lda + i] : A[(size_t)i*lda + j];
+         Si[(size_t)j*N + i] = in ? AI[(size_t)j*ldi + i] : AI[(size_t)i*ldi + j];
+      }
+   F_(gemm_fast)(ORC_NOTRANS, ORC_NOTRANS, N, N, N, (T)1, Sa, N, Si, N, (T)0, C, N);
+   for (i = 0; i < N; i++) C[(size_t)i*N + i] -= 1;
+   num = F_(genrm1)(N, N, C, N);
+   den = F_(lltnrm1)(uplo, N, A, lda) * F_(lltnrm1)(uplo, N, AI, ldi) * (T)N * F_(epsilon)();
+   free(Sa); free(Si); free(C);
+   return num / den;
+}
+
+/* bin/uumtst.c:300-351 (uumtest's residual): the lauum result R (uplo
+ * triangle) against U*U^T (upper) or L^T*L (lower) of the original
+ * triangle T0, lltnrm1 of the difference / (lltnrm1(T0) * eps * N). */
+T F_(lauum_resid)(int uplo, int N, const T *R, int ldr, const T *T0, int ld0)
+{
+   T *Tt = (T *)calloc((size_t)N * N > 0 ? (size_t)N * N : 1, sizeof(T));
+   T *P = (T *)calloc((size_t)N * N > 0 ? (size_t)N * N : 1, sizeof(T));
+   T num, nrm;
+   int i, j;
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+      {
+         const int in = (uplo == ORC_LOWER) ? (i >= j) : (i <= j);
+         Tt[(size_t)j*N + i] = in ? T0[(size_t)j*ld0 + i] : 0;
+      }
+   if (uplo == ORC_LOWER)
+      F_(gemm_fast)(ORC_TRANS, ORC_NOTRANS, N, N, N, (T)1, Tt, N, Tt, N, (T)0, P, N);
+   else
+      F_(gemm_fast)(ORC_NOTRANS, ORC_TRANS, N, N, N, (T)1, Tt, N, Tt, N, (T)0, P, N);
+   for (j = 0; j < N; j++)
+      for (i = 0; i < N; i++)
+      {
+         const int in = (uplo == ORC_LOWER) ? (i >= j) : (i <= j);
+         if (in) P[(size_t)j*N + i] = R[(size_t)j*ldr + i] - P[(size_t)j*N + i];
+      }
+   num = F_(lltnrm1)(uplo, N, P, N);
+   nrm = F_(lltnrm1)(uplo, N, T0, ld0);
+   free(Tt); free(P);
+   return num / (nrm * F_(epsilon)() * (T)N);
+}
+
+/* ======================================================================= */
+/* QR (src/lapack/ATL_geqr2.c, ATL_larfg.c) and bin/qrtst.c's residual      */
+/* ======================================================================= */
+
+/* 2-norm with a running scale (the reference BLAS nrm2 algorithm) */
+static T F_(nrm2)(int N, const T *X, int incX)
+{
+   T scale = 0, ssq = 1, a;
+   int i;
+   for (i = 0; i < N; i++)
+   {
+      a = ORC_ABS(X[(size_t)i*incX]);
+      if (a != 0)
+      {
+         if (scale < a) { ssq = 1 + ssq * (scale / a) * (scale / a); scale = a; }
+         else ssq += (a / scale) * (a / scale);
+      }
+   }
+   return scale * (T)sqrt((double)ssq);
+}
+
+/* src/lapack/ATL_larfg.c (real): H (alpha; x) = (beta; 0) with
+ * H = I - tau (1; v)(1; v)^T, beta = -sign(alpha) hypot(alpha, ||x||) */
+static void F_(larfg)(int N, T *alpha, T *X, int incX, T *tau)
+{
+   const T safmin = ORC_SAFMIN / F_(epsilon)();
+   T xnorm, beta, betap, rsafmn;
+   int j, knt = 0;
+   if (N < 1) { *tau = 0; return; }
+   xnorm = F_(nrm2)(N - 1, X, incX);
+   if (xnorm == 0) { *tau = 0; return; }
+   betap = (T)hypot((double)*alpha, (double)xnorm);
+   beta = (*alpha >= 0) ? -betap : betap;
+   if (betap < safmin)
+   {
+      rsafmn = 1 / safmin;
+      do
+      {
+         knt++;
+         for (j = 0; j < N - 1; j++) X[(size_t)j*incX] *= rsafmn;
+         beta *= rsafmn;
+         *alpha *= rsafmn;
+         betap = ORC_ABS(beta);
+      } while (betap < safmin && knt < 20);
+      xnorm = F_(nrm2)(N - 1, X, incX);
+      beta = (T)hypot((double)*alpha, (double)xnorm);
+      if (*alpha > 0) beta = -beta;
+   }
+   *tau = (beta - *alpha) / beta;
+   for (j = 0; j < N - 1; j++) X[(size_t)j*incX] *= 1 / (*alpha - beta);
+   for (j = 0; j < knt; j++) beta *= ORC_SAFMIN;
+   *alpha = beta;
+}
+
+/* src/lapack/ATL_geqr2.c: unblocked Householder QR, A(i:M, i+1:N) -=
+ * tau v (v^T A(i:M, i+1:N)) after each column's larfg. */
+void F_(geqr2)(int M, int N, T *A, int lda, T *tau)
+{
+   const int K = M < N ? M : N;
+   int i, j, r;
+   for (i = 0; i < K; i++)
+   {
+      T *col = A + (size_t)i*lda + i;
+      F_(larfg)(M - i, col, col + 1, 1, tau + i);
+      if (tau[i] == 0) continue;
+      for (j = i + 1; j < N; j++)
+      {
+         T *cj = A + (size_t)j*lda + i, w = cj[0];
+         for (r = 1; r < M - i; r++) w += col[r] * cj[r];
+         w *= tau[i];
+         cj[0] -= w;
+         for (r = 1; r < M - i; r++) cj[r] -= w * col[r];
+      }
+   }
+}
+
+/* bin/qrtst.c:1105-1307 (the QR case): Q R rebuilt from the factored
+ * storage (R on and above the diagonal, reflectors below) by applying
+ * H_{k-1} .. H_0 to R, then ||A0 - Q R||_1 / (||A0||_1 eps min(M, N)). */
+T F_(qr_resid)(int M, int N, const T *QR, int lda, const T *tau, const T *A0, int lda0)
+{
+   const int K = M < N ? M : N;
+   T *W = (T *)calloc((size_t)M * N > 0 ? (size_t)M * N : 1, sizeof(T));
+   T r, nrm;
+   int i, j, q;
+   for (j = 0; j < N; j++)
+      for (i = 0; i <= (j < M - 1 ? j : M - 1); i++) W[(size_t)j*M + i] = QR[(size_t)j*lda + i];
+   for (i = K - 1; i >= 0; i--)
+   {
+      const T *v = QR + (size_t)i*lda + i;
+      if (tau[i] == 0) continue;
+      for (j = 0; j < N; j++)
+      {
+         T *wj = W + (size_t)j*M + i, s = wj[0];
+         for (q = 1; q < M - i; q++) s += v[q] * wj[q];
+         s *= tau[i];
+         wj[0] -= s;
+         for (q = 1; q < M - i; q++) wj[q] -= s * v[q];
+      }
+   }
+   nrm = F_(genrm1)(M, N, A0, lda0);
+   r = F_(gediffnrm1)(M, N, A0, lda0, W, M);
+   free(W);
+   return r / (nrm * F_(epsilon)() * (T)(K > 0 ? K : 1));
 }
 
 #undef F_

@@ -15,6 +15,9 @@ it.
   scipy.linalg.lu_factor) for ATLAS lutst inputs (gegen, seed M*N+lda),
   truncated before the first column whose two largest candidates are within
   a relative 1e-9 of each other.
+* c2_getrf_pivots_n8192.npz (`python tests/golden/make_golden.py c2`, about
+  3 minutes) -- LAPACK's pivots for the n = 8192 LU configuration's
+  U[-1,1) input, with the near-tie screen of a blocked numpy elimination.
 """
 import ctypes
 import json
@@ -98,9 +101,62 @@ def lu_fixture():
     return out
 
 
-if __name__ == "__main__":
+if __name__ == "__main__" and len(__import__("sys").argv) == 1:
     with open(os.path.join(HERE, "lcg_streams.json"), "w") as f:
         json.dump(lcg_fixture(), f, indent=0)
     with open(os.path.join(HERE, "lu_pivots.json"), "w") as f:
         json.dump(lu_fixture(), f, indent=0)
     print("wrote", os.listdir(HERE))
+
+
+def c2_pivot_fixture(n=8192, seed=20260801, nb=64, tie=1e-9):
+    """Partial-pivoting pivots of the LU configuration (n = 8192, A ~
+    U[-1,1) from numpy's PCG64 stream with `seed`, as
+    atlas_amd.synth.uniform_matrix makes it), from LAPACK dgetrf through
+    scipy, cross-checked against a blocked numpy elimination that also
+    records the relative gap between the two largest candidates of every
+    pivot column; columns from the first gap below `tie` on are not
+    compared.  Returns a dict of arrays for np.savez_compressed."""
+    import scipy.linalg as sla
+    rng = np.random.default_rng(seed)
+    A = np.asfortranarray(rng.uniform(-1.0, 1.0, (n, n)))
+    _, piv = sla.lu_factor(A.copy(order="F"), check_finite=False)
+    W = A.copy(order="F")
+    gaps = np.ones(n)
+    mypiv = np.zeros(n, dtype=np.int64)
+    for j0 in range(0, n, nb):
+        j1 = min(n, j0 + nb)
+        for j in range(j0, j1):             # panel, right-looking inside
+            col = np.abs(W[j:, j])
+            p = int(np.argmax(col))
+            top = col[p]
+            if col.size > 1 and top > 0:
+                rest = col.copy()
+                rest[p] = -1.0
+                gaps[j] = (top - rest.max()) / top
+            p += j
+            mypiv[j] = p
+            if p != j:
+                W[[j, p], :] = W[[p, j], :]
+            if W[j, j] != 0:
+                W[j + 1:, j] /= W[j, j]
+                W[j + 1:, j + 1:j1] -= np.outer(W[j + 1:, j], W[j, j + 1:j1])
+        if j1 < n:                          # U12 and the trailing update
+            L11 = np.tril(W[j0:j1, j0:j1], -1) + np.eye(j1 - j0)
+            W[j0:j1, j1:] = sla.solve_triangular(L11, W[j0:j1, j1:], lower=True,
+                                                 unit_diagonal=True)
+            W[j1:, j1:] -= W[j1:, j0:j1] @ W[j0:j1, j1:]
+    near = np.nonzero(gaps < tie)[0]
+    stop = int(near[0]) if near.size else n
+    assert np.array_equal(mypiv[:stop], piv[:stop]), "numpy elimination disagrees with LAPACK"
+    return {"n": np.int64(n), "seed": np.int64(seed), "ipiv": piv.astype(np.int32),
+            "checked_columns": np.int64(stop), "min_gap": np.float64(gaps[:stop].min()),
+            "a_checksum": np.float64(A.sum()), "a_corner": A[:4, :4].copy()}
+
+
+if __name__ == "__main__" and len(__import__("sys").argv) > 1 and \
+        __import__("sys").argv[1] == "c2":
+    fx = c2_pivot_fixture()
+    np.savez_compressed(os.path.join(HERE, "c2_getrf_pivots_n8192.npz"), **fx)
+    print("c2 fixture: checked columns", int(fx["checked_columns"]), "min gap",
+          float(fx["min_gap"]))

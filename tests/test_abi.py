@@ -12,10 +12,10 @@ LIB = os.path.join(ROOT, "atlas_amd", "libatlas_mi355x.so")
 
 def _declared():
     names = set()
-    for h in ("cblas.h", "clapack.h", "atlmx.h"):
+    for h in ("cblas.h", "clapack.h", "atlmx.h", "atlas_f77.h"):
         txt = open(os.path.join(ROOT, "include", h)).read()
         txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-        for m in re.finditer(r"\b((?:cblas|clapack|atlmx)_\w+)\s*\(", txt):
+        for m in re.finditer(r"\b((?:cblas|clapack|atlmx)_\w+|[a-z]\w*_)\s*\(", txt):
             names.add(m.group(1))
     return names
 
@@ -61,3 +61,48 @@ def test_xerbla_exits_on_bad_argument():
                        text=True)
     assert p.returncode == 255
     assert "Parameter 6 to routine clapack_dgetrf was incorrect" in p.stderr
+
+
+def test_batch_plan_order_limits_without_gpu():
+    """The batched plans refuse orders the factor kernels cannot hold in
+    LDS before any device call: a NULL plan and a message."""
+    code = ("import ctypes as C, sys; sys.path.insert(0, %r); import atlas_amd as a; L = a.lib();"
+            "L.atlmx_batch_max_order.restype = C.c_int;"
+            "lim = L.atlmx_batch_max_order(1, 1);"
+            "n = (C.c_int * 2)(5, lim + 1);"
+            "L.atlmx_dgesv_batch_plan.restype = C.c_void_p;"
+            "h = L.atlmx_dgesv_batch_plan(2, n, None, n, None, 1, None, n, None);"
+            "print('LIM', lim, 'PLAN', h)" % ROOT)
+    p = subprocess.run(["python", "-c", code], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "PLAN None" in p.stdout and "LIM 768" in p.stdout
+    assert "instance 1 has order 769" in p.stderr
+
+
+
+def test_f77_xerbla_messages_without_gpu():
+    """F77 argument errors are caught before any device call: XERBLA's
+    message with the Fortran argument position (interfaces/blas/F77/src/
+    xerbla.f), then STOP (exit status 0)."""
+    cases = [
+        ("L.dgemm_(b'X', b'N', i(2), i(2), i(2), d(1.0), None, i(2), None, i(2), d(0.0), None,"
+         " i(2), 1, 1)", "DGEMM  parameter number  1"),
+        ("L.dtrsm_(b'L', b'U', b'N', b'N', i(5), i(2), d(1.0), None, i(4), None, i(5), 1, 1, 1, 1)",
+         "DTRSM  parameter number  9"),
+        ("L.dgetrf_(i(5), i(5), None, i(3), None, C.byref(C.c_int(0)))",
+         "DGETRF parameter number  4"),
+        ("L.spotrf_(b'Q', i(5), None, i(5), C.byref(C.c_int(0)), 1)",
+         "SPOTRF parameter number  1"),
+        ("L.dgetri_(i(5), None, i(5), None, None, i(2), C.byref(C.c_int(0)))",
+         "DGETRI parameter number  6"),
+    ]
+    for call, msg in cases:
+        code = ("import ctypes as C, sys; sys.path.insert(0, %r); import atlas_amd as a;"
+                "L = a.lib(); i = lambda v: C.byref(C.c_int(v));"
+                "d = lambda v: C.byref(C.c_double(v)); %s; print('RETURNED')" % (ROOT, call))
+        p = subprocess.run(["python", "-c", code], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=120)
+        assert p.returncode == 0, p.stderr
+        assert "** On entry to %s had an illegal value" % msg in p.stdout, p.stdout
+        assert "RETURNED" not in p.stdout

@@ -128,3 +128,74 @@ def test_config_size_vs_reference(torch, dt, spd):
         err = np.abs(g - r_).max(axis=(1, 2)) / np.maximum(np.abs(r_).max(axis=(1, 2)), 1e-300)
         bound = 4.0 * kappa * n * eps
         assert np.all(err <= np.maximum(bound, 1e3 * eps)), (name, float((err / bound).max()))
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_cholesky_solve_order_300(torch, dt):
+    """Orders above 256 take the adjoint kernel's global-scratch branch:
+    forward and both gradients against the numpy reference at n = 300."""
+    from atlas_amd.autograd import CholeskySolve
+    rng = np.random.default_rng(31)
+    b, n, r = 4, 300, 3
+    A = _spd_batch(rng, b, n)
+    B = rng.standard_normal((b, n, r))
+    W = rng.standard_normal((b, n, r))
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    if dt == "f32":
+        A, B = A.astype(np.float32).astype(np.float64), B.astype(np.float32).astype(np.float64)
+    At = torch.tensor(A, dtype=tdt, device="cuda", requires_grad=True)
+    Bt = torch.tensor(B, dtype=tdt, device="cuda", requires_grad=True)
+    Xt = CholeskySolve.apply(At, Bt)
+    (Xt * torch.tensor(W, dtype=tdt, device="cuda")).sum().backward()
+    Xr, Gr, Bbr = _ref_grads(A, B, W, True)
+    eps = 2.0 ** -52 if dt == "f64" else 2.0 ** -23
+    kappa = np.linalg.cond(A)
+    got = (Xt.detach().cpu().numpy(), Bt.grad.cpu().numpy(), At.grad.cpu().numpy())
+    for name, g, r_ in zip(("x", "b-bar", "A-bar"), got, (Xr, Bbr, Gr)):
+        err = np.abs(g - r_).max(axis=(1, 2)) / np.abs(r_).max(axis=(1, 2))
+        assert np.all(err <= 4.0 * kappa * n * eps), (name, err)
+
+
+@pytest.mark.parametrize("dt", ["f64", "f32"])
+def test_cholesky_adjoint_uneven_batch(torch, dt):
+    """posv_adjoint_batch on instances of orders 9, 200 and 300 in one
+    plan, from L, X and Bb computed in numpy: the gradient of <Xb, X> on
+    the lower triangle against the closed form tril(2S) - diag(S),
+    S = -(Bb X^T + X Bb^T) / 2."""
+    rng = np.random.default_rng(33)
+    tdt = torch.float64 if dt == "f64" else torch.float32
+    npdt = np.float64 if dt == "f64" else np.float32
+    sizes, r = [9, 200, 300], 2
+    Ls, Xs, Bbs, Gs, Ws, refs = [], [], [], [], [], []
+    for n in sizes:
+        A = _spd_batch(rng, 1, n)[0].astype(npdt).astype(np.float64)
+        Xm = np.linalg.solve(A, rng.standard_normal((n, r)))
+        Bb = np.linalg.solve(A, rng.standard_normal((n, r)))
+        L = np.linalg.cholesky(A)
+        S = -(Bb @ Xm.T + Xm @ Bb.T) / 2
+        G = np.tril(2 * S)
+        G[np.arange(n), np.arange(n)] = np.diag(S)
+        refs.append(G)
+        Ls.append(torch.tensor(np.ascontiguousarray(L.T), dtype=tdt, device="cuda"))
+        Xs.append(torch.tensor(np.ascontiguousarray(Xm.T), dtype=tdt, device="cuda"))
+        Bbs.append(torch.tensor(np.ascontiguousarray(Bb.T), dtype=tdt, device="cuda"))
+        Gs.append(torch.zeros(n, n, dtype=tdt, device="cuda"))
+        Ws.append(torch.zeros(n, n, dtype=tdt, device="cuda"))
+    with X.posv_adjoint_batch(Ls, Xs, Bbs, Gs, Ws) as plan:
+        plan.run()
+        torch.cuda.synchronize()
+    tol = 1e-11 if dt == "f64" else 2e-4
+    for n, g, ref in zip(sizes, Gs, refs):
+        got = np.tril(g.cpu().numpy().T.astype(np.float64))
+        assert np.abs(got - ref).max() <= tol * np.abs(ref).max(), n
+
+
+def test_batch_plan_rejects_orders_beyond_lds():
+    """A gesv order above the LDS limit gives a NULL plan (no launch)."""
+    import torch as t
+    lim = X.lib().atlmx_batch_max_order(1, 1)
+    A = [t.zeros(lim + 1, lim + 1, dtype=t.float64, device="cuda")]
+    piv = [t.zeros(lim + 1, dtype=t.int32, device="cuda")]
+    info = t.zeros(1, dtype=t.int32, device="cuda")
+    plan = X.gesv_batch(A, piv, [], info)
+    assert not plan.h

@@ -1,0 +1,150 @@
+"""The Fortran 77 ABI (include/atlas_f77.h) through ctypes with the
+gfortran calling convention: dgemm_/sgemm_, dtrsm_, dtrmm_, dsyrk_,
+dsyr2k_, dsymm_, idamax_ against the oracle; dgetrf_ returning 1-based
+pivots equal to clapack_dgetrf's plus one
+(interfaces/lapack/F77/src/f77wrap/ATL_f77wrap_getrf.c:27), dgetrs_ /
+dgesv_ / dpotrf_ / dpotrs_ / dposv_ under slvtst, and the inverses."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import atlas_amd as X
+import oracle as O
+from tests.checks import solve_ok
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(77)
+I = C.c_int
+
+
+def _f(a):
+    return np.asfortranarray(a)
+
+
+def _i(v):
+    return C.byref(I(v))
+
+
+def _d(v, dt=np.float64):
+    return C.byref(C.c_double(v) if dt == np.float64 else C.c_float(v))
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _c(ch):
+    return C.c_char_p(ch.encode())
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("ta,tb", [("N", "N"), ("t", "N"), ("N", "T"), ("C", "t")])
+def test_gemm_f77(dt, ta, tb):
+    M, N, K = 130, 70, 90
+    A = _f(RNG.uniform(-0.5, 0.5, (M, K) if ta == "N" else (K, M)).astype(dt))
+    B = _f(RNG.uniform(-0.5, 0.5, (K, N) if tb == "N" else (N, K)).astype(dt))
+    C0 = _f(RNG.uniform(-0.5, 0.5, (M, N)).astype(dt))
+    D = C0.copy(order="F")
+    f = X.lib().dgemm_ if dt == np.float64 else X.lib().sgemm_
+    f(_c(ta), _c(tb), _i(M), _i(N), _i(K), _d(1.5, dt), _p(A), _i(A.shape[0]), _p(B),
+      _i(B.shape[0]), _d(-0.5, dt), _p(D), _i(M), C.c_size_t(1), C.c_size_t(1))
+    T = O.gemm(X.NoTrans if ta == "N" else X.Trans, X.NoTrans if tb == "N" else X.Trans, 1.5, A,
+               B, -0.5, C0.copy(order="F"), exact_order=True)
+    assert O.gemm_ferr_check(K, 1.5, -0.5, T, D)[0] == 0
+
+
+def test_level3_f77_vs_oracle():
+    L = X.lib()
+    n, m = 120, 50
+    A = _f(RNG.uniform(-1, 1, (n, n)) + 4 * np.eye(n))
+    B0 = _f(RNG.uniform(-1, 1, (n, m)))
+    D = B0.copy(order="F")
+    L.dtrsm_(_c("L"), _c("U"), _c("T"), _c("N"), _i(n), _i(m), _d(2.0), _p(A), _i(n), _p(D),
+             _i(n), C.c_size_t(1), C.c_size_t(1), C.c_size_t(1), C.c_size_t(1))
+    assert O.trsm_resid(X.Left, X.Upper, X.Trans, X.NonUnit, 2.0, A, D, B0) < 1
+    D = B0.copy(order="F")
+    L.dtrmm_(_c("l"), _c("L"), _c("n"), _c("u"), _i(n), _i(m), _d(0.5), _p(A), _i(n), _p(D),
+             _i(n), C.c_size_t(1), C.c_size_t(1), C.c_size_t(1), C.c_size_t(1))
+    T = O.trmm(X.Left, X.Lower, X.NoTrans, X.Unit, 0.5, A, B0.copy(order="F"))
+    assert O.gemm_ferr_check(n, 0.5, 0.0, T, D)[0] == 0
+    Cs = _f(RNG.uniform(-1, 1, (n, n)))
+    D = Cs.copy(order="F")
+    L.dsyrk_(_c("U"), _c("N"), _i(n), _i(m), _d(1.0), _p(B0), _i(n), _d(0.5), _p(D), _i(n),
+             C.c_size_t(1), C.c_size_t(1))
+    T = O.syrk(X.Upper, X.NoTrans, 1.0, B0, 0.5, Cs.copy(order="F"))
+    assert np.allclose(np.triu(D), np.triu(T), atol=1e-12)
+    B2 = _f(RNG.uniform(-1, 1, (n, m)))
+    D = Cs.copy(order="F")
+    L.dsyr2k_(_c("L"), _c("N"), _i(n), _i(m), _d(1.0), _p(B0), _i(n), _p(B2), _i(n), _d(0.0),
+              _p(D), _i(n), C.c_size_t(1), C.c_size_t(1))
+    T = O.syr2k(X.Lower, X.NoTrans, 1.0, B0, B2, 0.0, Cs.copy(order="F"))
+    assert np.allclose(np.tril(D), np.tril(T), atol=1e-12)
+    Cm = _f(RNG.uniform(-1, 1, (n, m)))
+    D = Cm.copy(order="F")
+    L.dsymm_(_c("L"), _c("U"), _i(n), _i(m), _d(1.0), _p(A), _i(n), _p(B0), _i(n), _d(1.0),
+             _p(D), _i(n), C.c_size_t(1), C.c_size_t(1))
+    T = O.symm(X.Left, X.Upper, 1.0, A, B0, 1.0, Cm.copy(order="F"))
+    assert O.gemm_ferr_check(n, 1.0, 1.0, T, D)[0] == 0
+    x = np.ascontiguousarray(RNG.uniform(-1, 1, 999))
+    L.idamax_.restype = I
+    assert L.idamax_(_i(999), _p(x), _i(1)) == O.iamax(x) + 1
+    assert L.idamax_(_i(0), _p(x), _i(1)) == 0
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_lapack_f77_pivots_solves_inverses(dt):
+    L = X.lib()
+    pre = "d" if dt == np.float64 else "s"
+    fn = lambda name: getattr(L, pre + name + "_")   # noqa: E731
+    n, nrhs = 300, 3
+    A0 = O.gegen(n, n, dtype=dt)
+    A = A0.copy(order="F")
+    ipiv = np.zeros(n, dtype=np.int32)
+    info = I(-9)
+    fn("getrf")(_i(n), _i(n), _p(A), _i(n), _p(ipiv), C.byref(info))
+    assert info.value == 0
+    A2 = A0.copy(order="F")
+    inf2, ipiv0 = X.getrf(A2)
+    assert np.array_equal(ipiv, ipiv0 + 1) and np.array_equal(A, A2)
+    Xc = O.gegen(n, nrhs, dtype=dt)
+    B = _f((A0.astype(np.float64) @ Xc).astype(dt))
+    fn("getrs")(_c("N"), _i(n), _i(nrhs), _p(A), _i(n), _p(ipiv), _p(B), _i(n), C.byref(info),
+                C.c_size_t(1))
+    assert info.value == 0 and solve_ok(O.genrm1(A0), Xc, B)
+    G = A0.copy(order="F")
+    B = _f((A0.astype(np.float64) @ Xc).astype(dt))
+    piv = np.zeros(n, dtype=np.int32)
+    fn("gesv")(_i(n), _i(nrhs), _p(G), _i(n), _p(piv), _p(B), _i(n), C.byref(info))
+    assert info.value == 0 and np.array_equal(piv, ipiv) and solve_ok(O.genrm1(A0), Xc, B)
+    work = np.zeros(1, dtype=dt)
+    fn("getri")(_i(n), _p(A), _i(n), _p(ipiv), _p(work), _i(-1), C.byref(info))
+    assert info.value == 0 and work[0] >= n
+    work = np.zeros(int(work[0]), dtype=dt)
+    fn("getri")(_i(n), _p(A), _i(n), _p(ipiv), _p(work), _i(work.size), C.byref(info))
+    assert info.value == 0 and O.inv_resid(A0, A) < 100
+    for u, uplo in (("L", X.Lower), ("u", X.Upper)):
+        S = O.lltgen(uplo, n, dtype=dt)
+        Sf = np.tril(S) + np.tril(S, -1).T if uplo == X.Lower else np.triu(S) + np.triu(S, 1).T
+        F = S.copy(order="F")
+        fn("potrf")(_c(u), _i(n), _p(F), _i(n), C.byref(info), C.c_size_t(1))
+        assert info.value == 0 and O.llt_resid(uplo, F, S) < 100
+        B = _f((Sf.astype(np.float64) @ Xc).astype(dt))
+        fn("potrs")(_c(u), _i(n), _i(nrhs), _p(F), _i(n), _p(B), _i(n), C.byref(info),
+                    C.c_size_t(1))
+        assert info.value == 0 and solve_ok(O.genrm1(_f(Sf)), Xc, B)
+        F2 = S.copy(order="F")
+        B = _f((Sf.astype(np.float64) @ Xc).astype(dt))
+        fn("posv")(_c(u), _i(n), _i(nrhs), _p(F2), _i(n), _p(B), _i(n), C.byref(info),
+                   C.c_size_t(1))
+        assert info.value == 0 and solve_ok(O.genrm1(_f(Sf)), Xc, B)
+        fn("potri")(_c(u), _i(n), _p(F), _i(n), C.byref(info), C.c_size_t(1))
+        assert info.value == 0 and O.poinv_resid(uplo, S, F) < 100
+        R = S.copy(order="F")
+        fn("lauum")(_c(u), _i(n), _p(R), _i(n), C.byref(info), C.c_size_t(1))
+        assert info.value == 0 and O.lauum_resid(uplo, R, S) < 100
+        Tm = O.trigen(uplo, X.NonUnit, n, dtype=dt)
+        Ti = Tm.copy(order="F")
+        fn("trtri")(_c(u), _c("N"), _i(n), _p(Ti), _i(n), C.byref(info), C.c_size_t(1),
+                    C.c_size_t(1))
+        assert info.value == 0 and O.trtri_resid(uplo, X.NonUnit, Ti, Tm) < 100

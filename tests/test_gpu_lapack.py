@@ -7,6 +7,7 @@ import pytest
 
 import atlas_amd as X
 import oracle as O
+from tests.checks import solve_ok
 
 pytestmark = pytest.mark.gpu
 RNG = np.random.default_rng(99)
@@ -45,13 +46,22 @@ def test_getrf_lutst_and_pivots(M, N, lda):
     assert r < 100, r
 
 
-def test_getrf_float():
-    n = 500
-    A0 = O.gegen(n, n, dtype=np.float32)
+@pytest.mark.parametrize("M,N,lda", [(5, 5, 5), (100, 100, 100), (257, 257, 260),
+                                     (300, 200, 300), (200, 300, 200), (500, 500, 500)])
+def test_sgetrf_lutst_and_pivots(M, N, lda):
+    """clapack_sgetrf against the oracle's FP32 getrfC: pivots up to the
+    first near-tied column (relative gap 1e-4 in FP32), lutst residual."""
+    A0 = O.gegen(M, N, lda, dtype=np.float32)
     A = A0.copy(order="F")
-    info, ipiv = X.getrf(A)
+    info, ipiv = X.getrf(A[:M, :N])
     assert info == 0
-    assert O.lu_resid(A, ipiv, A0) < 100
+    Ao = A0.copy(order="F")
+    gaps = np.ones(min(M, N))
+    _, opiv = O.getrf(Ao[:M, :N], gaps=gaps)
+    near = np.nonzero(gaps < 1e-4)[0]
+    stop = int(near[0]) if near.size else min(M, N)
+    assert np.array_equal(ipiv[:stop], opiv[:stop]), (stop, np.nonzero(ipiv != opiv)[0][:5])
+    assert O.lu_resid(A[:M, :N], ipiv, A0[:M, :N]) < 100
 
 
 def test_getrf_singular_column_info():
@@ -74,7 +84,7 @@ def test_getrs_slvtst(nrhs, trans):
     info, ipiv = X.getrf(A)
     assert info == 0
     X.getrs(A, ipiv, B, trans=trans)
-    assert O.slv_resid(O.genrm1(A0), Xc, B) < 100
+    assert solve_ok(O.genrm1(A0), Xc, B)
 
 
 def test_gesv_rowmajor():
@@ -124,9 +134,110 @@ def test_potrs_posv(uplo):
     A = S.copy(order="F")
     assert X.potrf(A, uplo) == 0
     X.potrs(A, B, uplo)
-    assert O.slv_resid(O.genrm1(_f(Sf)), Xc, B) < 100
+    assert solve_ok(O.genrm1(_f(Sf)), Xc, B)
     A = S.copy(order="F")
     B = _f(Sf @ Xc)
     info = X.lib().clapack_dposv(X.ColMajor, uplo, n, nrhs, A.ctypes.data, n, B.ctypes.data, n)
     assert info == 0
-    assert O.slv_resid(O.genrm1(_f(Sf)), Xc, B) < 100
+    assert solve_ok(O.genrm1(_f(Sf)), Xc, B)
+
+
+@pytest.mark.parametrize("nrhs", [1, 9])
+@pytest.mark.parametrize("trans", [X.NoTrans, X.Trans])
+def test_sgetrs_slvtst(nrhs, trans):
+    n = 400
+    A0 = O.gegen(n, n, dtype=np.float32)
+    Xc = O.gegen(n, nrhs, dtype=np.float32)
+    B = _f(((A0 if trans == X.NoTrans else A0.T).astype(np.float64) @ Xc).astype(np.float32))
+    A = A0.copy(order="F")
+    info, ipiv = X.getrf(A)
+    assert info == 0
+    assert X.getrs(A, ipiv, B, trans=trans) == 0
+    assert solve_ok(O.genrm1(A0), Xc, B)
+
+
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+@pytest.mark.parametrize("N,lda", [(1, 1), (127, 130), (129, 129), (600, 600)])
+def test_spotrf_llttst(uplo, N, lda):
+    A0 = O.lltgen(uplo, N, lda, dtype=np.float32)
+    A = A0.copy(order="F")
+    assert X.potrf(A[:N, :N], uplo) == 0
+    unref = np.triu(np.ones((N, N), bool), 1) if uplo == X.Lower else np.tril(np.ones((N, N), bool), -1)
+    assert np.all(A[:N, :N][unref] == A0[:N, :N][unref])
+    assert O.llt_resid(uplo, A[:N, :N], A0[:N, :N]) < 100
+
+
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+def test_spotrs_sposv_sgesv(uplo):
+    n, nrhs = 300, 5
+    S = O.lltgen(uplo, n, dtype=np.float32)
+    Sf = np.tril(S) + np.tril(S, -1).T if uplo == X.Lower else np.triu(S) + np.triu(S, 1).T
+    Xc = O.gegen(n, nrhs, dtype=np.float32)
+    Bw = _f((Sf.astype(np.float64) @ Xc).astype(np.float32))
+    A = S.copy(order="F")
+    B = Bw.copy(order="F")
+    assert X.potrf(A, uplo) == 0
+    assert X.potrs(A, B, uplo) == 0
+    assert solve_ok(O.genrm1(_f(Sf)), Xc, B)
+    A = S.copy(order="F")
+    B = Bw.copy(order="F")
+    assert X.lib().clapack_sposv(X.ColMajor, uplo, n, nrhs, A.ctypes.data, n, B.ctypes.data,
+                                 n) == 0
+    assert solve_ok(O.genrm1(_f(Sf)), Xc, B)
+    G = O.gegen(n, n, dtype=np.float32)
+    B = _f((G.astype(np.float64) @ Xc).astype(np.float32))
+    Gw = G.copy(order="F")
+    ipiv = np.zeros(n, dtype=np.int32)
+    assert X.lib().clapack_sgesv(X.ColMajor, n, nrhs, Gw.ctypes.data, n, ipiv.ctypes.data,
+                                 B.ctypes.data, n) == 0
+    assert solve_ok(O.genrm1(G), Xc, B)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_getrf_tall_panel_beyond_131072_rows(dt):
+    """A 200,000 x 64 LU: taller than one cooperative panel launch holds;
+    pivots against the oracle and the lutst residual."""
+    M, N = 200000, 64
+    A0 = _f(RNG.uniform(-1, 1, (M, N)).astype(dt))
+    A = A0.copy(order="F")
+    info, ipiv = X.getrf(A)
+    assert info == 0
+    Ao = A0.copy(order="F")
+    gaps = np.ones(N)
+    _, opiv = O.getrf(Ao, gaps=gaps)
+    near = np.nonzero(gaps < (1e-9 if dt == np.float64 else 1e-4))[0]
+    stop = int(near[0]) if near.size else N
+    assert np.array_equal(ipiv[:stop], opiv[:stop])
+    assert O.lu_resid(A, ipiv, A0) < 100
+
+
+def test_two_host_threads():
+    """cblas_dgemm and clapack_dgetrf called from two host threads at once
+    (each thread has its own stream and workspaces)."""
+    import threading
+    res = {}
+
+    def gemm_job():
+        A = _f(RNG.uniform(-0.5, 0.5, (300, 200)))
+        B = _f(RNG.uniform(-0.5, 0.5, (200, 250)))
+        for _ in range(20):
+            Cm = np.zeros((300, 250), order="F")
+            X.gemm(X.NoTrans, X.NoTrans, 1.0, A, B, 0.0, Cm)
+        res["gemm"] = np.abs(Cm - A @ B).max()
+
+    def lu_job():
+        worst = 0.0
+        for k in range(10):
+            A0 = O.gegen(400, 400, seed=k + 5)
+            A = A0.copy(order="F")
+            info, ipiv = X.getrf(A)
+            worst = max(worst, O.lu_resid(A, ipiv, A0) if info == 0 else 1e9)
+        res["lu"] = worst
+
+    th = [threading.Thread(target=gemm_job), threading.Thread(target=lu_job)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    assert res["gemm"] < 1e-12
+    assert res["lu"] < 100

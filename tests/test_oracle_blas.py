@@ -175,3 +175,126 @@ def test_laswp_forward_backward_inverse():
     assert np.array_equal(A, A0[perm])
     B = O.laswp(A.copy(order="F"), 0, 10, piv, -1)
     assert np.array_equal(B, A0)
+
+
+def _ref_fn(R, name, dt, nint, nscal_first=True):
+    ct = C.c_double if dt == np.float64 else C.c_float
+    return getattr(R, ("ATL_d" if dt == np.float64 else "ATL_s") + name), ct
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("alpha", [1.5, -0.75, 0.0])
+def test_trmm_ref_bit_exact_vs_atlas_reftrmm(dt, alpha):
+    """All 16 side/uplo/trans/diag variants, bit for bit against ATLAS's own
+    reference TRMM (src/blas/reference/level3/ATL_dreftrmm*.c)."""
+    R = _ref_or_skip()
+    fn, ct = _ref_fn(R, "reftrmm", dt, 6)
+    fn.argtypes = [C.c_int] * 6 + [ct, C.c_void_p, C.c_int, C.c_void_p, C.c_int]
+    for side, uplo, ta, diag in itertools.product([O.LEFT, O.RIGHT], [O.UPPER, O.LOWER],
+                                                  [O.NOTRANS, O.TRANS], [O.NONUNIT, O.UNIT]):
+        M, N = 13, 9
+        na = M if side == O.LEFT else N
+        A = _f(RNG.uniform(-1, 1, (na + 3, na)).astype(dt))      # lda = na + 3
+        B0 = _f(RNG.uniform(-1, 1, (M + 2, N)).astype(dt))       # ldb = M + 2
+        b1, b2 = B0.copy(order="F"), B0.copy(order="F")
+        O.trmm(side, uplo, ta, diag, alpha, A[:na, :na], b1[:M, :])
+        fn(side, uplo, ta, diag, M, N, alpha, A.ctypes.data, na + 3, b2.ctypes.data, M + 2)
+        assert np.array_equal(b1, b2), (side, uplo, ta, diag)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("alpha,beta", [(1.25, 0.5), (-1.0, 0.0), (0.7, 1.0), (0.0, 0.3)])
+def test_symm_syr2k_bit_exact_vs_atlas_ref(dt, alpha, beta):
+    R = _ref_or_skip()
+    ct = C.c_double if dt == np.float64 else C.c_float
+    symm = R.ATL_drefsymm if dt == np.float64 else R.ATL_srefsymm
+    symm.argtypes = [C.c_int] * 4 + [ct, C.c_void_p, C.c_int, C.c_void_p, C.c_int, ct,
+                                     C.c_void_p, C.c_int]
+    for side, uplo in itertools.product([O.LEFT, O.RIGHT], [O.UPPER, O.LOWER]):
+        M, N = 11, 7
+        na = M if side == O.LEFT else N
+        A = _f(RNG.uniform(-1, 1, (na, na)).astype(dt))
+        B = _f(RNG.uniform(-1, 1, (M, N)).astype(dt))
+        C0 = _f(RNG.uniform(-1, 1, (M, N)).astype(dt))
+        c1, c2 = C0.copy(order="F"), C0.copy(order="F")
+        O.symm(side, uplo, alpha, A, B, beta, c1)
+        symm(side, uplo, M, N, alpha, A.ctypes.data, na, B.ctypes.data, M, beta,
+             c2.ctypes.data, M)
+        assert np.array_equal(c1, c2), (side, uplo)
+    syr2k = R.ATL_drefsyr2k if dt == np.float64 else R.ATL_srefsyr2k
+    syr2k.argtypes = [C.c_int] * 4 + [ct, C.c_void_p, C.c_int, C.c_void_p, C.c_int, ct,
+                                      C.c_void_p, C.c_int]
+    for uplo, tr in itertools.product([O.UPPER, O.LOWER], [O.NOTRANS, O.TRANS]):
+        N, K = 10, 6
+        shp = (N, K) if tr == O.NOTRANS else (K, N)
+        A = _f(RNG.uniform(-1, 1, shp).astype(dt))
+        B = _f(RNG.uniform(-1, 1, shp).astype(dt))
+        C0 = _f(RNG.uniform(-1, 1, (N, N)).astype(dt))
+        c1, c2 = C0.copy(order="F"), C0.copy(order="F")
+        O.syr2k(uplo, tr, alpha, A, B, beta, c1)
+        syr2k(uplo, tr, N, K, alpha, A.ctypes.data, shp[0], B.ctypes.data, shp[0], beta,
+              c2.ctypes.data, N)
+        assert np.array_equal(c1, c2), (uplo, tr)
+
+
+def test_trmm_symm_syr2k_closed_forms_vs_numpy():
+    """The restatements against the mathematics: dense products of the
+    expanded triangle / symmetric matrix (independent of _ref)."""
+    n, m = 17, 6
+    for side, uplo, ta, diag in itertools.product([O.LEFT, O.RIGHT], [O.UPPER, O.LOWER],
+                                                  [O.NOTRANS, O.TRANS], [O.NONUNIT, O.UNIT]):
+        A = _f(RNG.uniform(-1, 1, (n, n)))
+        Tm = np.tril(A) if uplo == O.LOWER else np.triu(A)
+        if diag == O.UNIT:
+            np.fill_diagonal(Tm, 1.0)
+        opT = Tm if ta == O.NOTRANS else Tm.T
+        B0 = _f(RNG.uniform(-1, 1, (n, m) if side == O.LEFT else (m, n)))
+        got = O.trmm(side, uplo, ta, diag, 0.5, A, B0.copy(order="F"))
+        want = 0.5 * (opT @ B0 if side == O.LEFT else B0 @ opT)
+        assert np.allclose(got, want, atol=1e-13, rtol=0), (side, uplo, ta, diag)
+    for side, uplo in itertools.product([O.LEFT, O.RIGHT], [O.UPPER, O.LOWER]):
+        A = _f(RNG.uniform(-1, 1, (n, n)))
+        S = np.tril(A) + np.tril(A, -1).T if uplo == O.LOWER else np.triu(A) + np.triu(A, 1).T
+        B = _f(RNG.uniform(-1, 1, (n, m) if side == O.LEFT else (m, n)))
+        C0 = _f(RNG.uniform(-1, 1, B.shape))
+        got = O.symm(side, uplo, 1.5, A, B, -0.5, C0.copy(order="F"))
+        want = 1.5 * (S @ B if side == O.LEFT else B @ S) - 0.5 * C0
+        assert np.allclose(got, want, atol=1e-13, rtol=0), (side, uplo)
+    for uplo, tr in itertools.product([O.UPPER, O.LOWER], [O.NOTRANS, O.TRANS]):
+        A = _f(RNG.uniform(-1, 1, (n, m) if tr == O.NOTRANS else (m, n)))
+        B = _f(RNG.uniform(-1, 1, A.shape))
+        C0 = _f(RNG.uniform(-1, 1, (n, n)))
+        got = O.syr2k(uplo, tr, 0.75, A, B, 2.0, C0.copy(order="F"))
+        oA, oB = (A, B) if tr == O.NOTRANS else (A.T, B.T)
+        full = 0.75 * (oA @ oB.T + oB @ oA.T) + 2.0 * C0
+        tri = np.tril(np.ones((n, n), bool)) if uplo == O.LOWER else np.triu(np.ones((n, n), bool))
+        assert np.allclose(got[tri], full[tri], atol=1e-13, rtol=0)
+        assert np.array_equal(got[~tri], C0[~tri])          # other triangle untouched
+
+
+def test_norm_helpers_vs_numpy():
+    """genrm1 (max column sum), gediffnrm1, infnrm and lltnrm1 are the
+    denominators of every lutst/llttst/slvtst threshold: pin them to numpy's
+    norms on non-symmetric inputs, where a row-sum/column-sum mix-up shows."""
+    A = _f(RNG.uniform(-1, 1, (23, 17)))
+    A[:, 3] *= 9.0                           # one heavy column: norm1 != norm inf
+    assert abs(np.linalg.norm(A, 1) - np.linalg.norm(A, np.inf)) > 1.0
+    assert O.genrm1(A) == pytest.approx(np.linalg.norm(A, 1), rel=1e-14)
+    # a planted mistake (the row-sum norm, i.e. genrm1 of the transpose)
+    # must not pass the same check
+    assert O.genrm1(_f(A.T)) != pytest.approx(np.linalg.norm(A, 1), rel=1e-3)
+    B = _f(RNG.uniform(-1, 1, A.shape))
+    assert O.gediffnrm1(A, B) == pytest.approx(np.linalg.norm(A - B, 1), rel=1e-14)
+    x = RNG.uniform(-1, 1, 41)
+    assert O.infnrm(x) == np.abs(x).max()
+    x[7] = np.nan
+    assert np.isnan(O.infnrm(x))             # ATL_infnrm.c:30-31 returns the NaN
+    for uplo in (O.LOWER, O.UPPER):
+        n = 19
+        T = _f(RNG.uniform(-1, 1, (n, n)))
+        T[:, 2] *= 7.0
+        S = np.tril(T) + np.tril(T, -1).T if uplo == O.LOWER else np.triu(T) + np.triu(T, 1).T
+        assert O.lltnrm1(uplo, T) == pytest.approx(np.linalg.norm(S, 1), rel=1e-14)
+        # reading the other triangle (the planted mistake) gives another norm
+        other = O.UPPER if uplo == O.LOWER else O.LOWER
+        assert O.lltnrm1(other, T) != pytest.approx(np.linalg.norm(S, 1), rel=1e-3)

@@ -220,3 +220,131 @@ def test_trsm_resid_flags_planted_error():
     assert O.trsm_resid(O.LEFT, O.LOWER, O.NOTRANS, O.NONUNIT, 0.5, L, X, B0) < 1
     X[7, 3] *= 1 + 1e-9
     assert O.trsm_resid(O.LEFT, O.LOWER, O.NOTRANS, O.NONUNIT, 0.5, L, X, B0) > 1
+
+
+def test_lltgen_diagonal_rule_and_spd():
+    """bin/llttst.c:225-259: gegen, the unreferenced triangle zeroed, then
+    a_jj = |5.3 a_jj| + ||A||_1 of the symmetric matrix.  Checked through
+    numpy: the off-diagonal part is gegen's, the diagonal exceeds every
+    row's off-diagonal absolute sum (so A is SPD and Cholesky succeeds)."""
+    import oracle as Or
+    for uplo in (Or.LOWER, Or.UPPER):
+        n = 37
+        A = Or.lltgen(uplo, n)
+        G = Or.gegen(n, n, seed=n * 1029 + n)
+        ref_tri = np.tril(np.ones((n, n), bool), -1) if uplo == Or.LOWER else \
+            np.triu(np.ones((n, n), bool), 1)
+        assert np.array_equal(A[ref_tri], G[ref_tri])
+        assert np.all(A[~ref_tri & ~np.eye(n, dtype=bool)] == 0)
+        Gt = np.where(ref_tri | np.eye(n, dtype=bool), G, 0.0)
+        S0 = Gt + Gt.T - np.diag(np.diag(Gt))
+        nrm = np.linalg.norm(S0, 1)
+        assert np.allclose(np.diag(A), np.abs(5.3 * np.diag(G)) + nrm, rtol=1e-15, atol=0)
+        S = A + A.T - np.diag(np.diag(A))
+        off = np.abs(S).sum(axis=1) - np.abs(np.diag(S))
+        assert np.all(np.diag(S) > off)
+        np.linalg.cholesky(S)
+
+
+def test_nan_solution_fails_the_solve_check():
+    """slvtst's difference norm skips NaN (bin/slvtst.c:400-415 compares with
+    '>'), so the tests' solve check also asserts finiteness: a NaN solution
+    must fail it."""
+    import oracle as Or
+    from tests.checks import solve_ok
+    n = 50
+    A = Or.gegen(n, n)
+    Xc = Or.gegen(n, 2)
+    Xn = np.full_like(Xc, np.nan)
+    assert Or.slv_resid(Or.genrm1(A), Xc, Xn) == 0.0     # the tester alone passes it
+    assert not solve_ok(Or.genrm1(A), Xc, Xn)
+    assert solve_ok(Or.genrm1(A), Xc, Xc.copy(order="F"))
+
+
+def test_inverse_testers_pass_exact_and_fail_planted():
+    """The inverse testers (bin/trtritst.c, bin/invtst.c, bin/uumtst.c):
+    small residual for numpy's inverse / product, large for a planted error
+    of one entry in the 1e-8 range."""
+    import oracle as Or
+    n = 60
+    for uplo in (Or.LOWER, Or.UPPER):
+        for diag in (Or.NONUNIT, Or.UNIT):
+            A0 = Or.trigen(uplo, diag, n)
+            T = np.tril(A0) if uplo == Or.LOWER else np.triu(A0)
+            if diag == Or.UNIT:
+                np.fill_diagonal(T, 1.0)
+            Ti = np.asfortranarray(np.linalg.inv(T))
+            assert Or.trtri_resid(uplo, diag, Ti, A0) < 10
+            Ti[n - 1 if uplo == Or.LOWER else 0, n // 2] += 1e-8
+            assert Or.trtri_resid(uplo, diag, Ti, A0) > 100
+    A = Or.gegen(n, n)
+    Ai = np.asfortranarray(np.linalg.inv(A))
+    assert Or.inv_resid(A, Ai) < 10
+    Ai[3, 7] += 1e-8 * np.abs(Ai).max()
+    assert Or.inv_resid(A, Ai) > 100
+    for uplo in (Or.LOWER, Or.UPPER):
+        S = Or.lltgen(uplo, n)
+        Sf = np.tril(S) + np.tril(S, -1).T if uplo == Or.LOWER else np.triu(S) + np.triu(S, 1).T
+        Si = np.asfortranarray(np.linalg.inv(Sf))
+        assert Or.poinv_resid(uplo, S, Si) < 10
+        Si[n - 2, 1] += 1e-8 * np.abs(Si).max()
+        Si[1, n - 2] += 1e-8 * np.abs(Si).max()
+        assert Or.poinv_resid(uplo, S, Si) > 100
+        L = np.tril(S) if uplo == Or.LOWER else np.triu(S)
+        R = np.asfortranarray(L.T @ L if uplo == Or.LOWER else L @ L.T)
+        assert Or.lauum_resid(uplo, R, S) < 10
+        R[n - 1, 0] += 1e-6
+        R[0, n - 1] += 1e-6
+        assert Or.lauum_resid(uplo, R, S) > 100
+
+
+def test_trinrm1_and_trigen_vs_numpy():
+    import oracle as Or
+    n = 31
+    for uplo in (Or.LOWER, Or.UPPER):
+        for diag in (Or.NONUNIT, Or.UNIT):
+            A = Or.trigen(uplo, diag, n, padval=-7.0)
+            tri = np.tril(np.ones((n, n), bool)) if uplo == Or.LOWER else np.triu(np.ones((n, n), bool))
+            ref_part = tri & ~np.eye(n, dtype=bool) if diag == Or.UNIT else tri
+            assert np.all(A[~ref_part] == -7.0)              # padding outside what is referenced
+            T = np.where(ref_part, A, 0.0)
+            assert Or.trinrm1(uplo, diag, A) == pytest.approx(np.linalg.norm(T, 1), rel=1e-14)
+            if diag == Or.NONUNIT:                         # diagonally dominant by columns and rows
+                off = np.abs(T - np.diag(np.diag(T)))
+                assert np.all(np.abs(np.diag(T)) > off.sum(axis=0))
+                assert np.all(np.abs(np.diag(T)) > off.sum(axis=1))
+
+
+@pytest.mark.parametrize("M,N", [(1, 1), (7, 5), (5, 7), (64, 64), (130, 40), (40, 130)])
+def test_geqr2_vs_lapack_geqrf(M, N):
+    """The oracle's unblocked Householder QR against LAPACK's dgeqrf (raw
+    storage and tau through scipy), and qrtst's residual: small for the
+    factors, large for a planted error in one reflector entry."""
+    import oracle as Or
+    import scipy.linalg as sla
+    A0 = Or.gegen(M, N)
+    A = A0.copy(order="F")
+    tau = Or.geqr2(A)
+    (qr_l, tau_l), _ = sla.qr(A0, mode="raw")
+    assert np.allclose(A, qr_l, atol=1e-12, rtol=0)
+    assert np.allclose(tau, tau_l, atol=1e-13, rtol=0)
+    assert Or.qr_resid(A, tau, A0) < 10
+    if M > 2 and min(M, N) > 0:
+        B = A.copy(order="F")
+        B[M - 1, 0] += 1e-9
+        assert Or.qr_resid(B, tau, A0) > 100
+
+
+def test_larfg_zero_and_tiny_columns():
+    """larfg's special cases: a zero column below the diagonal gives tau = 0
+    and leaves the column alone; a column scaled to 1e-300 (below
+    safmin/eps) is rescaled first and still factors exactly."""
+    import oracle as Or
+    A = np.asfortranarray(np.array([[3.0, 1.0], [0.0, 2.0], [0.0, 5.0]]))
+    tau = Or.geqr2(A)
+    assert tau[0] == 0.0 and A[0, 0] == 3.0
+    T = np.asfortranarray(np.array([[3e-300, 1.0], [4e-300, 1.0]]))
+    T0 = T.copy(order="F")
+    tau = Or.geqr2(T)
+    assert T[0, 0] == pytest.approx(-5e-300, rel=1e-14)
+    assert Or.qr_resid(T, tau, T0) < 10
