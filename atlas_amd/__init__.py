@@ -133,6 +133,10 @@ def _declare(L):
     L.atlmx_set_mixed_method.argtypes = [_I]
     L.atlmx_set_mixed_method.restype = None
     L.atlmx_dbackward_error.restype = _D
+    L.atlmx_dgetrs_noalloc.argtypes = [_I, _I, _I, _P, _I, _P, _P, _I]
+    L.atlmx_dgetrs_noalloc.restype = _I
+    L.atlmx_noalloc_scratch_bytes.argtypes = []
+    L.atlmx_noalloc_scratch_bytes.restype = C.c_size_t
 
 
 def _pre(dt):

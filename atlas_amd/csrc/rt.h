@@ -53,6 +53,7 @@ typedef struct atlmx_ctx {
   hipStream_t aux;          /* second stream for lookahead work (lazily created) */
   hipEvent_t ev[3];         /* ordering events between stream and aux   */
   int ncu;                  /* compute units of the device (set with aux) */
+  int noprof;               /* 1: the profiler records nothing for this context */
 } atlmx_ctx;
 
 atlmx_ctx *rt_ctx(void);

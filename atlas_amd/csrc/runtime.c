@@ -143,7 +143,7 @@ int rt_prof_begin(atlmx_ctx *c, int cls, double work)
 int rt_prof_begin2(atlmx_ctx *c, int cls, double work, double bytes)
 {
   prof_rec *r;
-  if (!(prof_enabled & (1 << cls)) || prof_n >= PROF_MAX) return -1;
+  if (!(prof_enabled & (1 << cls)) || prof_n >= PROF_MAX || c->noprof) return -1;
   r = &prof_buf[prof_n];
   if (!r->a) {
     RT_CHECK("atlmx_prof", hipEventCreate(&r->a));
@@ -163,9 +163,12 @@ void rt_prof_end(atlmx_ctx *c, int id)
 }
 
 /* ---- public runtime entry points (declared in include/atlmx.h) ---- */
+void atlmx_noalloc_setup(void);   /* noalloc.hip */
+
 int atlmx_init(void)
 {
   (void)rt_ctx();
+  atlmx_noalloc_setup();
   return 0;
 }
 

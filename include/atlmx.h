@@ -11,6 +11,8 @@
 #ifndef ATLMX_H
 #define ATLMX_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -173,6 +175,24 @@ int atlmx_dsgesv(int N, int NRHS, const double *A, int lda, const double *B, int
                  double *X, int ldx, int iter_max, atlmx_ir_info *info);
 double atlmx_dbackward_error(int N, int NRHS, const double *A, int lda, const double *B,
                              int ldb, const double *X, int ldx);   /* synchronizes */
+
+/* ---- allocation-free LU solve --------------------------------------------
+ * No counterpart in ATLAS (nor a reference interface): it is for callers
+ * that cannot allocate device memory -- code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc.  The same solve as
+ * atlmx_dgetrs (same kernels, same order, bit-identical results), but its
+ * scratch (the inverted 64 x 64 diagonal blocks of the factor and the
+ * solve's flags) lives in one static device array of
+ * atlmx_noalloc_scratch_bytes() bytes per GPU, held until the process ends;
+ * no hipMalloc, hipFree or other allocation is made.  N <= ATLMX_NOALLOC_MAX_N:
+ * a larger N (or another illegal argument) returns minus its position, with
+ * a message on stderr, and does nothing.  Asynchronous on the calling
+ * thread's stream; returns 0.  Calls that overlap on one GPU (other threads
+ * or streams) race on the shared array. */
+#define ATLMX_NOALLOC_MAX_N 16384
+int atlmx_dgetrs_noalloc(int Trans, int N, int NRHS, const double *A, int lda,
+                         const int *d_ipiv, double *B, int ldb);
+size_t atlmx_noalloc_scratch_bytes(void);
 
 #ifdef __cplusplus
 }

@@ -81,6 +81,19 @@ def test_batch_plan_order_limits_without_gpu():
 
 
 
+def test_noalloc_refuses_too_large_order_without_gpu():
+    """The allocation-free solve checks its arguments before any device call:
+    an order beyond its static scratch array is refused with a message."""
+    code = ("import sys; sys.path.insert(0, %r); import atlas_amd as a; L = a.lib();"
+            "print('RC', L.atlmx_dgetrs_noalloc(111, 16385, 1, None, 16385, None, None, 16385),"
+            " L.atlmx_dgetrs_noalloc(111, 10, 1, None, 10, None, None, 9))" % ROOT)
+    p = subprocess.run(["python", "-c", code], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "RC -2 -8" in p.stdout
+    assert "N = 16385 is outside [0, 16384]" in p.stderr
+
+
 def test_f77_xerbla_messages_without_gpu():
     """F77 argument errors are caught before any device call: XERBLA's
     message with the Fortran argument position (interfaces/blas/F77/src/
