@@ -13,8 +13,9 @@
  * calling thread's stream.
  *
  * The autotuner times every candidate configuration -- panel width 16 or
- * 32, one to three persistent workgroups per CU; for GEMM the workgroups per
- * CU -- on a replica of up to AT_SAMPLE instances drawn evenly from the
+ * 32, one to three persistent workgroups per CU; for GEMM the kernel
+ * (vector FMAs, or MFMA with one or four wavefronts per 64 x 64 tile) and
+ * the workgroups per CU -- on a replica of up to AT_SAMPLE instances drawn evenly from the
  * ordered batch (well-conditioned synthetic data of the same sizes, so
  * tuning never touches the caller's arrays), and keeps the fastest.
  * Results are cached for the process by (kind, precision, nrhs, size
@@ -89,15 +90,18 @@ static hipError_t launch(const atlmx_batch *p, int count, const int *meta, void 
   if (p->kind == BK_GEMM) {
     const int *m = meta + 4 * count, *k = meta + 5 * count, *ldc = meta + 6 * count;
     const int *ts = meta + 7 * count;
-    /* meta layout for GEMM: order | n | lda | ldb | m | k | ldc | tile_start */
+    /* meta layout for GEMM: order | n | lda | ldb | m | k | ldc | tile_start;
+     * nb is the kernel variant (0 vector FMAs, 1 / 2 MFMA with one / four
+     * wavefronts per tile) */
     if (p->dbl)
       return atlmx_k_dbatch_gemm(count, p->ta, p->tb, order, m, n, k, p->alpha,
                                  (const double *const *)A, lda, (const double *const *)B, ldb,
-                                 p->beta, (double *const *)C, ldc, ts, p->d_counter, grid, s);
+                                 p->beta, (double *const *)C, ldc, ts, p->d_counter, nb, grid,
+                                 s);
     return atlmx_k_sbatch_gemm(count, p->ta, p->tb, order, m, n, k, (float)p->alpha,
                                (const float *const *)A, lda, (const float *const *)B, ldb,
-                               (float)p->beta, (float *const *)C, ldc, ts, p->d_counter, grid,
-                               s);
+                               (float)p->beta, (float *const *)C, ldc, ts, p->d_counter, nb,
+                               grid, s);
   }
   if (p->kind == BK_CHOLADJ) {
     /* meta: order | n | lda | ldx | ldg; A = L, B = X, C = Bb */
@@ -271,11 +275,15 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   RT_CHECK(rout, hipEventCreate(&e0));
   RT_CHECK(rout, hipEventCreate(&e1));
 
-  p->nb = 32;
+  p->nb = p->kind == BK_GEMM ? 2 : 32;
   p->grid = cus;
-  for (c = 0; c < (p->kind == BK_GEMM ? 1 : 2); ++c)
+  /* factorizations: panel width 16 or 32 x one to three workgroups per CU;
+   * GEMM: the kernel variant (vector FMAs, MFMA with one or four
+   * wavefronts per 64 x 64 tile) x workgroups per CU */
+  for (c = 0; c < (p->kind == BK_GEMM ? 3 : 2); ++c)
     for (reps = 1; reps <= (p->kind == BK_GEMM ? 4 : 3); ++reps) {
-      const int nb = nbs[c], grid = reps * cus;
+      const int nb = p->kind == BK_GEMM ? c : nbs[c];
+      const int grid = (p->kind == BK_GEMM && c == 1 ? 3 : 1) * reps * cus;
       float ms = 0.0f;
       hipError_t e;
       RT_CHECK(rout, hipMemcpy(pool, pool0, total, hipMemcpyDeviceToDevice));

@@ -819,6 +819,140 @@ hipError_t batch_factor(int kind, int nb, int count, const int *order, const int
   return hipErrorInvalidValue;
 }
 
+/* The same work items on the matrix cores: a 64 x 64 tile of C per item,
+ * NW wavefronts per tile (1: one wavefront holds the whole tile as 4 x 4
+ * MFMA blocks of 16 x 16; 4: a 2 x 2 grid of wavefronts, each 32 x 32 =
+ * 2 x 2 blocks), K in slices of 16 staged through LDS as [k][row], the next
+ * slice's loads issued before the current slice's MFMAs.  FP64:
+ * v_mfma_f64_16x16x4f64 (lane l, element r of the accumulator holds
+ * D[(l>>4) + 4r][l&15]); FP32: v_mfma_f32_16x16x4f32 (D[4(l>>4) + r][l&15]).
+ * In one MFMA lane l supplies op(A)(row l&15, k slot l>>4) and op(B)(k slot
+ * l>>4, column l&15); over the four sub-steps s4 the slots l>>4 = 0..3 take
+ * k = 4 (l>>4) + s4, so the 16 k of a slice are each used once.  The
+ * choice of NW is the autotuner's. */
+typedef double bd4 __attribute__((ext_vector_type(4)));
+typedef float bf4 __attribute__((ext_vector_type(4)));
+template <typename T> struct BMfma;
+template <> struct BMfma<double> {
+  typedef bd4 acc_t;
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  __device__ static int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct BMfma<float> {
+  typedef bf4 acc_t;
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  __device__ static int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T, int NW>
+__global__ __launch_bounds__(64 * NW) void gemm_batched_mfma_kernel(GemmBatchArgs<T> g) {
+  typedef BMfma<T> MF;
+  typedef typename MF::acc_t acc_t;
+  constexpr int NTH = 64 * NW;
+  constexpr int KS = 16;                       /* K slice */
+  constexpr int LDT = TS + 4;                  /* LDS row: [k][row], padded */
+  constexpr int PER = KS * TS / NTH;           /* elements of a slice per thread */
+  constexpr int BPW = (NW == 1) ? 4 : 2;       /* 16 x 16 blocks per wavefront side */
+  __shared__ T s_a[KS * LDT], s_b[KS * LDT];
+  __shared__ int s_item;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (NW == 1) ? 0 : (wave & 1) * 32, wn = (NW == 1) ? 0 : (wave >> 1) * 32;
+  const int lr = lane & 15, kq = (lane >> 4) * 4;
+  const int total = g.tile_start[g.count];
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) s_item = atomicAdd(g.counter, 1);
+    __syncthreads();
+    const int it = s_item;
+    if (it >= total) return;
+    int lo = 0, hi = g.count - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (g.tile_start[mid] <= it) lo = mid; else hi = mid - 1;
+    }
+    const int i = g.order[lo], t = it - g.tile_start[lo];
+    const int M = g.m[i], N = g.n[i], K = g.k[i];
+    const int tm = (M + TS - 1) / TS;
+    const int r0 = (t % tm) * TS, c0 = (t / tm) * TS;
+    const T *A = g.A[i], *B = g.B[i];
+    const long lda = g.lda[i], ldb = g.ldb[i], ldc = g.ldc[i];
+    /* element e of a slice: op(A)(r0 + ra, k0 + ka) and op(B)(k0 + kb, c0 + cb),
+     * numbered so that consecutive threads read consecutive addresses */
+    T va[PER], vb[PER];
+    auto load = [&](int k0) {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + NTH * q;
+        int ra, ka, kb, cb;
+        if (g.ta == KX_NOTRANS) { ka = e / TS; ra = e % TS; } else { ra = e / KS; ka = e % KS; }
+        if (g.tb == KX_NOTRANS) { cb = e / KS; kb = e % KS; } else { kb = e / TS; cb = e % TS; }
+        const int gr = r0 + ra, gka = k0 + ka, gc = c0 + cb, gkb = k0 + kb;
+        va[q] = (gr < M && gka < K)
+                    ? (g.ta == KX_NOTRANS ? A[(size_t)gka * lda + gr] : A[(size_t)gr * lda + gka])
+                    : T(0);
+        vb[q] = (gc < N && gkb < K)
+                    ? (g.tb == KX_NOTRANS ? B[(size_t)gc * ldb + gkb] : B[(size_t)gkb * ldb + gc])
+                    : T(0);
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int q = 0; q < PER; ++q) {
+        const int e = tid + NTH * q;
+        int ra, ka, kb, cb;
+        if (g.ta == KX_NOTRANS) { ka = e / TS; ra = e % TS; } else { ra = e / KS; ka = e % KS; }
+        if (g.tb == KX_NOTRANS) { cb = e / KS; kb = e % KS; } else { kb = e / TS; cb = e % TS; }
+        s_a[ka * LDT + ra] = va[q];
+        s_b[kb * LDT + cb] = vb[q];
+      }
+    };
+    acc_t acc[BPW][BPW];
+#pragma unroll
+    for (int a = 0; a < BPW; ++a)
+#pragma unroll
+      for (int b = 0; b < BPW; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    if (K > 0) load(0);
+    for (int k0 = 0; k0 < K; k0 += KS) {
+      __syncthreads();
+      store();
+      __syncthreads();
+      if (k0 + KS < K) load(k0 + KS);
+#pragma unroll
+      for (int s4 = 0; s4 < 4; ++s4) {
+        T a[BPW], b[BPW];
+#pragma unroll
+        for (int x = 0; x < BPW; ++x) {
+          a[x] = s_a[(kq + s4) * LDT + wm + 16 * x + lr];
+          b[x] = s_b[(kq + s4) * LDT + wn + 16 * x + lr];
+        }
+#pragma unroll
+        for (int x = 0; x < BPW; ++x)
+#pragma unroll
+          for (int y = 0; y < BPW; ++y) acc[x][y] = MF::mma(a[x], b[y], acc[x][y]);
+      }
+    }
+    T *C = g.C[i];
+#pragma unroll
+    for (int x = 0; x < BPW; ++x)
+#pragma unroll
+      for (int y = 0; y < BPW; ++y)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+          const int r = r0 + wm + 16 * x + MF::drow(lane, rr), c = c0 + wn + 16 * y + lr;
+          if (r < M && c < N) {
+            T *pc = C + (size_t)c * ldc + r;
+            T v = g.alpha * acc[x][y][rr];
+            if (g.beta != T(0)) v += g.beta * *pc;
+            *pc = v;
+          }
+        }
+  }
+}
+
 /* Largest instance order whose LDS working set fits (panel width 16, the
  * smaller of the two the autotuner tries). */
 template <typename T>
@@ -834,17 +968,23 @@ hipError_t batch_gemm(int count, int ta, int tb, const int *order, const int *m,
                       const int *k,
                       T alpha, const T *const *A, const int *lda, const T *const *B,
                       const int *ldb, T beta, T *const *C, const int *ldc,
-                      const int *tile_start, int *counter, int grid, hipStream_t s) {
+                      const int *tile_start, int *counter, int variant, int grid,
+                      hipStream_t s) {
   if (count <= 0) return hipSuccess;
   if ((ta != KX_NOTRANS && ta != KX_TRANS) || (tb != KX_NOTRANS && tb != KX_TRANS) ||
-      grid <= 0)
+      grid <= 0 || variant < 0 || variant > 2)
     return hipErrorInvalidValue;
   GemmBatchArgs<T> g;
   g.count = count; g.ta = ta; g.tb = tb; g.order = order; g.m = m; g.n = n; g.k = k;
   g.alpha = alpha;
   g.beta = beta; g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
   g.tile_start = tile_start; g.counter = counter;
-  hipLaunchKernelGGL(gemm_batched_kernel<T>, dim3(grid), dim3(NT), 0, s, g);
+  if (variant == 0)
+    hipLaunchKernelGGL(gemm_batched_kernel<T>, dim3(grid), dim3(NT), 0, s, g);
+  else if (variant == 1)
+    hipLaunchKernelGGL((gemm_batched_mfma_kernel<T, 1>), dim3(grid), dim3(64), 0, s, g);
+  else
+    hipLaunchKernelGGL((gemm_batched_mfma_kernel<T, 4>), dim3(grid), dim3(256), 0, s, g);
   return hipGetLastError();
 }
 
@@ -904,19 +1044,19 @@ extern "C" hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *
                                           const int *lda, const double *const *B,
                                           const int *ldb, double beta, double *const *C,
                                           const int *ldc, const int *tile_start, int *counter,
-                                          int grid, hipStream_t s) {
+                                          int variant, int grid, hipStream_t s) {
   return batch_gemm<double>(count, ta, tb, order, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
-                            tile_start, counter, grid, s);
+                            tile_start, counter, variant, grid, s);
 }
 extern "C" hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *order,
                                           const int *m, const int *n,
                                           const int *k, float alpha, const float *const *A,
                                           const int *lda, const float *const *B, const int *ldb,
                                           float beta, float *const *C, const int *ldc,
-                                          const int *tile_start, int *counter, int grid,
-                                          hipStream_t s) {
+                                          const int *tile_start, int *counter, int variant,
+                                          int grid, hipStream_t s) {
   return batch_gemm<float>(count, ta, tb, order, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
-                           tile_start, counter, grid, s);
+                           tile_start, counter, variant, grid, s);
 }
 
 /* kind 0 Cholesky, 1 LU (2, the transposed solve, as 1) */

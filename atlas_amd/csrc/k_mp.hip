@@ -186,9 +186,78 @@ __global__ void scale_conv_kernel(int M, int N, const float *D, long ldd, const 
   }
 }
 
+/* Dot products out[j] = sum_i V(i, j) w(i), j < kc (the Gram-Schmidt
+ * coefficients of GMRES).  Grid (ceil(M / GT_ROWS), ceil(kc / GT_CH)): each
+ * block sums GT_CH columns over its GT_ROWS rows (w read once per block and
+ * kept in a register across the columns), reduces across its 256 threads
+ * and writes part[j][bx]; a second pass adds the row blocks in fixed order,
+ * so the result is deterministic. */
+constexpr int GT_ROWS = 1024;
+constexpr int GT_CH = 8;
+__global__ __launch_bounds__(256) void gemv_t_part_kernel(int M, int kc, int nbx,
+                                                          const double *__restrict__ V,
+                                                          long ldv,
+                                                          const double *__restrict__ w,
+                                                          double *__restrict__ part) {
+  const int bx = blockIdx.x, j0 = blockIdx.y * GT_CH;
+  const int r0 = bx * GT_ROWS, r1 = min(M, r0 + GT_ROWS);
+  double acc[GT_CH];
+#pragma unroll
+  for (int c = 0; c < GT_CH; ++c) acc[c] = 0.0;
+  for (int i = r0 + (int)threadIdx.x; i < r1; i += 256) {
+    const double wi = w[i];
+#pragma unroll
+    for (int c = 0; c < GT_CH; ++c)
+      if (j0 + c < kc) acc[c] += V[(size_t)(j0 + c) * ldv + i] * wi;
+  }
+  __shared__ double red[GT_CH][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < GT_CH; ++c) {
+    double v = acc[c];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) red[c][wave] = v;
+  }
+  __syncthreads();
+  const int c = threadIdx.x;
+  if (c < GT_CH && j0 + c < kc)
+    part[(size_t)(j0 + c) * nbx + bx] = (red[c][0] + red[c][1]) + (red[c][2] + red[c][3]);
+}
+
+__global__ void gemv_t_reduce_kernel(int kc, int nbx, const double *__restrict__ part,
+                                     double *__restrict__ out) {
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= kc) return;
+  double s = 0.0;
+  for (int b = 0; b < nbx; ++b) s += part[(size_t)j * nbx + b];
+  out[j] = s;
+}
+
 }  // namespace
 
 extern "C" {
+
+size_t atlmx_k_dots_ws_bytes(int M, int kc) {
+  const size_t nbx = (size_t)(M > 0 ? (M + GT_ROWS - 1) / GT_ROWS : 1);
+  return nbx * (size_t)(kc > 0 ? kc : 1) * sizeof(double) + 256;
+}
+
+hipError_t atlmx_k_ddots(int M, int kc, const double *V, long ldv, const double *w,
+                         double *out, void *ws, hipStream_t s) {
+  if (kc <= 0) return hipSuccess;
+  if (M < 0 || !ws || (M > 0 && ldv < M)) return hipErrorInvalidValue;
+  const int nbx = M > 0 ? (M + GT_ROWS - 1) / GT_ROWS : 0;
+  double *part = (double *)ws;
+  if (nbx > 0) {
+    hipLaunchKernelGGL(gemv_t_part_kernel, dim3(nbx, (kc + GT_CH - 1) / GT_CH), dim3(256), 0, s,
+                       M, kc, nbx, V, ldv, w, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(gemv_t_reduce_kernel, dim3((kc + 255) / 256), dim3(256), 0, s, kc, nbx,
+                     part, out);
+  return hipGetLastError();
+}
 
 hipError_t atlmx_k_pow2_scale(int n, const double *mx, double *scale, hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -84,6 +84,10 @@ hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const float *A,
  * to prev's moves followed by this panel's. */
 #define ATLMX_TSLU_W 64
 size_t atlmx_k_tslu_ws_ints(int n);
+/* The tournament tree of a panel of m rows: rows per node (256 or 512,
+ * whichever needs fewer levels) and the number of leaves (workgroups). */
+int atlmx_k_tslu_node_rows(int m);
+int atlmx_k_tslu_leaves(int m);
 hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w, int *ws,
                                int *plan, int *perm, float *lu11, float *invl, float *invu,
                                int *info, const int *prev, int *plan2, hipStream_t s);
@@ -161,20 +165,24 @@ hipError_t atlmx_k_sbatch_chol_adjoint(int count, const int *order, const int *n
                                        int *counter, int grid, hipStream_t s);
 /* C[i] = alpha op(A[i]) op(B[i]) + beta C[i] for count instances of sizes
  * m[i] x n[i] x k[i]; tiles are numbered instance by instance in `order`,
- * tile_start[p] = first tile of instance order[p] (count + 1 entries). */
+ * tile_start[p] = first tile of instance order[p] (count + 1 entries).
+ * variant: 0 vector FMAs (256 threads per tile), 1 MFMA with one
+ * wavefront per tile, 2 MFMA with four wavefronts per tile. */
 #define ATLMX_BATCH_TILE 64
 hipError_t atlmx_k_dbatch_gemm(int count, int ta, int tb, const int *order, const int *m,
                                const int *n,
                                const int *k, double alpha, const double *const *A,
                                const int *lda, const double *const *B, const int *ldb,
                                double beta, double *const *C, const int *ldc,
-                               const int *tile_start, int *counter, int grid, hipStream_t s);
+                               const int *tile_start, int *counter, int variant, int grid,
+                               hipStream_t s);
 hipError_t atlmx_k_sbatch_gemm(int count, int ta, int tb, const int *order, const int *m,
                                const int *n,
                                const int *k, float alpha, const float *const *A,
                                const int *lda, const float *const *B, const int *ldb,
                                float beta, float *const *C, const int *ldc,
-                               const int *tile_start, int *counter, int grid, hipStream_t s);
+                               const int *tile_start, int *counter, int variant, int grid,
+                               hipStream_t s);
 
 /* S = (float) (A diag(colscale))^T (colscale may be NULL), perm = 0..n-1 */
 hipError_t atlmx_k_tslu_convert(int n, const double *A, long lda, float *S, long lds, int *perm,

@@ -32,6 +32,11 @@ hipError_t atlmx_k_dcolmax(int M, int N, const double *X, long ldx, double *out,
 /* rowsum[i] = sum_j |A(i,j)|; ws of atlmx_k_resid_ws_bytes(M, 1) bytes */
 hipError_t atlmx_k_drowsum(int M, int N, const double *A, long lda, double *rowsum,
                            void *ws, hipStream_t s);
+/* out[j] = sum_i V(i,j) w(i) for j < kc (V: M x kc), summed in a fixed
+ * order; ws of atlmx_k_dots_ws_bytes(M, kc) bytes. */
+size_t atlmx_k_dots_ws_bytes(int M, int kc);
+hipError_t atlmx_k_ddots(int M, int kc, const double *V, long ldv, const double *w,
+                         double *out, void *ws, hipStream_t s);
 /* X += (double)D */
 hipError_t atlmx_k_axpy_s2d(int M, int N, const float *D, long ldd, double *X, long ldx,
                             hipStream_t s);

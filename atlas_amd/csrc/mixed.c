@@ -295,7 +295,7 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm, i
     rt_on_aux(c, main, 0);
     RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
     tslu_update_beside(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, lp,
-                       (n - k2 + 255) / 256, d_info, main, rout);
+                       atlmx_k_tslu_leaves(n - k2), d_info, main, rout);
     RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[2], 0));   /* head through Q's choice */
     k = k2;
   }
@@ -405,12 +405,13 @@ static int refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const double 
   return 1;
 }
 
-/* <u, v> of two device vectors, via the GEMM kernel (synchronizes) */
+/* <u, v> of two device vectors, split over row blocks and summed in a fixed
+ * order (synchronizes) */
 static double ddot_dev(atlmx_ctx *c, int n, const double *u, const double *v, double *tmp,
-                       const char *rout)
+                       void *ws, const char *rout)
 {
   double h = 0.0;
-  dgemm_dev(c, KX_TRANS, KX_NOTRANS, 1, 1, n, 1.0, u, n, v, n, 0.0, tmp, 1, 0, NULL, rout);
+  RT_CHECK(rout, atlmx_k_ddots(n, 1, u, n, v, tmp, ws, c->stream));
   RT_CHECK(rout, hipMemcpyAsync(&h, tmp, sizeof(double), hipMemcpyDeviceToHost, c->stream));
   rt_sync(c, rout);
   return h;
@@ -434,9 +435,11 @@ static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const d
   const int m = GM_RESTART;
   double *V = (double *)rt_ws(c, WS_GMRES, sizeof(double) * ((size_t)N * (m + 3) + 4 * m + 64),
                               rout);
-  double *T = V + (size_t)N * (m + 1), *Z = T + N, *hd = Z + N, *yd = hd + m + 2;
+  /* hd: the two Gram-Schmidt passes' coefficients and the new vector's
+   * squared norm, [0, m+1), [m+1, 2m+2), 2m+2; yd: the correction's weights */
+  double *T = V + (size_t)N * (m + 1), *Z = T + N, *hd = Z + N, *yd = hd + 2 * m + 4;
   double H[GM_RESTART + 1][GM_RESTART], g[GM_RESTART + 1], cs[GM_RESTART], sn[GM_RESTART];
-  double y[GM_RESTART], h1[GM_RESTART + 1], h2[GM_RESTART + 1];
+  double y[GM_RESTART], hh[2 * GM_RESTART + 3];
   int j, it, k, i, conv = 1;
   RT_CHECK(rout, atlmx_k_fill_i32((int *)Z, 2 * N, 0, c->stream));      /* Z = 0.0 */
   st->backward_error = 0.0;
@@ -462,7 +465,7 @@ static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const d
       if (it == outer_max || stalled) break;
       /* GMRES on M^{-1} A d = M^{-1} r from d = 0 */
       lp_solve(c, f, 1, R, N, V, N, 0, rout);
-      beta = sqrt(ddot_dev(c, N, V, V, hd, rout));
+      beta = sqrt(ddot_dev(c, N, V, V, hd, ws, rout));
       if (!(beta > 0.0) || !(beta < 1e300)) break;
       RT_CHECK(rout, atlmx_k_dscal2d(N, 1, 1.0 / beta, V, N, c->stream));
       memset(g, 0, sizeof(g));
@@ -473,21 +476,21 @@ static int gmres_refine(atlmx_ctx *c, const lp_fact *f, int N, int NRHS, const d
         RT_CHECK(rout, atlmx_k_dresid(N, N, 1, A, lda, vk, N, Z, N, T, N, ws, c->stream));
         lp_solve(c, f, 1, T, N, w, N, 0, rout);
         RT_CHECK(rout, atlmx_k_dscal2d(N, 1, -1.0, w, N, c->stream));
-        /* classical Gram-Schmidt, twice */
-        dgemm_dev(c, KX_TRANS, KX_NOTRANS, k + 1, 1, N, 1.0, V, N, w, N, 0.0, hd, k + 1, 0, NULL,
-                  rout);
-        RT_CHECK(rout, hipMemcpyAsync(h1, hd, sizeof(double) * (k + 1), hipMemcpyDeviceToHost,
+        /* classical Gram-Schmidt, twice: h = V^T w (split dot products),
+         * then w -= V h (the residual GEMV with w as both B and R: every
+         * entry is read and written by one thread, and V's columns 0..k do
+         * not include w); one host synchronization per step */
+        RT_CHECK(rout, atlmx_k_ddots(N, k + 1, V, N, w, hd, ws, c->stream));
+        RT_CHECK(rout, atlmx_k_dresid(N, k + 1, 1, V, N, hd, k + 1, w, N, w, N, ws, c->stream));
+        RT_CHECK(rout, atlmx_k_ddots(N, k + 1, V, N, w, hd + (m + 1), ws, c->stream));
+        RT_CHECK(rout, atlmx_k_dresid(N, k + 1, 1, V, N, hd + (m + 1), k + 1, w, N, w, N, ws,
                                       c->stream));
-        dgemm_dev(c, KX_NOTRANS, KX_NOTRANS, N, 1, k + 1, -1.0, V, N, hd, k + 1, 1.0, w, N, 0,
-                  NULL, rout);
-        dgemm_dev(c, KX_TRANS, KX_NOTRANS, k + 1, 1, N, 1.0, V, N, w, N, 0.0, hd, k + 1, 0, NULL,
-                  rout);
-        RT_CHECK(rout, hipMemcpyAsync(h2, hd, sizeof(double) * (k + 1), hipMemcpyDeviceToHost,
-                                      c->stream));
-        dgemm_dev(c, KX_NOTRANS, KX_NOTRANS, N, 1, k + 1, -1.0, V, N, hd, k + 1, 1.0, w, N, 0,
-                  NULL, rout);
-        hn = sqrt(ddot_dev(c, N, w, w, hd + m, rout));      /* synchronizes: h1, h2 ready */
-        for (i = 0; i <= k; ++i) H[i][k] = h1[i] + h2[i];
+        RT_CHECK(rout, atlmx_k_ddots(N, 1, w, N, w, hd + 2 * (m + 1), ws, c->stream));
+        RT_CHECK(rout, hipMemcpyAsync(hh, hd, sizeof(double) * (2 * (m + 1) + 1),
+                                      hipMemcpyDeviceToHost, c->stream));
+        rt_sync(c, rout);
+        hn = sqrt(hh[2 * (m + 1)]);
+        for (i = 0; i <= k; ++i) H[i][k] = hh[i] + hh[m + 1 + i];
         H[k + 1][k] = hn;
         for (i = 0; i < k; ++i) {                           /* previous rotations */
           t = cs[i] * H[i][k] + sn[i] * H[i + 1][k];

@@ -16,8 +16,55 @@
 #include <string.h>
 #include "rt.h"
 #include "../../include/cblas.h"
+#include "../../include/atlmx.h"
 
 static __thread atlmx_ctx *tls_ctx = NULL;
+
+/* Every context and profiler buffer ever created, so that process exit can
+ * release them while the HIP runtime is still up (an atexit handler
+ * registered after the runtime's own initialisation runs before its
+ * teardown): streams and events destroyed, workspaces freed, after one
+ * device synchronisation. */
+#define RT_REG_MAX 1024
+static pthread_mutex_t reg_mu = PTHREAD_MUTEX_INITIALIZER;
+static atlmx_ctx *reg_ctx[RT_REG_MAX];
+static int reg_nctx = 0;
+static void *reg_prof[RT_REG_MAX];
+static int reg_nprof = 0;
+static int reg_hooked = 0;
+static void rt_release_prof(void *buf);
+
+static void rt_teardown(void)
+{
+  int i, b, s;
+  pthread_mutex_lock(&reg_mu);
+  if (reg_nctx > 0 || reg_nprof > 0) (void)hipDeviceSynchronize();
+  for (i = 0; i < reg_nctx; ++i) {
+    atlmx_ctx *c = reg_ctx[i];
+    for (b = 0; b < 2; ++b)
+      for (s = 0; s < WS_NSLOTS; ++s)
+        if (c->ws[b][s]) { (void)hipFree(c->ws[b][s]); c->ws[b][s] = NULL; c->ws_bytes[b][s] = 0; }
+    if (c->aux) {
+      for (s = 0; s < 3; ++s)
+        if (c->ev[s]) { (void)hipEventDestroy(c->ev[s]); c->ev[s] = NULL; }
+      (void)hipStreamDestroy(c->aux);
+      c->aux = NULL;
+    }
+    c->panel_buf = NULL;
+  }
+  for (i = 0; i < reg_nprof; ++i) rt_release_prof(reg_prof[i]);
+  reg_nctx = reg_nprof = 0;
+  pthread_mutex_unlock(&reg_mu);
+}
+
+static void rt_register(atlmx_ctx *c, void *prof)
+{
+  pthread_mutex_lock(&reg_mu);
+  if (!reg_hooked) { atexit(rt_teardown); reg_hooked = 1; }
+  if (c && reg_nctx < RT_REG_MAX) reg_ctx[reg_nctx++] = c;
+  if (prof && reg_nprof < RT_REG_MAX) reg_prof[reg_nprof++] = prof;
+  pthread_mutex_unlock(&reg_mu);
+}
 
 void rt_fail(const char *rout, const char *what, hipError_t e)
 {
@@ -40,6 +87,7 @@ atlmx_ctx *rt_ctx(void)
   c->own = 0;
   c->stream = 0;
   tls_ctx = c;
+  rt_register(c, NULL);
   return c;
 }
 
@@ -133,6 +181,16 @@ static __thread prof_rec *prof_buf = NULL;
 static __thread int prof_n = 0;
 static __thread int prof_enabled = 0;   /* bit mask of recorded classes */
 
+static void rt_release_prof(void *buf)
+{
+  prof_rec *r = (prof_rec *)buf;
+  int i;
+  for (i = 0; i < PROF_MAX; ++i) {
+    if (r[i].a) { (void)hipEventDestroy(r[i].a); r[i].a = NULL; }
+    if (r[i].b) { (void)hipEventDestroy(r[i].b); r[i].b = NULL; }
+  }
+}
+
 int rt_prof_on(void) { return prof_enabled; }
 
 int rt_prof_begin(atlmx_ctx *c, int cls, double work)
@@ -198,6 +256,7 @@ void atlmx_prof_enable(int mask)
   if (mask && !prof_buf) {
     prof_buf = (prof_rec *)calloc(PROF_MAX, sizeof(prof_rec));
     if (!prof_buf) return;
+    rt_register(NULL, prof_buf);
   }
   prof_enabled = mask;
   prof_n = 0;

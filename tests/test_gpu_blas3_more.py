@@ -45,7 +45,7 @@ def test_trmm_alpha_zero_rowmajor_and_gaps(dt):
     B = _f(np.full((ld, N), 7.0, dt))
     B[:M] = _gen((M, N), dt)
     B[:M, 3] = np.nan                          # alpha == 0: B = 0, not 0 * NaN
-    X.lib().__getattr__("cblas_%strmm" % ("d" if dt == np.float64 else "s"))(
+    getattr(X.lib(), "cblas_%strmm" % ("d" if dt == np.float64 else "s"))(
         X.ColMajor, X.Left, X.Lower, X.NoTrans, X.NonUnit, M, N, 0.0, A.ctypes.data, ld,
         B.ctypes.data, ld)
     assert np.all(B[:M] == 0) and np.all(B[M:] == 7.0)
@@ -53,7 +53,7 @@ def test_trmm_alpha_zero_rowmajor_and_gaps(dt):
     Ar = np.ascontiguousarray(_gen((N, N), dt))
     Br = np.ascontiguousarray(_gen((M, N), dt))
     D = Br.copy()
-    X.lib().__getattr__("cblas_%strmm" % ("d" if dt == np.float64 else "s"))(
+    getattr(X.lib(), "cblas_%strmm" % ("d" if dt == np.float64 else "s"))(
         X.RowMajor, X.Right, X.Upper, X.Trans, X.Unit, M, N, 2.0, Ar.ctypes.data, N,
         D.ctypes.data, N)
     U = np.triu(Ar.astype(np.float64), 1) + np.eye(N)

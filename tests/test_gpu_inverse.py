@@ -27,6 +27,11 @@ def test_trtri_trtritst(dt, uplo, diag, N, lda):
         ref &= ~np.eye(N, dtype=bool)
     assert np.array_equal(A[:N, :N][~ref], A0[:N, :N][~ref])     # padding and unit diag kept
     assert np.all(A[N:, :] == A0[N:, :])
+    if N == 1 and diag == X.Unit:
+        # trtritst's norm leaves a unit diagonal out, so for N = 1 it is 0 and
+        # its residual 0/0 (bin/trtritst.c:338,394); the 1 x 1 unit triangle
+        # is its own inverse and nothing in it may change (checked above)
+        return
     r = O.trtri_resid(uplo, diag, A[:N, :N], A0[:N, :N])
     assert np.isfinite(r) and r < 100, r
 
