@@ -20,9 +20,11 @@
 #define K_IAMAX atlmx_k_idamax
 #define K_TRSV atlmx_k_dtrsv
 #define K_TRICOPY atlmx_k_dtricopy
+#define K_UPDATE atlmx_k_dupdate
 #include "blas3_tmpl.h"
 #undef T
 #undef PFX
+#undef K_UPDATE
 #undef K_GEMM
 #undef K_TRI_INV
 #undef K_SCAL

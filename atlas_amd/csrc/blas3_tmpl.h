@@ -38,6 +38,28 @@ void GEMM_DEV(atlmx_ctx *c, int ta, int tb, int M, int N, int K, T alpha, const 
   rt_prof_end(c, pid);
 }
 
+/* C += alpha * A * op(B) with A not transposed (beta = 1): the short-K
+ * trailing updates of the factorizations.  The FP64 instantiation has a
+ * kernel for this form that loads C behind the MFMAs (K_UPDATE); the other
+ * runs the GEMM.  tri as GEMM_DEV. */
+void NM(PFX, update_dev)(atlmx_ctx *c, int tb, int M, int N, int K, T alpha, const T *A,
+                         long lda, const T *B, long ldb, T *C, long ldc, int tri,
+                         const int *abort_flag, const char *rout)
+{
+#ifdef K_UPDATE
+  int pid;
+  if (M <= 0 || N <= 0 || K <= 0 || alpha == (T)0) return;
+  pid = rt_prof_begin2(c, PROF_GEMM, 2.0 * (double)M * N * K,
+                      ((double)M * K + (double)K * N + 2.0 * M * N) * sizeof(T));
+  RT_CHECK(rout, K_UPDATE(tb, M, N, K, alpha, A, lda, B, ldb, C, ldc, tri, abort_flag,
+                          c->stream));
+  rt_prof_end(c, pid);
+#else
+  GEMM_DEV(c, KX_NOTRANS, tb, M, N, K, alpha, A, lda, B, ldb, (T)1, C, ldc, tri, abort_flag,
+           rout);
+#endif
+}
+
 /* ------------------------------------------------------------------------
  * TRSM: op(A) X = alpha B (Left) or X op(A) = alpha B (Right), in place.
  * Replaces src/blas/ulevel3/ATL_trsm.c:13-228 (recursive halving with GEMM

@@ -12,6 +12,9 @@
 void P##gemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, T alpha,          \
                  const T *A, long lda, const T *B, long ldb, T beta, T *C, long ldc,  \
                  int tri, const int *abort_flag, const char *rout);                   \
+void P##update_dev(atlmx_ctx *c, int tb, int M, int N, int K, T alpha, const T *A,     \
+                   long lda, const T *B, long ldb, T *C, long ldc, int tri,           \
+                   const int *abort_flag, const char *rout);                          \
 void P##trsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,    \
                  T alpha, const T *A, long lda, T *B, long ldb,                       \
                  const int *abort_flag, const char *rout);                            \

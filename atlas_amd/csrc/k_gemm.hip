@@ -366,6 +366,110 @@ __global__ __launch_bounds__(NT, 1) void update_kernel(GemmArgs<T> g) {
     }
 }
 
+/* The FP64 short-K update, C += alpha op(A) op(B) with beta = 1 (the LU
+ * and Cholesky trailing updates, K <= 128 or so), op(B) either form.  The
+ * FP64 MFMA's accumulator holds rows (l>>4) + 4r of a column, so the
+ * product is formed transposed as in gemm_kernel (a lane's elements run down
+ * one row of C, and the 16 lanes of a row group read 128 contiguous bytes of
+ * a column); the C tile (64 doubles a thread, in registers) is loaded right
+ * after the first K slice's operands and added after the last MFMA, so its
+ * HBM traffic overlaps the MFMAs.  That takes about 300 registers a lane,
+ * so one workgroup runs per CU.  tri: only the referenced triangle of C is
+ * read and written (the SYRK form). */
+template <int TB>
+__global__ __launch_bounds__(NT, 1) void update_d_kernel(GemmArgs<double> g) {
+  typedef Mfma<double> MF;
+  typedef MF::acc_t acc_t;
+  constexpr int BK = MF::BK;
+  constexpr bool B_KCONT = (TB == KX_NOTRANS);
+  typedef SliceLoader<double, false, BM, BK> LA;
+  typedef SliceLoader<double, B_KCONT, BN, BK> LB;
+  if (g.abort_flag && *g.abort_flag) return;
+  const int bid = blockIdx.x;
+  const int tile = (bid & 7) * g.per_xcd + (bid >> 3);
+  const int ntiles = g.tiles_m * g.tiles_n;
+  if (tile >= ntiles) return;
+  const int per_group = GROUP_M * g.tiles_n;
+  const int group = tile / per_group;
+  const int first_m = group * GROUP_M;
+  const int gsz = min(g.tiles_m - first_m, GROUP_M);
+  const int tm = first_m + (tile % per_group) % gsz;
+  const int tn = (tile % per_group) / gsz;
+  const int m0 = tm * BM, n0 = tn * BN;
+  if (g.tri == KX_LOWER && m0 + BM - 1 < n0) return;
+  if (g.tri == KX_UPPER && n0 + BN - 1 < m0) return;
+
+  __shared__ __attribute__((aligned(16))) double As[2][LA::SIZE];
+  __shared__ __attribute__((aligned(16))) double Bs[2][LB::SIZE];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 64;
+  const int kq = (lane >> 4) * 4, lr = lane & 15;
+  const int nk = (g.K + BK - 1) / BK;
+
+  LA la;
+  LB lb;
+  la.load(g.A, g.lda, m0, 0, g.M, g.K, g.vec_a, tid);
+  lb.load(g.B, g.ldb, n0, 0, g.N, g.K, g.vec_b, tid);
+  /* the C tile, element r of block (i, j) = C(row, col) as the accumulator
+   * holds it: row m0 + wm + 16 i + lr, column n0 + wn + 16 j + drow(r) */
+  acc_t cv[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lr, col = n0 + wn + j * 16 + MF::drow(lane, r);
+        const bool in = row < g.M && col < g.N &&
+                        !(g.tri == KX_LOWER && row < col) && !(g.tri == KX_UPPER && row > col);
+        cv[i][j][r] = in ? g.C[(size_t)col * g.ldc + row] : 0.0;
+      }
+  la.store(As[0], tid);
+  lb.store(Bs[0], tid);
+  __syncthreads();
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  for (int kt = 0; kt < nk; ++kt) {
+    const int cur = kt & 1;
+    if (kt + 1 < nk) {
+      la.load(g.A, g.lda, m0, (kt + 1) * BK, g.M, g.K, g.vec_a, tid);
+      lb.load(g.B, g.ldb, n0, (kt + 1) * BK, g.N, g.K, g.vec_b, tid);
+    }
+#pragma unroll
+    for (int s4 = 0; s4 < 4; ++s4) {
+      double a[4], b[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        a[i] = As[cur][LA::at(wm + i * 16 + lr, kq + s4)];
+        b[i] = Bs[cur][LB::at(wn + i * 16 + lr, kq + s4)];
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j], a[i], acc[i][j]);
+    }
+    if (kt + 1 < nk) {
+      la.store(As[cur ^ 1], tid);
+      lb.store(Bs[cur ^ 1], tid);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int row = m0 + wm + i * 16 + lr, col = n0 + wn + j * 16 + MF::drow(lane, r);
+        const bool in = row < g.M && col < g.N &&
+                        !(g.tri == KX_LOWER && row < col) && !(g.tri == KX_UPPER && row > col);
+        if (in) g.C[(size_t)col * g.ldc + row] = g.alpha * acc[i][j][r] + cv[i][j][r];
+      }
+}
+
 /* The same update with its operands rounded to 16 bits: FP32 A and B are
  * converted (round to nearest even) as they are staged into LDS, the
  * products run on v_mfma_f32_16x16x16_{bf16,f16} and accumulate in FP32
@@ -566,6 +670,35 @@ extern "C" hipError_t atlmx_k_supdate(int M, int N, int K, float alpha, const fl
   g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % 4 == 0);
   hipLaunchKernelGGL((update_kernel<float, 32>), dim3((unsigned)(8 * g.per_xcd)), dim3(NT), 0, s,
                      g);
+  return hipGetLastError();
+}
+
+/* C += alpha A op(B) in FP64 with the C tile prefetched (update_d_kernel);
+ * tri as atlmx_k_dgemm. */
+extern "C" hipError_t atlmx_k_dupdate(int tb, int M, int N, int K, double alpha,
+                                      const double *A, long lda, const double *B, long ldb,
+                                      double *C, long ldc, int tri, const int *abort_flag,
+                                      hipStream_t s) {
+  if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+  if ((tb != KX_NOTRANS && tb != KX_TRANS) || lda < M || ldb < (tb == KX_NOTRANS ? K : N) ||
+      ldc < M || !A || !B || !C)
+    return hipErrorInvalidValue;
+  GemmArgs<double> g;
+  g.M = M; g.N = N; g.K = K; g.alpha = alpha; g.beta = 1.0;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.tri = tri; g.abort_flag = abort_flag;
+  g.tiles_m = (M + BM - 1) / BM;
+  g.tiles_n = (N + BN - 1) / BN;
+  const long ntiles = (long)g.tiles_m * g.tiles_n;
+  if (ntiles > (1L << 30)) return hipErrorInvalidValue;
+  g.per_xcd = (int)((ntiles + 7) / 8);
+  g.vec_a = (((uintptr_t)A % 16) == 0) && (lda % 2 == 0);
+  g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % 2 == 0);
+  const dim3 grid((unsigned)(8 * g.per_xcd)), block(NT);
+  if (tb == KX_NOTRANS)
+    hipLaunchKernelGGL(update_d_kernel<KX_NOTRANS>, grid, block, 0, s, g);
+  else
+    hipLaunchKernelGGL(update_d_kernel<KX_TRANS>, grid, block, 0, s, g);
   return hipGetLastError();
 }
 

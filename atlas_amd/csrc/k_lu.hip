@@ -17,15 +17,19 @@
  *
  * Cross-workgroup exchange, per column: every workgroup reduces its rows to
  * one candidate (wavefront shuffles, then LDS across the 4 wavefronts) and
- * publishes it -- the candidate row into a parity slot, then |value| and
- * row index into a per-column header slot that starts out holding a
- * sentinel.  Readers poll all G headers at once (one lane per workgroup)
- * until none holds the sentinel, reduce them in one wavefront and fetch the
- * winning row.  All exchanged data moves with device-coherent (sc1)
- * accesses, so no L2 writeback/invalidate fence is needed; the measured
- * cost is ~1.3 us per column instead of ~4-10 us for a fenced barrier.
- * The header slots of the next launch are reset at the start of this one
- * (two alternating slot sets), so no extra fill launch is needed.
+ * publishes it in its record for the column's parity: the row index, |value|
+ * and the candidate row, every 32 bits of payload in a 64-bit word whose
+ * upper half is a tag unique to (launch, column).  A word is valid when its
+ * tag is the current one, so no store has to wait for an earlier one to
+ * land and no slot is ever reset.  Readers poll the G records' headers at
+ * once (one lane per workgroup), reduce them in one wavefront, and then
+ * fetch the one row they need: the winning candidate row, or, in the
+ * winner's own workgroup, the row it swaps with.  Two record sets
+ * (column parity) suffice: a workgroup can publish column c + 2 only after
+ * reading every record of column c + 1, which each writer publishes only
+ * after reading all of column c.  All exchanged words move with
+ * device-coherent (sc1) accesses, so no L2 writeback/invalidate fence is
+ * needed.
  */
 #include <hip/hip_runtime.h>
 #include <float.h>
@@ -38,6 +42,7 @@ namespace {
 constexpr int NT = 256;
 constexpr int MAXG = 256;   /* cooperative workgroups at most (1 per CU) */
 constexpr int MAXW = ATLMX_PANEL_MAXW;
+constexpr int TALL_PARTS = 1024;   /* partial arg-maxes of the tall path */
 
 template <typename T> struct Lim;
 template <> struct Lim<double> { __device__ static double safmin() { return DBL_MIN; } };
@@ -48,51 +53,53 @@ __device__ inline void amax_combine(T &v, int &i, T v2, int i2) {
   if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
 }
 
-/* coherent (sc1) 64-bit accesses for the exchange buffers */
-__device__ inline void st_coh(double *p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline double ld_coh(double *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_coh(float *p, float v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline float ld_coh(float *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline void st_coh(long long *p, long long v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ inline long long ld_coh(long long *p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+/* Record of one workgroup for one column parity: 64-bit words tag << 32 |
+ * payload, [0] the candidate's row index, [1], [2] the low and high halves
+ * of its |value| as a double (-1 for NaN, -2 when the workgroup has no row
+ * left), [RH + 2j], [RH + 2j + 1] the halves of entry j of the candidate
+ * row.  One more record per parity holds the current row c (published by
+ * workgroup 0, which holds rows 0..R-1 of the panel). */
+constexpr int RH = 4;
+constexpr int RECW = RH + 2 * MAXW;
+typedef unsigned long long u64;
 
-/* Header word: |value| as the bit pattern of a double (both precisions
- * publish it widened to double), and the row index.  SENT is a NaN payload
- * no published value can have (NaN inputs are published as -1). */
-constexpr long long SENT = (long long)0x7ff8dead0badf00dLL;
-
-/* scratch layout (bytes): [0, 64*MAXG*MAXW*... ) see panel_scratch_bytes */
 struct PanelSlots {
-  long long *hval;   /* [2 sets][MAXW cols][MAXG]  |value| bits     */
-  long long *hidx;   /* [2 sets][MAXW cols][MAXG]  row index        */
-  double *crow;      /* [2 parity][MAXG][MAXW]     candidate rows   */
-  double *drow;      /* [2 parity][MAXW]           diagonal row     */
+  u64 *rec;          /* [2 parity][MAXG + 1][RECW]; record MAXG: the current row */
+  double *part_v;    /* tall path: per-workgroup arg-max partials */
+  int *part_i;
 };
 
 __host__ __device__ inline size_t slots_bytes() {
-  return (size_t)2 * MAXW * MAXG * 8 * 2 + (size_t)2 * MAXG * MAXW * 8 + (size_t)2 * MAXW * 8;
+  return (size_t)2 * (MAXG + 1) * RECW * 8 + (size_t)TALL_PARTS * (8 + 4);
 }
 
 __device__ inline PanelSlots slots_view(void *base) {
   PanelSlots s;
   char *p = (char *)base;
-  s.hval = (long long *)p;  p += (size_t)2 * MAXW * MAXG * 8;
-  s.hidx = (long long *)p;  p += (size_t)2 * MAXW * MAXG * 8;
-  s.crow = (double *)p;     p += (size_t)2 * MAXG * MAXW * 8;
-  s.drow = (double *)p;
+  s.rec = (u64 *)p;        p += (size_t)2 * (MAXG + 1) * RECW * 8;
+  s.part_v = (double *)p;  p += (size_t)TALL_PARTS * 8;
+  s.part_i = (int *)p;
   return s;
+}
+
+__device__ inline void put_u32(u64 *p, unsigned tag, unsigned v) {
+  __hip_atomic_store(p, ((u64)tag << 32) | v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void put_f64(u64 *p, unsigned tag, double v) {
+  const u64 b = (u64)__double_as_longlong(v);
+  put_u32(p, tag, (unsigned)b);
+  put_u32(p + 1, tag, (unsigned)(b >> 32));
+}
+/* the 32-bit payload of word p once it carries tag */
+__device__ inline unsigned take_u32(u64 *p, unsigned tag) {
+  u64 w;
+  while (((w = __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) >> 32) != tag)
+    __builtin_amdgcn_s_sleep(1);
+  return (unsigned)w;
+}
+__device__ inline double take_f64(u64 *p, unsigned tag) {
+  const unsigned lo = take_u32(p, tag), hi = take_u32(p + 1, tag);
+  return __longlong_as_double((long long)(((u64)hi << 32) | lo));
 }
 
 /* The panel rows of one workgroup live in LDS, row-major with a padded row
@@ -101,30 +108,19 @@ __device__ inline PanelSlots slots_view(void *base) {
 template <typename T, int W, int R>
 __global__ __launch_bounds__(NT) void getf2_panel_lds_kernel(int m, int w, T *P, long ld,
                                                          int row0, int *ipiv, int *info,
-                                                         void *scratch, int set) {
+                                                         void *scratch, int seq) {
   constexpr int RPT = R / NT;
   constexpr int LDW = W + 1;   /* odd stride: lanes hit distinct LDS banks */
   extern __shared__ __attribute__((aligned(16))) unsigned char smem[];
   T *rows = (T *)smem;                         /* [R][LDW] */
   __shared__ double s_wv[NT / 64];
   __shared__ int s_wi[NT / 64];
-  __shared__ T s_prow[W], s_drow[W];
+  __shared__ T s_prow[W];
   __shared__ int s_piv;
   const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
   const int rbase = g * R;                     /* first panel row of this workgroup */
   PanelSlots S = slots_view(scratch);
-  long long *hv = S.hval + (size_t)set * MAXW * MAXG;
-  long long *hi = S.hidx + (size_t)set * MAXW * MAXG;
-
-  /* reset the header slots of the other set (the next launch's), for every
-   * workgroup slot g, g+G, ..., whatever the next launch's G is */
-  for (int gg = g; gg < MAXG; gg += G)
-    for (int c = tid; c < MAXW; c += NT) {
-      const size_t o = (size_t)(set ^ 1) * MAXW * MAXG + (size_t)c * MAXG + gg;
-      st_coh(S.hval + o, SENT);
-      st_coh(S.hidx + o, SENT);
-    }
 
   for (int k = 0; k < w; ++k)
 #pragma unroll
@@ -159,48 +155,57 @@ __global__ __launch_bounds__(NT) void getf2_panel_lds_kernel(int m, int w, T *P,
 #pragma unroll
       for (int k = 1; k < NT / 64; ++k) atlmx::argmax_combine(wv, wi, bz, s_wv[k], s_wi[k], 0);
       if (wv < 0.0) wi = 0x7fffffff;    /* only NaN or no rows: nothing to offer */
-      /* ---- publish: one coherent store per lane, then the header ---- */
+      /* ---- publish this workgroup's record (and workgroup 0 the current
+       * row c), each word tagged; no store waits for another ---- */
+      const unsigned tag = (unsigned)seq * MAXW + (unsigned)c + 1u;
+      u64 *recs = S.rec + (size_t)par * (MAXG + 1) * RECW;
+      u64 *mine = recs + (size_t)g * RECW, *cur = recs + (size_t)MAXG * RECW;
       if (lane < w) {
-        if (wi != 0x7fffffff)
-          st_coh(&S.crow[((size_t)par * MAXG + g) * MAXW + lane],
-                 (double)rows[(wi - rbase) * LDW + lane]);
-        if (g == 0) st_coh(&S.drow[(size_t)par * MAXW + lane], (double)rows[c * LDW + lane]);
+        if (wi != 0x7fffffff) put_f64(mine + RH + 2 * lane, tag, (double)rows[(wi - rbase) * LDW + lane]);
+        if (g == 0) put_f64(cur + RH + 2 * lane, tag, (double)rows[c * LDW + lane]);
       }
-      /* the row must reach the coherence point before the header can:
-       * wait for the stores' acknowledgements (vmcnt = 0) */
-      __builtin_amdgcn_s_waitcnt(0);
       if (lane == 0) {
-        st_coh(&hv[(size_t)c * MAXG + g], __double_as_longlong((double)wv));
-        st_coh(&hi[(size_t)c * MAXG + g], (long long)wi);
+        put_u32(mine, tag, (unsigned)wi);
+        put_f64(mine + 1, tag, wv);
       }
-      /* ---- poll the G headers, reduce, fetch the pivot row ---- */
+      /* ---- read the G headers (one lane per workgroup), reduce ---- */
       double v = -2.0;
       int i = 0x7fffffff, gi = 0;
       for (int k0 = 0; k0 < G; k0 += 64) {
         const int k = k0 + lane;
         if (k < G) {
-          long long hx, hy;
-          do {
-            hx = ld_coh(&hv[(size_t)c * MAXG + k]);
-            hy = ld_coh(&hi[(size_t)c * MAXG + k]);
-          } while (hx == SENT || hy == SENT);
-          atlmx::argmax_combine(v, i, gi, __longlong_as_double(hx), (int)hy, k);
+          u64 *rk = recs + (size_t)k * RECW;
+          const int ik = (int)take_u32(rk, tag);
+          atlmx::argmax_combine(v, i, gi, take_f64(rk + 1, tag), ik, k);
         }
       }
       atlmx::wave_argmax(v, i, gi);
       /* all |a| zero or NaN: iamax answers the first row, i.e. c */
       const int p = (v > 0.0) ? i : c;
       if (lane < w) {
-        const T dr = (T)ld_coh(&S.drow[(size_t)par * MAXW + lane]);
-        const T pr =
-            (p == c) ? dr : (T)ld_coh(&S.crow[((size_t)par * MAXG + gi) * MAXW + lane]);
-        s_drow[lane] = dr;
-        s_prow[lane] = pr;
-        /* swap the whole panel rows c and p where this workgroup holds them,
-         * one lane per column (p != c implies a nonzero pivot) */
+        /* the pivot row: the winner's candidate (its own workgroup has it in
+         * LDS); the current row c: needed only where row p lives, i.e. in
+         * the winner's workgroup, which also has it in LDS when g == 0 */
+        const bool own = (p != c) && gi == g;
+        const T pr = (p == c) ? T(0)
+                              : (own ? rows[(p - rbase) * LDW + lane]
+                                     : (T)take_f64(recs + (size_t)gi * RECW + RH + 2 * lane, tag));
         if (p != c) {
-          if (p >= rbase && p < rbase + R) rows[(p - rbase) * LDW + lane] = dr;
+          s_prow[lane] = pr;
+          /* swap the whole panel rows c and p where this workgroup holds
+           * them, one lane per column (p != c implies a nonzero pivot) */
+          if (p >= rbase && p < rbase + R) {
+            const T dr = (c >= rbase && c < rbase + R)
+                             ? rows[(c - rbase) * LDW + lane]
+                             : (T)take_f64(cur + RH + 2 * lane, tag);
+            rows[(p - rbase) * LDW + lane] = dr;
+          }
           if (c >= rbase && c < rbase + R) rows[(c - rbase) * LDW + lane] = pr;
+        } else {
+          /* no interchange: the pivot row is row c itself, held by
+           * workgroup 0, published as the current row */
+          s_prow[lane] = (c >= rbase && c < rbase + R) ? rows[(c - rbase) * LDW + lane]
+                                                       : (T)take_f64(cur + RH + 2 * lane, tag);
         }
       }
       if (lane == 0) {
@@ -241,10 +246,10 @@ __global__ __launch_bounds__(NT) void getf2_panel_lds_kernel(int m, int w, T *P,
     }
 }
 
-__global__ void slots_init_kernel(long long *h, size_t n) {
+__global__ void slots_init_kernel(u64 *h, size_t n) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
-    h[i] = SENT;
+    h[i] = 0ull;                      /* tag 0: never a valid word */
 }
 
 /* ------------------------------------------------------------------------
@@ -253,18 +258,11 @@ __global__ void slots_init_kernel(long long *h, size_t n) {
  * panel in HBM -- a partial arg-max per workgroup, a one-wavefront kernel
  * that reduces the partials (same tie and NaN rules), swaps the two panel
  * rows and records the pivot, and the scaling + rank-1 update of the rows
- * below.  The partials live in the candidate-row area of the scratch, which
- * holds nothing between launches; the other header set is reset as the
- * cooperative kernel would, so the two paths can alternate.
+ * below.  The partials have their own area of the scratch; the tagged
+ * records of the cooperative kernel are not touched, so the two paths can
+ * alternate.
  * ---------------------------------------------------------------------- */
-constexpr int TALL_G = 1024;
-
-__global__ void tall_reset_kernel(long long *hv, long long *hi, int n) {
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    st_coh(hv + i, SENT);
-    st_coh(hi + i, SENT);
-  }
-}
+constexpr int TALL_G = TALL_PARTS;
 
 template <typename T>
 __global__ __launch_bounds__(NT) void tall_amax_kernel(int m, int c, const T *P, long ld,
@@ -332,19 +330,10 @@ __global__ __launch_bounds__(NT) void tall_update_kernel(int m, int w, int c, T 
 
 template <typename T>
 hipError_t launch_tall(int m, int w, T *P, long ld, int row0, int *ipiv, int *info,
-                       void *scratch, int set, hipStream_t s) {
-  PanelSlots S;
-  {
-    char *b = (char *)scratch;
-    S.hval = (long long *)b;  b += (size_t)2 * MAXW * MAXG * 8;
-    S.hidx = (long long *)b;  b += (size_t)2 * MAXW * MAXG * 8;
-    S.crow = (double *)b;
-  }
-  double *pv = S.crow;                       /* TALL_G doubles, then TALL_G ints */
-  int *pi = (int *)(S.crow + TALL_G);
-  const size_t o = (size_t)(set ^ 1) * MAXW * MAXG;
-  hipLaunchKernelGGL(tall_reset_kernel, dim3(64), dim3(256), 0, s, S.hval + o, S.hidx + o,
-                     MAXW * MAXG);
+                       void *scratch, hipStream_t s) {
+  char *b = (char *)scratch + (size_t)2 * (MAXG + 1) * RECW * 8;   /* past the records */
+  double *pv = (double *)b;                  /* TALL_G doubles, then TALL_G ints */
+  int *pi = (int *)(pv + TALL_G);
   for (int c = 0; c < w; ++c) {
     long rows = (long)m - c;
     int G = (int)((rows + NT - 1) / NT);
@@ -374,11 +363,11 @@ hipError_t launch_wr(int G, void **args, hipStream_t s) {
 
 template <typename T>
 hipError_t launch_panel(int m, int w, T *P, long ld, int row0, int *ipiv, int *info,
-                        void *scratch, int set, hipStream_t s) {
+                        void *scratch, int seq, hipStream_t s) {
   if (m <= 0 || w <= 0) return hipSuccess;
   if (w > MAXW || ld < m || !P || !ipiv || !info || !scratch) return hipErrorInvalidValue;
-  if (w > m || (set != 0 && set != 1)) return hipErrorInvalidValue;
-  void *args[] = {&m, &w, &P, &ld, &row0, &ipiv, &info, &scratch, &set};
+  if (w > m || seq < 0 || seq >= ATLMX_PANEL_SEQ_MAX) return hipErrorInvalidValue;
+  void *args[] = {&m, &w, &P, &ld, &row0, &ipiv, &info, &scratch, &seq};
   /* rows per workgroup: 256, or 512 for taller panels (then w <= 32 in FP64
    * so the LDS tile stays within 160 KB) */
   if ((long)m <= (long)NT * MAXG) {
@@ -386,7 +375,7 @@ hipError_t launch_panel(int m, int w, T *P, long ld, int row0, int *ipiv, int *i
     return launch_wr<T, MAXW, NT>(G, args, s);
   }
   if ((long)m > (long)2 * NT * MAXG) return launch_tall<T>(m, w, P, ld, row0, ipiv, info,
-                                                          scratch, set, s);
+                                                          scratch, s);
   if (w > MAXW / 2) return hipErrorInvalidValue;
   const int G = (m + 2 * NT - 1) / (2 * NT);
   return launch_wr<T, MAXW / 2, 2 * NT>(G, args, s);
@@ -551,21 +540,21 @@ extern "C" size_t atlmx_k_panel_scratch_bytes(int m, int w) {
 }
 
 extern "C" hipError_t atlmx_k_panel_scratch_init(void *scratch, hipStream_t s) {
-  const size_t n = (size_t)2 * 2 * MAXW * MAXG;    /* both header arrays, both sets */
-  hipLaunchKernelGGL(slots_init_kernel, dim3(256), dim3(256), 0, s, (long long *)scratch, n);
+  const size_t n = (size_t)2 * (MAXG + 1) * RECW;  /* the records of both parities */
+  hipLaunchKernelGGL(slots_init_kernel, dim3(256), dim3(256), 0, s, (u64 *)scratch, n);
   return hipGetLastError();
 }
 
 extern "C" hipError_t atlmx_k_dgetf2_panel(int m, int w, double *P, long ld, int row0,
-                                           int *ipiv, int *info, void *scratch, int set,
+                                           int *ipiv, int *info, void *scratch, int seq,
                                            hipStream_t s) {
-  return launch_panel<double>(m, w, P, ld, row0, ipiv, info, scratch, set, s);
+  return launch_panel<double>(m, w, P, ld, row0, ipiv, info, scratch, seq, s);
 }
 
 extern "C" hipError_t atlmx_k_sgetf2_panel(int m, int w, float *P, long ld, int row0,
-                                           int *ipiv, int *info, void *scratch, int set,
+                                           int *ipiv, int *info, void *scratch, int seq,
                                            hipStream_t s) {
-  return launch_panel<float>(m, w, P, ld, row0, ipiv, info, scratch, set, s);
+  return launch_panel<float>(m, w, P, ld, row0, ipiv, info, scratch, seq, s);
 }
 
 extern "C" hipError_t atlmx_k_laswp_plan(int k1, int k2, int rows_end, const int *ipiv,

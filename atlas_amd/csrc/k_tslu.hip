@@ -33,14 +33,9 @@
 namespace {
 
 constexpr int TW = ATLMX_TSLU_W;   /* 64 */
-/* Rows per tree node = threads per workgroup (TR), wavefronts NWV = TR / 64,
- * children per node FAN = TR / TW: 256 (4 children) or 512 (8 children).
- * A panel of m rows uses the node size that needs fewer tree levels (256 on
- * a tie): each level is TW dependent elimination steps, the critical path of
- * the whole solve.  A 512-row node keeps one row per thread in at most 128
- * registers, so it still fits on a compute unit beside one workgroup of the
- * trailing update. */
-constexpr int TR_MIN = 256, TR_MAX = 512;
+constexpr int TR = 256;            /* rows per elimination = threads */
+constexpr int NWV = TR / 64;       /* wavefronts per workgroup */
+constexpr int FAN = TR / TW;       /* children per tree node: 4 */
 
 __device__ inline int ldc_i(const int *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -105,61 +100,42 @@ __device__ inline void load_row(f4 (&r)[TW / 4], const TsluArgs &a, int row) {
   }
 }
 
-/* rank of an entry: bits(|x|) + 1 (orders like |x|), 1 for NaN (never
- * better than a number), 0 for a row already chosen; branch-free */
-__device__ __forceinline__ unsigned rank_of(float x, bool act) {
-  const unsigned b = __float_as_uint(x) & 0x7fffffffu;
-  const unsigned rk = b > 0x7f800000u ? 1u : b + 1u;
-  return act ? rk : 0u;
-}
-
 /* Gaussian elimination with partial pivoting on the rows held by the
  * workgroup (one per thread, in registers as float4), recording in s_sel the
- * rows chosen as pivots, in order.  Each step ranks a row by rank_of; every
- * wavefront takes the maximum (ties: lowest lane), its owner stores the key
- * rank << 32 | (TR - 1 - tid) << 2 | isnan << 1 | sign and its row in the
- * wavefront's slot, and after one barrier every thread takes the largest
- * key -- the largest entry, ties to the lower thread.  The key carries the
- * pivot itself (NaN, or its sign and bits(|p|) = rank - 1), so the
- * multiplier needs no dependent LDS read; the winner's row is read for the update.  The key,
- * the winner and the pivot are wavefront-uniform and kept in scalar
- * registers.  The multiplier is the entry times the hardware reciprocal of
- * the pivot (within an ulp of the quotient), which keeps the IEEE
- * division's dependent chain off every step.  Column c + 1 is updated first
- * and ranked for the next step, so its wavefront reduction can run beside
- * the rest of this step's row update.  Each thread notes the step that
- * chose its row; s_sel is written from those after the loop (-1 where no
- * row was left).  The multipliers are kept in place, so a row retired at
- * column c holds its L entries in r[0..c) and its U entries in r[c..TW);
- * *mypos = c.  All TW columns are eliminated: columns past the panel width
- * hold zeros, so their steps change nothing that is used (their choices are
- * ignored), and the loop body has no branch that would make the compiler
- * copy r. */
-template <int TR>
-__device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[TR / 64][TW / 4],
-                            unsigned long long (*s_key)[TR / 64], int *s_sel, int *mypos) {
-  constexpr int NWV = TR / 64;
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+ * rows chosen as pivots, in order.  Each step ranks a row by the integer
+ * bits(|entry|) + 1 (0 for a row already chosen, 1 for NaN: never better
+ * than a number), which orders like |entry|; every wavefront takes the
+ * maximum (ties: lowest lane), its owner stores the key
+ * rank << 32 | (TR - 1 - tid) and its row in the wavefront's slot, and
+ * after one barrier every thread takes the largest key -- the largest
+ * entry, ties to the lower thread -- and reads the winner's row.  The
+ * multiplier is the entry times the hardware reciprocal of the pivot
+ * (within an ulp of the quotient), which keeps the IEEE division's
+ * dependent chain off every step.  Each
+ * thread notes the step that chose its row; s_sel is written from those
+ * after the loop (-1 where no row was left).  The multipliers are kept in
+ * place, so a row retired at column c holds its L entries in r[0..c) and its
+ * U entries in r[c..TW); *mypos = c.  All TW columns are eliminated:
+ * columns past the panel width hold zeros, so their steps change nothing
+ * that is used (their choices are ignored), and the loop body has no branch
+ * that would make the compiler copy r. */
+__device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[NWV][TW / 4],
+                            unsigned long long (*s_key)[NWV], int *s_sel, int *mypos) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   bool act = myrow >= 0;
   int pos = -1;
   if (tid < TW) s_sel[tid] = -1;
-  unsigned u = rank_of(r[0][0], act);
 #pragma unroll
   for (int c = 0; c < TW; ++c) {
     const int par = c & 1;
     const float rc = r[c / 4][c % 4];
+    const float av = fabsf(rc);
+    const unsigned u = act ? ((av == av) ? __float_as_uint(av) + 1u : 1u) : 0u;
     const unsigned wm = wave_max(u);
     const unsigned long long bal = __ballot(u == wm);
     const int wl = __ffsll((long long)bal) - 1;
-    /* the winner's sign and NaN bits, then the key */
-    const unsigned wb = (unsigned)__builtin_amdgcn_readlane(__float_as_int(rc), wl);
-    const unsigned tag = ((wb & 0x7fffffffu) > 0x7f800000u ? 2u : 0u) | (wb >> 31);
-    const unsigned long long wkey =
-        wm ? ((unsigned long long)wm << 32) | ((unsigned)(TR - 1 - 64 * wave - wl) << 2) | tag
-           : 0ull;
     if (lane == wl) {                             /* stores only */
-      s_key[par][wave] = wkey;
+      s_key[par][wave] = wm ? ((unsigned long long)wm << 32) | (unsigned)(TR - 1 - tid) : 0ull;
 #pragma unroll
       for (int q = c / 4; q < TW / 4; ++q) s_row[par][wave][q] = r[q];
     }
@@ -171,36 +147,22 @@ __device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[TR / 64][TW 
     for (int h = NWV / 2; h > 0; h /= 2)
 #pragma unroll
       for (int q = 0; q < h; ++q) kv[q] = kv[q] > kv[q + h] ? kv[q] : kv[q + h];
-    /* every thread read the same keys: the winner is uniform (scalar; the
-     * empty asm keeps the compiler from folding the tests below back into
-     * 64-bit vector compares) */
-    unsigned klo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)kv[0]);
-    unsigned rk = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(kv[0] >> 32));
-    asm volatile("" : "+s"(klo), "+s"(rk));
-    const bool any = rk != 0u;                    /* some row was still active */
-    const int wt = TR - 1 - (int)((klo >> 2) & (TR - 1));
-    /* the pivot: sign | bits(|p|) = rank - 1 (a zero pivot is +-0), or NaN */
-    const float piv = (klo & 2u) ? __uint_as_float(0x7fc00000u)
-                                 : __uint_as_float((klo << 31) | (rk - 1u));
+    const unsigned long long key = kv[0];
+    const bool any = key != 0ull;                 /* some row was still active */
+    const int wt = TR - 1 - (int)(unsigned)(key & 0xffffffffull);
     const bool me = any && tid == wt;
     pos = me ? c : pos;
     act = act && !me;
     const int bw = any ? (wt >> 6) : 0;
-    const float rp = __builtin_amdgcn_rcpf(piv);
-    const float l = (act && piv != 0.0f) ? rc * rp : 0.0f;
+    const float piv = s_row[par][bw][c / 4][c % 4];
+    const float l = (act && piv != 0.0f) ? rc * __builtin_amdgcn_rcpf(piv) : 0.0f;
     r[c / 4][c % 4] = act ? l : rc;
-    if (c + 1 < TW) {
-      const int c1 = c + 1;
-      const f4 p1 = s_row[par][bw][c1 / 4];
-      r[c1 / 4][c1 % 4] -= l * p1[c1 % 4];
-      u = rank_of(r[c1 / 4][c1 % 4], act);
-    }
 #pragma unroll
     for (int q = c / 4; q < TW / 4; ++q) {
       const f4 pr = s_row[par][bw][q];
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        if (4 * q + e > c + 1) r[q][e] -= l * pr[e];
+        if (4 * q + e > c) r[q][e] -= l * pr[e];
     }
   }
   if (pos >= 0) s_sel[pos] = myrow;
@@ -215,7 +177,7 @@ __device__ void gepp_select(f4 (&r)[TW / 4], int myrow, f4 (*s_row)[TR / 64][TW 
  * X12 = -X11 U12 X22 (upper), from the inverses of its two b x b halves.
  * The sums run over all b terms (the triangles' zeros make up the rest), so
  * every loop has a fixed trip count. */
-template <int TR, int B>
+template <int B>
 __device__ __forceinline__ void tri_level(const float (*s_lu)[TW + 1], float (*s_xl)[TW + 1],
                                           float (*s_xu)[TW + 1], float *s_t) {
   constexpr int nblk = TW / (2 * B), per = B * B, tot = 2 * nblk * per;
@@ -260,7 +222,6 @@ __device__ __forceinline__ void tri_level(const float (*s_lu)[TW + 1], float (*s
  * one block: threads 0..63 the unit lower blocks, 64..127 the upper ones,
  * the upper diagonal applied as its reciprocal), then the doubling levels
  * 16 and 32 above them. */
-template <int TR>
 __device__ void tri_inverses(const float (*s_lu)[TW + 1], int w, float (*s_xl)[TW + 1],
                              float (*s_xu)[TW + 1], float *s_t) {
   const int tid = threadIdx.x;
@@ -297,18 +258,11 @@ __device__ void tri_inverses(const float (*s_lu)[TW + 1], int w, float (*s_xl)[T
     }
   }
   __syncthreads();
-  tri_level<TR, 16>(s_lu, s_xl, s_xu, s_t);
-  tri_level<TR, 32>(s_lu, s_xl, s_xu, s_t);
+  tri_level<16>(s_lu, s_xl, s_xu, s_t);
+  tri_level<32>(s_lu, s_xl, s_xu, s_t);
 }
 
-/* One launch per panel, nleaves workgroups of TR threads.  512-thread
- * nodes ask for four wavefronts per SIMD (the second bound), i.e. at most
- * 128 registers, so one such workgroup (two wavefronts per SIMD) fits
- * beside a trailing-update workgroup (one wavefront of 256 registers). */
-template <int TR>
-__global__ __launch_bounds__(TR, TR == 512 ? 4 : 1) void tslu_select_kernel(TsluArgs a) {
-  constexpr int NWV = TR / 64;       /* wavefronts per workgroup */
-  constexpr int FAN = TR / TW;       /* children per tree node */
+__global__ __launch_bounds__(TR) void tslu_select_kernel(TsluArgs a) {
   __shared__ f4 s_row[2][NWV][TW / 4];
   __shared__ unsigned long long s_key[2][NWV];
   __shared__ int s_sel[TW];
@@ -332,7 +286,7 @@ __global__ __launch_bounds__(TR, TR == 512 ? 4 : 1) void tslu_select_kernel(Tslu
   load_row(r, a, myrow);
   int node = g, mypos = -1;
   for (;;) {
-    gepp_select<TR>(r, myrow, s_row, s_key, s_sel, &mypos);
+    gepp_select(r, myrow, s_row, s_key, s_sel, &mypos);
     if (lsz == 1) break;                          /* root: s_sel is the answer */
     /* publish this node's choice, then count in at the parent */
     if (tid < TW) stc_i(&a.cand[(size_t)(loff + node) * TW + tid], tid < a.w ? s_sel[tid] : -1);
@@ -381,7 +335,7 @@ __global__ __launch_bounds__(TR, TR == 512 ? 4 : 1) void tslu_select_kernel(Tslu
   }
   __syncthreads();
   if (tid == 0 && s_aux[0] < ws) atomicCAS(a.info, 0, a.k + s_aux[0] + 1);
-  tri_inverses<TR>(s_lu, ws, s_xl, s_xu, s_t);
+  tri_inverses(s_lu, ws, s_xl, s_xu, s_t);
   for (int e = tid; e < TW * TW; e += TR) {
     const int i = e % TW, j = e / TW;             /* column-major (i, j) */
     a.lu11[e] = s_lu[i][j];
@@ -476,18 +430,17 @@ __global__ __launch_bounds__(TR, TR == 512 ? 4 : 1) void tslu_select_kernel(Tslu
     }
     __syncthreads();
     const int ne = a.sw_e1 - a.sw_e0;
-    constexpr int NMV = 2 * TW * 64 / TR;        /* covers the 2 TW moves at most */
     for (int h = 0; h < ne; h += 64) {
-      float v[NMV];
+      float v[32];
 #pragma unroll
-      for (int i = 0; i < NMV; ++i) {
+      for (int i = 0; i < 32; ++i) {
         const int idx = tid + TR * i, mv = idx >> 6, e = h + (idx & 63);
         v[i] = (mv < np && e < ne) ? a.Sw[(size_t)s_mv[2 * mv + 1] * a.lds + a.sw_e0 + e]
                                    : 0.0f;
       }
       __syncthreads();
 #pragma unroll
-      for (int i = 0; i < NMV; ++i) {
+      for (int i = 0; i < 32; ++i) {
         const int idx = tid + TR * i, mv = idx >> 6, e = h + (idx & 63);
         if (mv < np && e < ne) a.Sw[(size_t)s_mv[2 * mv] * a.lds + a.sw_e0 + e] = v[i];
       }
@@ -508,7 +461,7 @@ __global__ __launch_bounds__(TR, TR == 512 ? 4 : 1) void tslu_select_kernel(Tslu
         ut[i][j] = (i < nr && j < ws) ? a.Sw[(size_t)(a.k + j) * a.lds + r0 + i] : 0.0f;
       }
       __syncthreads();
-      const int ti = (tid >> 4) & 15, tj = tid & 15;
+      const int ti = tid >> 4, tj = tid & 15;
       float acc[4][4];
 #pragma unroll
       for (int x = 0; x < 4; ++x)
@@ -531,7 +484,7 @@ __global__ __launch_bounds__(TR, TR == 512 ? 4 : 1) void tslu_select_kernel(Tslu
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
           const int i = 4 * ti + x, j = 4 * tj + y;
-          if (tid < 256 && i < nr && j < ws) a.Sw[(size_t)(a.k + j) * a.lds + r0 + i] = acc[x][y];
+          if (i < nr && j < ws) a.Sw[(size_t)(a.k + j) * a.lds + r0 + i] = acc[x][y];
         }
     }
   }
@@ -886,36 +839,11 @@ __global__ __launch_bounds__(256) void tslu_gather_kernel(int n, int nrhs, const
 
 }  // namespace
 
-namespace {
-/* tree levels of a panel of m rows with nodes of tr rows */
-int tree_levels(int m, int tr) {
-  int sz = (m + tr - 1) / tr, lv = 1;
-  while (sz > 1) { sz = (sz + tr / TW - 1) / (tr / TW); ++lv; }
-  return lv;
-}
-/* tree nodes (all levels) of a panel of m rows with nodes of tr rows */
-int tree_nodes(int m, int tr) {
-  int sz = (m + tr - 1) / tr, nodes = sz;
-  while (sz > 1) { sz = (sz + tr / TW - 1) / (tr / TW); nodes += sz; }
-  return nodes;
-}
-}  // namespace
-
-/* node size for a panel of m rows: the one with fewer tree levels, the
- * smaller on a tie */
-extern "C" int atlmx_k_tslu_node_rows(int m) {
-  return tree_levels(m, TR_MAX) < tree_levels(m, TR_MIN) ? TR_MAX : TR_MIN;
-}
-
-extern "C" int atlmx_k_tslu_leaves(int m) {
-  const int tr = atlmx_k_tslu_node_rows(m);
-  return m > 0 ? (m + tr - 1) / tr : 0;
-}
-
 extern "C" size_t atlmx_k_tslu_ws_ints(int n) {
-  /* candidates for every tree node + counters, for the largest panel with
-   * the smaller node size (the most nodes) */
-  const int nodes = tree_nodes(n, TR_MIN);
+  /* candidates for every tree node + counters, for the largest panel */
+  const int leaves = (n + TR - 1) / TR;
+  int nodes = leaves, sz = leaves;
+  while (sz > 1) { sz = (sz + FAN - 1) / FAN; nodes += sz; }
   return (size_t)nodes * TW + (size_t)nodes + 2 + 4 * TW + 64;
 }
 
@@ -937,19 +865,19 @@ extern "C" hipError_t atlmx_k_tslu_select_swap(float *S, long lds, int n, int k,
   TsluArgs a;
   a.lu11 = lu11; a.invl = invl; a.invu = invu; a.info = info;
   a.S = S; a.lds = lds; a.k = k; a.n = n; a.w = min(w, n - k);
-  const int tr = atlmx_k_tslu_node_rows(n - k);
-  a.nleaves = (n - k + tr - 1) / tr;
-  /* counters first, sized for the largest tree any panel of this matrix
-   * uses (the first panel's with the smaller nodes) */
-  a.cnt = ws;
-  a.cand = ws + tree_nodes(n, TR_MIN) + 2;
+  a.nleaves = (n - k + TR - 1) / TR;
+  {
+    /* counters first; the tree of the first panel (k = 0) is the largest */
+    const int leaves_max = (n + TR - 1) / TR;
+    int nmax = leaves_max, szm = leaves_max;
+    while (szm > 1) { szm = (szm + FAN - 1) / FAN; nmax += szm; }
+    a.cnt = ws;
+    a.cand = ws + nmax + 2;
+  }
   if (prev && !plan2) return hipErrorInvalidValue;
   a.plan = plan; a.perm = perm; a.prev = prev; a.plan2 = plan2;
   a.Sw = S; a.sw_e0 = sw_e0; a.sw_e1 = sw_e1; a.u12_r1 = u12_r1;
-  if (tr == TR_MAX)
-    hipLaunchKernelGGL(tslu_select_kernel<TR_MAX>, dim3(a.nleaves), dim3(TR_MAX), 0, s, a);
-  else
-    hipLaunchKernelGGL(tslu_select_kernel<TR_MIN>, dim3(a.nleaves), dim3(TR_MIN), 0, s, a);
+  hipLaunchKernelGGL(tslu_select_kernel, dim3(a.nleaves), dim3(TR), 0, s, a);
   return hipGetLastError();
 }
 

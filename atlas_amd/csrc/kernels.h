@@ -38,6 +38,13 @@ hipError_t atlmx_k_sgemm(int ta, int tb, int M, int N, int K, float alpha,
                          float beta, float *C, long ldc, int tri,
                          const int *abort_flag, hipStream_t s);
 
+/* C += alpha A op(B) in FP64 (A not transposed, beta = 1) with the C tile
+ * loaded behind the MFMAs: the short-K trailing updates of getrf / potrf.
+ * tb: KX_NOTRANS or KX_TRANS; tri as atlmx_k_dgemm. */
+hipError_t atlmx_k_dupdate(int tb, int M, int N, int K, double alpha, const double *A,
+                           long lda, const double *B, long ldb, double *C, long ldc, int tri,
+                           const int *abort_flag, hipStream_t s);
+
 /* Largest row count one in-place GEMM tile holds: an in-place product
  * B := inv(T)*B with M <= ATLMX_GEMM_BM rows (or B := B*inv(T) with
  * N <= ATLMX_GEMM_BN columns) reads all of its input before any write. */
@@ -84,10 +91,6 @@ hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const float *A,
  * to prev's moves followed by this panel's. */
 #define ATLMX_TSLU_W 64
 size_t atlmx_k_tslu_ws_ints(int n);
-/* The tournament tree of a panel of m rows: rows per node (256 or 512,
- * whichever needs fewer levels) and the number of leaves (workgroups). */
-int atlmx_k_tslu_node_rows(int m);
-int atlmx_k_tslu_leaves(int m);
 hipError_t atlmx_k_tslu_select(const float *S, long lds, int n, int k, int w, int *ws,
                                int *plan, int *perm, float *lu11, float *invl, float *invu,
                                int *info, const int *prev, int *plan2, hipStream_t s);
@@ -218,18 +221,21 @@ hipError_t atlmx_k_sscal2d(int M, int N, float alpha, float *B, long ldb,
  * three launches per column on the panel in HBM.  ipiv[c] receives the
  * absolute pivot row (row0 + panel row); info (device int) gets row0+c+1
  * for the first exactly-zero pivot if it is still 0.  scratch: device
- * buffer of atlmx_k_panel_scratch_bytes() bytes, initialised once with
- * atlmx_k_panel_scratch_init(); consecutive launches on one scratch must
- * alternate set = 0, 1, 0, ... starting from 0 after the init. */
+ * buffer of atlmx_k_panel_scratch_bytes() bytes, initialised with
+ * atlmx_k_panel_scratch_init() (which clears every exchange tag); the
+ * launches on one scratch take increasing sequence numbers seq = 1, 2, ...
+ * below ATLMX_PANEL_SEQ_MAX, and after that the scratch is initialised again
+ * and the numbering restarts at 1. */
 #define ATLMX_PANEL_MAXW 64
 #define ATLMX_PANEL_TALL 65536
+#define ATLMX_PANEL_SEQ_MAX (1 << 26)
 size_t atlmx_k_panel_scratch_bytes(int m, int w);
 hipError_t atlmx_k_panel_scratch_init(void *scratch, hipStream_t s);
 hipError_t atlmx_k_dgetf2_panel(int m, int w, double *P, long ld, int row0,
-                                int *ipiv, int *info, void *scratch, int set,
+                                int *ipiv, int *info, void *scratch, int seq,
                                 hipStream_t s);
 hipError_t atlmx_k_sgetf2_panel(int m, int w, float *P, long ld, int row0,
-                                int *ipiv, int *info, void *scratch, int set,
+                                int *ipiv, int *info, void *scratch, int seq,
                                 hipStream_t s);
 
 /* Apply the row interchanges ipiv[k1..k2) (absolute row numbers below

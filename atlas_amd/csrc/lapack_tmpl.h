@@ -53,10 +53,13 @@ static void PANEL_REC(atlmx_ctx *c, int M, T *A, long lda, int r0, int w, int *i
   const int leaf = (M - r0 > ATLMX_PANEL_TALL) ? LU_PW / 2 : LU_PW;
   if (w <= leaf) {
     int pid;
+    if (++c->panel_set >= ATLMX_PANEL_SEQ_MAX) {     /* restart the exchange tags */
+      RT_CHECK(rout, atlmx_k_panel_scratch_init(scratch, c->stream));
+      c->panel_set = 1;
+    }
     pid = rt_prof_begin(c, PROF_PANEL, (double)(M - r0) * w * 2.0 * sizeof(T));
     RT_CHECK(rout, K_PANEL(M - r0, w, A + r0 + (size_t)r0 * lda, lda, r0, ipiv + r0, info,
                            scratch, c->panel_set, c->stream));
-    c->panel_set ^= 1;
     rt_prof_end(c, pid);
     return;
   } else {
@@ -96,9 +99,9 @@ static void GETRF_UPDATE(atlmx_ctx *c, int M, T *A, long lda, int j, int jb, int
   NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, jb, nc, (T)1,
                     A + j + (size_t)j * lda, lda, A + j + (size_t)c0 * lda, lda, NULL, rout);
   if (jn < M)
-    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - jn, nc, jb, (T)-1,
-                      A + jn + (size_t)j * lda, lda, A + j + (size_t)c0 * lda, lda, (T)1,
-                      A + jn + (size_t)c0 * lda, lda, 0, NULL, rout);
+    NM(PFX, update_dev)(c, KX_NOTRANS, M - jn, nc, jb, (T)-1, A + jn + (size_t)j * lda, lda,
+                        A + j + (size_t)c0 * lda, lda, A + jn + (size_t)c0 * lda, lda, 0, NULL,
+                        rout);
 }
 
 int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const char *rout)
@@ -212,9 +215,9 @@ static void NM(PFX, potrf_update)(atlmx_ctx *c, int uplo, int N, T *A, long lda,
 {
   if (nc <= 0) return;
   if (uplo == KX_LOWER)
-    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_TRANS, N - r0, nc, jb, (T)-1,
-                      A + r0 + (size_t)j * lda, lda, A + r0 + (size_t)j * lda, lda, (T)1,
-                      A + r0 + (size_t)r0 * lda, lda, KX_LOWER, d_info, rout);
+    NM(PFX, update_dev)(c, KX_TRANS, N - r0, nc, jb, (T)-1, A + r0 + (size_t)j * lda, lda,
+                        A + r0 + (size_t)j * lda, lda, A + r0 + (size_t)r0 * lda, lda,
+                        KX_LOWER, d_info, rout);
   else
     NM(PFX, gemm_dev)(c, KX_TRANS, KX_NOTRANS, nc, N - r0, jb, (T)-1,
                       A + j + (size_t)r0 * lda, lda, A + j + (size_t)r0 * lda, lda, (T)1,

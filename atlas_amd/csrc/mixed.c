@@ -295,7 +295,7 @@ static int sgetrf_tslu_dev(atlmx_ctx *c, int n, float *S, long lds, int *perm, i
     rt_on_aux(c, main, 0);
     RT_CHECK(rout, hipEventRecord(c->ev[1], aux));
     tslu_update_beside(c, S, lds, k2 + R, n - k2 - R, k2, n - k2, k, k2 - k, lp,
-                       atlmx_k_tslu_leaves(n - k2), d_info, main, rout);
+                       (n - k2 + 255) / 256, d_info, main, rout);
     RT_CHECK(rout, hipStreamWaitEvent(main, c->ev[2], 0));   /* head through Q's choice */
     k = k2;
   }

@@ -6,7 +6,7 @@
  *
  * atlmx_dgetrs needs two scratch buffers: the inverted 64 x 64 diagonal
  * blocks of the triangular factor (nblk * 64 * 64 doubles) and the
- * few-right-hand-side solve's block counter and flags (nblk + 1 ints).  It
+ * few-right-hand-side solve's block counter and published solution words.  It
  * takes them from the calling thread's grow-only workspaces.  Here they are
  * slices of one file-scope device array, `scratch`, sized for N <=
  * ATLMX_NOALLOC_MAX_N.  Each GPU holds its own copy of the array until the
@@ -15,7 +15,7 @@
  * handed to a kernel or a fill.  The kernels are those of atlmx_dgetrs, in
  * the same order, so the two give bit-identical results.
  *
- * Every call on a GPU shares that GPU's array: the flags are cleared by a
+ * Every call on a GPU shares that GPU's array: the solve's words are cleared by a
  * fill kernel at the start of each solve and the inverses are rewritten in
  * full, so one call never sees the last call's contents, but calls that
  * overlap on one GPU (two threads, or two streams) race on it.
@@ -36,11 +36,15 @@ namespace {
 constexpr int NB = ATLMX_TRI_NB;                                     /* 64 */
 constexpr size_t INV_BYTES =
     (size_t)((ATLMX_NOALLOC_MAX_N + NB - 1) / NB) * NB * NB * sizeof(double);
-constexpr size_t CTL_BYTES = 4096;     /* (nblk + 1) ints, nblk <= 256 */
+/* the solve's block counter and tagged words: atlmx_k_trsv_ctl_ints(N, 1)
+ * ints, 2 + 4 * ATLMX_TRSV_MAXRHS * N for FP64 */
+constexpr size_t CTL_BYTES =
+    ((size_t)(2 + 4 * ATLMX_TRSV_MAXRHS * (size_t)ATLMX_NOALLOC_MAX_N) * sizeof(int) + 255) &
+    ~(size_t)255;
 constexpr size_t SCRATCH_BYTES = INV_BYTES + CTL_BYTES;
 constexpr int MAX_DEV = 64;
 
-static_assert(INV_BYTES % 256 == 0, "the flags slice must stay 256-byte aligned");
+static_assert(INV_BYTES % 256 == 0, "the solve's slice must stay 256-byte aligned");
 
 __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
 

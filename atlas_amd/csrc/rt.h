@@ -49,7 +49,7 @@ typedef struct atlmx_ctx {
   size_t ws_bytes[2][WS_NSLOTS];
   int bank;
   void *panel_buf;          /* LU panel exchange buffer last initialised */
-  int panel_set;            /* header set the next panel launch uses     */
+  int panel_set;            /* sequence number of the last panel launch  */
   hipStream_t aux;          /* second stream for lookahead work (lazily created) */
   hipEvent_t ev[3];         /* ordering events between stream and aux   */
   int ncu;                  /* compute units of the device (set with aux) */

@@ -182,7 +182,7 @@ double atlmx_dbackward_error(int N, int NRHS, const double *A, int lda, const do
  * serving loops, harnesses that forbid hipMalloc.  The same solve as
  * atlmx_dgetrs (same kernels, same order, bit-identical results), but its
  * scratch (the inverted 64 x 64 diagonal blocks of the factor and the
- * solve's flags) lives in one static device array of
+ * solve's published words) lives in one static device array of
  * atlmx_noalloc_scratch_bytes() bytes per GPU, held until the process ends;
  * no hipMalloc, hipFree or other allocation is made.  N <= ATLMX_NOALLOC_MAX_N:
  * a larger N (or another illegal argument) returns minus its position, with

@@ -81,33 +81,6 @@ def test_batch_plan_order_limits_without_gpu():
 
 
 
-def test_tournament_node_size_rule_without_gpu():
-    """Tournament trees use 512-row nodes (8 children) where that takes fewer
-    levels than 256-row nodes (4 children), 256 on a tie; pure host logic."""
-    import ctypes as C
-    import atlas_amd
-    L = atlas_amd.lib()
-    f = L.atlmx_k_tslu_node_rows
-    f.argtypes = [C.c_int]
-    f.restype = C.c_int
-    g = L.atlmx_k_tslu_leaves
-    g.argtypes = [C.c_int]
-    g.restype = C.c_int
-
-    def levels(m, tr):
-        sz, lv = -(-m // tr), 1
-        while sz > 1:
-            sz, lv = -(-sz // (tr // 64)), lv + 1
-        return lv
-
-    for m in (1, 64, 256, 257, 300, 512, 513, 1024, 1025, 1500, 2048, 4096, 4097, 4872, 8192,
-              16384, 16385, 20000, 65536):
-        want = 512 if levels(m, 512) < levels(m, 256) else 256
-        assert f(m) == want, m
-        assert g(m) == -(-m // want), m
-    assert f(16384) == 512 and f(1024) == 256 and f(200) == 256
-
-
 def test_noalloc_refuses_too_large_order_without_gpu():
     """The allocation-free solve checks its arguments before any device call:
     an order beyond its static scratch array is refused with a message."""

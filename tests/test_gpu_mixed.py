@@ -39,25 +39,11 @@ def _k(name, *args):
     assert r == 0, (name, r)
 
 
-def _node_rows(m):
-    """rows per tree node the library uses for a panel of m rows (256 or 512,
-    whichever needs fewer levels)"""
-    f = X.lib().atlmx_k_tslu_node_rows
-    f.argtypes = [C.c_int]
-    f.restype = C.c_int
-    return f(m)
-
-
-def _tournament(A, k, w, leaf=None, fan=None):
+def _tournament(A, k, w, leaf=256, fan=4):
     """numpy restatement of the tournament (the checker of the GPU kernel):
-    partial-pivoting elimination in FP32 on leaves of `leaf` rows of the
-    panel A[k:, k:k+w], then on the original rows of up to leaf / 64
-    children's choices, level by level; returns the root's chosen rows in
-    pivot order."""
-    if leaf is None:
-        leaf = _node_rows(A.shape[0] - k)
-    if fan is None:
-        fan = leaf // 64
+    partial-pivoting elimination in FP32 on 256-row leaves of the panel
+    A[k:, k:k+w], then on the original rows of up to 4 children's choices,
+    level by level; returns the root's chosen rows in pivot order."""
     def gepp(rows):
         P = A[rows, k:k + w].astype(np.float32).copy()
         act = np.ones(len(rows), bool)
