@@ -21,8 +21,11 @@
  * Results are cached for the process by (kind, precision, nrhs, size
  * buckets), so later plans of a similar shape skip the timing.
  */
+#include <fcntl.h>
 #include <math.h>
 #include <pthread.h>
+#include <sys/stat.h>
+#include <unistd.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -57,6 +60,97 @@ static int at_n = 0;
 /* plans may be made from any host thread: the cache is read and extended
  * under this lock (the timing itself runs outside it) */
 static pthread_mutex_t at_lock = PTHREAD_MUTEX_INITIALIZER;
+static int at_file_read = 0;
+
+/* The tuner's choices also persist across processes in a text file, one
+ * JSON object per line keyed by the GPU's architecture name and the shape
+ * class: {"arch": "gfx950", "kind": 2, "dbl": 1, "nrhs": 0, "b_max": 9,
+ * "b_mean": 8, "nb": 2, "grid": 768}.  The file is $ATLMX_TUNE_CACHE, else
+ * $XDG_CACHE_HOME/atlas_mi355x/batch_tune.jsonl, else
+ * $HOME/.cache/atlas_mi355x/batch_tune.jsonl; ATLMX_TUNE_CACHE set to the
+ * empty string turns persistence off.  Lines for other architectures, or
+ * that do not parse, are ignored; a new choice is appended as one short
+ * write. */
+static int at_path(char *buf, size_t len, int make_dirs)
+{
+  const char *e = getenv("ATLMX_TUNE_CACHE"), *base;
+  char dir[4064];
+  if (e) {
+    if (!*e) return 0;
+    snprintf(buf, len, "%.4000s", e);
+    return 1;
+  }
+  base = getenv("XDG_CACHE_HOME");
+  if (base && *base) {
+    snprintf(dir, sizeof dir, "%.4000s/atlas_mi355x", base);
+  } else {
+    base = getenv("HOME");
+    if (!base || !*base) return 0;
+    snprintf(dir, sizeof dir, "%.4000s/.cache", base);
+    if (make_dirs) (void)mkdir(dir, 0755);
+    snprintf(dir, sizeof dir, "%.4000s/.cache/atlas_mi355x", base);
+  }
+  if (make_dirs) (void)mkdir(dir, 0755);
+  snprintf(buf, len, "%.4000s/batch_tune.jsonl", dir);
+  return 1;
+}
+
+static void at_arch(char *buf, size_t len)
+{
+  int dev = 0;
+  hipDeviceProp_t p;
+  size_t i;
+  snprintf(buf, len, "unknown");
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&p, dev) != hipSuccess) return;
+  snprintf(buf, len, "%.63s", p.gcnArchName);
+  for (i = 0; buf[i]; ++i)                 /* "gfx950:sramecc+:xnack-" -> "gfx950" */
+    if (buf[i] == ':' || buf[i] == '"') { buf[i] = 0; break; }
+}
+
+static int cu_count(void);
+
+/* called with at_lock held */
+static void at_file_load(void)
+{
+  char path[4200], arch[64], line[512], a2[64];
+  FILE *f;
+  if (at_file_read) return;
+  at_file_read = 1;
+  if (!at_path(path, sizeof path, 0)) return;
+  f = fopen(path, "r");
+  if (!f) return;
+  at_arch(arch, sizeof arch);
+  while (fgets(line, sizeof line, f) && at_n < AT_CACHE) {
+    at_entry a;
+    if (sscanf(line, " {\"arch\": \"%63[^\"]\", \"kind\": %d, \"dbl\": %d, \"nrhs\": %d, "
+                     "\"b_max\": %d, \"b_mean\": %d, \"nb\": %d, \"grid\": %d}",
+               a2, &a.kind, &a.dbl, &a.nrhs, &a.b_max, &a.b_mean, &a.nb, &a.grid) != 8)
+      continue;
+    if (strcmp(a2, arch) != 0) continue;
+    /* only configurations the tuner itself could have chosen: the GEMM
+     * variant 0..2 or the panel width 16 / 32, at most 16 workgroups per CU */
+    if (a.kind == BK_GEMM ? (a.nb < 0 || a.nb > 2) : (a.nb != 16 && a.nb != 32)) continue;
+    if (a.grid <= 0 || a.grid > 16 * cu_count()) continue;
+    at_cache[at_n++] = a;
+  }
+  fclose(f);
+}
+
+static void at_file_append(const at_entry *a)
+{
+  char path[4200], arch[64], line[512];
+  int fd, len;
+  if (!at_path(path, sizeof path, 1)) return;
+  at_arch(arch, sizeof arch);
+  len = snprintf(line, sizeof line,
+                 "{\"arch\": \"%s\", \"kind\": %d, \"dbl\": %d, \"nrhs\": %d, \"b_max\": %d, "
+                 "\"b_mean\": %d, \"nb\": %d, \"grid\": %d}\n",
+                 arch, a->kind, a->dbl, a->nrhs, a->b_max, a->b_mean, a->nb, a->grid);
+  fd = open(path, O_WRONLY | O_CREAT | O_APPEND, 0644);
+  if (fd < 0) return;
+  if (len > 0 && len < (int)sizeof line) (void)!write(fd, line, (size_t)len);
+  close(fd);
+}
 
 static int cu_count(void)
 {
@@ -179,6 +273,7 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   b_max = bucket(p->maxn);
   b_mean = bucket(mean);
   pthread_mutex_lock(&at_lock);
+  at_file_load();
   for (i = 0; i < at_n; ++i) {
     const at_entry *a = &at_cache[i];
     if (a->kind == p->kind && a->dbl == p->dbl && a->nrhs == p->nrhs && a->b_max == b_max &&
@@ -312,9 +407,10 @@ static void autotune(atlmx_batch *p, const int *h_order, const int *h_n, const i
   if (hipiv) RT_CHECK(rout, hipFree(hipiv));
   free(sn); free(offA); free(hA);
   pthread_mutex_lock(&at_lock);
-  if (at_n < AT_CACHE) {
+  {
     at_entry a = {p->kind, p->dbl, p->nrhs, b_max, b_mean, p->nb, p->grid};
-    at_cache[at_n++] = a;
+    if (at_n < AT_CACHE) at_cache[at_n++] = a;
+    at_file_append(&a);
   }
   pthread_mutex_unlock(&at_lock);
 }

@@ -159,3 +159,52 @@ def test_graph_replays_match_eager(torch):
         for i, n in enumerate(sizes):
             x = outs[rep][i].T
             assert np.abs(As0[i] @ x - (rep + 1) * Bs0[i]).max() < 1e-10 * n, (rep, i)
+
+
+_TUNE_CODE = r"""
+import sys, numpy as np
+sys.path.insert(0, %r)
+import torch, atlas_amd as X
+torch.cuda.set_device(0)
+rng = np.random.default_rng(5)
+M, N, K = (16 * rng.integers(1, 9, 40) for _ in range(3))
+As = [torch.rand(int(K[i]), int(M[i]), dtype=torch.float64, device="cuda") for i in range(40)]
+Bs = [torch.rand(int(N[i]), int(K[i]), dtype=torch.float64, device="cuda") for i in range(40)]
+Cs = [torch.rand(int(N[i]), int(M[i]), dtype=torch.float64, device="cuda") for i in range(40)]
+with X.gemm_batch(X.NoTrans, X.NoTrans, 1.0, As, Bs, 0.0, Cs) as plan:
+    plan.run()
+    torch.cuda.synchronize()
+    print("CONFIG", *plan.config())
+"""
+
+
+def test_autotune_choice_persists_across_processes(tmp_path):
+    """A tuned choice is appended to the cache file ($ATLMX_TUNE_CACHE), keyed
+    by the GPU architecture and the shape class; a fresh process with the
+    same shape reads it back instead of timing (checked by editing the
+    stored grid to another valid value and seeing the plan use it)."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cache = tmp_path / "tune.jsonl"
+    env = dict(os.environ, ATLMX_TUNE_CACHE=str(cache))
+
+    def run():
+        p = subprocess.run([sys.executable, "-c", _TUNE_CODE % root], env=env, text=True,
+                           stdout=subprocess.PIPE, stderr=subprocess.PIPE, timeout=300)
+        assert p.returncode == 0, p.stderr[-2000:]
+        nb, grid = (int(v) for v in p.stdout.split("CONFIG")[1].split())
+        return nb, grid
+
+    nb, grid = run()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1
+    rec = json.loads(lines[0])
+    assert rec["arch"].startswith("gfx") and rec["kind"] == 2 and rec["dbl"] == 1
+    assert (rec["nb"], rec["grid"]) == (nb, grid)
+    rec["nb"], rec["grid"] = 0, grid + 8              # another valid configuration
+    cache.write_text(json.dumps(rec) + "\n")
+    assert run() == (0, grid + 8)
+    assert len(cache.read_text().splitlines()) == 1   # read back, nothing tuned or appended
