@@ -19,6 +19,7 @@
 #define K_SCAL atlmx_k_dscal2d
 #define K_IAMAX atlmx_k_idamax
 #define K_TRSV atlmx_k_dtrsv
+#define K_TRSM_TAG atlmx_k_dtrsm_tag
 #define K_TRICOPY atlmx_k_dtricopy
 #define K_UPDATE atlmx_k_dupdate
 #include "blas3_tmpl.h"
@@ -30,6 +31,7 @@
 #undef K_SCAL
 #undef K_IAMAX
 #undef K_TRSV
+#undef K_TRSM_TAG
 #undef K_TRICOPY
 
 #define T float
@@ -39,5 +41,6 @@
 #define K_SCAL atlmx_k_sscal2d
 #define K_IAMAX atlmx_k_isamax
 #define K_TRSV atlmx_k_strsv
+#define K_TRSM_TAG atlmx_k_strsm_tag
 #define K_TRICOPY atlmx_k_stricopy
 #include "blas3_tmpl.h"

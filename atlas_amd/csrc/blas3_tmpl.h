@@ -68,7 +68,10 @@ void NM(PFX, update_dev)(atlmx_ctx *c, int tb, int M, int N, int K, T alpha, con
  * are inverted in one launch, then per diagonal block one in-place GEMM
  * (X_k = op(inv A_kk) * B_k, a single 128-row tile, so it reads B_k before
  * writing it) and one GEMM update of the rest of B.  Left-side solves with
- * at most ATLMX_TRSV_MAXRHS columns run as one flag-chained launch.
+ * one column (or a few, within one diagonal block) run as one flag-chained
+ * launch, and with up to ATLMX_TRSM_TAG_MAXRHS as one launch of work items
+ * of two 64-row blocks by 16 columns chained the same way (matrix-core
+ * products).
  * ---------------------------------------------------------------------- */
 /* inv_ws / ctl_ws: NULL, or caller-provided workspaces of at least
  * nblk*64*64 elements / atlmx_k_trsv_ctl_ints(na, T is double) ints (the
@@ -96,7 +99,7 @@ void NM(PFX, trsm_dev_ws)(atlmx_ctx *c, int side, int uplo, int ta, int diag, in
   RT_CHECK(rout, K_TRI_INV(uplo, diag, na, A, lda, inv, abort_flag, c->stream));
   rt_prof_end(c, pid);
 
-  if (side == KX_LEFT && N <= ATLMX_TRSV_MAXRHS && !abort_flag) {
+  if (side == KX_LEFT && (N == 1 || (N <= ATLMX_TRSV_MAXRHS && nblk < 2)) && !abort_flag) {
     /* few right-hand sides: the solve is bound by reading the factor once,
      * so one flag-chained launch replaces 2*nblk small GEMMs */
     const size_t nctl = atlmx_k_trsv_ctl_ints(na, sizeof(T) == sizeof(double));
@@ -105,6 +108,26 @@ void NM(PFX, trsm_dev_ws)(atlmx_ctx *c, int side, int uplo, int ta, int diag, in
     pid = rt_prof_begin(c, PROF_TRSV, (double)na * na / 2 * sizeof(T));
     RT_CHECK(rout, K_TRSV(uplo, tinv, M, N, A, lda, inv, B, ldb, ctl, c->stream));
     rt_prof_end(c, pid);
+  } else if (side == KX_LEFT && N <= ATLMX_TRSM_TAG_MAXRHS && nblk >= 2 && !abort_flag) {
+    /* two to a few hundred right-hand sides: still one launch, 64x16 blocks
+     * of the solution chained through tagged words; with a caller's workspace
+     * (sized for ATLMX_TRSM_TAG_CHUNK columns) one launch per that many
+     * columns, which gives the same result since columns are independent */
+    const int chunk = ctl_ws ? ATLMX_TRSM_TAG_CHUNK : N;
+    const size_t nctl =
+        atlmx_k_trsm_tag_ctl_ints(na, chunk < N ? chunk : N, sizeof(T) == sizeof(double));
+    int *ctl = ctl_ws ? ctl_ws : (int *)rt_ws(c, WS_TRSV, nctl * sizeof(int), rout);
+    int q;
+    for (q = 0; q < N; q += chunk) {
+      const int nq = (N - q < chunk) ? N - q : chunk;
+      RT_CHECK(rout, atlmx_k_fill_i32(ctl, (int)atlmx_k_trsm_tag_ctl_ints(
+                                               na, nq, sizeof(T) == sizeof(double)),
+                                      0, c->stream));
+      pid = rt_prof_begin(c, PROF_TRSV, (double)na * na / 2 * sizeof(T));
+      RT_CHECK(rout, K_TRSM_TAG(uplo, tinv, M, nq, A, lda, inv, B + (size_t)q * ldb, ldb, ctl,
+                                c->stream));
+      rt_prof_end(c, pid);
+    }
   } else if (side == KX_LEFT) {
     const int fwd = opA_lower;
     for (step = 0; step < nblk; ++step) {

@@ -498,6 +498,120 @@ __global__ __launch_bounds__(256) void laswp_lds_kernel(T *A, long lda, int k1, 
   for (int i = threadIdx.x; i < n; i += 256) g[i] = col[i];
 }
 
+/* The interchanges of getrs as one permutation (for right-hand sides: the
+ * sequential replay above costs one LDS round trip per interchange).  With
+ * p_k >= k (as getrf leaves them), interchange k is the last to touch row
+ * k, so after the forward sequence row j holds what row p_j held just
+ * before interchange j.  Let c(j) be what row j holds just before its own
+ * interchange: the row last moved there, i.e. c(pred(j)) with pred(j) the
+ * last k < j with p_k = j, or j itself.  Then the gather map is
+ *   src[j] = c(j)                                  if j has no interchange
+ *   src[j] = c(q) (q = the last k < j with p_k = p_j), else p_j.
+ * One workgroup: the pairs (p_k, k) are sorted in LDS (bitonic), which
+ * gives pred and q from neighbours, and c follows by pointer jumping
+ * (log2 n rounds).  The reverse sequence (inc < 0) is the inverse map.  Any
+ * p_k < k or out of range falls back to replaying the interchanges on row
+ * labels in LDS (the map is then exact for whatever the sequence does,
+ * entries out of range skipped). */
+constexpr int PERM_NT = 1024;
+__global__ __launch_bounds__(PERM_NT) void laswp_perm_kernel(int m, int nr, int npow,
+                                                             const int *ipiv, int k1, int inc,
+                                                             int *src, int *fwd) {
+  extern __shared__ unsigned s_dyn[];
+  unsigned *key = s_dyn;                          /* npow sorted (p << 16 | k) */
+  int *r = reinterpret_cast<int *>(s_dyn + npow); /* nr: row labels, then c() */
+  __shared__ int s_bad;
+  const int tid = threadIdx.x;
+  if (tid == 0) s_bad = 0;
+  __syncthreads();
+  for (int j = tid; j < nr; j += PERM_NT) r[j] = j;
+  for (int k = tid; k < npow; k += PERM_NT) {
+    unsigned v = 0xffffffffu;
+    if (k < m) {
+      const int p = ipiv[k1 + k] - k1;
+      if (p < k || p >= nr) s_bad = 1;
+      else if (p != k) v = ((unsigned)p << 16) | (unsigned)k;
+    }
+    key[k] = v;
+  }
+  __syncthreads();
+  if (s_bad) {
+    if (tid == 0) {
+      for (int t = 0; t < m; ++t) {
+        const int k = inc > 0 ? t : m - 1 - t;
+        const int p = ipiv[k1 + k] - k1;
+        if (p >= 0 && p < nr && p != k) { const int x = r[k]; r[k] = r[p]; r[p] = x; }
+      }
+    }
+    __syncthreads();
+    for (int j = tid; j < nr; j += PERM_NT) src[j] = r[j];
+    return;
+  }
+  for (int size = 2; size <= npow; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int i = tid; i < npow / 2; i += PERM_NT) {
+        const int lo = 2 * stride * (i / stride) + (i % stride), hi = lo + stride;
+        const bool up = (lo & size) == 0;
+        const unsigned a = key[lo], b = key[hi];
+        if ((a > b) == up) { key[lo] = b; key[hi] = a; }
+      }
+      __syncthreads();
+    }
+  /* pred(p) = the largest k of p's run */
+  for (int s = tid; s < m; s += PERM_NT) {
+    const unsigned v = key[s];
+    if (v == 0xffffffffu) continue;
+    const unsigned nx = s + 1 < m ? key[s + 1] : 0xffffffffu;
+    if (nx == 0xffffffffu || (nx >> 16) != (v >> 16)) r[v >> 16] = (int)(v & 0xffffu);
+  }
+  __syncthreads();
+  /* r[j] = pred(j) or j; follow to the root: c(j) */
+  for (int len = 1; len < nr; len <<= 1) {
+    for (int j = tid; j < nr; j += PERM_NT) r[j] = r[r[j]];
+    __syncthreads();
+  }
+  int *out = inc > 0 ? src : fwd;
+  for (int j = tid; j < nr; j += PERM_NT) {
+    const bool moved = j < m && (ipiv[k1 + j] - k1) != j;
+    if (!moved) out[j] = r[j];
+  }
+  for (int s = tid; s < m; s += PERM_NT) {
+    const unsigned v = key[s];
+    if (v == 0xffffffffu) continue;
+    const int p = (int)(v >> 16), k = (int)(v & 0xffffu);
+    const unsigned pv = s > 0 ? key[s - 1] : 0xffffffffu;
+    out[k] = (pv != 0xffffffffu && (int)(pv >> 16) == p) ? r[pv & 0xffffu] : p;
+  }
+  if (inc < 0) {
+    __syncthreads();
+    for (int j = tid; j < nr; j += PERM_NT) src[fwd[j]] = j;
+  }
+}
+
+/* tmp(i, c) = A(k1 + src[i], c), then A(k1 + i, c) = tmp(i, c) */
+template <typename T>
+__global__ void gather_rows_kernel(int nr, const T *A, long lda, int k1, const int *src,
+                                   T *tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (i < nr) tmp[(size_t)c * nr + i] = A[(size_t)c * lda + k1 + src[i]];
+}
+template <typename T>
+__global__ void copy_rows_kernel(int nr, T *A, long lda, int k1, const T *tmp) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (i < nr) A[(size_t)c * lda + k1 + i] = tmp[(size_t)c * nr + i];
+}
+
+template <typename T>
+hipError_t launch_permute_rows(int N, T *A, long lda, int k1, int nr, const int *src, T *tmp,
+                               hipStream_t s) {
+  if (N <= 0 || nr <= 0) return hipSuccess;
+  if (N > 65535 || lda < k1 + nr || !src || !tmp) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)((nr + 255) / 256), (unsigned)N);
+  hipLaunchKernelGGL(gather_rows_kernel<T>, grid, dim3(256), 0, s, nr, A, lda, k1, src, tmp);
+  hipLaunchKernelGGL(copy_rows_kernel<T>, grid, dim3(256), 0, s, nr, A, lda, k1, tmp);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_laswp(int N, T *A, long lda, int k1, int k2, const int *ipiv, int inc,
                         int rows_end, hipStream_t s) {
@@ -590,4 +704,34 @@ extern "C" hipError_t atlmx_k_dlaswp(int N, double *A, long lda, int k1, int k2,
 extern "C" hipError_t atlmx_k_slaswp(int N, float *A, long lda, int k1, int k2,
                                      const int *ipiv, int inc, int rows_end, hipStream_t s) {
   return launch_laswp<float>(N, A, lda, k1, k2, ipiv, inc, rows_end, s);
+}
+
+extern "C" hipError_t atlmx_k_laswp_perm(int k1, int k2, int rows_end, const int *ipiv, int inc,
+                                         int *src, int *tmp, hipStream_t s) {
+  static int attr_set = 0;
+  const int m = k2 - k1, nr = rows_end - k1;
+  if (nr <= 0) return hipSuccess;
+  if (k1 < 0 || m < 0 || m > nr || nr > ATLMX_LASWP_PERM_MAXROWS || !ipiv || !src || !tmp)
+    return hipErrorInvalidValue;
+  int npow = 1;
+  while (npow < (m > 1 ? m : 1)) npow <<= 1;
+  const size_t lds = (size_t)(npow + nr) * sizeof(int);
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute((const void *)laswp_perm_kernel,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       2 * ATLMX_LASWP_PERM_MAXROWS * (int)sizeof(int));
+    if (e != hipSuccess) return e;
+    attr_set = 1;
+  }
+  hipLaunchKernelGGL(laswp_perm_kernel, dim3(1), dim3(PERM_NT), lds, s, m, nr, npow, ipiv, k1,
+                     inc, src, tmp);
+  return hipGetLastError();
+}
+extern "C" hipError_t atlmx_k_dpermute_rows(int N, double *A, long lda, int k1, int nr,
+                                            const int *src, double *tmp, hipStream_t s) {
+  return launch_permute_rows<double>(N, A, lda, k1, nr, src, tmp, s);
+}
+extern "C" hipError_t atlmx_k_spermute_rows(int N, float *A, long lda, int k1, int nr,
+                                            const int *src, float *tmp, hipStream_t s) {
+  return launch_permute_rows<float>(N, A, lda, k1, nr, src, tmp, s);
 }

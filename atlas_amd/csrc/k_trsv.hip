@@ -273,6 +273,220 @@ hipError_t launch_trsv_tag(int uplo, int trans, int n, int nrhs, const T *A, lon
   return hipGetLastError();
 }
 
+/* ---- many right-hand sides ----------------------------------------------
+ * op(A) X = B with up to ATLMX_TRSM_TAG_MAXRHS columns in ONE launch, the
+ * case getrs/potrs with tens of right-hand sides hit: a chain of 2 * nblk
+ * small GEMM launches otherwise.  The work item is (R = 2 consecutive
+ * 64-row blocks in solve order, group of RC = 16 columns); items are taken
+ * from the atomic counter in solve order, group-minor, so every item an item
+ * waits for was claimed by a running workgroup.  An item subtracts
+ * op(A)_kj X_j from both its blocks for every earlier block j as X_j's
+ * columns of its group are published, each 64x64 by 64x16 product on the
+ * matrix cores (wavefront w owns rows 16w..16w+15; the MFMA's four k lane
+ * groups take k = 16 lk + t in step t).  Then it solves its first block
+ * (B_k minus the sum, times the inverted diagonal block), publishes it,
+ * applies it to its second block inside the workgroup and solves that one:
+ * two blocks per cross-workgroup hand-over, which is what bounds the chain.
+ * Solutions are published as tagged words (as in the few-RHS solve) and
+ * stored in place; a published block is staged in LDS once per workgroup
+ * and read by all four wavefronts. */
+constexpr int RC = 16;
+constexpr int RB = 2;     /* 64-row blocks per work item */
+
+template <typename T> struct TMfma;
+template <> struct TMfma<double> {
+  typedef double acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  /* v_mfma_f64_16x16x4f64: lane l, element r holds D[(l>>4) + 4r][l&15] */
+  __device__ static int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct TMfma<float> {
+  typedef float acc_t __attribute__((ext_vector_type(4)));
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  /* v_mfma_f32_16x16x4f32: lane l, element r holds D[4(l>>4) + r][l&15] */
+  __device__ static int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+template <typename T>
+__global__ __launch_bounds__(NT) void trsm_tag_kernel(int n, int nrhs, int lower, int trans,
+                                                      const T *__restrict__ A, long lda,
+                                                      const T *__restrict__ inv, T *X, long ldx,
+                                                      int *ctl, unsigned long long *pub) {
+  typedef TMfma<T> MF;
+  typedef typename MF::acc_t acc_t;
+  constexpr int W = Tag<T>::W;
+  __shared__ int s_idx;
+  __shared__ T s_x[2][RB][TB][RC + 1];            /* an earlier item's blocks of X, [row][col] */
+  __shared__ T s_c[TB][RC + 1];                   /* B_k minus the partial sums */
+  __shared__ T s_iv[RB][TB][TB + 1];              /* op(inv_k)(row, k) stored [k][row] */
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int nblk = (n + TB - 1) / TB;
+  const int ngrp = (nrhs + RC - 1) / RC;
+  const int nitem = (nblk + RB - 1) / RB;
+  const bool fwd = (lower != 0) != (trans != 0);
+  if (tid == 0) s_idx = atomicAdd(&ctl[0], 1);
+  __syncthreads();
+  const int ticket = s_idx;
+  if (ticket >= nitem * ngrp) return;
+  const int item = ticket / ngrp, g = ticket % ngrp;
+  const int o0 = item * RB;                       /* solve-order position of block 0 */
+  const int nbh = min(RB, nblk - o0);             /* blocks in this item */
+  auto blk = [&](int o) { return fwd ? o : nblk - 1 - o; };
+  int kb[RB], rbk[RB];
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    kb[b] = blk(min(o0 + b, nblk - 1));
+    rbk[b] = b < nbh ? min(TB, n - kb[b] * TB) : 0;
+  }
+  const int q0 = g * RC, qb = min(RC, nrhs - q0);
+  const int li = lane & 15, lk = lane >> 4;
+  const int arow = wave * 16 + li;                /* row this lane gives the A operand */
+
+  /* the item's inverted diagonal blocks, staged once */
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    const T *ik = inv + (size_t)kb[b] * TB * TB;
+    for (int e = tid; e < TB * TB; e += NT) {
+      const int r = e % TB, kk = e / TB;          /* op(inv)(r, kk) */
+      s_iv[b][kk][r] = trans ? ik[(size_t)r * TB + kk] : ik[(size_t)kk * TB + r];
+    }
+  }
+  /* op(A)(64 kr + arow, 64 j + 16 lk + t), t = 0..15: the lane's elements
+   * of tile (kr, j); zero outside the matrix or for a missing block */
+  auto load_a = [&](int kr, int rb, int j, T (&a)[16]) {
+    const int r0 = kr * TB, c0 = j * TB, cb = min(TB, n - c0);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int kk = 16 * lk + t;
+      const bool ok = arow < rb && kk < cb;
+      a[t] = !ok ? T(0)
+                 : (trans ? A[(size_t)(r0 + arow) * lda + c0 + kk]
+                          : A[(size_t)(c0 + kk) * lda + r0 + arow]);
+    }
+  };
+  /* A(k1, k0), for the second block once the first is solved */
+  T a10[16];
+  load_a(kb[1], rbk[1], kb[0], a10);
+
+  acc_t acc[RB];
+#pragma unroll
+  for (int b = 0; b < RB; ++b) acc[b] = acc_t{0, 0, 0, 0};
+  const int pr = tid >> 2, pc = (tid & 3) * 4;    /* this thread's 4 entries of a block */
+  /* every earlier item: both of its blocks polled at once (they are
+   * published within a few microseconds of each other), the four tiles'
+   * operands loaded meanwhile */
+  for (int i = 0; i < item; ++i) {
+    const int par = i & 1;
+    int jb[RB], cbj[RB];
+#pragma unroll
+    for (int jj = 0; jj < RB; ++jj) {
+      jb[jj] = blk(i * RB + jj);                  /* i < item, so both blocks exist */
+      cbj[jj] = min(TB, n - jb[jj] * TB);
+    }
+    T a[RB][RB][16];
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+#pragma unroll
+      for (int jj = 0; jj < RB; ++jj) load_a(kb[b], rbk[b], jb[jj], a[b][jj]);
+    unsigned long long w[RB][4 * W];
+#pragma unroll
+    for (int jj = 0; jj < RB; ++jj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int h = 0; h < W; ++h) {
+          const bool ok = pr < cbj[jj] && pc + e < qb;
+          w[jj][e * W + h] =
+              ok ? __hip_atomic_load(&pub[((size_t)(q0 + pc + e) * n + jb[jj] * TB + pr) * W + h],
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                 : (1ull << 32);
+        }
+    for (;;) {
+      bool done = true;
+#pragma unroll
+      for (int jj = 0; jj < RB; ++jj)
+#pragma unroll
+        for (int t = 0; t < 4 * W; ++t) done = done && (w[jj][t] != 0ull);
+      if (done) break;
+      __builtin_amdgcn_s_sleep(1);
+#pragma unroll
+      for (int jj = 0; jj < RB; ++jj)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+          for (int h = 0; h < W; ++h)
+            if (w[jj][e * W + h] == 0ull)
+              w[jj][e * W + h] = __hip_atomic_load(
+                  &pub[((size_t)(q0 + pc + e) * n + jb[jj] * TB + pr) * W + h], __ATOMIC_RELAXED,
+                  __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int jj = 0; jj < RB; ++jj)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        s_x[par][jj][pr][pc + e] =
+            (pr < cbj[jj] && pc + e < qb) ? Tag<T>::get(&w[jj][e * W]) : T(0);
+    __syncthreads();
+#pragma unroll
+    for (int jj = 0; jj < RB; ++jj)
+#pragma unroll
+      for (int t = 0; t < 16; ++t) {
+        const T xv = s_x[par][jj][16 * lk + t][li];
+#pragma unroll
+        for (int b = 0; b < RB; ++b) acc[b] = MF::mma(a[b][jj][t], xv, acc[b]);
+      }
+  }
+  /* block b: c = B_k - sum, X_k = op(inv_k) c, published; block 0's
+   * solution then enters block 1's sum (s_x[0][0] is free: the last
+   * iteration above read s_x[par] before the barrier below) */
+#pragma unroll
+  for (int b = 0; b < RB; ++b) {
+    if (b >= nbh) break;
+    const int r0 = kb[b] * TB, rb = rbk[b];
+    if (b > 0) {
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 16; ++t) acc[b] = MF::mma(a10[t], s_x[0][0][16 * lk + t][li], acc[b]);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = wave * 16 + MF::drow(lane, r);
+      const T bv = (row < rb && li < qb) ? X[(size_t)(q0 + li) * ldx + r0 + row] : T(0);
+      s_c[row][li] = bv - acc[b][r];
+    }
+    __syncthreads();
+    acc_t y = {0, 0, 0, 0};
+#pragma unroll
+    for (int t = 0; t < 16; ++t) y = MF::mma(s_iv[b][16 * lk + t][arow], s_c[16 * lk + t][li], y);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int row = wave * 16 + MF::drow(lane, r);
+      if (row < rb && li < qb) {
+        X[(size_t)(q0 + li) * ldx + r0 + row] = y[r];
+        Tag<T>::put(&pub[((size_t)(q0 + li) * n + r0 + row) * W], y[r]);
+      }
+      if (b + 1 < nbh) s_x[0][0][row][li] = (row < rb && li < qb) ? y[r] : T(0);
+    }
+  }
+}
+
+template <typename T>
+hipError_t launch_trsm_tag(int uplo, int trans, int n, int nrhs, const T *A, long lda,
+                           const T *inv, T *X, long ldx, int *ctl, hipStream_t s) {
+  if (n <= 0 || nrhs <= 0) return hipSuccess;
+  if (nrhs > ATLMX_TRSM_TAG_MAXRHS || lda < n || ldx < n || !ctl || ((uintptr_t)ctl & 7))
+    return hipErrorInvalidValue;
+  const int nitem = (n + RB * TB - 1) / (RB * TB), ngrp = (nrhs + RC - 1) / RC;
+  unsigned long long *pub = reinterpret_cast<unsigned long long *>(ctl + 2);
+  hipLaunchKernelGGL(trsm_tag_kernel<T>, dim3((unsigned)(nitem * ngrp)), dim3(NT), 0, s, n, nrhs,
+                     uplo == KX_LOWER, trans == KX_TRANS, A, lda, inv, X, ldx, ctl, pub);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 extern "C" hipError_t atlmx_k_dtrsv(int uplo, int trans, int n, int nrhs, const double *A,
@@ -289,4 +503,21 @@ extern "C" hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const 
 /* block counter (and padding to 8 bytes), then the tagged words */
 extern "C" size_t atlmx_k_trsv_ctl_ints(int n, int dbl) {
   return 2 + (size_t)2 * MAXR * n * (dbl ? 2 : 1);
+}
+
+extern "C" hipError_t atlmx_k_dtrsm_tag(int uplo, int trans, int n, int nrhs, const double *A,
+                                        long lda, const double *inv, double *X, long ldx,
+                                        int *ctl, hipStream_t s) {
+  return launch_trsm_tag<double>(uplo, trans, n, nrhs, A, lda, inv, X, ldx, ctl, s);
+}
+extern "C" hipError_t atlmx_k_strsm_tag(int uplo, int trans, int n, int nrhs, const float *A,
+                                        long lda, const float *inv, float *X, long ldx,
+                                        int *ctl, hipStream_t s) {
+  return launch_trsm_tag<float>(uplo, trans, n, nrhs, A, lda, inv, X, ldx, ctl, s);
+}
+
+/* block counter (and padding to 8 bytes), then the tagged words of every
+ * right-hand side */
+extern "C" size_t atlmx_k_trsm_tag_ctl_ints(int n, int nrhs, int dbl) {
+  return 2 + (size_t)2 * nrhs * n * (dbl ? 2 : 1);
 }

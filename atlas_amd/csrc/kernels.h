@@ -76,6 +76,22 @@ hipError_t atlmx_k_strsv(int uplo, int trans, int n, int nrhs, const float *A,
                          long lda, const float *inv, float *X, long ldx,
                          int *ctl, hipStream_t s);
 
+/* The same for up to ATLMX_TRSM_TAG_MAXRHS right-hand sides (any number of
+ * 64-row blocks; work item = one block of 16 columns, products on the matrix
+ * cores).  ctl: atlmx_k_trsm_tag_ctl_ints(n, nrhs, is_double) ints, 8-byte
+ * aligned, zero on entry.  The columns are independent, so a solve split
+ * into launches over column ranges gives the same result. */
+#define ATLMX_TRSM_TAG_MAXRHS 256
+/* columns per launch when the caller brings the workspace */
+#define ATLMX_TRSM_TAG_CHUNK 64
+size_t atlmx_k_trsm_tag_ctl_ints(int n, int nrhs, int dbl);
+hipError_t atlmx_k_dtrsm_tag(int uplo, int trans, int n, int nrhs, const double *A,
+                             long lda, const double *inv, double *X, long ldx,
+                             int *ctl, hipStream_t s);
+hipError_t atlmx_k_strsm_tag(int uplo, int trans, int n, int nrhs, const float *A,
+                             long lda, const float *inv, float *X, long ldx,
+                             int *ctl, hipStream_t s);
+
 /* Tournament pivoting (FP32) on S = A^T (column-major, A(i,j) = S[j+i*lds],
  * lds and k multiples of 4, S 16-byte aligned): chooses the w <=
  * ATLMX_TSLU_W pivot rows of the panel rows k..n-1, columns k..k+w-1,
@@ -247,6 +263,19 @@ hipError_t atlmx_k_dlaswp(int N, double *A, long lda, int k1, int k2,
                           const int *ipiv, int inc, int rows_end, hipStream_t s);
 hipError_t atlmx_k_slaswp(int N, float *A, long lda, int k1, int k2,
                           const int *ipiv, int inc, int rows_end, hipStream_t s);
+
+/* The same interchanges as one permutation of rows [k1, rows_end)
+ * (rows_end-k1 <= ATLMX_LASWP_PERM_MAXROWS): src[i] = the row (relative to
+ * k1) whose contents end up in row k1+i; tmp: rows_end-k1 ints of scratch.
+ * permute_rows then applies such a map to N columns through tmp
+ * (rows_end-k1 by N elements). */
+#define ATLMX_LASWP_PERM_MAXROWS 16384
+hipError_t atlmx_k_laswp_perm(int k1, int k2, int rows_end, const int *ipiv, int inc,
+                              int *src, int *tmp, hipStream_t s);
+hipError_t atlmx_k_dpermute_rows(int N, double *A, long lda, int k1, int nr,
+                                 const int *src, double *tmp, hipStream_t s);
+hipError_t atlmx_k_spermute_rows(int N, float *A, long lda, int k1, int nr,
+                                 const int *src, float *tmp, hipStream_t s);
 
 /* Forward interchanges k1..k2 (k2-k1 <= 2*ATLMX_PANEL_MAXW) of rows below
  * rows_end (rows_end-k1 <= ATLMX_LASWP_PLAN_MAXROWS) as a move list:

@@ -22,6 +22,7 @@
 #define PFX d
 #define K_PANEL atlmx_k_dgetf2_panel
 #define K_LASWP atlmx_k_dlaswp
+#define K_PERMUTE_ROWS atlmx_k_dpermute_rows
 #define K_LASWP_APPLY atlmx_k_dlaswp_apply
 #define K_POTF2 atlmx_k_dpotf2
 #define K_TRICOPY atlmx_k_dtricopy
@@ -34,6 +35,7 @@
 #undef PFX
 #undef K_PANEL
 #undef K_LASWP
+#undef K_PERMUTE_ROWS
 #undef K_LASWP_APPLY
 #undef K_POTF2
 #undef K_TRICOPY
@@ -46,6 +48,7 @@
 #define PFX s
 #define K_PANEL atlmx_k_sgetf2_panel
 #define K_LASWP atlmx_k_slaswp
+#define K_PERMUTE_ROWS atlmx_k_spermute_rows
 #define K_LASWP_APPLY atlmx_k_slaswp_apply
 #define K_POTF2 atlmx_k_spotf2
 #define K_TRICOPY atlmx_k_stricopy

@@ -150,13 +150,37 @@ int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const cha
  * solve; Trans: U^T solve, L^T-unit solve, P^T*B.  inv / ctl: NULL, or the
  * two TRSMs' workspaces (see trsm_dev_ws; the solves run one after the
  * other on the stream, so they share them). */
+/* The row interchanges of getrs on the right-hand sides (ATL_getrs.c's
+ * ATL_laswp calls).  Replayed one by one they are a chain of N dependent
+ * swaps; here, for up to ATLMX_LASWP_PERM_MAXROWS rows, they become one
+ * permutation computed in parallel and a gather of B's rows (through a
+ * workspace, at most 1024 columns at a time).  The allocation-free solve
+ * (with caller workspaces) keeps the replay; both move the same values. */
+static void NM(PFX, laswp_rhs)(atlmx_ctx *c, int NRHS, T *B, long ldb, int N,
+                               const int *d_ipiv, int inc, int ws_given, const char *rout)
+{
+  if (!ws_given && N <= ATLMX_LASWP_PERM_MAXROWS) {
+    const int cw = NRHS < 1024 ? NRHS : 1024;
+    int *src = (int *)rt_ws(c, WS_LASWP, 2 * (size_t)N * sizeof(int), rout);
+    T *tmp = (T *)rt_ws(c, WS_LASWP_B, (size_t)N * cw * sizeof(T), rout);
+    int q;
+    RT_CHECK(rout, atlmx_k_laswp_perm(0, N, N, d_ipiv, inc, src, src + N, c->stream));
+    for (q = 0; q < NRHS; q += cw) {
+      const int nq = NRHS - q < cw ? NRHS - q : cw;
+      RT_CHECK(rout, K_PERMUTE_ROWS(nq, B + (size_t)q * ldb, ldb, 0, N, src, tmp, c->stream));
+    }
+  } else {
+    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, inc, N, c->stream));
+  }
+}
+
 void NM(PFX, getrs_dev_ws)(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
                            const int *d_ipiv, T *B, long ldb, T *inv, int *ctl,
                            const char *rout)
 {
   if (N <= 0 || NRHS <= 0) return;
   if (trans == KX_NOTRANS) {
-    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, 1, N, c->stream));
+    NM(PFX, laswp_rhs)(c, NRHS, B, ldb, N, d_ipiv, 1, inv != NULL, rout);
     NM(PFX, trsm_dev_ws)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
                          ldb, NULL, inv, ctl, rout);
     NM(PFX, trsm_dev_ws)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, (T)1, A, lda,
@@ -166,7 +190,7 @@ void NM(PFX, getrs_dev_ws)(atlmx_ctx *c, int trans, int N, int NRHS, const T *A,
                          ldb, NULL, inv, ctl, rout);
     NM(PFX, trsm_dev_ws)(c, KX_LEFT, KX_LOWER, KX_TRANS, KX_UNIT, N, NRHS, (T)1, A, lda, B,
                          ldb, NULL, inv, ctl, rout);
-    RT_CHECK(rout, K_LASWP(NRHS, B, ldb, 0, N, d_ipiv, -1, N, c->stream));
+    NM(PFX, laswp_rhs)(c, NRHS, B, ldb, N, d_ipiv, -1, inv != NULL, rout);
   }
 }
 

@@ -5,8 +5,9 @@
  * hipMalloc).  It has no counterpart in ATLAS.
  *
  * atlmx_dgetrs needs two scratch buffers: the inverted 64 x 64 diagonal
- * blocks of the triangular factor (nblk * 64 * 64 doubles) and the
- * few-right-hand-side solve's block counter and published solution words.  It
+ * blocks of the triangular factor (nblk * 64 * 64 doubles) and the chained
+ * solve's block counter and published solution words (for up to 64
+ * right-hand sides; more run 64 per launch, with the same result).  It
  * takes them from the calling thread's grow-only workspaces.  Here they are
  * slices of one file-scope device array, `scratch`, sized for N <=
  * ATLMX_NOALLOC_MAX_N.  Each GPU holds its own copy of the array until the
@@ -37,9 +38,14 @@ constexpr int NB = ATLMX_TRI_NB;                                     /* 64 */
 constexpr size_t INV_BYTES =
     (size_t)((ATLMX_NOALLOC_MAX_N + NB - 1) / NB) * NB * NB * sizeof(double);
 /* the solve's block counter and tagged words: atlmx_k_trsv_ctl_ints(N, 1)
- * ints, 2 + 4 * ATLMX_TRSV_MAXRHS * N for FP64 */
+ * ints (2 + 4 * ATLMX_TRSV_MAXRHS * N) for a few right-hand sides,
+ * atlmx_k_trsm_tag_ctl_ints(N, ATLMX_TRSM_TAG_CHUNK, 1) ints
+ * (2 + 4 * ATLMX_TRSM_TAG_CHUNK * N) for up to ATLMX_TRSM_TAG_MAXRHS, which
+ * are then solved that many columns per launch */
+constexpr size_t CTL_RHS =
+    ATLMX_TRSM_TAG_CHUNK > ATLMX_TRSV_MAXRHS ? ATLMX_TRSM_TAG_CHUNK : ATLMX_TRSV_MAXRHS;
 constexpr size_t CTL_BYTES =
-    ((size_t)(2 + 4 * ATLMX_TRSV_MAXRHS * (size_t)ATLMX_NOALLOC_MAX_N) * sizeof(int) + 255) &
+    ((size_t)(2 + 4 * CTL_RHS * (size_t)ATLMX_NOALLOC_MAX_N) * sizeof(int) + 255) &
     ~(size_t)255;
 constexpr size_t SCRATCH_BYTES = INV_BYTES + CTL_BYTES;
 constexpr int MAX_DEV = 64;

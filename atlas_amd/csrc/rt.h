@@ -36,6 +36,8 @@ enum {
   WS_QR,            /* QR: panel partial sums, larfg coefficients */
   WS_QRV,           /* QR: V, V^T V, T and the two W blocks     */
   WS_TAU,           /* QR: device copy of TAU                   */
+  WS_LASWP,         /* getrs: row-interchange gather map        */
+  WS_LASWP_B,       /* getrs: right-hand sides being permuted   */
   WS_NSLOTS
 };
 

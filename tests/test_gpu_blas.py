@@ -116,6 +116,34 @@ def test_trsm_all_variants_vs_oracle(dt):
 
 
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
+def test_trsm_many_rhs_vs_oracle(dt):
+    """Left-side solves with 5..256 right-hand sides (the one-launch path of
+    64x16 work items on the matrix cores): every uplo/trans/diag, ragged
+    block and column-group counts, a gap between M and ldb that must stay
+    untouched, and more work items than one wave of the chip holds."""
+    for uplo, ta, diag in itertools.product([X.Upper, X.Lower], [X.NoTrans, X.Trans],
+                                            [X.NonUnit, X.Unit]):
+        for M, N in ((65, 5), (128, 16), (257, 17), (1000, 64), (2100, 256), (12000, 40)):
+            if M == 12000 and (diag == X.Unit or dt == np.float32):
+                continue
+            A = _gen((M, M), dt) * np.asarray(1.0 / M, dt)
+            A[np.diag_indices(M)] = np.asarray(1000.0, dt) if diag == X.Unit else \
+                np.asarray(1.0, dt) + np.abs(RNG.uniform(-0.5, 0.5, M)).astype(dt)
+            B0 = _gen((M, N), dt)
+            Bbuf = _f(np.full((M + 3, N), 7.0, dt))
+            Bbuf[:M] = B0
+            Xg = X.trsm(X.Left, uplo, ta, diag, 0.75, A, Bbuf[:M])
+            assert np.all(Bbuf[M:] == 7.0)
+            r = O.trsm_resid(X.Left, uplo, ta, diag, 0.75, A, Xg, B0)
+            # as for a few right-hand sides: two roundings per block solution
+            assert r < 2.0, (uplo, ta, diag, M, N, r)
+            if M <= 1000:
+                Xo = O.trsm(X.Left, uplo, ta, diag, 0.75, A, B0.copy(order="F"))
+                tol = 1e-10 if dt == np.float64 else 2e-3
+                assert np.allclose(Xg, Xo, rtol=tol, atol=tol), (uplo, ta, diag, M, N)
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
 def test_trsm_few_rhs_vs_oracle(dt):
     """Left-side solves with 1..4 right-hand sides (the one-launch
     flag-chained path): every uplo/trans/diag, ragged block counts, a gap

@@ -87,6 +87,44 @@ def test_getrs_slvtst(nrhs, trans):
     assert solve_ok(O.genrm1(A0), Xc, B)
 
 
+def _replay_swaps(ipiv, B, forward):
+    """ATLAS's laswp on the rows of B: interchange k with ipiv[k] in order
+    (forward) or in reverse order."""
+    B = B.copy()
+    ks = range(len(ipiv)) if forward else range(len(ipiv) - 1, -1, -1)
+    for k in ks:
+        p = int(ipiv[k])
+        if p != k:
+            B[[k, p]] = B[[p, k]]
+    return B
+
+
+@pytest.mark.parametrize("dt", [np.float64, np.float32])
+@pytest.mark.parametrize("kind", ["lu_like", "repeated_targets", "below_diagonal"])
+def test_getrs_row_interchanges_match_replay(dt, kind):
+    """getrs with the identity as its LU factor is its row interchanges
+    alone: P B (NoTrans, forward order) and P^T B (Trans, reverse order),
+    compared exactly with replaying the interchanges one by one.  LU-like
+    ipiv (ipiv[k] >= k) takes the parallel permutation; targets hit many
+    times stress its chains; an interchange with a row above (allowed by
+    laswp, never made by getrf) takes the replay inside the same kernel."""
+    rng = np.random.default_rng(11)
+    for n in (1, 2, 65, 700, 4097):
+        if kind == "lu_like":
+            ipiv = np.array([rng.integers(k, n) for k in range(n)], np.int32)
+        elif kind == "repeated_targets":
+            ipiv = np.array([max(k, n - 1 - (k % 3)) if rng.random() < 0.7 else k
+                             for k in range(n)], np.int32)
+        else:
+            ipiv = np.array([rng.integers(0, n) for k in range(n)], np.int32)
+        B0 = rng.uniform(-1, 1, (n, 5)).astype(dt)
+        eye = np.asfortranarray(np.eye(n, dtype=dt))
+        for trans, forward in ((X.NoTrans, True), (X.Trans, False)):
+            B = _f(B0.copy())
+            X.getrs(eye, ipiv, B, trans=trans)
+            assert np.array_equal(B, _replay_swaps(ipiv, B0, forward)), (n, trans)
+
+
 def test_gesv_rowmajor():
     n, nrhs = 300, 3
     A0 = np.ascontiguousarray(RNG.uniform(-1, 1, (n, n)))

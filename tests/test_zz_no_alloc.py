@@ -55,7 +55,7 @@ def _solve_both(torch, A0, Ad, ipiv, nrhs, trans, seed):
     assert solve_ok(O.genrm1(A0), Xc, np.asfortranarray(xn.T))
 
 
-@pytest.mark.parametrize("nrhs", [1, 3, 16])
+@pytest.mark.parametrize("nrhs", [1, 3, 16, 70])     # 70: two launches of the chained solve
 @pytest.mark.parametrize("trans", [X.NoTrans, X.Trans], ids=["N", "T"])
 def test_noalloc_matches_getrs_two_calls(torch, nrhs, trans):
     for i, n in enumerate((1500, 333)):              # the second call is smaller
