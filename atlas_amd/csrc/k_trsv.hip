@@ -323,6 +323,7 @@ __global__ __launch_bounds__(NT) void trsm_tag_kernel(int n, int nrhs, int lower
   __shared__ T s_x[2][RB][TB][RC + 1];            /* an earlier item's blocks of X, [row][col] */
   __shared__ T s_c[TB][RC + 1];                   /* B_k minus the partial sums */
   __shared__ T s_iv[RB][TB][TB + 1];              /* op(inv_k)(row, k) stored [k][row] */
+  __shared__ T s_b[RB][TB][RC + 1];               /* the item's right-hand sides */
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int nblk = (n + TB - 1) / TB;
   const int ngrp = (nrhs + RC - 1) / RC;
@@ -346,7 +347,8 @@ __global__ __launch_bounds__(NT) void trsm_tag_kernel(int n, int nrhs, int lower
   const int li = lane & 15, lk = lane >> 4;
   const int arow = wave * 16 + li;                /* row this lane gives the A operand */
 
-  /* the item's inverted diagonal blocks, staged once */
+  /* the item's inverted diagonal blocks and right-hand sides, staged once
+   * (off the chain from one item to the next) */
 #pragma unroll
   for (int b = 0; b < RB; ++b) {
     const T *ik = inv + (size_t)kb[b] * TB * TB;
@@ -354,7 +356,12 @@ __global__ __launch_bounds__(NT) void trsm_tag_kernel(int n, int nrhs, int lower
       const int r = e % TB, kk = e / TB;          /* op(inv)(r, kk) */
       s_iv[b][kk][r] = trans ? ik[(size_t)r * TB + kk] : ik[(size_t)kk * TB + r];
     }
+    for (int e = tid; e < TB * RC; e += NT) {
+      const int r = e % TB, cc = e / TB;
+      s_b[b][r][cc] = (r < rbk[b] && cc < qb) ? X[(size_t)(q0 + cc) * ldx + kb[b] * TB + r] : T(0);
+    }
   }
+  __syncthreads();
   /* op(A)(64 kr + arow, 64 j + 16 lk + t), t = 0..15: the lane's elements
    * of tile (kr, j); zero outside the matrix or for a missing block */
   auto load_a = [&](int kr, int rb, int j, T (&a)[16]) {
@@ -372,39 +379,34 @@ __global__ __launch_bounds__(NT) void trsm_tag_kernel(int n, int nrhs, int lower
   T a10[16];
   load_a(kb[1], rbk[1], kb[0], a10);
 
-  acc_t acc[RB];
+  /* one accumulator per (block, earlier block of the pair): four
+   * independent MFMA chains (a chain of dependent FP64 MFMAs is bound by
+   * their latency, not their issue rate) */
+  acc_t acc[RB][RB];
 #pragma unroll
-  for (int b = 0; b < RB; ++b) acc[b] = acc_t{0, 0, 0, 0};
+  for (int b = 0; b < RB; ++b)
+#pragma unroll
+    for (int jj = 0; jj < RB; ++jj) acc[b][jj] = acc_t{0, 0, 0, 0};
   const int pr = tid >> 2, pc = (tid & 3) * 4;    /* this thread's 4 entries of a block */
-  /* every earlier item: both of its blocks polled at once (they are
-   * published within a few microseconds of each other), the four tiles'
-   * operands loaded meanwhile */
-  for (int i = 0; i < item; ++i) {
-    const int par = i & 1;
-    int jb[RB], cbj[RB];
-#pragma unroll
-    for (int jj = 0; jj < RB; ++jj) {
-      jb[jj] = blk(i * RB + jj);                  /* i < item, so both blocks exist */
-      cbj[jj] = min(TB, n - jb[jj] * TB);
-    }
-    T a[RB][RB][16];
-#pragma unroll
-    for (int b = 0; b < RB; ++b)
-#pragma unroll
-      for (int jj = 0; jj < RB; ++jj) load_a(kb[b], rbk[b], jb[jj], a[b][jj]);
+  /* stage blocks [j0, j1) of earlier item i in s_x[par] (the thread's
+   * words all requested at once, only missing ones polled again; entries
+   * outside a block read as zero), then the products with them */
+  auto consume = [&](int i, int par, int j0, int j1, const T (&a)[RB][RB][16]) {
     unsigned long long w[RB][4 * W];
 #pragma unroll
-    for (int jj = 0; jj < RB; ++jj)
+    for (int jj = 0; jj < RB; ++jj) {
+      const int jbk = blk(i * RB + jj), cb = min(TB, n - jbk * TB);
 #pragma unroll
       for (int e = 0; e < 4; ++e)
 #pragma unroll
         for (int h = 0; h < W; ++h) {
-          const bool ok = pr < cbj[jj] && pc + e < qb;
+          const bool ok = jj >= j0 && jj < j1 && pr < cb && pc + e < qb;
           w[jj][e * W + h] =
-              ok ? __hip_atomic_load(&pub[((size_t)(q0 + pc + e) * n + jb[jj] * TB + pr) * W + h],
+              ok ? __hip_atomic_load(&pub[((size_t)(q0 + pc + e) * n + jbk * TB + pr) * W + h],
                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                  : (1ull << 32);
         }
+    }
     for (;;) {
       bool done = true;
 #pragma unroll
@@ -414,54 +416,84 @@ __global__ __launch_bounds__(NT) void trsm_tag_kernel(int n, int nrhs, int lower
       if (done) break;
       __builtin_amdgcn_s_sleep(1);
 #pragma unroll
-      for (int jj = 0; jj < RB; ++jj)
+      for (int jj = 0; jj < RB; ++jj) {
+        const int jbk = blk(i * RB + jj);
 #pragma unroll
         for (int e = 0; e < 4; ++e)
 #pragma unroll
           for (int h = 0; h < W; ++h)
             if (w[jj][e * W + h] == 0ull)
               w[jj][e * W + h] = __hip_atomic_load(
-                  &pub[((size_t)(q0 + pc + e) * n + jb[jj] * TB + pr) * W + h], __ATOMIC_RELAXED,
+                  &pub[((size_t)(q0 + pc + e) * n + jbk * TB + pr) * W + h], __ATOMIC_RELAXED,
                   __HIP_MEMORY_SCOPE_AGENT);
+      }
     }
 #pragma unroll
-    for (int jj = 0; jj < RB; ++jj)
+    for (int jj = 0; jj < RB; ++jj) {
+      if (jj < j0 || jj >= j1) continue;
+      const int cb = min(TB, n - blk(i * RB + jj) * TB);
 #pragma unroll
       for (int e = 0; e < 4; ++e)
-        s_x[par][jj][pr][pc + e] =
-            (pr < cbj[jj] && pc + e < qb) ? Tag<T>::get(&w[jj][e * W]) : T(0);
+        s_x[par][jj][pr][pc + e] = (pr < cb && pc + e < qb) ? Tag<T>::get(&w[jj][e * W]) : T(0);
+    }
     __syncthreads();
 #pragma unroll
-    for (int jj = 0; jj < RB; ++jj)
+    for (int jj = 0; jj < RB; ++jj) {
+      if (jj < j0 || jj >= j1) continue;
 #pragma unroll
       for (int t = 0; t < 16; ++t) {
         const T xv = s_x[par][jj][16 * lk + t][li];
 #pragma unroll
-        for (int b = 0; b < RB; ++b) acc[b] = MF::mma(a[b][jj][t], xv, acc[b]);
+        for (int b = 0; b < RB; ++b) acc[b][jj] = MF::mma(a[b][jj][t], xv, acc[b][jj]);
       }
+    }
+  };
+  /* every earlier item; both of its blocks polled at once, except for the
+   * item just before, whose first block is used while its second is still
+   * being solved.  The four tiles' operands load meanwhile. */
+  for (int i = 0; i < item; ++i) {
+    const int par = i & 1;
+    T a[RB][RB][16];
+#pragma unroll
+    for (int b = 0; b < RB; ++b)
+#pragma unroll
+      for (int jj = 0; jj < RB; ++jj) load_a(kb[b], rbk[b], blk(i * RB + jj), a[b][jj]);
+    if (i + 1 < item) {
+      consume(i, par, 0, RB, a);
+    } else {
+      consume(i, par, 0, 1, a);
+      consume(i, par, 1, RB, a);
+    }
   }
   /* block b: c = B_k - sum, X_k = op(inv_k) c, published; block 0's
    * solution then enters block 1's sum (s_x[0][0] is free: the last
-   * iteration above read s_x[par] before the barrier below) */
+   * products above read s_x before the barrier below).  Each 16-step
+   * product runs as four chains of four. */
 #pragma unroll
   for (int b = 0; b < RB; ++b) {
     if (b >= nbh) break;
     const int r0 = kb[b] * TB, rb = rbk[b];
+    acc_t sum = acc[b][0];
+#pragma unroll
+    for (int jj = 1; jj < RB; ++jj) sum += acc[b][jj];
     if (b > 0) {
       __syncthreads();
+      acc_t u[4] = {acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}};
 #pragma unroll
-      for (int t = 0; t < 16; ++t) acc[b] = MF::mma(a10[t], s_x[0][0][16 * lk + t][li], acc[b]);
+      for (int t = 0; t < 16; ++t) u[t & 3] = MF::mma(a10[t], s_x[0][0][16 * lk + t][li], u[t & 3]);
+      sum += (u[0] + u[1]) + (u[2] + u[3]);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = wave * 16 + MF::drow(lane, r);
-      const T bv = (row < rb && li < qb) ? X[(size_t)(q0 + li) * ldx + r0 + row] : T(0);
-      s_c[row][li] = bv - acc[b][r];
+      s_c[row][li] = s_b[b][row][li] - sum[r];
     }
     __syncthreads();
-    acc_t y = {0, 0, 0, 0};
+    acc_t yv[4] = {acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}, acc_t{0, 0, 0, 0}};
 #pragma unroll
-    for (int t = 0; t < 16; ++t) y = MF::mma(s_iv[b][16 * lk + t][arow], s_c[16 * lk + t][li], y);
+    for (int t = 0; t < 16; ++t)
+      yv[t & 3] = MF::mma(s_iv[b][16 * lk + t][arow], s_c[16 * lk + t][li], yv[t & 3]);
+    const acc_t y = (yv[0] + yv[1]) + (yv[2] + yv[3]);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int row = wave * 16 + MF::drow(lane, r);
