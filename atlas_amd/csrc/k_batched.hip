@@ -127,6 +127,104 @@ __device__ void tile_update_nn(T *C, long ldc, const T *L, long ldl, const T *U,
 }
 
 /* ---------------------------------------------------------------------- */
+/* Triangular solve of one instance's right-hand side by one workgroup     */
+/* ---------------------------------------------------------------------- */
+/* op(A) x = x for one vector xv in LDS, A's triangle in global memory
+ * (lower: the stored triangle is the lower one; trans: op(A) = A^T; unit:
+ * the diagonal is taken as 1).  Left-looking over blocks of SB rows in solve
+ * order (top first when lower != trans): the block's entries less the
+ * solved entries' contributions (all four wavefronts, a GEMV, then one
+ * barrier), then the SB x SB diagonal triangle -- staged in LDS -- by
+ * wavefront 0 alone, one unknown per lane, each solved value broadcast from
+ * its lane: no barrier per row.  s_blk holds SB x (SB + 1), s_part 8 x SB
+ * elements. */
+constexpr int SB = 32;
+template <typename T>
+__device__ void wg_trsv(int n, const T *A, long lda, bool lower, bool trans, bool unit, T *xv,
+                        T *s_blk, T *s_part) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool fwd = lower != trans;
+  const int nb = (n + SB - 1) / SB;
+  for (int bs = 0; bs < nb; ++bs) {
+    const int b = fwd ? bs : nb - 1 - bs;
+    const int R0 = b * SB, rb = min(SB, n - R0);
+    /* solved range: [0, R0) forward, [R0 + rb, n) backward */
+    const int k0 = fwd ? 0 : R0 + rb, k1 = fwd ? R0 : n;
+    __syncthreads();
+    if (!trans) {
+      /* op(A)(i, k) = A[k lda + i]: thread (row l, chunk q) sums k = k0 + q, k0 + q + 8, ... */
+      constexpr int ST = NT / SB, U = 8;          /* stride, loads in flight */
+      const int l = tid & (SB - 1), q = tid / SB;
+      T part = T(0);
+      if (l < rb) {
+        int k = k0 + q;
+        for (; k + (U - 1) * ST < k1; k += U * ST) {
+          T a[U];
+#pragma unroll
+          for (int u = 0; u < U; ++u) a[u] = A[(size_t)(k + u * ST) * lda + R0 + l];
+#pragma unroll
+          for (int u = 0; u < U; ++u) part += a[u] * xv[k + u * ST];
+        }
+        for (; k < k1; k += ST) part += A[(size_t)k * lda + R0 + l] * xv[k];
+      }
+      s_part[q * SB + l] = part;
+    } else {
+      /* op(A)(i, k) = A[i lda + k]: wavefront w takes rows w, w + 4, ..., lanes run along k */
+      constexpr int RW = SB / (NT / 64);          /* rows per wavefront, all loads issued together */
+      T part[RW];
+#pragma unroll
+      for (int m = 0; m < RW; ++m) part[m] = T(0);
+      for (int k = k0 + lane; k < k1; k += 64) {
+        T a[RW];
+#pragma unroll
+        for (int m = 0; m < RW; ++m) {
+          const int l = wave + m * (NT / 64);
+          a[m] = l < rb ? A[(size_t)(R0 + l) * lda + k] : T(0);
+        }
+        const T xk = xv[k];
+#pragma unroll
+        for (int m = 0; m < RW; ++m) part[m] += a[m] * xk;
+      }
+#pragma unroll
+      for (int m = 0; m < RW; ++m) {
+        T v = part[m];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        if (lane == 0) s_part[wave + m * (NT / 64)] = v;
+      }
+    }
+    /* the diagonal block of op(A), [row][col] */
+    for (int e = tid; e < SB * SB; e += NT) {
+      const int i = e % SB, j = e / SB;
+      T v = T(0);
+      if (i < rb && j < rb && (fwd ? i >= j : i <= j))
+        v = trans ? A[(size_t)(R0 + i) * lda + R0 + j] : A[(size_t)(R0 + j) * lda + R0 + i];
+      s_blk[i * (SB + 1) + j] = v;
+    }
+    __syncthreads();
+    if (wave == 0) {
+      T c = T(0);
+      if (lane < rb) {
+        T sum = s_part[lane];
+        if (!trans)
+          for (int q = 1; q < NT / SB; ++q) sum += s_part[q * SB + lane];
+        c = xv[R0 + lane] - sum;
+      }
+#pragma unroll
+      for (int s = 0; s < SB; ++s) {
+        const int j = fwd ? s : SB - 1 - s;
+        if (j < rb) {
+          if (lane == j && !unit) c /= s_blk[j * (SB + 1) + j];
+          const T xj = __shfl(c, j);
+          if (lane < rb && (fwd ? (lane > j) : (lane < j))) c -= s_blk[lane * (SB + 1) + j] * xj;
+        }
+      }
+      if (lane < rb) xv[R0 + lane] = c;
+    }
+  }
+  __syncthreads();
+}
+
+/* ---------------------------------------------------------------------- */
 /* Cholesky factor + solve of one instance by one workgroup                */
 /* ---------------------------------------------------------------------- */
 template <typename T, int NB>
@@ -201,33 +299,13 @@ __device__ void wg_posv(int n, T *A, long lda, int nrhs, T *B, long ldb, int *in
   }
   if (tid == 0) *info = 0;
   if (nrhs <= 0) return;
-  /* ---- potrs: L y = b, L^T x = y, right-hand sides in LDS ---- */
+  /* ---- potrs: L y = b, L^T x = y, right-hand sides in LDS; the factor
+   * phase's LDS (s_d, s_a, s_b, contiguous) is the solve's staging ---- */
   __syncthreads();
   for (int e = tid; e < n * nrhs; e += NT) s_x[e] = B[(size_t)(e / n) * ldb + e % n];
-  __syncthreads();
-  for (int j = 0; j < n; ++j) {            /* forward, column-oriented */
-    const T d = A[(size_t)j * lda + j];
-    if (tid < nrhs) s_x[tid * n + j] /= d;
-    __syncthreads();
-    for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
-      const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
-      s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
-    }
-    __syncthreads();
-  }
-  for (int j = n - 1; j >= 0; --j) {       /* backward with L^T: dot products */
-    for (int r = 0; r < nrhs; ++r) {
-      T part = T(0);
-      for (int i = j + 1 + tid; i < n; i += NT) part += A[(size_t)j * lda + i] * s_x[r * n + i];
-      for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off);
-      if ((tid & 63) == 0) s_a[(tid >> 6) * MAXR + r] = part;
-    }
-    __syncthreads();
-    if (tid < nrhs) {
-      const T s = s_a[tid] + s_a[MAXR + tid] + s_a[2 * MAXR + tid] + s_a[3 * MAXR + tid];
-      s_x[tid * n + j] = (s_x[tid * n + j] - s) / A[(size_t)j * lda + j];
-    }
-    __syncthreads();
+  for (int r = 0; r < nrhs; ++r) {
+    wg_trsv(n, A, lda, true, false, false, s_x + (size_t)r * n, s_d, s_d + SB * (SB + 1));
+    wg_trsv(n, A, lda, true, true, false, s_x + (size_t)r * n, s_d, s_d + SB * (SB + 1));
   }
   for (int e = tid; e < n * nrhs; e += NT) B[(size_t)(e / n) * ldb + e % n] = s_x[e];
 }
@@ -340,61 +418,34 @@ __device__ void wg_gesv(int n, T *A, long lda, int *ipiv, int nrhs, T *B, long l
   if (tid == 0) *info = first0;
   if (nrhs <= 0 || first0) return;
   /* ---- getrs in LDS (ATL_getrs.c:8-89): NoTrans: P b, L y = b (unit),
-   * U x = y; Trans (A^T = U^T L^T P): U^T z = b, L^T w = z (unit), x = P^T w ---- */
+   * U x = y; Trans (A^T = U^T L^T P): U^T z = b, L^T w = z (unit), x = P^T w.
+   * The triangular solves stage through s_a, s_b (contiguous) ---- */
   __syncthreads();
   for (int e = tid; e < n * nrhs; e += NT) s_x[e] = B[(size_t)(e / n) * ldb + e % n];
   __syncthreads();
-  if (!trans) {
-    if (tid < nrhs) {
-      for (int j = 0; j < n; ++j) {
-        const int p = ipiv[j];
-        if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
-      }
-    }
-    __syncthreads();
+  if (!trans && tid < nrhs) {
     for (int j = 0; j < n; ++j) {
-      for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
-        const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
-        s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
-      }
-      __syncthreads();
+      const int p = ipiv[j];
+      if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
     }
-    for (int j = n - 1; j >= 0; --j) {
-      const T d = A[(size_t)j * lda + j];
-      if (tid < nrhs) s_x[tid * n + j] /= d;
-      __syncthreads();
-      for (int e = tid; e < j * nrhs; e += NT) {
-        const int i = e % j, r = e / j;
-        s_x[r * n + i] -= A[(size_t)j * lda + i] * s_x[r * n + j];
-      }
-      __syncthreads();
-    }
-  } else {
-    for (int j = 0; j < n; ++j) {            /* U^T z = b: U^T(i, j) = A(j, i) */
-      const T d = A[(size_t)j * lda + j];
-      if (tid < nrhs) s_x[tid * n + j] /= d;
-      __syncthreads();
-      for (int e = tid; e < (n - j - 1) * nrhs; e += NT) {
-        const int i = j + 1 + e % (n - j - 1), r = e / (n - j - 1);
-        s_x[r * n + i] -= A[(size_t)i * lda + j] * s_x[r * n + j];
-      }
-      __syncthreads();
-    }
-    for (int j = n - 1; j >= 0; --j) {       /* L^T w = z: L^T(i, j) = A(j, i), i < j */
-      for (int e = tid; e < j * nrhs; e += NT) {
-        const int i = e % j, r = e / j;
-        s_x[r * n + i] -= A[(size_t)i * lda + j] * s_x[r * n + j];
-      }
-      __syncthreads();
-    }
-    if (tid < nrhs) {                        /* x = P^T w: the swaps in reverse */
-      for (int j = n - 1; j >= 0; --j) {
-        const int p = ipiv[j];
-        if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
-      }
-    }
-    __syncthreads();
   }
+  for (int r = 0; r < nrhs; ++r) {
+    T *xv = s_x + (size_t)r * n;
+    if (!trans) {
+      wg_trsv(n, A, lda, true, false, true, xv, s_a, s_a + SB * (SB + 1));
+      wg_trsv(n, A, lda, false, false, false, xv, s_a, s_a + SB * (SB + 1));
+    } else {
+      wg_trsv(n, A, lda, false, true, false, xv, s_a, s_a + SB * (SB + 1));
+      wg_trsv(n, A, lda, true, true, true, xv, s_a, s_a + SB * (SB + 1));
+    }
+  }
+  if (trans && tid < nrhs) {                 /* x = P^T w: the swaps in reverse */
+    for (int j = n - 1; j >= 0; --j) {
+      const int p = ipiv[j];
+      if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
+    }
+  }
+  __syncthreads();
   for (int e = tid; e < n * nrhs; e += NT) B[(size_t)(e / n) * ldb + e % n] = s_x[e];
 }
 
