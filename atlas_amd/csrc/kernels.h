@@ -299,6 +299,16 @@ hipError_t atlmx_k_dpotf2(int uplo, int n, double *A, long lda, int info_base,
                           int *info, hipStream_t s);
 hipError_t atlmx_k_spotf2(int uplo, int n, float *A, long lda, int info_base,
                           int *info, hipStream_t s);
+/* The panel's triangular solve after a jb x jb (jb <= 128) diagonal block D
+ * is factored (uplo as for potf2): nr vectors of P (Lower: the rows of the
+ * nr x jb block below D; Upper: the columns of the jb x nr block right of
+ * it) are solved in place against D's factor, using inv = the inverted
+ * 64 x 64 diagonal blocks of D from atlmx_k_?tri_inv.  Does nothing when
+ * *info is nonzero. */
+hipError_t atlmx_k_dchol_panel(int uplo, int nr, int jb, const double *D, long lda,
+                               const double *inv, double *P, const int *info, hipStream_t s);
+hipError_t atlmx_k_schol_panel(int uplo, int nr, int jb, const float *D, long lda,
+                               const float *inv, float *P, const int *info, hipStream_t s);
 
 /* First index of max |x_i| (cblas_i?amax semantics) into *out. */
 hipError_t atlmx_k_idamax(int n, const double *x, int incx, int *out,

@@ -25,6 +25,8 @@
 #define K_PERMUTE_ROWS atlmx_k_dpermute_rows
 #define K_LASWP_APPLY atlmx_k_dlaswp_apply
 #define K_POTF2 atlmx_k_dpotf2
+#define K_TRI_INV atlmx_k_dtri_inv
+#define K_CHOL_PANEL atlmx_k_dchol_panel
 #define K_TRICOPY atlmx_k_dtricopy
 #define K_DIAG_ZERO atlmx_k_ddiag_zero
 #define K_GEQR2 atlmx_k_dgeqr2_panel
@@ -38,6 +40,8 @@
 #undef K_PERMUTE_ROWS
 #undef K_LASWP_APPLY
 #undef K_POTF2
+#undef K_TRI_INV
+#undef K_CHOL_PANEL
 #undef K_TRICOPY
 #undef K_DIAG_ZERO
 #undef K_GEQR2
@@ -51,6 +55,8 @@
 #define K_PERMUTE_ROWS atlmx_k_spermute_rows
 #define K_LASWP_APPLY atlmx_k_slaswp_apply
 #define K_POTF2 atlmx_k_spotf2
+#define K_TRI_INV atlmx_k_stri_inv
+#define K_CHOL_PANEL atlmx_k_schol_panel
 #define K_TRICOPY atlmx_k_stricopy
 #define K_DIAG_ZERO atlmx_k_sdiag_zero
 #define K_GEQR2 atlmx_k_sgeqr2_panel

@@ -202,9 +202,10 @@ void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
 
 /* Blocked right-looking Cholesky (Lower: A = L*L^T, Upper: A = U^T*U).
  * Replaces the recursion of src/lapack/ATL_potrfL.c:204-262 /
- * ATL_potrfU.c:199-257: diagonal block in registers (k_chol.hip), TRSM of
- * the panel, SYRK trailing update on the GEMM kernel restricted to one
- * triangle.  One block of lookahead: step j first updates only the next
+ * ATL_potrfU.c:199-257: diagonal block in registers (k_chol.hip), the
+ * panel's TRSM in one launch (k_chol.hip's panel kernel on the block's
+ * inverted 64 x 64 diagonal blocks), SYRK trailing update on the GEMM kernel
+ * restricted to one triangle.  One block of lookahead: step j first updates only the next
  * block column (block row for Upper) from panel j, then factors that block
  * and solves its panel on the second stream while the main stream applies
  * panel j to the rest of the trailing matrix.  Every block sees the same
@@ -221,12 +222,20 @@ static void NM(PFX, potrf_panel)(atlmx_ctx *c, int uplo, int N, T *A, long lda, 
   RT_CHECK(rout, K_POTF2(uplo, jb, Ad, lda, j, d_info, c->stream));
   rt_prof_end(c, pid);
   if (nr <= 0) return;
-  if (uplo == KX_LOWER)
-    NM(PFX, trsm_dev)(c, KX_RIGHT, KX_LOWER, KX_TRANS, KX_NONUNIT, nr, jb, (T)1, Ad, lda,
-                      A + (j + jb) + (size_t)j * lda, lda, d_info, rout);
-  else
-    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_TRANS, KX_NONUNIT, jb, nr, (T)1, Ad, lda,
-                      A + j + (size_t)(j + jb) * lda, lda, d_info, rout);
+  {
+    /* the panel's TRSM (Lower: X L^T = A21; Upper: U^T X = A12) in one
+     * launch after the diagonal block's 64 x 64 inverses */
+    const int nbi = (jb + ATLMX_TRI_NB - 1) / ATLMX_TRI_NB;
+    T *inv = (T *)rt_ws(c, WS_TRI_INV, (size_t)nbi * ATLMX_TRI_NB * ATLMX_TRI_NB * sizeof(T),
+                        rout);
+    T *P = (uplo == KX_LOWER) ? A + (j + jb) + (size_t)j * lda : A + j + (size_t)(j + jb) * lda;
+    pid = rt_prof_begin(c, PROF_TRSM_INV, (double)jb * ATLMX_TRI_NB * sizeof(T));
+    RT_CHECK(rout, K_TRI_INV(uplo, KX_NONUNIT, jb, Ad, lda, inv, d_info, c->stream));
+    rt_prof_end(c, pid);
+    pid = rt_prof_begin(c, PROF_MISC, (double)nr * jb * jb);
+    RT_CHECK(rout, K_CHOL_PANEL(uplo, nr, jb, Ad, lda, inv, P, d_info, c->stream));
+    rt_prof_end(c, pid);
+  }
 }
 
 /* Trailing update from panel j (columns/rows j..j+jb) of the part of the
