@@ -146,10 +146,6 @@ int GETRF_DEV(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv, const cha
   return info;
 }
 
-/* src/lapack/ATL_getrs.c:8-89 (ColMajor).  NoTrans: P*B, L-unit solve, U
- * solve; Trans: U^T solve, L^T-unit solve, P^T*B.  inv / ctl: NULL, or the
- * two TRSMs' workspaces (see trsm_dev_ws; the solves run one after the
- * other on the stream, so they share them). */
 /* The row interchanges of getrs on the right-hand sides (ATL_getrs.c's
  * ATL_laswp calls).  Replayed one by one they are a chain of N dependent
  * swaps; here, for up to ATLMX_LASWP_PERM_MAXROWS rows, they become one
@@ -174,6 +170,10 @@ static void NM(PFX, laswp_rhs)(atlmx_ctx *c, int NRHS, T *B, long ldb, int N,
   }
 }
 
+/* src/lapack/ATL_getrs.c:8-89 (ColMajor).  NoTrans: P*B, L-unit solve, U
+ * solve; Trans: U^T solve, L^T-unit solve, P^T*B.  inv / ctl: NULL, or the
+ * two TRSMs' workspaces (see trsm_dev_ws; the solves run one after the
+ * other on the stream, so they share them). */
 void NM(PFX, getrs_dev_ws)(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
                            const int *d_ipiv, T *B, long ldb, T *inv, int *ctl,
                            const char *rout)
@@ -205,12 +205,12 @@ void GETRS_DEV(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
  * ATL_potrfU.c:199-257: diagonal block in registers (k_chol.hip), the
  * panel's TRSM in one launch (k_chol.hip's panel kernel on the block's
  * inverted 64 x 64 diagonal blocks), SYRK trailing update on the GEMM kernel
- * restricted to one triangle.  One block of lookahead: step j first updates only the next
- * block column (block row for Upper) from panel j, then factors that block
- * and solves its panel on the second stream while the main stream applies
- * panel j to the rest of the trailing matrix.  Every block sees the same
- * updates in the same order as without lookahead, so the result is the
- * same.  A non-positive pivot stops the factorization: the device info flag
+ * restricted to one triangle.  One block of lookahead: step j first
+ * updates only the next block column (block row for Upper) from panel j,
+ * then factors that block and solves its panel on the second stream while
+ * the main stream applies panel j to the rest of the trailing matrix.
+ * Every block sees the same updates in the same order as without
+ * lookahead, so the result is the same.  A non-positive pivot stops the factorization: the device info flag
  * makes every queued kernel after it return at once.  Synchronizes;
  * returns info. */
 static void NM(PFX, potrf_panel)(atlmx_ctx *c, int uplo, int N, T *A, long lda, int j,
