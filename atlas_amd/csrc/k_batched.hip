@@ -34,6 +34,16 @@ constexpr int NT = 256;
 constexpr int TS = 64;                 /* trailing-update / GEMM tile */
 constexpr int MAXR = ATLMX_BATCH_MAXRHS;
 
+/* v from lane `src` (a constant) to every lane of the wavefront */
+__device__ inline double bcast(double v, int src) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ inline float bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
 template <typename T> __device__ inline T dsqrt(T x);
 template <> __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
 template <> __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
@@ -124,6 +134,80 @@ __device__ void tile_update_nn(T *C, long ldc, const T *L, long ldl, const T *U,
       const int r = ty * 4 + i, c = tx * 4 + j;
       if (r < mr && c < mc) C[(size_t)c * ldc + r] -= acc[i][j];
     }
+}
+
+/* ---------------------------------------------------------------------- */
+/* Trailing updates on the matrix cores, one 64 x 64 tile per wavefront    */
+/* ---------------------------------------------------------------------- */
+typedef double td4 __attribute__((ext_vector_type(4)));
+typedef float tf4 __attribute__((ext_vector_type(4)));
+template <typename T> struct TMfma;
+template <> struct TMfma<double> {
+  typedef td4 acc_t;
+  __device__ static acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  /* v_mfma_f64_16x16x4f64: lane l, element r holds D[(l>>4) + 4r][l&15] */
+  __device__ static int drow(int lane, int r) { return (lane >> 4) + 4 * r; }
+};
+template <> struct TMfma<float> {
+  typedef tf4 acc_t;
+  __device__ static acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  /* v_mfma_f32_16x16x4f32: lane l, element r holds D[4(l>>4) + r][l&15] */
+  __device__ static int drow(int lane, int r) { return 4 * (lane >> 4) + r; }
+};
+
+/* C(tile) -= X Y^T for a 64 x 64 tile by one wavefront, K <= KMAX: X(r, k)
+ * = X[r + k ldx] (rows r0 of the tile), Y(c, k) = Y[c*ycs + k*yks] (NT:
+ * ycs = 1, yks = ld; NN with Y = U^T: ycs = ld, yks = 1).  Operands go from
+ * memory straight into the MFMA lanes (lane l: row or column l&15 of a
+ * 16-block, k = 4s + (l>>4)), no LDS and no barrier, so the wavefronts of a
+ * workgroup work on different tiles at once.  lower_only: only r0 + r >=
+ * c0 + c is written. */
+template <typename T, int KMAX>
+__device__ void wave_tile_update(T *C, long ldc, const T *X, long ldx, const T *Y, long ycs,
+                                 long yks, int K, int mr, int mc, bool lower_only, int r0,
+                                 int c0) {
+  typedef TMfma<T> MF;
+  typedef typename MF::acc_t acc_t;
+  const int lane = threadIdx.x & 63, li = lane & 15, lk = lane >> 4;
+  acc_t acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < KMAX / 4; ++s) {
+    const int k = 4 * s + lk;
+    if (4 * s >= K) break;                        /* uniform */
+    T a[4], b[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int r = 16 * i + li;
+      a[i] = (r < mr && k < K) ? X[(size_t)r + (size_t)k * ldx] : T(0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = 16 * j + li;
+      b[j] = (c < mc && k < K) ? Y[(size_t)c * ycs + (size_t)k * yks] : T(0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(a[i], b[j], acc[i][j]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = 16 * i + MF::drow(lane, r), cc = 16 * j + li;
+        if (rr < mr && cc < mc && (!lower_only || r0 + rr >= c0 + cc))
+          C[(size_t)cc * ldc + rr] -= acc[i][j][r];
+      }
 }
 
 /* ---------------------------------------------------------------------- */
@@ -241,24 +325,39 @@ __device__ void wg_posv(int n, T *A, long lda, int nrhs, T *B, long ldb, int *in
     }
     if (tid == 0) *s_flag = 0;
     __syncthreads();
-    for (int j = 0; j < kb; ++j) {
-      if (tid == 0) {
-        const T d = s_d[j * (NB + 1) + j];
-        if (!(d > T(0))) *s_flag = j + 1;
-        else s_d[j * (NB + 1) + j] = dsqrt(d);
+    /* wavefront 0 alone, one row per lane in registers: column j's values
+     * reach the other lanes by readlane, so no barrier per column */
+    if (threadIdx.x < 64) {
+      const int lane = threadIdx.x;
+      T a[NB];
+#pragma unroll
+      for (int c = 0; c < NB; ++c) a[c] = (lane < kb && c <= lane) ? s_d[c * (NB + 1) + lane] : T(0);
+      int fail = 0;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        if (j < kb && !fail) {
+          const T d = bcast(a[j], j);
+          if (!(d > T(0))) {
+            fail = j + 1;                         /* uniform: every lane saw the same d */
+          } else {
+            const T sd = dsqrt(d);
+            if (lane == j) a[j] = sd;
+            else if (lane > j) a[j] = a[j] / sd;
+#pragma unroll
+            for (int c = j + 1; c < NB; ++c) {
+              const T lcj = bcast(a[j], c);
+              if (lane >= c && lane < kb) a[c] -= a[j] * lcj;
+            }
+          }
+        }
       }
-      __syncthreads();
-      if (*s_flag) break;
-      const T djj = s_d[j * (NB + 1) + j];
-      if (tid > j && tid < kb) s_d[j * (NB + 1) + tid] /= djj;
-      __syncthreads();
-      for (int e = tid; e < NB * NB; e += NT) {
-        const int r = e % NB, c = e / NB;
-        if (c > j && r >= c && r < kb)
-          s_d[c * (NB + 1) + r] -= s_d[j * (NB + 1) + r] * s_d[j * (NB + 1) + c];
-      }
-      __syncthreads();
+      if (lane < kb)
+#pragma unroll
+        for (int c = 0; c < NB; ++c)
+          if (c <= lane) s_d[c * (NB + 1) + lane] = a[c];
+      if (lane == 0) *s_flag = fail;
     }
+    __syncthreads();
     if (*s_flag) {
       if (tid == 0) *info = k + *s_flag;
       return;                               /* ATL_potrfL.c:238-240: stop */
@@ -290,12 +389,18 @@ __device__ void wg_posv(int n, T *A, long lda, int nrhs, T *B, long ldb, int *in
     const int m0 = k + kb, m = n - m0;
     const int nt = (m + TS - 1) / TS;
     const T *P = A + (size_t)k * lda;
-    for (int tj = 0; tj < nt; ++tj)
-      for (int ti = tj; ti < nt; ++ti) {
-        const int r0 = m0 + ti * TS, c0 = m0 + tj * TS;
-        tile_update_nt(A + (size_t)c0 * lda + r0, lda, P + r0, P + c0, lda, kb,
-                       min(TS, n - r0), min(TS, n - c0), ti == tj, r0, c0, s_a, s_b);
-      }
+    {
+      /* the lower triangle's tiles, wavefront w taking every fourth */
+      const int wave = threadIdx.x >> 6;
+      int t = 0;
+      for (int tj = 0; tj < nt; ++tj)
+        for (int ti = tj; ti < nt; ++ti, ++t) {
+          if ((t & 3) != wave) continue;
+          const int r0 = m0 + ti * TS, c0 = m0 + tj * TS;
+          wave_tile_update<T, NB>(A + (size_t)c0 * lda + r0, lda, P + r0, lda, P + c0, 1, lda,
+                                  kb, min(TS, n - r0), min(TS, n - c0), ti == tj, r0, c0);
+        }
+    }
   }
   if (tid == 0) *info = 0;
   if (nrhs <= 0) return;
@@ -407,13 +512,13 @@ __device__ void wg_gesv(int n, T *A, long lda, int *ipiv, int nrhs, T *B, long l
     /* ---- A22 -= L21 U12 ---- */
     const int m0 = k + kb, m = n - m0;
     const int nt = (m + TS - 1) / TS;
-    for (int tj = 0; tj < nt; ++tj)
-      for (int ti = 0; ti < nt; ++ti) {
-        const int r0 = m0 + ti * TS, c0 = m0 + tj * TS;
-        tile_update_nn(A + (size_t)c0 * lda + r0, lda, A + (size_t)k * lda + r0, lda,
-                       A + (size_t)c0 * lda + k, lda, kb, min(TS, n - r0), min(TS, n - c0),
-                       s_a, s_b);
-      }
+    for (int t = wave; t < nt * nt; t += NT / 64) {
+      const int ti = t % nt, tj = t / nt;         /* wavefront w takes every fourth tile */
+      const int r0 = m0 + ti * TS, c0 = m0 + tj * TS;
+      wave_tile_update<T, NB>(A + (size_t)c0 * lda + r0, lda, A + (size_t)k * lda + r0, lda,
+                              A + (size_t)c0 * lda + k, lda, 1, kb, min(TS, n - r0),
+                              min(TS, n - c0), false, r0, c0);
+    }
   }
   if (tid == 0) *info = first0;
   if (nrhs <= 0 || first0) return;
