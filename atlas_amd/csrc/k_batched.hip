@@ -213,19 +213,22 @@ __device__ void wave_tile_update(T *C, long ldc, const T *X, long ldx, const T *
 /* ---------------------------------------------------------------------- */
 /* Triangular solve of one instance's right-hand side by one workgroup     */
 /* ---------------------------------------------------------------------- */
-/* op(A) x = x for one vector xv in LDS, A's triangle in global memory
- * (lower: the stored triangle is the lower one; trans: op(A) = A^T; unit:
- * the diagonal is taken as 1).  Left-looking over blocks of SB rows in solve
- * order (top first when lower != trans): the block's entries less the
- * solved entries' contributions (all four wavefronts, a GEMV, then one
- * barrier), then the SB x SB diagonal triangle -- staged in LDS -- by
- * wavefront 0 alone, one unknown per lane, each solved value broadcast from
- * its lane: no barrier per row.  s_blk holds SB x (SB + 1), s_part 8 x SB
- * elements. */
+/* op(A) X = X for the nrhs (<= MAXR) vectors of X in LDS (column r at
+ * X + r n), A's triangle in global memory (lower: the stored triangle is
+ * the lower one; trans: op(A) = A^T; unit: the diagonal is taken as 1).
+ * Left-looking over blocks of SB rows in solve order (top first when
+ * lower != trans), RCH right-hand sides at a time: the block's entries less
+ * the solved entries' contributions (all four wavefronts, one pass over A
+ * for the RCH vectors, then one barrier), then the SB x SB diagonal
+ * triangle -- staged in LDS -- by wavefront 0 alone, one row per lane,
+ * each solved value broadcast from its lane: no barrier per row.  s_blk
+ * holds SB x (SB + 1), s_part 4 RCH SB elements. */
 constexpr int SB = 32;
+constexpr int RCH = 4;                             /* right-hand sides per pass */
+constexpr int RCT = 2;                             /* the transposed GEMV's accumulator set */
 template <typename T>
-__device__ void wg_trsv(int n, const T *A, long lda, bool lower, bool trans, bool unit, T *xv,
-                        T *s_blk, T *s_part) {
+__device__ void wg_trsv(int n, int nrhs, const T *A, long lda, bool lower, bool trans, bool unit,
+                        T *X, T *s_blk, T *s_part) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const bool fwd = lower != trans;
   const int nb = (n + SB - 1) / SB;
@@ -235,47 +238,6 @@ __device__ void wg_trsv(int n, const T *A, long lda, bool lower, bool trans, boo
     /* solved range: [0, R0) forward, [R0 + rb, n) backward */
     const int k0 = fwd ? 0 : R0 + rb, k1 = fwd ? R0 : n;
     __syncthreads();
-    if (!trans) {
-      /* op(A)(i, k) = A[k lda + i]: thread (row l, chunk q) sums k = k0 + q, k0 + q + 8, ... */
-      constexpr int ST = NT / SB, U = 8;          /* stride, loads in flight */
-      const int l = tid & (SB - 1), q = tid / SB;
-      T part = T(0);
-      if (l < rb) {
-        int k = k0 + q;
-        for (; k + (U - 1) * ST < k1; k += U * ST) {
-          T a[U];
-#pragma unroll
-          for (int u = 0; u < U; ++u) a[u] = A[(size_t)(k + u * ST) * lda + R0 + l];
-#pragma unroll
-          for (int u = 0; u < U; ++u) part += a[u] * xv[k + u * ST];
-        }
-        for (; k < k1; k += ST) part += A[(size_t)k * lda + R0 + l] * xv[k];
-      }
-      s_part[q * SB + l] = part;
-    } else {
-      /* op(A)(i, k) = A[i lda + k]: wavefront w takes rows w, w + 4, ..., lanes run along k */
-      constexpr int RW = SB / (NT / 64);          /* rows per wavefront, all loads issued together */
-      T part[RW];
-#pragma unroll
-      for (int m = 0; m < RW; ++m) part[m] = T(0);
-      for (int k = k0 + lane; k < k1; k += 64) {
-        T a[RW];
-#pragma unroll
-        for (int m = 0; m < RW; ++m) {
-          const int l = wave + m * (NT / 64);
-          a[m] = l < rb ? A[(size_t)(R0 + l) * lda + k] : T(0);
-        }
-        const T xk = xv[k];
-#pragma unroll
-        for (int m = 0; m < RW; ++m) part[m] += a[m] * xk;
-      }
-#pragma unroll
-      for (int m = 0; m < RW; ++m) {
-        T v = part[m];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-        if (lane == 0) s_part[wave + m * (NT / 64)] = v;
-      }
-    }
     /* the diagonal block of op(A), [row][col] */
     for (int e = tid; e < SB * SB; e += NT) {
       const int i = e % SB, j = e / SB;
@@ -284,25 +246,109 @@ __device__ void wg_trsv(int n, const T *A, long lda, bool lower, bool trans, boo
         v = trans ? A[(size_t)(R0 + i) * lda + R0 + j] : A[(size_t)(R0 + j) * lda + R0 + i];
       s_blk[i * (SB + 1) + j] = v;
     }
-    __syncthreads();
-    if (wave == 0) {
-      T c = T(0);
-      if (lane < rb) {
-        T sum = s_part[lane];
-        if (!trans)
-          for (int q = 1; q < NT / SB; ++q) sum += s_part[q * SB + lane];
-        c = xv[R0 + lane] - sum;
-      }
+    for (int rc = 0; rc < nrhs; rc += RCH) {
+      if (rc > 0) __syncthreads();               /* wavefront 0 is done with s_part */
+      if (!trans) {
+        /* op(A)(i, k) = A[k lda + i]: thread (row l, chunk q) sums k = k0 + q, k0 + q + 8, ...;
+         * the two chunks of a wavefront are added by a lane exchange */
+        constexpr int ST = NT / SB, U = 8;       /* stride, loads in flight */
+        const int l = tid & (SB - 1), q = tid / SB;
+        T part[RCH];
 #pragma unroll
-      for (int s = 0; s < SB; ++s) {
-        const int j = fwd ? s : SB - 1 - s;
-        if (j < rb) {
-          if (lane == j && !unit) c /= s_blk[j * (SB + 1) + j];
-          const T xj = __shfl(c, j);
-          if (lane < rb && (fwd ? (lane > j) : (lane < j))) c -= s_blk[lane * (SB + 1) + j] * xj;
+        for (int r = 0; r < RCH; ++r) part[r] = T(0);
+        if (l < rb) {
+          int k = k0 + q;
+          for (; k + (U - 1) * ST < k1; k += U * ST) {
+            T a[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) a[u] = A[(size_t)(k + u * ST) * lda + R0 + l];
+#pragma unroll
+            for (int r = 0; r < RCH; ++r)
+              if (rc + r < nrhs)
+#pragma unroll
+                for (int u = 0; u < U; ++u) part[r] += a[u] * X[(size_t)(rc + r) * n + k + u * ST];
+          }
+          for (; k < k1; k += ST) {
+            const T a = A[(size_t)k * lda + R0 + l];
+#pragma unroll
+            for (int r = 0; r < RCH; ++r)
+              if (rc + r < nrhs) part[r] += a * X[(size_t)(rc + r) * n + k];
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < RCH; ++r) {
+          const T v = part[r] + __shfl_xor(part[r], 32);
+          if (lane < 32) s_part[(r * 4 + wave) * SB + l] = v;
+        }
+      } else {
+        /* op(A)(i, k) = A[i lda + k]: wavefront w takes rows w, w + 4, ...,
+         * lanes run along k, all loads of a step issued together */
+        constexpr int RW = SB / (NT / 64);
+        for (int rt = 0; rt < RCH && rc + rt < nrhs; rt += RCT) {
+          T part[RW][RCT];
+#pragma unroll
+          for (int m = 0; m < RW; ++m)
+#pragma unroll
+            for (int r = 0; r < RCT; ++r) part[m][r] = T(0);
+          for (int k = k0 + lane; k < k1; k += 64) {
+            T a[RW];
+#pragma unroll
+            for (int m = 0; m < RW; ++m) {
+              const int l = wave + m * (NT / 64);
+              a[m] = l < rb ? A[(size_t)(R0 + l) * lda + k] : T(0);
+            }
+#pragma unroll
+            for (int r = 0; r < RCT; ++r)
+              if (rc + rt + r < nrhs) {
+                const T xk = X[(size_t)(rc + rt + r) * n + k];
+#pragma unroll
+                for (int m = 0; m < RW; ++m) part[m][r] += a[m] * xk;
+              }
+          }
+#pragma unroll
+          for (int m = 0; m < RW; ++m)
+#pragma unroll
+            for (int r = 0; r < RCT; ++r) {
+              T v = part[m][r];
+              for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+              if (lane == 0) s_part[((rt + r) * 4) * SB + wave + m * (NT / 64)] = v;
+            }
         }
       }
-      if (lane < rb) xv[R0 + lane] = c;
+      __syncthreads();
+      if (wave == 0) {
+        T c[RCH];
+#pragma unroll
+        for (int r = 0; r < RCH; ++r) {
+          c[r] = T(0);
+          if (rc + r < nrhs && lane < rb) {
+            T sum = s_part[(r * 4) * SB + lane];
+            if (!trans)
+              for (int w = 1; w < NT / 64; ++w) sum += s_part[(r * 4 + w) * SB + lane];
+            c[r] = X[(size_t)(rc + r) * n + R0 + lane] - sum;
+          }
+        }
+        for (int s = 0; s < SB; ++s) {
+          const int j = fwd ? s : SB - 1 - s;
+          if (j < rb) {
+            const T dj = unit ? T(1) : s_blk[j * (SB + 1) + j];
+            const T lj = (lane < rb && (fwd ? (lane > j) : (lane < j)))
+                             ? s_blk[lane * (SB + 1) + j] : T(0);
+#pragma unroll
+            for (int r = 0; r < RCH; ++r) {
+              if (rc + r < nrhs) {
+                if (lane == j) c[r] /= dj;
+                const T xj = __shfl(c[r], j);
+                c[r] -= lj * xj;
+              }
+            }
+          }
+        }
+        if (lane < rb)
+#pragma unroll
+          for (int r = 0; r < RCH; ++r)
+            if (rc + r < nrhs) X[(size_t)(rc + r) * n + R0 + lane] = c[r];
+      }
     }
   }
   __syncthreads();
@@ -408,10 +454,8 @@ __device__ void wg_posv(int n, T *A, long lda, int nrhs, T *B, long ldb, int *in
    * phase's LDS (s_d, s_a, s_b, contiguous) is the solve's staging ---- */
   __syncthreads();
   for (int e = tid; e < n * nrhs; e += NT) s_x[e] = B[(size_t)(e / n) * ldb + e % n];
-  for (int r = 0; r < nrhs; ++r) {
-    wg_trsv(n, A, lda, true, false, false, s_x + (size_t)r * n, s_d, s_d + SB * (SB + 1));
-    wg_trsv(n, A, lda, true, true, false, s_x + (size_t)r * n, s_d, s_d + SB * (SB + 1));
-  }
+  wg_trsv(n, nrhs, A, lda, true, false, false, s_x, s_d, s_d + SB * (SB + 1));
+  wg_trsv(n, nrhs, A, lda, true, true, false, s_x, s_d, s_d + SB * (SB + 1));
   for (int e = tid; e < n * nrhs; e += NT) B[(size_t)(e / n) * ldb + e % n] = s_x[e];
 }
 
@@ -534,15 +578,12 @@ __device__ void wg_gesv(int n, T *A, long lda, int *ipiv, int nrhs, T *B, long l
       if (p != j) { const T t = s_x[tid * n + j]; s_x[tid * n + j] = s_x[tid * n + p]; s_x[tid * n + p] = t; }
     }
   }
-  for (int r = 0; r < nrhs; ++r) {
-    T *xv = s_x + (size_t)r * n;
-    if (!trans) {
-      wg_trsv(n, A, lda, true, false, true, xv, s_a, s_a + SB * (SB + 1));
-      wg_trsv(n, A, lda, false, false, false, xv, s_a, s_a + SB * (SB + 1));
-    } else {
-      wg_trsv(n, A, lda, false, true, false, xv, s_a, s_a + SB * (SB + 1));
-      wg_trsv(n, A, lda, true, true, true, xv, s_a, s_a + SB * (SB + 1));
-    }
+  if (!trans) {
+    wg_trsv(n, nrhs, A, lda, true, false, true, s_x, s_a, s_a + SB * (SB + 1));
+    wg_trsv(n, nrhs, A, lda, false, false, false, s_x, s_a, s_a + SB * (SB + 1));
+  } else {
+    wg_trsv(n, nrhs, A, lda, false, true, false, s_x, s_a, s_a + SB * (SB + 1));
+    wg_trsv(n, nrhs, A, lda, true, true, true, s_x, s_a, s_a + SB * (SB + 1));
   }
   if (trans && tid < nrhs) {                 /* x = P^T w: the swaps in reverse */
     for (int j = n - 1; j >= 0; --j) {
