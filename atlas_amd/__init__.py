@@ -119,6 +119,13 @@ def _declare(L):
         f = getattr(L, "atlmx_%spotrs" % p)
         f.argtypes = [_I, _I, _I, _P, _I, _P, _I]
         f.restype = None
+    for p in ("z", "c"):
+        f = getattr(L, "cblas_%sgemm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%strsm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I]
+        f.restype = None
     L.atlmx_init.restype = _I
     L.atlmx_set_stream.argtypes = [_P]
     L.atlmx_get_stream.restype = _P
@@ -162,6 +169,41 @@ def _ld(a):
 # --------------------------------------------------------------------------
 # host (numpy, column-major) wrappers of the ATLAS C ABI
 # --------------------------------------------------------------------------
+def _cpre(dt):
+    dt = np.dtype(dt)
+    if dt == np.complex128:
+        return "z", np.float64
+    if dt == np.complex64:
+        return "c", np.float32
+    raise TypeError("unsupported complex dtype %s" % dt)
+
+
+def _cscalar(x, rt):
+    return np.array([complex(x).real, complex(x).imag], dtype=rt)
+
+
+def cgemm(ta, tb, alpha, A, B, beta, Cm):
+    """Complex Cm (Fortran-ordered, in place) = alpha op(A) op(B) + beta Cm
+    through cblas_zgemm / cblas_cgemm (op may be ConjTrans)."""
+    p, rt = _cpre(Cm.dtype)
+    M, N = Cm.shape
+    K = A.shape[1] if ta == NoTrans else A.shape[0]
+    al, be = _cscalar(alpha, rt), _cscalar(beta, rt)
+    getattr(lib(), "cblas_%sgemm" % p)(ColMajor, ta, tb, M, N, K, _ptr(al), _ptr(A), _ld(A),
+                                       _ptr(B), _ld(B), _ptr(be), _ptr(Cm), _ld(Cm))
+    return Cm
+
+
+def ctrsm(side, uplo, ta, diag, alpha, A, B):
+    """Complex B (Fortran-ordered, in place) through cblas_ztrsm / cblas_ctrsm."""
+    p, rt = _cpre(B.dtype)
+    M, N = B.shape
+    al = _cscalar(alpha, rt)
+    getattr(lib(), "cblas_%strsm" % p)(ColMajor, side, uplo, ta, diag, M, N, _ptr(al),
+                                       _ptr(A), _ld(A), _ptr(B), _ld(B))
+    return B
+
+
 def gemm(ta, tb, alpha, A, B, beta, Cm):
     """Cm (Fortran-ordered, in place) = alpha op(A) op(B) + beta Cm."""
     M, N = Cm.shape

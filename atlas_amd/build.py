@@ -21,9 +21,9 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 
 C_SOURCES = ["runtime.c", "xerbla.c", "blas3.c", "lapack.c", "mixed.c", "batched.c",
-             "f77.c"]
+             "f77.c", "cblas3c.c"]
 HIP_SOURCES = ["k_gemm.hip", "k_aux.hip", "k_lu.hip", "k_chol.hip", "k_mp.hip", "k_trsv.hip", "k_tslu.hip", "k_batched.hip",
-               "k_qr.hip", "noalloc.hip"]
+               "k_qr.hip", "noalloc.hip", "k_cplx.hip"]
 
 CFLAGS = ["-O2", "-fPIC", "-std=gnu99", "-Wall", "-Wno-unused-function",
           "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),

@@ -296,65 +296,13 @@ void NM3(cblas_, PFX, gemm)(const enum CBLAS_ORDER Order, const enum CBLAS_TRANS
                             const T *B, const int ldb, const T beta, T *C, const int ldc)
 {
   static const char *rout = "cblas_" STR(PFX) "gemm";
-  int info = 2000;
+  const int info = atlmx_gemm_check(Order, TA, TB, M, N, K, lda, ldb, ldc);
   atlmx_ctx *c;
   int ta, tb, m, n, ar, ac, br, bc;
   const T *pa, *pb;
   long la, lb, lc, dla, dlb, dlc;
   T *dA, *dB, *dC;
 
-  if (M < 0) info = cblas_errprn(4, info, "M cannot be less than zero 0,; is set to %d.", M);
-  if (N < 0) info = cblas_errprn(5, info, "N cannot be less than zero 0,; is set to %d.", N);
-  if (K < 0) info = cblas_errprn(6, info, "K cannot be less than zero 0,; is set to %d.", K);
-  if (Order == CblasRowMajor) {
-    if (TA == CblasNoTrans) {
-      if (lda < K || lda < 1)
-        info = cblas_errprn(9, info, "lda must be >= MAX(K,1): lda=%d K=%d", lda, K);
-    } else {
-      if (TA != CblasConjTrans && TA != CblasTrans)
-        info = cblas_errprn(2, info, "TransA must be %d, %d or %d, but is set to %d",
-                            CblasNoTrans, CblasTrans, CblasConjTrans, TA);
-      if (lda < M || lda < 1)
-        info = cblas_errprn(9, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
-    }
-    if (TB == CblasNoTrans) {
-      if (ldb < N || ldb < 1)
-        info = cblas_errprn(11, info, "ldb must be >= MAX(N,1): ldb=%d N=%d", ldb, N);
-    } else {
-      if (TB != CblasConjTrans && TB != CblasTrans)
-        info = cblas_errprn(3, info, "TransB must be %d, %d or %d, but is set to %d",
-                            CblasNoTrans, CblasTrans, CblasConjTrans, TB);
-      if (ldb < K || ldb < 1)
-        info = cblas_errprn(11, info, "ldb must be >= MAX(K,1): ldb=%d K=%d", ldb, K);
-    }
-    if (ldc < N || ldc < 1)
-      info = cblas_errprn(14, info, "ldc must be >= MAX(N,1): ldc=%d N=%d", ldc, N);
-  } else if (Order == CblasColMajor) {
-    if (TA == CblasNoTrans) {
-      if (lda < M || lda < 1)
-        info = cblas_errprn(9, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
-    } else {
-      if (TA != CblasConjTrans && TA != CblasTrans)
-        info = cblas_errprn(2, info, "TransA must be %d, %d or %d, but is set to %d",
-                            CblasNoTrans, CblasTrans, CblasConjTrans, TA);
-      if (lda < K || lda < 1)
-        info = cblas_errprn(9, info, "lda must be >= MAX(K,1): lda=%d K=%d", lda, K);
-    }
-    if (TB == CblasNoTrans) {
-      if (ldb < K || ldb < 1)
-        info = cblas_errprn(11, info, "ldb must be >= MAX(K,1): ldb=%d K=%d", ldb, K);
-    } else {
-      if (TB != CblasConjTrans && TB != CblasTrans)
-        info = cblas_errprn(3, info, "TransB must be %d, %d or %d, but is set to %d",
-                            CblasNoTrans, CblasTrans, CblasConjTrans, TB);
-      if (ldb < N || ldb < 1)
-        info = cblas_errprn(11, info, "ldb must be >= MAX(N,1): ldb=%d N=%d", ldb, N);
-    }
-    if (ldc < M || ldc < 1)
-      info = cblas_errprn(14, info, "ldc must be >= MAX(M,1): ldc=%d M=%d", ldc, M);
-  } else
-    info = cblas_errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
-                        CblasColMajor, Order);
   if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
 
   if (M == 0 || N == 0 || ((alpha == (T)0 || K == 0) && beta == (T)1)) return;
@@ -380,55 +328,6 @@ void NM3(cblas_, PFX, gemm)(const enum CBLAS_ORDER Order, const enum CBLAS_TRANS
   rt_sync(c, rout);
 }
 
-/* Argument checks shared by cblas_?trsm and cblas_?trmm, whose checks are
- * the same (interfaces/blas/C/src/cblas_dtrsm.c:23-75, cblas_dtrmm.c). */
-static int NM(PFX, trxm_check)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
-                               const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
-                               const enum CBLAS_DIAG Diag, const int M, const int N,
-                               const int lda, const int ldb)
-{
-  int info = 2000;
-  if (Order == CblasColMajor) {
-    if (Side == CblasLeft) {
-      if (lda < M || lda < 1)
-        info = cblas_errprn(10, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
-    } else if (Side == CblasRight) {
-      if (lda < N || lda < 1)
-        info = cblas_errprn(10, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
-    } else
-      info = cblas_errprn(2, info, "SIDE must be %d or %d, but is set to %d", CblasRight,
-                          CblasLeft, Side);
-    if (ldb < M || ldb < 1)
-      info = cblas_errprn(12, info, "ldb must be >= MAX(M,1): ldb=%d M=%d", ldb, M);
-  } else if (Order == CblasRowMajor) {
-    if (Side == CblasLeft) {
-      if (lda < M || lda < 1)
-        info = cblas_errprn(10, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
-    } else if (Side == CblasRight) {
-      if (lda < N || lda < 1)
-        info = cblas_errprn(10, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
-    } else
-      info = cblas_errprn(2, info, "SIDE must be %d or %d, but is set to %d", CblasRight,
-                          CblasLeft, Side);
-    if (ldb < N || ldb < 1)
-      info = cblas_errprn(12, info, "ldb must be >= MAX(N,1): ldb=%d N=%d", ldb, N);
-  } else
-    info = cblas_errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
-                        CblasColMajor, Order);
-  if (Uplo != CblasUpper && Uplo != CblasLower)
-    info = cblas_errprn(3, info, "UPLO must be %d or %d, but is set to %d", CblasUpper,
-                        CblasLower, Uplo);
-  if (TA != CblasNoTrans && TA != CblasTrans && TA != CblasConjTrans)
-    info = cblas_errprn(4, info, "TransA must be %d, %d or %d, but is set to %d",
-                        CblasNoTrans, CblasTrans, CblasConjTrans, TA);
-  if (Diag != CblasUnit && Diag != CblasNonUnit)
-    info = cblas_errprn(5, info, "UPLO must be %d or %d, but is set to %d", CblasUnit,
-                        CblasNonUnit, Diag);
-  if (M < 0) info = cblas_errprn(6, info, "M cannot be less than zero; it is set to %d.", M);
-  if (N < 0) info = cblas_errprn(7, info, "N cannot be less than zero; it is set to %d.", N);
-  return info;
-}
-
 /* interfaces/blas/C/src/cblas_dtrsm.c:13-106 */
 void NM3(cblas_, PFX, trsm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
                             const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
@@ -436,7 +335,7 @@ void NM3(cblas_, PFX, trsm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE 
                             const T alpha, const T *A, const int lda, T *B, const int ldb)
 {
   static const char *rout = "cblas_" STR(PFX) "trsm";
-  const int info = NM(PFX, trxm_check)(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
+  const int info = atlmx_trxm_check(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
   int side, uplo, m, n, na;
   atlmx_ctx *c;
   long dla, dlb;
@@ -527,7 +426,7 @@ void NM3(cblas_, PFX, trmm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE 
                             const T alpha, const T *A, const int lda, T *B, const int ldb)
 {
   static const char *rout = "cblas_" STR(PFX) "trmm";
-  const int info = NM(PFX, trxm_check)(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
+  const int info = atlmx_trxm_check(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
   int side, uplo, m, n, na;
   atlmx_ctx *c;
   long dla, dlb;

@@ -1,0 +1,36 @@
+/*
+ * cblas3c.c -- complex Level-3 BLAS (C host code) of libatlas_mi355x:
+ * cblas_{c,z}{gemm,trsm} and their device-pointer twins, instantiated from
+ * cblas3c_tmpl.h per precision over the real MFMA GEMM kernels.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include "rt.h"
+#include "drivers.h"
+#include "../../include/cblas.h"
+#include "../../include/atlmx.h"
+
+#define STR_(x) #x
+#define STR(x) STR_(x)
+
+#define T double
+#define PFX z
+#define K_GEMM atlmx_k_dgemm
+#define K_CGATHER atlmx_k_zgather
+#define K_CSCAL atlmx_k_zscal2d
+#define K_CTRI_INV atlmx_k_ztri_inv
+#include "cblas3c_tmpl.h"
+#undef T
+#undef PFX
+#undef K_GEMM
+#undef K_CGATHER
+#undef K_CSCAL
+#undef K_CTRI_INV
+
+#define T float
+#define PFX c
+#define K_GEMM atlmx_k_sgemm
+#define K_CGATHER atlmx_k_cgather
+#define K_CSCAL atlmx_k_cscal2d
+#define K_CTRI_INV atlmx_k_ctri_inv
+#include "cblas3c_tmpl.h"

@@ -53,4 +53,25 @@ void P##geqrf_dev(atlmx_ctx *c, int M, int N, T *A, long lda, T *d_tau,         
 ATLMX_DRV_DECLARE(d, double)
 ATLMX_DRV_DECLARE(s, float)
 
+/* cblas argument checks shared by the real and complex routines (defined in
+ * blas3.c); each returns 2000 when the arguments are valid, else the
+ * position xerbla reports, after printing ATLAS's message. */
+int atlmx_gemm_check(int Order, int TA, int TB, int M, int N, int K, int lda, int ldb,
+                     int ldc);
+int atlmx_trxm_check(int Order, int Side, int Uplo, int TA, int Diag, int M, int N, int lda,
+                     int ldb);
+
+/* complex drivers (defined in cblas3c.c); alpha and beta point at
+ * {real, imaginary}; matrices interleaved, leading dimensions in complex
+ * elements */
+#define ATLMX_CDRV_DECLARE(P, T)                                                      \
+void P##gemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const T *alpha,   \
+                 const T *A, long lda, const T *B, long ldb, const T *beta, T *C,     \
+                 long ldc, const char *rout);                                         \
+void P##trsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,    \
+                 const T *alpha, const T *A, long lda, T *B, long ldb,                \
+                 const char *rout);
+ATLMX_CDRV_DECLARE(z, double)
+ATLMX_CDRV_DECLARE(c, float)
+
 #endif

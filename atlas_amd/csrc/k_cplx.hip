@@ -1,0 +1,240 @@
+/*
+ * k_cplx.hip -- the complex (c/z) Level-3 pieces around the real MFMA GEMM.
+ *
+ * Column-major complex storage (real and imaginary parts interleaved) read
+ * as a real matrix of twice the rows is exactly the interleaved real view
+ * of the matrix, and a complex product C = A B is the real product
+ *     C_re-view = Ahat * B_re-view,
+ * where Ahat replaces every element a of A by the 2x2 block
+ *     [ Re a  -Im a ]
+ *     [ Im a   Re a ].
+ * The real product does 2m x 2k x n multiply-adds = 8mnk flops, the complex
+ * count, so a complex GEMM is one launch of the real MFMA kernel after an
+ * O(mk) expansion of the left operand.  The kernels here build that
+ * expansion (with op(A), conjugation, alpha and a triangle mask folded in),
+ * plain complex gathers of op(B) and transposes, the complex scaling of C by
+ * beta, and the inverses of the diagonal blocks of a complex triangle (for
+ * TRSM, stored already expanded so the solve stays a chain of real GEMMs).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "kernels.h"
+
+namespace {
+
+constexpr int CT = 32;   /* tile edge of the gather, in complex elements */
+
+/* ------------------------------------------------------------------------
+ * out = alpha * op(A) for an m x n op(A), where op(A)(i,p) is A(i,p)
+ * (trans = 0) or A(p,i) (trans = 1), conjugated when conj is set.
+ *   expand = 1: out is real, 2m x 2n with leading dimension ldo (reals),
+ *               element (i,p) written as its 2x2 block;
+ *   expand = 0: out is complex, m x n with leading dimension ldo (complex).
+ * tri = KX_LOWER / KX_UPPER keeps op(A)'s elements on and below / above the
+ * diagonal and writes zeros elsewhere; unit writes 1 on the diagonal.
+ * One 32x32 tile per 256-thread workgroup; the source tile is read along its
+ * columns and, for trans, turned through LDS so the writes stay contiguous.
+ * ---------------------------------------------------------------------- */
+template <typename T>
+__global__ __launch_bounds__(256) void cgather_kernel(int m, int n, const T *__restrict__ A,
+                                                      long lda, int trans, int conj, T ar,
+                                                      T ai, int expand, T *__restrict__ out,
+                                                      long ldo, int tri, int unit) {
+  __shared__ T sr[CT][CT + 1], si[CT][CT + 1];
+  const int tx = threadIdx.x & (CT - 1), ty = threadIdx.x >> 5;
+  const int i0 = blockIdx.x * CT, p0 = blockIdx.y * CT;
+  /* source rows run along tx in both cases: op row i (trans = 0) or op
+   * column p (trans = 1) */
+  for (int q = ty; q < CT; q += 8) {
+    int sr_, sc_;
+    if (!trans) { sr_ = i0 + tx; sc_ = p0 + q; }
+    else        { sr_ = p0 + tx; sc_ = i0 + q; }
+    const bool in = trans ? (sr_ < n && sc_ < m) : (sr_ < m && sc_ < n);
+    T vr = T(0), vi = T(0);
+    if (in) {
+      const T *a = A + 2 * ((size_t)sc_ * lda + sr_);
+      vr = a[0]; vi = a[1];
+    }
+    if (!trans) { sr[tx][q] = vr; si[tx][q] = vi; }
+    else        { sr[q][tx] = vr; si[q][tx] = vi; }
+  }
+  __syncthreads();
+  for (int q = ty; q < CT; q += 8) {
+    const int i = i0 + tx, p = p0 + q;
+    if (i >= m || p >= n) continue;
+    T xr = sr[tx][q], xi = si[tx][q];
+    if (conj) xi = -xi;
+    T vr = ar * xr - ai * xi, vi = ar * xi + ai * xr;
+    if ((tri == KX_LOWER && i < p) || (tri == KX_UPPER && i > p)) { vr = T(0); vi = T(0); }
+    if (unit && i == p) { vr = T(1); vi = T(0); }
+    if (expand) {
+      T *o0 = out + (size_t)(2 * p) * ldo + 2 * i;
+      T *o1 = o0 + ldo;
+      o0[0] = vr; o0[1] = vi;
+      o1[0] = -vi; o1[1] = vr;
+    } else {
+      T *o = out + 2 * ((size_t)p * ldo + i);
+      o[0] = vr; o[1] = vi;
+    }
+  }
+}
+
+/* C = beta * C for an m x n complex C; beta = 0 writes zeros without reading
+ * C (the Mzgescal of include/atlas_refmisc.h:341-361). */
+template <typename T>
+__global__ __launch_bounds__(256) void cscal_kernel(int m, int n, T br, T bi, T *C, long ldc) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long j = e / m, i = e - j * m;
+    T *c = C + 2 * (j * ldc + i);
+    if (br == T(0) && bi == T(0)) { c[0] = T(0); c[1] = T(0); continue; }
+    const T cr = c[0], ci = c[1];
+    c[0] = br * cr - bi * ci;
+    c[1] = br * ci + bi * cr;
+  }
+}
+
+/* 1/d by Smith's scaling (R. L. Smith, CACM 5(8), 1962): divide by the
+ * larger of |Re d|, |Im d| first so neither square overflows. */
+template <typename T>
+__device__ inline void crecip(T dr, T di, T &rr, T &ri) {
+  if (fabs(di) <= fabs(dr)) {
+    const T t = di / dr, den = dr + di * t;
+    rr = T(1) / den; ri = -t / den;
+  } else {
+    const T t = dr / di, den = di + dr * t;
+    rr = t / den; ri = T(-1) / den;
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * Inverses of the 32x32 complex diagonal blocks of a triangular matrix
+ * held expanded in H (2n x 2n reals, leading dimension ldh, the complex
+ * element (i,j) at H[2j*ldh + 2i], H[2j*ldh + 2i + 1]), written expanded:
+ * block b as a 64x64 real block at inv + b*64*64 (leading dimension 64).
+ * Past the matrix's edge the block is the identity.  One workgroup of 64
+ * threads per block; thread c < 32 builds column c of the inverse by
+ * substitution, with the block and the columns in LDS.
+ * ---------------------------------------------------------------------- */
+template <typename T>
+__global__ __launch_bounds__(64) void ctri_inv_kernel(int lower, int n, const T *__restrict__ H,
+                                                      long ldh, T *__restrict__ inv) {
+  __shared__ T Lr[CT][CT + 1], Li[CT][CT + 1];
+  __shared__ T Xr[CT][CT + 1], Xi[CT][CT + 1];
+  __shared__ T Dr[CT], Di[CT];
+  const int tid = threadIdx.x, b0 = blockIdx.x * CT, nb = min(CT, n - b0);
+  for (int e = tid; e < CT * CT; e += 64) {
+    const int r = e & (CT - 1), c = e >> 5;
+    T vr, vi;
+    if (r < nb && c < nb) {
+      const T *h = H + (size_t)(2 * (b0 + c)) * ldh + 2 * (b0 + r);
+      vr = h[0]; vi = h[1];
+    } else {
+      vr = (r == c) ? T(1) : T(0); vi = T(0);
+    }
+    Lr[r][c] = vr; Li[r][c] = vi;
+  }
+  __syncthreads();
+  if (tid < CT) crecip(Lr[tid][tid], Li[tid][tid], Dr[tid], Di[tid]);
+  __syncthreads();
+  if (tid < CT) {
+    const int c = tid;
+    for (int r = 0; r < CT; ++r) { Xr[r][c] = T(0); Xi[r][c] = T(0); }
+    Xr[c][c] = Dr[c]; Xi[c][c] = Di[c];
+    if (lower) {
+      for (int r = c + 1; r < CT; ++r) {
+        T sr = T(0), si = T(0);
+        for (int t = c; t < r; ++t) {
+          sr += Lr[r][t] * Xr[t][c] - Li[r][t] * Xi[t][c];
+          si += Lr[r][t] * Xi[t][c] + Li[r][t] * Xr[t][c];
+        }
+        Xr[r][c] = -(sr * Dr[r] - si * Di[r]);
+        Xi[r][c] = -(sr * Di[r] + si * Dr[r]);
+      }
+    } else {
+      for (int r = c - 1; r >= 0; --r) {
+        T sr = T(0), si = T(0);
+        for (int t = r + 1; t <= c; ++t) {
+          sr += Lr[r][t] * Xr[t][c] - Li[r][t] * Xi[t][c];
+          si += Lr[r][t] * Xi[t][c] + Li[r][t] * Xr[t][c];
+        }
+        Xr[r][c] = -(sr * Dr[r] - si * Di[r]);
+        Xi[r][c] = -(sr * Di[r] + si * Dr[r]);
+      }
+    }
+  }
+  __syncthreads();
+  T *o = inv + (size_t)blockIdx.x * 64 * 64;
+  for (int e = tid; e < CT * CT; e += 64) {
+    const int r = e & (CT - 1), c = e >> 5;
+    const T vr = Xr[r][c], vi = Xi[r][c];
+    T *o0 = o + (size_t)(2 * c) * 64 + 2 * r, *o1 = o0 + 64;
+    o0[0] = vr; o0[1] = vi;
+    o1[0] = -vi; o1[1] = vr;
+  }
+}
+
+template <typename T>
+hipError_t launch_gather(int m, int n, const T *A, long lda, int trans, int conj, T ar, T ai,
+                         int expand, T *out, long ldo, int tri, int unit, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (!A || !out) return hipErrorInvalidValue;
+  if (lda < (trans ? n : m)) return hipErrorInvalidValue;
+  if (ldo < (expand ? 2L * m : (long)m)) return hipErrorInvalidValue;
+  const unsigned gx = (unsigned)((m + CT - 1) / CT), gy = (unsigned)((n + CT - 1) / CT);
+  if (gy > 65535u) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((cgather_kernel<T>), dim3(gx, gy), dim3(256), 0, s, m, n, A, lda, trans,
+                     conj, ar, ai, expand, out, ldo, tri, unit);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_cscal(int m, int n, T br, T bi, T *C, long ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (!C || ldc < m) return hipErrorInvalidValue;
+  const long total = (long)m * n;
+  const long want = (total + 255) / 256;
+  const unsigned g = (unsigned)(want < 8192 ? want : 8192);
+  hipLaunchKernelGGL((cscal_kernel<T>), dim3(g), dim3(256), 0, s, m, n, br, bi, C, ldc);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ctri_inv(int lower, int n, const T *H, long ldh, T *inv, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!H || !inv || ldh < 2L * n) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((ctri_inv_kernel<T>), dim3((n + CT - 1) / CT), dim3(64), 0, s, lower, n,
+                     H, ldh, inv);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" hipError_t atlmx_k_zgather(int m, int n, const double *A, long lda, int trans,
+                                      int conj, double ar, double ai, int expand, double *out,
+                                      long ldo, int tri, int unit, hipStream_t s) {
+  return launch_gather<double>(m, n, A, lda, trans, conj, ar, ai, expand, out, ldo, tri, unit,
+                               s);
+}
+extern "C" hipError_t atlmx_k_cgather(int m, int n, const float *A, long lda, int trans,
+                                      int conj, float ar, float ai, int expand, float *out,
+                                      long ldo, int tri, int unit, hipStream_t s) {
+  return launch_gather<float>(m, n, A, lda, trans, conj, ar, ai, expand, out, ldo, tri, unit,
+                              s);
+}
+extern "C" hipError_t atlmx_k_zscal2d(int m, int n, double br, double bi, double *C, long ldc,
+                                      hipStream_t s) {
+  return launch_cscal<double>(m, n, br, bi, C, ldc, s);
+}
+extern "C" hipError_t atlmx_k_cscal2d(int m, int n, float br, float bi, float *C, long ldc,
+                                      hipStream_t s) {
+  return launch_cscal<float>(m, n, br, bi, C, ldc, s);
+}
+extern "C" hipError_t atlmx_k_ztri_inv(int lower, int n, const double *H, long ldh, double *inv,
+                                       hipStream_t s) {
+  return launch_ctri_inv<double>(lower, n, H, ldh, inv, s);
+}
+extern "C" hipError_t atlmx_k_ctri_inv(int lower, int n, const float *H, long ldh, float *inv,
+                                       hipStream_t s) {
+  return launch_ctri_inv<float>(lower, n, H, ldh, inv, s);
+}

@@ -17,6 +17,7 @@ extern "C" {
 /* Operand codes used by the launchers (values of ATLAS include/cblas.h:5-10). */
 #define KX_NOTRANS 111
 #define KX_TRANS   112
+#define KX_CONJTRANS 113
 #define KX_UPPER   121
 #define KX_LOWER   122
 #define KX_NONUNIT 131
@@ -359,6 +360,34 @@ hipError_t atlmx_k_slarft(int w, const float *G, const float *tau, float *Tm, hi
 
 /* Fill n ints with v (stream-ordered, no memset node). */
 hipError_t atlmx_k_fill_i32(int *p, int n, int v, hipStream_t s);
+
+/* ---- complex (z: double, c: float; interleaved real/imaginary parts;
+ * leading dimensions in complex elements unless marked as reals) ---- */
+/* out = (ar + i ai) * op(A), op(A) m x n: A(i,p) or, for trans, A(p,i),
+ * conjugated when conj.  expand = 1: out is the real 2m x 2n matrix with
+ * each element as the 2x2 block [Re -Im; Im Re] (ldo in reals); expand = 0:
+ * out is complex m x n (ldo in complex elements).  tri = KX_LOWER/KX_UPPER
+ * zeroes op(A) outside that triangle; unit puts 1 on the diagonal. */
+hipError_t atlmx_k_zgather(int m, int n, const double *A, long lda, int trans, int conj,
+                           double ar, double ai, int expand, double *out, long ldo, int tri,
+                           int unit, hipStream_t s);
+hipError_t atlmx_k_cgather(int m, int n, const float *A, long lda, int trans, int conj,
+                           float ar, float ai, int expand, float *out, long ldo, int tri,
+                           int unit, hipStream_t s);
+/* C = (br + i bi) C; zero beta writes zeros without reading C */
+hipError_t atlmx_k_zscal2d(int m, int n, double br, double bi, double *C, long ldc,
+                           hipStream_t s);
+hipError_t atlmx_k_cscal2d(int m, int n, float br, float bi, float *C, long ldc, hipStream_t s);
+/* Inverses of the 32x32 complex diagonal blocks of the triangle (lower or
+ * upper) held expanded in H (2n x 2n reals, ldh in reals), written expanded
+ * as 64x64 real blocks (ld 64) at inv + b*64*64; identity past the edge. */
+#define ATLMX_CTRI_NB 32
+/* largest expanded left operand a complex GEMM builds at once (row chunks) */
+#define ATLMX_CPLX_WS_BYTES (1024.0 * 1024.0 * 1024.0)
+hipError_t atlmx_k_ztri_inv(int lower, int n, const double *H, long ldh, double *inv,
+                            hipStream_t s);
+hipError_t atlmx_k_ctri_inv(int lower, int n, const float *H, long ldh, float *inv,
+                            hipStream_t s);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,17 @@ void atlmx_dtrsm(int Side, int Uplo, int TA, int Diag, int M, int N, double alph
                  const double *A, int lda, double *B, int ldb);
 void atlmx_strsm(int Side, int Uplo, int TA, int Diag, int M, int N, float alpha,
                  const float *A, int lda, float *B, int ldb);
+/* complex twins (interleaved storage, leading dimensions in complex
+ * elements, alpha/beta point at host {real, imaginary}; TA/TB may be
+ * ConjTrans = 113) */
+void atlmx_zgemm(int TA, int TB, int M, int N, int K, const double *alpha, const double *A,
+                 int lda, const double *B, int ldb, const double *beta, double *C, int ldc);
+void atlmx_cgemm(int TA, int TB, int M, int N, int K, const float *alpha, const float *A,
+                 int lda, const float *B, int ldb, const float *beta, float *C, int ldc);
+void atlmx_ztrsm(int Side, int Uplo, int TA, int Diag, int M, int N, const double *alpha,
+                 const double *A, int lda, double *B, int ldb);
+void atlmx_ctrsm(int Side, int Uplo, int TA, int Diag, int M, int N, const float *alpha,
+                 const float *A, int lda, float *B, int ldb);
 void atlmx_dsyrk(int Uplo, int Trans, int N, int K, double alpha, const double *A,
                  int lda, double beta, double *C, int ldc);
 void atlmx_ssyrk(int Uplo, int Trans, int N, int K, float alpha, const float *A, int lda,

@@ -112,6 +112,30 @@ void cblas_dsyr2k(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
                   const double alpha, const double *A, const int lda, const double *B,
                   const int ldb, const double beta, double *C, const int ldc);
 
+/* ---- complex (c: float, z: double; interleaved real/imaginary parts,
+ * alpha and beta point at {real, imaginary}; TransA/TransB may be
+ * CblasConjTrans) ---- */
+/* Replaces ATLAS include/cblas.h:501 / :531 and
+ * interfaces/blas/C/src/cblas_zgemm.c:16-136 (ATL_zgemm). */
+void cblas_cgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_TRANSPOSE TransB, const int M, const int N,
+                 const int K, const void *alpha, const void *A, const int lda,
+                 const void *B, const int ldb, const void *beta, void *C, const int ldc);
+void cblas_zgemm(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_TRANSPOSE TransB, const int M, const int N,
+                 const int K, const void *alpha, const void *A, const int lda,
+                 const void *B, const int ldb, const void *beta, void *C, const int ldc);
+/* Replaces ATLAS include/cblas.h:525 / :555 and
+ * interfaces/blas/C/src/cblas_ztrsm.c:13-106 (ATL_ztrsm). */
+void cblas_ctrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, void *B, const int ldb);
+void cblas_ztrsm(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                 const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TransA,
+                 const enum CBLAS_DIAG Diag, const int M, const int N, const void *alpha,
+                 const void *A, const int lda, void *B, const int ldb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,3 +137,22 @@ void orc_track_pivot_gaps(double *gaps, int n)
 #define ORC_SAFMIN FLT_MIN          /* LAPACK slamch('S') */
 #define ORC_GEMMTST_EPS 5.0e-7      /* include/atlas_misc.h:107 */
 #include "orc_tmpl.h"
+#undef T
+#undef P
+#undef ORC_ABS
+#undef ORC_SQRT
+#undef ORC_SAFMIN
+#undef ORC_GEMMTST_EPS
+
+/* ---- complex double / complex float ---- */
+#define T double
+#define P orc_z
+#define ORC_ABS(x) fabs(x)
+#include "orc_cplx.h"
+#undef T
+#undef P
+#undef ORC_ABS
+#define T float
+#define P orc_c
+#define ORC_ABS(x) fabsf(x)
+#include "orc_cplx.h"

@@ -124,6 +124,18 @@ T    P##qr_resid(int M, int N, const T *QR, int lda, const T *tau,            \
 ORC_DECLARE(orc_d, double)
 ORC_DECLARE(orc_s, float)
 
+/* complex (z: double, c: float): interleaved storage, leading dimensions in
+ * complex elements, alpha/beta point at {real, imaginary} */
+#define ORC_CDECLARE(P, T)                                                    \
+void P##gemm_ref(int TA, int TB, int M, int N, int K, const T *alpha,        \
+                  const T *A, int lda, const T *B, int ldb, const T *beta,    \
+                  T *C, int ldc);                                             \
+void P##trsm_ref(int side, int uplo, int TA, int diag, int M, int N,         \
+                  const T *alpha, const T *A, int lda, T *B, int ldb);
+
+ORC_CDECLARE(orc_z, double)
+ORC_CDECLARE(orc_c, float)
+
 #ifdef __cplusplus
 }
 #endif

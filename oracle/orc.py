@@ -127,6 +127,9 @@ def _declare(L):
         g("geqr2").argtypes = [_I, _I, _P, _I, _P]
         g("qr_resid").argtypes = [_I, _I, _P, _I, _P, _P, _I]
         g("qr_resid").restype = t
+    for p in ("orc_z", "orc_c"):
+        getattr(L, p + "gemm_ref").argtypes = [_I] * 5 + [_P, _P, _I, _P, _I, _P, _P, _I]
+        getattr(L, p + "trsm_ref").argtypes = [_I] * 6 + [_P, _P, _I, _P, _I]
 
 
 def _p(a):
@@ -209,6 +212,43 @@ def gemm(ta, tb, alpha, A, B, beta, C, exact_order=False, order=COL):
         _fn(dt, "gemm")(order, ta, tb, M, N, K, alpha, _p(A), _ld(A), _p(B), _ld(B),
                         beta, _p(C), _ld(C))
     return C
+
+
+def _cpre(dtype):
+    dt = np.dtype(dtype)
+    if dt == np.complex128:
+        return "orc_z", np.float64
+    if dt == np.complex64:
+        return "orc_c", np.float32
+    raise TypeError("complex128 or complex64 expected, not %s" % dt)
+
+
+def _cscalar(x, rt):
+    """A complex scalar as the {real, imaginary} pair the C side reads."""
+    return np.array([complex(x).real, complex(x).imag], dtype=rt)
+
+
+def cgemm(ta, tb, alpha, A, B, beta, Cm):
+    """Complex Cm (Fortran, in place) = alpha op(A) op(B) + beta Cm in the
+    reference loop order (ATL_zrefgemm*.c); op may be CONJTRANS."""
+    p, rt = _cpre(Cm.dtype)
+    M, N = Cm.shape
+    K = A.shape[1] if ta == NOTRANS else A.shape[0]
+    al, be = _cscalar(alpha, rt), _cscalar(beta, rt)
+    getattr(lib(), p + "gemm_ref")(ta, tb, M, N, K, _p(al), _p(A), _ld(A), _p(B), _ld(B),
+                                   _p(be), _p(Cm), _ld(Cm))
+    return Cm
+
+
+def ctrsm(side, uplo, ta, diag, alpha, A, B):
+    """Complex B (Fortran, in place) = alpha inv(op(A)) B (Left) or
+    alpha B inv(op(A)) (Right); op may be CONJTRANS."""
+    p, rt = _cpre(B.dtype)
+    M, N = B.shape
+    al = _cscalar(alpha, rt)
+    getattr(lib(), p + "trsm_ref")(side, uplo, ta, diag, M, N, _p(al), _p(A), _ld(A), _p(B),
+                                   _ld(B))
+    return B
 
 
 def trsm(side, uplo, ta, diag, alpha, A, B, order=COL):

@@ -1,0 +1,180 @@
+/*
+ * ORACLE -- test infrastructure only (see oracle.h).
+ *
+ * Complex precision template: included by oracle.c with T=double, P=orc_z
+ * and with T=float, P=orc_c.  Matrices are column-major with the real and
+ * imaginary parts interleaved (ATLAS's complex storage); leading dimensions
+ * count complex elements; alpha and beta point at {real, imaginary}.
+ */
+#define F_(name) CAT_(P, name)
+
+/* One complex multiply-add, c += a*b, in the operation order of the
+ * reference's Mmla (include/atlas_refmisc.h:170-174). */
+#define ORC_CMLA(ar, ai, br, bi, cr, ci)      \
+   do {                                     \
+      (cr) += (ar) * (br) - (ai) * (bi);    \
+      (ci) += (ar) * (bi) + (ai) * (br);    \
+   } while (0)
+
+/* c = beta*c as the reference's Mzelscal (include/atlas_refmisc.h:233-239):
+ * beta = 0 stores zeros, beta = 1 leaves c alone, else Mdscl. */
+static void F_(celscal)(const T *beta, T *c)
+{
+   if (beta[0] == 0 && beta[1] == 0) { c[0] = c[1] = 0; return; }
+   if (beta[0] == 1 && beta[1] == 0) return;
+   {
+      const T r = beta[0] * c[0] - beta[1] * c[1];
+      const T i = beta[0] * c[1] + beta[1] * c[0];
+      c[0] = r; c[1] = i;
+   }
+}
+
+/* src/blas/reference/level3/ATL_zrefgemm.c (dispatcher, quick returns,
+ * alpha = 0 -> Mzgescal(C, beta)) and the loop orders of
+ * ATL_zrefgemm{NN,NT,NC}.c (op(A) = A: per column j, C(:,j) = beta C(:,j),
+ * then for l = 0..K-1, t = alpha*op(B)(l,j) by Mmul and C(:,j) += A(:,l) t)
+ * and ATL_zrefgemm{T,C}{N,T,C}.c (op(A) = A^T or A^H: per (i,j) the dot
+ * product t = sum_l op(A)(i,l) op(B)(l,j) from zero, then C(i,j) =
+ * beta C(i,j) by Mzelscal and C(i,j) += alpha t).  Conjugation negates the
+ * imaginary part before the product, as the reference's -A[..+1]. */
+void F_(gemm_ref)(int TA, int TB, int M, int N, int K, const T *alpha, const T *A, int lda,
+                   const T *B, int ldb, const T *beta, T *C, int ldc)
+{
+   const int ta = (TA != ORC_NOTRANS), ca = (TA == ORC_CONJTRANS);
+   const int tb = (TB != ORC_NOTRANS), cb = (TB == ORC_CONJTRANS);
+   const int a_zero = (alpha[0] == 0 && alpha[1] == 0);
+   int i, j, l;
+   if (M == 0 || N == 0 ||
+       ((a_zero || K == 0) && beta[0] == 1 && beta[1] == 0)) return;
+   if (a_zero)
+   {
+      for (j = 0; j < N; j++)
+         for (i = 0; i < M; i++) F_(celscal)(beta, C + 2*((size_t)j*ldc + i));
+      return;
+   }
+   if (!ta)
+   {
+      for (j = 0; j < N; j++)
+      {
+         T *c = C + 2*(size_t)j*ldc;
+         for (i = 0; i < M; i++) F_(celscal)(beta, c + 2*i);
+         for (l = 0; l < K; l++)
+         {
+            const T *a = A + 2*(size_t)l*lda;
+            const T *b = tb ? B + 2*((size_t)l*ldb + j) : B + 2*((size_t)j*ldb + l);
+            const T br = b[0], bi = cb ? -b[1] : b[1];
+            const T tr = alpha[0] * br - alpha[1] * bi;
+            const T ti = alpha[0] * bi + alpha[1] * br;
+            for (i = 0; i < M; i++) ORC_CMLA(a[2*i], a[2*i+1], tr, ti, c[2*i], c[2*i+1]);
+         }
+      }
+   }
+   else
+   {
+      for (j = 0; j < N; j++)
+         for (i = 0; i < M; i++)
+         {
+            const T *a = A + 2*(size_t)i*lda;
+            T *c = C + 2*((size_t)j*ldc + i);
+            T tr = 0, ti = 0;
+            for (l = 0; l < K; l++)
+            {
+               const T *b = tb ? B + 2*((size_t)l*ldb + j) : B + 2*((size_t)j*ldb + l);
+               const T ar = a[2*l], ai = ca ? -a[2*l+1] : a[2*l+1];
+               const T br = b[0], bi = cb ? -b[1] : b[1];
+               ORC_CMLA(ar, ai, br, bi, tr, ti);
+            }
+            F_(celscal)(beta, c);
+            ORC_CMLA(alpha[0], alpha[1], tr, ti, c[0], c[1]);
+         }
+   }
+}
+
+/* (xr + i xi) / (dr + i di) by Smith's algorithm, the form of the
+ * reference's Mddiv (include/atlas_refmisc.h:131-155; D. Knuth, TAOCP
+ * vol. 2, p. 195). */
+static void F_(cdiv)(T dr, T di, T *xr, T *xi)
+{
+   T r, den, cr, ci;
+   if (ORC_ABS(di) < ORC_ABS(dr))
+   {
+      r = di / dr; den = dr + di * r;
+      cr = (*xr + *xi * r) / den;
+      ci = (*xi - *xr * r) / den;
+   }
+   else
+   {
+      r = dr / di; den = di + dr * r;
+      cr = (*xi + *xr * r) / den;
+      ci = (-*xr + *xi * r) / den;
+   }
+   *xr = cr; *xi = ci;
+}
+
+/* The effective left operand of a solve: E(i,k) = op(A)(i,k) for Left, and
+ * op(A)(k,i) (so that X op(A) = B is E X^T = B^T) for Right. */
+static void F_(trsm_elem)(const T *A, int lda, int tr, int cj, int i, int k, T *er, T *ei)
+{
+   const T *a = tr ? A + 2*((size_t)i*lda + k) : A + 2*((size_t)k*lda + i);
+   *er = a[0];
+   *ei = cj ? -a[1] : a[1];
+}
+
+/* Complex TRSM, all 24 cases of src/blas/reference/level3/ATL_zreftrsm.c
+ * (dispatcher: M = 0 or N = 0 returns, alpha = 0 zeroes B).  Each
+ * right-hand side (a column of B for Left, a row for Right) is scaled by
+ * alpha and solved against the effective triangle E by substitution in dot
+ * form, dividing by the diagonal with Smith's algorithm as Mddiv.  The
+ * value, not the reference's operation order, is restated: the tests pin it
+ * against ATL_zreftrsm (oracle/_ref) and numpy within rounding. */
+void F_(trsm_ref)(int side, int uplo, int TA, int diag, int M, int N, const T *alpha,
+                   const T *A, int lda, T *B, int ldb)
+{
+   const int left = (side == ORC_LEFT);
+   const int na = left ? M : N, nr = left ? N : M;
+   const int cj = (TA == ORC_CONJTRANS);
+   /* E = op(A) (Left) or op(A)^T (Right) in terms of A: transposed? */
+   const int tr = left ? (TA != ORC_NOTRANS) : (TA == ORC_NOTRANS);
+   const int lower = ((uplo == ORC_LOWER) != tr);
+   int r, i, k;
+   if (M == 0 || N == 0) return;
+   for (r = 0; r < nr; r++)
+   {
+      /* element q of this right-hand side */
+#define XE(q) (left ? B + 2*((size_t)r*ldb + (q)) : B + 2*((size_t)(q)*ldb + r))
+      for (i = 0; i < na; i++)
+      {
+         T *x = XE(i);
+         if (alpha[0] == 0 && alpha[1] == 0) { x[0] = x[1] = 0; continue; }
+         {
+            const T xr = alpha[0] * x[0] - alpha[1] * x[1];
+            const T xi = alpha[0] * x[1] + alpha[1] * x[0];
+            x[0] = xr; x[1] = xi;
+         }
+      }
+      if (alpha[0] == 0 && alpha[1] == 0) continue;
+      for (k = 0; k < na; k++)
+      {
+         const int ii = lower ? k : na - 1 - k;
+         T *x = XE(ii), sr = x[0], si = x[1], er, ei;
+         const int j0 = lower ? 0 : ii + 1, j1 = lower ? ii : na;
+         int jj;
+         for (jj = j0; jj < j1; jj++)
+         {
+            const T *y = XE(jj);
+            F_(trsm_elem)(A, lda, tr, cj, ii, jj, &er, &ei);
+            sr -= er * y[0] - ei * y[1];
+            si -= er * y[1] + ei * y[0];
+         }
+         if (diag != ORC_UNIT)
+         {
+            F_(trsm_elem)(A, lda, tr, cj, ii, ii, &er, &ei);
+            F_(cdiv)(er, ei, &sr, &si);
+         }
+         x[0] = sr; x[1] = si;
+      }
+#undef XE
+   }
+}
+
+#undef F_

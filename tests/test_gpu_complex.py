@@ -1,0 +1,218 @@
+"""GPU parity of the complex Level-3 BLAS (cblas_{z,c}gemm, cblas_{z,c}trsm)
+through the C ABI against the oracle (whose GEMM is bit-exact with ATLAS's
+ATL_zrefgemm / ATL_crefgemm), with the complex form of gemmtst's elementwise
+bound (bin/gemmtst.c:395): |D - T| <= 4 K eps (|alpha| |op A| |op B| +
+|beta| |C|) element by element, and l3blastst's TRSM residual."""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+import atlas_amd as X
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(77)
+OPS = [X.NoTrans, X.Trans, X.ConjTrans]
+
+
+def _cplx(shape, dt):
+    a = RNG.uniform(-0.5, 0.5, shape) + 1j * RNG.uniform(-0.5, 0.5, shape)
+    return np.asfortranarray(a.astype(dt))
+
+
+def _op(M, t):
+    M = M.astype(np.complex128)
+    if t == X.NoTrans:
+        return M
+    if t == X.Trans:
+        return M.T
+    return M.conj().T
+
+
+def _eps(dt):
+    return np.finfo(np.float64 if dt == np.complex128 else np.float32).eps
+
+
+def _check_gemm(D, T, ta, tb, alpha, A, B, beta, C0):
+    K = A.shape[1] if ta == X.NoTrans else A.shape[0]
+    bound = 4 * max(K, 1) * _eps(D.dtype) * (
+        abs(alpha) * (np.abs(_op(A, ta)) @ np.abs(_op(B, tb))) + abs(beta) * np.abs(C0)) + 1e-300
+    err = np.abs(D.astype(np.complex128) - T.astype(np.complex128))
+    assert np.all(err <= bound), (float(np.max(err / bound)), ta, tb, alpha, beta)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("ta,tb", list(itertools.product(OPS, repeat=2)))
+@pytest.mark.parametrize("M,N,K", [(1, 1, 1), (17, 9, 5), (130, 129, 33), (255, 64, 257),
+                                   (3, 200, 300), (128, 128, 0)])
+def test_cgemm_shapes_vs_oracle(dt, ta, tb, M, N, K):
+    A = _cplx((M, K) if ta == X.NoTrans else (K, M), dt)
+    B = _cplx((K, N) if tb == X.NoTrans else (N, K), dt)
+    C0 = _cplx((M, N), dt)
+    alpha, beta = 1.25 - 0.5j, -0.5 + 0.25j
+    D = X.cgemm(ta, tb, alpha, A, B, beta, C0.copy(order="F"))
+    T = O.cgemm(ta, tb, alpha, A, B, beta, C0.copy(order="F"))
+    _check_gemm(D, T, ta, tb, alpha, A, B, beta, C0)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_cgemm_alpha_beta_special_cases(dt):
+    """alpha = 0 only scales C; beta = 0 never reads C (NaN there stays out);
+    real, imaginary and unit betas; imaginary alpha."""
+    M, N, K = 70, 45, 38
+    for ta, tb in [(X.NoTrans, X.NoTrans), (X.ConjTrans, X.Trans), (X.Trans, X.ConjTrans)]:
+        A = _cplx((M, K) if ta == X.NoTrans else (K, M), dt)
+        B = _cplx((K, N) if tb == X.NoTrans else (N, K), dt)
+        for alpha, beta in [(0.0, 0.5 - 1j), (0.0, 0.0), (0.0, 1.0), (1.0, 0.0), (0.5j, 1.0),
+                            (-1.0, 2.0), (0.3 + 0.7j, -1j)]:
+            C0 = _cplx((M, N), dt)
+            if beta == 0:
+                C0[:, 3] = np.nan
+            D = X.cgemm(ta, tb, alpha, A, B, beta, C0.copy(order="F"))
+            T = O.cgemm(ta, tb, alpha, A, B, beta, C0.copy(order="F"))
+            assert np.all(np.isfinite(D)), (alpha, beta)
+            if alpha == 0:
+                assert np.array_equal(D, T), (alpha, beta)
+            _check_gemm(D, T, ta, tb, alpha, A, B, beta, np.nan_to_num(C0))
+
+
+def test_zgemm_integer_exact():
+    """Gaussian integers: every MFMA partial sum is exact, so the GPU must
+    agree bit for bit with int64 arithmetic for every op pair."""
+    M, N, K = 150, 90, 200
+    for ta, tb in itertools.product(OPS, repeat=2):
+        Ai = RNG.integers(-4, 5, (M, K)) + 1j * RNG.integers(-4, 5, (M, K))
+        Bi = RNG.integers(-4, 5, (K, N)) + 1j * RNG.integers(-4, 5, (K, N))
+        # stored so that op(A) == Ai and op(B) == Bi
+        A = np.asfortranarray(Ai if ta == X.NoTrans else (Ai.T if ta == X.Trans else Ai.conj().T))
+        B = np.asfortranarray(Bi if tb == X.NoTrans else (Bi.T if tb == X.Trans else Bi.conj().T))
+        for dt in (np.complex128, np.complex64):
+            D = X.cgemm(ta, tb, 1.0, A.astype(dt), B.astype(dt), 0.0,
+                        np.zeros((M, N), dtype=dt, order="F"))
+            assert np.array_equal(D, (Ai @ Bi).astype(dt)), (ta, tb, dt)
+
+
+def test_zgemm_rowmajor_and_lda_gaps():
+    M, N, K, ld = 61, 47, 35, 80
+    L = X.lib()
+    for ta, tb in [(X.NoTrans, X.NoTrans), (X.ConjTrans, X.NoTrans), (X.Trans, X.ConjTrans)]:
+        # RowMajor: C-ordered arrays
+        A = np.ascontiguousarray(_cplx((M, K) if ta == X.NoTrans else (K, M), np.complex128))
+        B = np.ascontiguousarray(_cplx((K, N) if tb == X.NoTrans else (N, K), np.complex128))
+        C0 = np.ascontiguousarray(_cplx((M, N), np.complex128))
+        D = C0.copy()
+        al = np.array([0.5, -1.5]); be = np.array([2.0, 0.5])
+        L.cblas_zgemm(X.RowMajor, ta, tb, M, N, K, al.ctypes.data, A.ctypes.data, A.shape[1],
+                      B.ctypes.data, B.shape[1], be.ctypes.data, D.ctypes.data, N)
+        want = (0.5 - 1.5j) * (_op(A, ta) @ _op(B, tb)) + (2.0 + 0.5j) * C0
+        assert np.max(np.abs(D - want)) < 1e-12 * K
+    # ColMajor with leading-dimension gaps: the gaps stay untouched
+    A = np.asfortranarray(np.full((ld, K), 7 + 7j))
+    A[:M] = _cplx((M, K), np.complex128)
+    B = np.asfortranarray(np.full((ld, N), 7 + 7j))
+    B[:K] = _cplx((K, N), np.complex128)
+    Cm = np.asfortranarray(np.full((ld, N), -9e27 + 0j))
+    Cm[:M] = _cplx((M, N), np.complex128)
+    C0 = Cm.copy(order="F")
+    al = np.array([1.0, 0.0]); be = np.array([0.0, 1.0])
+    L.cblas_zgemm(X.ColMajor, X.NoTrans, X.NoTrans, M, N, K, al.ctypes.data, A.ctypes.data, ld,
+                  B.ctypes.data, ld, be.ctypes.data, Cm.ctypes.data, ld)
+    assert np.all(Cm[M:] == -9e27 + 0j)
+    want = A[:M] @ B[:K] + 1j * C0[:M]
+    assert np.max(np.abs(Cm[:M] - want)) < 1e-12 * K
+
+
+def test_zgemm_2048_vs_numpy():
+    n = 2048
+    A = _cplx((n, n), np.complex128)
+    B = _cplx((n, n), np.complex128)
+    C0 = _cplx((n, n), np.complex128)
+    D = X.cgemm(X.NoTrans, X.ConjTrans, 1.0 + 0.5j, A, B, 0.5, C0.copy(order="F"))
+    want = (1.0 + 0.5j) * (A @ B.conj().T) + 0.5 * C0
+    bound = 4 * n * _eps(np.complex128) * (abs(1 + 0.5j) * (np.abs(A) @ np.abs(B).T)
+                                           + 0.5 * np.abs(C0))
+    assert np.all(np.abs(D - want) <= bound)
+
+
+def test_cgemm_row_chunks_of_the_expanded_operand():
+    """K large enough that the expanded left operand is built in several row
+    chunks (1 GiB each): 600 rows of K = 2^18 in complex64 make three."""
+    M, N, K = 600, 3, 1 << 18
+    A = _cplx((M, K), np.complex64)
+    B = _cplx((K, N), np.complex64)
+    D = X.cgemm(X.NoTrans, X.NoTrans, 1.0, A, B, 0.0, np.zeros((M, N), np.complex64, order="F"))
+    want = A.astype(np.complex128) @ B.astype(np.complex128)
+    bound = 4 * K * _eps(np.complex64) * (np.abs(A) @ np.abs(B))
+    assert np.all(np.abs(D - want) <= bound)
+
+
+def _tri(n, uplo, dt, junk=True):
+    A = _cplx((n, n), dt)
+    A[np.diag_indices(n)] += np.asarray(2.0 + 1.0j, dtype=dt)
+    if junk:
+        mask = np.tril(np.ones((n, n), bool), -1) if uplo == X.Upper else \
+            np.triu(np.ones((n, n), bool), 1)
+        A[mask] = np.asarray(1e30 - 1e30j, dtype=dt)
+    return A
+
+
+def _trsm_resid(side, uplo, ta, diag, alpha, A, Xs, B0):
+    """l3blastst's TRSM check: ||op(A) X - alpha B|| / (||A|| ||X|| eps n)."""
+    n = A.shape[0]
+    T = np.tril(A) if uplo == X.Lower else np.triu(A)
+    T = T.astype(np.complex128)
+    if diag == X.Unit:
+        T[np.diag_indices(n)] = 1.0
+    E = _op(T, ta)
+    Xd = Xs.astype(np.complex128)
+    R = E @ Xd if side == X.Left else Xd @ E
+    R -= alpha * B0.astype(np.complex128)
+    den = np.abs(E).sum(0).max() * max(np.abs(Xd).sum(0).max(), 1e-300) * _eps(Xs.dtype) * n
+    return np.abs(R).sum(0).max() / den
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("side", [X.Left, X.Right])
+@pytest.mark.parametrize("uplo", [X.Upper, X.Lower])
+@pytest.mark.parametrize("ta", OPS)
+@pytest.mark.parametrize("diag", [X.NonUnit, X.Unit])
+def test_ctrsm_all_cases_vs_oracle(dt, side, uplo, ta, diag):
+    for M, N in [(1, 1), (37, 5), (100, 70), (257, 33)]:
+        na = M if side == X.Left else N
+        A = _tri(na, uplo, dt)
+        B0 = _cplx((M, N), dt)
+        alpha = 0.75 - 0.25j
+        D = X.ctrsm(side, uplo, ta, diag, alpha, A, B0.copy(order="F"))
+        T = O.ctrsm(side, uplo, ta, diag, alpha, A, B0.copy(order="F"))
+        assert _trsm_resid(side, uplo, ta, diag, alpha, A, D, B0) < 1.0
+        tol = 1e-9 if dt == np.complex128 else 2e-3
+        assert np.max(np.abs(D - T)) <= tol * max(1.0, np.max(np.abs(T))), (M, N)
+
+
+def test_ztrsm_alpha_zero_rowmajor_and_large():
+    L = X.lib()
+    # alpha = 0: B = 0 whatever it held
+    A = _tri(20, X.Lower, np.complex128)
+    B = _cplx((20, 6), np.complex128)
+    B[0, 0] = np.nan
+    X.ctrsm(X.Left, X.Lower, X.NoTrans, X.NonUnit, 0.0, A, B)
+    assert np.all(B == 0)
+    # RowMajor: C-ordered arrays, solve X op(A) = alpha B on the right
+    M, N = 50, 40
+    A = np.ascontiguousarray(_tri(N, X.Upper, np.complex128, junk=False))
+    B0 = np.ascontiguousarray(_cplx((M, N), np.complex128))
+    Bm = B0.copy()
+    al = np.array([1.0, 1.0])
+    L.cblas_ztrsm(X.RowMajor, X.Right, X.Upper, X.ConjTrans, X.NonUnit, M, N, al.ctypes.data,
+                  A.ctypes.data, N, Bm.ctypes.data, N)
+    E = np.triu(A).conj().T
+    assert np.max(np.abs(Bm @ E - (1 + 1j) * B0)) < 1e-11
+    # 2000 x 300, the blocked path over many 32-row blocks
+    n, nr = 2000, 300
+    A = _tri(n, X.Lower, np.complex128, junk=False)
+    A[np.tril_indices(n, -1)] /= n    # well conditioned
+    B0 = _cplx((n, nr), np.complex128)
+    D = X.ctrsm(X.Left, X.Lower, X.ConjTrans, X.NonUnit, 1.0, A, B0.copy(order="F"))
+    assert _trsm_resid(X.Left, X.Lower, X.ConjTrans, X.NonUnit, 1.0, A, D, B0) < 1.0

@@ -1,0 +1,156 @@
+"""Pins the oracle's complex BLAS restatement (oracle/orc_cplx.h).
+
+* GEMM bit-exact against oracle/_ref's ATL_zrefgemm / ATL_crefgemm (ATLAS's
+  own C reference BLAS, compiled from src/blas/reference where
+  /root/reference exists; skipped elsewhere), every op pair with conjugates
+  and the alpha/beta special cases;
+* GEMM exact on small Gaussian-integer inputs (every product and partial sum
+  is an integer below 2^24, so any summation order gives the same bits)
+  against numpy int64 arithmetic -- this pins the conjugation and transpose
+  conventions without the reference;
+* TRSM against ATL_ztrsm's reference (ATL_zreftrsm / ATL_creftrsm) within
+  rounding, all 24 cases, and against numpy's dense solve of op(A).
+"""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+RNG = np.random.default_rng(4321)
+OPS = [O.NOTRANS, O.TRANS, O.CONJTRANS]
+
+
+def _ref_or_skip():
+    r = O.ref()
+    if r is None:
+        pytest.skip("oracle/_ref not built (needs /root/reference)")
+    return r
+
+
+def _cplx(shape, dt, lo=-0.5, hi=0.5):
+    a = RNG.uniform(lo, hi, shape) + 1j * RNG.uniform(lo, hi, shape)
+    return np.asfortranarray(a.astype(dt))
+
+
+def _op(M, t):
+    if t == O.NOTRANS:
+        return M
+    if t == O.TRANS:
+        return M.T
+    return M.conj().T
+
+
+def _scal(x, rt):
+    return np.array([complex(x).real, complex(x).imag], dtype=rt)
+
+
+ALBE = [(1.0, 0.5 - 0.25j), (-0.7 + 0.3j, 0.0), (1.3, 1.0), (0.0, 1.3 + 0.2j), (0.0, 0.0),
+        (0.5j, 1.0), (0.25 - 1.5j, -0.5j)]
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("ta,tb", list(itertools.product(OPS, repeat=2)))
+def test_cgemm_ref_bit_exact_vs_atlas_refgemm(dt, ta, tb):
+    R = _ref_or_skip()
+    rt = np.float64 if dt == np.complex128 else np.float32
+    fn = R.ATL_zrefgemm if dt == np.complex128 else R.ATL_crefgemm
+    fn.argtypes = [C.c_int] * 5 + [C.c_void_p] * 2 + [C.c_int, C.c_void_p, C.c_int,
+                                                      C.c_void_p, C.c_void_p, C.c_int]
+    M, N, K = 13, 9, 11
+    A = _cplx((M, K) if ta == O.NOTRANS else (K, M), dt)
+    B = _cplx((K, N) if tb == O.NOTRANS else (N, K), dt)
+    C0 = _cplx((M, N), dt)
+    for alpha, beta in ALBE:
+        c1, c2 = C0.copy(order="F"), C0.copy(order="F")
+        O.cgemm(ta, tb, alpha, A, B, beta, c1)
+        al, be = _scal(alpha, rt), _scal(beta, rt)
+        fn(ta, tb, M, N, K, al.ctypes.data, A.ctypes.data, A.shape[0], B.ctypes.data,
+           B.shape[0], be.ctypes.data, c2.ctypes.data, M)
+        assert np.array_equal(c1, c2), (alpha, beta)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_cgemm_ref_integer_exact_vs_numpy(dt):
+    """Gaussian integers: the result is exact in any order, so the oracle must
+    equal numpy's int64 product bit for bit for every op pair."""
+    M, N, K = 17, 12, 23
+    for ta, tb in itertools.product(OPS, repeat=2):
+        Ai = RNG.integers(-4, 5, (M, K)) + 1j * RNG.integers(-4, 5, (M, K))
+        Bi = RNG.integers(-4, 5, (K, N)) + 1j * RNG.integers(-4, 5, (K, N))
+        Ci = RNG.integers(-4, 5, (M, N)) + 1j * RNG.integers(-4, 5, (M, N))
+        # A and B are stored so that op(A) == Ai and op(B) == Bi
+        A = np.asfortranarray((Ai if ta == O.NOTRANS else
+                               (Ai.T if ta == O.TRANS else Ai.conj().T)).astype(dt))
+        B = np.asfortranarray((Bi if tb == O.NOTRANS else
+                               (Bi.T if tb == O.TRANS else Bi.conj().T)).astype(dt))
+        alpha, beta = 2 - 1j, 1 + 3j
+        got = O.cgemm(ta, tb, alpha, A, B, beta, np.asfortranarray(Ci.astype(dt)))
+        want = alpha * (Ai @ Bi) + beta * Ci
+        assert np.array_equal(got, want.astype(dt)), (ta, tb)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_cgemm_ref_vs_numpy_rounding(dt):
+    tol = 1e-13 if dt == np.complex128 else 2e-5
+    M, N, K = 40, 33, 57
+    for ta, tb in itertools.product(OPS, repeat=2):
+        A = _cplx((M, K) if ta == O.NOTRANS else (K, M), dt)
+        B = _cplx((K, N) if tb == O.NOTRANS else (N, K), dt)
+        C0 = _cplx((M, N), dt)
+        alpha, beta = 0.75 - 0.5j, -0.25 + 1j
+        got = O.cgemm(ta, tb, alpha, A, B, beta, C0.copy(order="F"))
+        want = alpha * (_op(A.astype(np.complex128), ta) @ _op(B.astype(np.complex128), tb)) \
+            + beta * C0.astype(np.complex128)
+        assert np.max(np.abs(got - want)) <= tol * K, (ta, tb)
+
+
+def _tri(n, uplo, dt):
+    A = _cplx((n, n), dt)
+    A[np.diag_indices(n)] += np.asarray(3.0 + 1.0j, dtype=dt)
+    # the other triangle holds junk the solve must not read
+    mask = np.tril(np.ones((n, n), bool), -1) if uplo == O.UPPER else \
+        np.triu(np.ones((n, n), bool), 1)
+    A[mask] = np.asarray(1e30 + 1e30j, dtype=dt)
+    return A
+
+
+def _dense_op(A, uplo, ta, diag):
+    T = np.tril(A) if uplo == O.LOWER else np.triu(A)
+    T = T.astype(np.complex128)
+    if diag == O.UNIT:
+        T[np.diag_indices(T.shape[0])] = 1.0
+    return _op(T, ta)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_ctrsm_ref_vs_atlas_reftrsm_and_numpy(dt):
+    R = O.ref()   # without oracle/_ref only the numpy pin runs
+    rt = np.float64 if dt == np.complex128 else np.float32
+    tol = 1e-12 if dt == np.complex128 else 5e-5
+    fn = None
+    if R is not None:
+        fn = R.ATL_zreftrsm if dt == np.complex128 else R.ATL_creftrsm
+        fn.argtypes = [C.c_int] * 6 + [C.c_void_p] * 2 + [C.c_int, C.c_void_p, C.c_int]
+    for side, uplo, ta, diag in itertools.product([O.LEFT, O.RIGHT], [O.UPPER, O.LOWER], OPS,
+                                                  [O.NONUNIT, O.UNIT]):
+        M, N = 11, 7
+        na = M if side == O.LEFT else N
+        A = _tri(na, uplo, dt)
+        B0 = _cplx((M, N), dt)
+        for alpha in (1.0, -0.6 + 0.8j, 0.0):
+            b1 = O.ctrsm(side, uplo, ta, diag, alpha, A, B0.copy(order="F"))
+            if fn is not None:
+                b2 = B0.copy(order="F")
+                al = _scal(alpha, rt)
+                fn(side, uplo, ta, diag, M, N, al.ctypes.data, A.ctypes.data, na,
+                   b2.ctypes.data, M)
+                assert np.max(np.abs(b1 - b2)) <= tol, (side, uplo, ta, diag, alpha)
+            E = _dense_op(A, uplo, ta, diag)
+            if side == O.LEFT:
+                want = np.linalg.solve(E, alpha * B0.astype(np.complex128))
+            else:
+                want = np.linalg.solve(E.T, (alpha * B0.astype(np.complex128)).T).T
+            assert np.max(np.abs(b1 - want)) <= tol, (side, uplo, ta, diag, alpha)
