@@ -126,6 +126,15 @@ def _declare(L):
         f = getattr(L, "cblas_%strsm" % p)
         f.argtypes = [_I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I]
         f.restype = None
+        f = getattr(L, "clapack_%sgetrf" % p)
+        f.argtypes = [_I, _I, _I, _P, _I, _P]
+        f.restype = _I
+        f = getattr(L, "clapack_%sgetrs" % p)
+        f.argtypes = [_I, _I, _I, _I, _P, _I, _P, _P, _I]
+        f.restype = _I
+        f = getattr(L, "clapack_%sgesv" % p)
+        f.argtypes = [_I, _I, _I, _P, _I, _P, _P, _I]
+        f.restype = _I
     L.atlmx_init.restype = _I
     L.atlmx_set_stream.argtypes = [_P]
     L.atlmx_get_stream.restype = _P

@@ -1,13 +1,15 @@
 /*
- * cblas3c.c -- complex Level-3 BLAS (C host code) of libatlas_mi355x:
- * cblas_{c,z}{gemm,trsm} and their device-pointer twins, instantiated from
- * cblas3c_tmpl.h per precision over the real MFMA GEMM kernels.
+ * cblas3c.c -- complex Level-3 BLAS and LU (C host code) of libatlas_mi355x:
+ * cblas_{c,z}{gemm,trsm}, clapack_{c,z}{getrf,getrs,gesv} and their
+ * device-pointer twins, instantiated from cblas3c_tmpl.h and lapackc_tmpl.h
+ * per precision over the real MFMA GEMM kernels.
  */
 #include <stdio.h>
 #include <stdlib.h>
 #include "rt.h"
 #include "drivers.h"
 #include "../../include/cblas.h"
+#include "../../include/clapack.h"
 #include "../../include/atlmx.h"
 
 #define STR_(x) #x
@@ -19,13 +21,18 @@
 #define K_CGATHER atlmx_k_zgather
 #define K_CSCAL atlmx_k_zscal2d
 #define K_CTRI_INV atlmx_k_ztri_inv
+#define K_CGETF2 atlmx_k_zgetf2
+#define K_RLASWP atlmx_k_dlaswp
 #include "cblas3c_tmpl.h"
+#include "lapackc_tmpl.h"
 #undef T
 #undef PFX
 #undef K_GEMM
 #undef K_CGATHER
 #undef K_CSCAL
 #undef K_CTRI_INV
+#undef K_CGETF2
+#undef K_RLASWP
 
 #define T float
 #define PFX c
@@ -33,4 +40,7 @@
 #define K_CGATHER atlmx_k_cgather
 #define K_CSCAL atlmx_k_cscal2d
 #define K_CTRI_INV atlmx_k_ctri_inv
+#define K_CGETF2 atlmx_k_cgetf2
+#define K_RLASWP atlmx_k_slaswp
 #include "cblas3c_tmpl.h"
+#include "lapackc_tmpl.h"

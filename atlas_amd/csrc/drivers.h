@@ -60,6 +60,12 @@ int atlmx_gemm_check(int Order, int TA, int TB, int M, int N, int K, int lda, in
                      int ldc);
 int atlmx_trxm_check(int Order, int Side, int Uplo, int TA, int Diag, int M, int N, int lda,
                      int ldb);
+/* clapack argument checks shared by the real and complex routines (defined
+ * in lapack.c): 0, or the negated position of the last bad argument */
+int atlmx_getrf_check(const char *rout, int Order, int M, int N, int lda);
+int atlmx_getrs_check(const char *rout, int Order, int Trans, int N, int NRHS, int lda,
+                      int ldb);
+int atlmx_gesv_check(const char *rout, int Order, int N, int NRHS, int lda, int ldb);
 
 /* complex drivers (defined in cblas3c.c); alpha and beta point at
  * {real, imaginary}; matrices interleaved, leading dimensions in complex
@@ -70,7 +76,11 @@ void P##gemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const T *alp
                  long ldc, const char *rout);                                         \
 void P##trsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,    \
                  const T *alpha, const T *A, long lda, T *B, long ldb,                \
-                 const char *rout);
+                 const char *rout);                                                   \
+int  P##getrf_dev(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv,            \
+                  const char *rout);                                                  \
+void P##getrs_dev(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,     \
+                  const int *d_ipiv, T *B, long ldb, const char *rout);
 ATLMX_CDRV_DECLARE(z, double)
 ATLMX_CDRV_DECLARE(c, float)
 

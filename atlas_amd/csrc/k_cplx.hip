@@ -174,6 +174,136 @@ __global__ __launch_bounds__(64) void ctri_inv_kernel(int lower, int n, const T 
   }
 }
 
+/* x / d by Smith's algorithm (the same scaling as crecip) */
+template <typename T>
+__device__ inline void cdivide(T dr, T di, T &xr, T &xi) {
+  T cr, ci;
+  if (fabs(di) < fabs(dr)) {
+    const T t = di / dr, den = dr + di * t;
+    cr = (xr + xi * t) / den; ci = (xi - xr * t) / den;
+  } else {
+    const T t = dr / di, den = di + dr * t;
+    cr = (xi + xr * t) / den; ci = (-xr + xi * t) / den;
+  }
+  xr = cr; xi = ci;
+}
+
+template <typename T> struct SafMin;
+template <> struct SafMin<double> { static constexpr double v = 2.2250738585072014e-308; };
+template <> struct SafMin<float> { static constexpr float v = 1.17549435e-38f; };
+
+/* ------------------------------------------------------------------------
+ * Unblocked complex LU of an m x n leaf (n <= CLEAF_MAXW columns) whose top
+ * row is global row row0, in place, one 1024-thread workgroup: the column
+ * loop of src/lapack/ATL_getf2.c:60-95 with LU1 (:10-58) for complex data.
+ * Per column j: the pivot is the first row of largest |Re| + |Im| at or
+ * below j (cblas_izamax, ATL_izrefamax.c: NaN never wins, all-zero gives
+ * row j); ipiv[j] = row0 + pivot row.  A nonzero pivot swaps its row with
+ * row j across the leaf and the rows below are multiplied by its
+ * reciprocal, or divided by it when |Re| + |Im| < SAFMIN; a zero pivot sets
+ * *info = row0 + j + 1 when *info is 0.  Then the rank-1 update of the
+ * trailing columns, which runs in both cases.
+ * Thread t owns rows j + t, j + t + 1024, ...; pivot row j of the trailing
+ * columns is staged in LDS for the update.
+ * ---------------------------------------------------------------------- */
+constexpr int CLEAF_MAXW = 16;
+template <typename T>
+__global__ __launch_bounds__(1024) void cgetf2_kernel(int m, int n, T *__restrict__ A, long lda,
+                                                      int row0, int *__restrict__ ipiv,
+                                                      int *__restrict__ info) {
+  __shared__ T s_v[16];
+  __shared__ int s_i[16];
+  __shared__ int s_p;
+  __shared__ T s_pr, s_pi;
+  __shared__ T s_rr[CLEAF_MAXW], s_ri[CLEAF_MAXW];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int mn = min(m, n);
+  for (int j = 0; j < mn; ++j) {
+    T *Aj = A + 2 * (size_t)j * lda;
+    T bv = T(0);
+    int bi = j;
+    for (int i = j + tid; i < m; i += 1024) {
+      const T v = fabs(Aj[2 * i]) + fabs(Aj[2 * i + 1]);
+      if (v > bv) { bv = v; bi = i; }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { s_v[wid] = bv; s_i[wid] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      T v = s_v[0];
+      int p = s_i[0];
+      for (int w = 1; w < 16; ++w)
+        if (s_v[w] > v || (s_v[w] == v && s_i[w] < p)) { v = s_v[w]; p = s_i[w]; }
+      s_p = p;
+      s_pr = Aj[2 * p];
+      s_pi = Aj[2 * p + 1];
+      ipiv[j] = row0 + p;
+    }
+    __syncthreads();
+    const int p = s_p;
+    const T pr = s_pr, pi = s_pi;
+    const T pv = fabs(pr) + fabs(pi);   /* the same in every thread */
+    if (pv != T(0)) {
+      if (p != j && tid < n) {
+        T *a = A + 2 * ((size_t)tid * lda + j), *b = A + 2 * ((size_t)tid * lda + p);
+        const T ar = a[0], ai = a[1];
+        a[0] = b[0]; a[1] = b[1];
+        b[0] = ar; b[1] = ai;
+      }
+      __syncthreads();
+      if (pv >= SafMin<T>::v) {
+        T rr, ri;
+        crecip(pr, pi, rr, ri);
+        for (int i = j + 1 + tid; i < m; i += 1024) {
+          const T xr = Aj[2 * i], xi = Aj[2 * i + 1];
+          Aj[2 * i] = xr * rr - xi * ri;
+          Aj[2 * i + 1] = xr * ri + xi * rr;
+        }
+      } else {
+        for (int i = j + 1 + tid; i < m; i += 1024) {
+          T xr = Aj[2 * i], xi = Aj[2 * i + 1];
+          cdivide(pr, pi, xr, xi);
+          Aj[2 * i] = xr; Aj[2 * i + 1] = xi;
+        }
+      }
+    } else if (tid == 0 && *info == 0) {
+      *info = row0 + j + 1;
+    }
+    /* row j of the trailing columns (after the swap) to LDS */
+    __syncthreads();
+    if (tid > j && tid < n) {
+      s_rr[tid] = A[2 * ((size_t)tid * lda + j)];
+      s_ri[tid] = A[2 * ((size_t)tid * lda + j) + 1];
+    }
+    __syncthreads();
+    for (int i = j + 1 + tid; i < m; i += 1024) {
+      const T lr = Aj[2 * i], li = Aj[2 * i + 1];
+      for (int c = j + 1; c < n; ++c) {
+        T *a = A + 2 * ((size_t)c * lda + i);
+        a[0] -= lr * s_rr[c] - li * s_ri[c];
+        a[1] -= lr * s_ri[c] + li * s_rr[c];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+/* ipiv2[2k + s] = 2 ipiv[k] + s for k in [k1, k2): the row interchanges of
+ * a complex matrix as those of its interleaved real view. */
+__global__ __launch_bounds__(256) void cpiv2_kernel(int k1, int k2, const int *__restrict__ ipiv,
+                                                    int *__restrict__ ipiv2) {
+  const int k = k1 + blockIdx.x * 256 + threadIdx.x;
+  if (k >= k2) return;
+  const int p = ipiv[k];
+  ipiv2[2 * k] = 2 * p;
+  ipiv2[2 * k + 1] = 2 * p + 1;
+}
+
 template <typename T>
 hipError_t launch_gather(int m, int n, const T *A, long lda, int trans, int conj, T ar, T ai,
                          int expand, T *out, long ldo, int tri, int unit, hipStream_t s) {
@@ -237,4 +367,33 @@ extern "C" hipError_t atlmx_k_ztri_inv(int lower, int n, const double *H, long l
 extern "C" hipError_t atlmx_k_ctri_inv(int lower, int n, const float *H, long ldh, float *inv,
                                        hipStream_t s) {
   return launch_ctri_inv<float>(lower, n, H, ldh, inv, s);
+}
+
+namespace {
+template <typename T>
+hipError_t launch_cgetf2(int m, int n, T *A, long lda, int row0, int *ipiv, int *info,
+                         hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (n > CLEAF_MAXW || !A || !ipiv || !info || lda < m) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((cgetf2_kernel<T>), dim3(1), dim3(1024), 0, s, m, n, A, lda, row0, ipiv,
+                     info);
+  return hipGetLastError();
+}
+}  // namespace
+
+extern "C" hipError_t atlmx_k_zgetf2(int m, int n, double *A, long lda, int row0, int *ipiv,
+                                     int *info, hipStream_t s) {
+  return launch_cgetf2<double>(m, n, A, lda, row0, ipiv, info, s);
+}
+extern "C" hipError_t atlmx_k_cgetf2(int m, int n, float *A, long lda, int row0, int *ipiv,
+                                     int *info, hipStream_t s) {
+  return launch_cgetf2<float>(m, n, A, lda, row0, ipiv, info, s);
+}
+extern "C" hipError_t atlmx_k_cpiv2(int k1, int k2, const int *ipiv, int *ipiv2,
+                                    hipStream_t s) {
+  if (k2 <= k1) return hipSuccess;
+  if (!ipiv || !ipiv2) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(cpiv2_kernel, dim3((k2 - k1 + 255) / 256), dim3(256), 0, s, k1, k2, ipiv,
+                     ipiv2);
+  return hipGetLastError();
 }

@@ -388,6 +388,18 @@ hipError_t atlmx_k_ztri_inv(int lower, int n, const double *H, long ldh, double 
                             hipStream_t s);
 hipError_t atlmx_k_ctri_inv(int lower, int n, const float *H, long ldh, float *inv,
                             hipStream_t s);
+/* Unblocked complex LU (ATL_getf2's column loop) of an m x n leaf, n <=
+ * ATLMX_CLEAF_MAXW, whose top row is global row row0: ipiv[j] receives
+ * row0 + the pivot row, *info = row0 + j + 1 at the first zero pivot (when
+ * *info is 0).  One workgroup. */
+#define ATLMX_CLEAF_MAXW 16
+hipError_t atlmx_k_zgetf2(int m, int n, double *A, long lda, int row0, int *ipiv, int *info,
+                          hipStream_t s);
+hipError_t atlmx_k_cgetf2(int m, int n, float *A, long lda, int row0, int *ipiv, int *info,
+                          hipStream_t s);
+/* ipiv2[2k + s] = 2 ipiv[k] + s, k in [k1, k2): a complex matrix's row
+ * interchanges as those of its interleaved real view */
+hipError_t atlmx_k_cpiv2(int k1, int k2, const int *ipiv, int *ipiv2, hipStream_t s);
 
 #ifdef __cplusplus
 }

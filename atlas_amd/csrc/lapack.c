@@ -18,6 +18,99 @@
 
 #include "drivers.h"
 
+/* Argument checks of clapack_?getrf / ?getrs / ?gesv for every type (real
+ * and complex), in ATLAS's order, positions and messages
+ * (interfaces/lapack/C/src/clapack_dgetrf.c:48-86, clapack_dgetrs.c,
+ * clapack_dgesv.c; the z/c files are the same).  Each returns 0 or the
+ * negated position of the last bad argument, after xerbla for each. */
+int atlmx_getrf_check(const char *rout, int Order, int M, int N, int lda)
+{
+  int ierr = 0;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (M < 0) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "M cannot be less than zero 0,; is set to %d.\n", M);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (Order == CblasColMajor) {
+    if (lda < M || lda < 1) {
+      ierr = -6;
+      cblas_xerbla(6, rout, "lda must be >= MAX(M,1): lda=%d M=%d\n", lda, M);
+    }
+  } else if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  return ierr;
+}
+
+int atlmx_getrs_check(const char *rout, int Order, int Trans, int N, int NRHS, int lda,
+                      int ldb)
+{
+  int ierr = 0;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Trans != CblasNoTrans && Trans != CblasTrans && Trans != CblasConjTrans) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "Trans must be %d, %d, or %d, but is set to %d\n", CblasNoTrans,
+                 CblasTrans, CblasConjTrans, Trans);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (NRHS < 0) {
+    ierr = -4;
+    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ldb < N || ldb < 1) {
+    ierr = -9;
+    cblas_xerbla(9, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
+  }
+  return ierr;
+}
+
+int atlmx_gesv_check(const char *rout, int Order, int N, int NRHS, int lda, int ldb)
+{
+  int ierr = 0;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (N < 0) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (NRHS < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -5;
+    cblas_xerbla(5, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ldb < N || ldb < 1) {
+    ierr = -8;
+    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
+  }
+  return ierr;
+}
+
 #define T double
 #define PFX d
 #define K_PANEL atlmx_k_dgetf2_panel

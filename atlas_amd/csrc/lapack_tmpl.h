@@ -451,33 +451,12 @@ int NM3(clapack_, PFX, getrf)(const enum CBLAS_ORDER Order, const int M, const i
                               T *A, const int lda, int *ipiv)
 {
   static const char *rout = "clapack_" STR(PFX) "getrf";
-  int ierr = 0, m, n, mn, info;
+  const int ierr = atlmx_getrf_check(rout, Order, M, N, lda);
+  int m, n, mn, info;
   atlmx_ctx *c;
   long dla;
   T *dA;
   int *dp;
-  if (Order != CblasRowMajor && Order != CblasColMajor) {
-    ierr = -1;
-    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
-                 CblasColMajor, Order);
-  }
-  if (M < 0) {
-    ierr = -2;
-    cblas_xerbla(2, rout, "M cannot be less than zero 0,; is set to %d.\n", M);
-  }
-  if (N < 0) {
-    ierr = -3;
-    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
-  }
-  if (Order == CblasColMajor) {
-    if (lda < M || lda < 1) {
-      ierr = -6;
-      cblas_xerbla(6, rout, "lda must be >= MAX(M,1): lda=%d M=%d\n", lda, M);
-    }
-  } else if (lda < N || lda < 1) {
-    ierr = -6;
-    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
-  }
   if (ierr) return ierr;
   m = (Order == CblasColMajor) ? M : N;
   n = (Order == CblasColMajor) ? N : M;
@@ -513,36 +492,11 @@ int NM3(clapack_, PFX, getrs)(const enum CBLAS_ORDER Order, const enum CBLAS_TRA
                               const int *ipiv, T *B, const int ldb)
 {
   static const char *rout = "clapack_" STR(PFX) "getrs";
-  int ierr = 0, t;
+  const int ierr = atlmx_getrs_check(rout, Order, Trans, N, NRHS, lda, ldb);
+  int t;
   atlmx_ctx *c;
   long dla, dlb;
   T *dA, *dB;
-  if (Order != CblasRowMajor && Order != CblasColMajor) {
-    ierr = -1;
-    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
-                 CblasColMajor, Order);
-  }
-  if (Trans != CblasNoTrans && Trans != CblasTrans && Trans != CblasConjTrans) {
-    ierr = -2;
-    cblas_xerbla(2, rout, "Trans must be %d, %d, or %d, but is set to %d\n", CblasNoTrans,
-                 CblasTrans, CblasConjTrans, Trans);
-  }
-  if (N < 0) {
-    ierr = -3;
-    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
-  }
-  if (NRHS < 0) {
-    ierr = -4;
-    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
-  }
-  if (lda < N || lda < 1) {
-    ierr = -6;
-    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
-  }
-  if (ldb < N || ldb < 1) {
-    ierr = -9;
-    cblas_xerbla(9, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
-  }
   if (ierr) return ierr;
   if (N == 0 || NRHS == 0) return 0;
   t = (Trans != CblasNoTrans);
@@ -651,32 +605,12 @@ int NM3(clapack_, PFX, gesv)(const enum CBLAS_ORDER Order, const int N, const in
                              T *A, const int lda, int *ipiv, T *B, const int ldb)
 {
   static const char *rout = "clapack_" STR(PFX) "gesv";
-  int ierr = 0, info;
+  const int ierr = atlmx_gesv_check(rout, Order, N, NRHS, lda, ldb);
+  int info;
   atlmx_ctx *c;
   long dla, dlb;
   T *dA, *dB;
   int *dp;
-  if (Order != CblasRowMajor && Order != CblasColMajor) {
-    ierr = -1;
-    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
-                 CblasColMajor, Order);
-  }
-  if (N < 0) {
-    ierr = -2;
-    cblas_xerbla(2, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
-  }
-  if (NRHS < 0) {
-    ierr = -3;
-    cblas_xerbla(3, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
-  }
-  if (lda < N || lda < 1) {
-    ierr = -5;
-    cblas_xerbla(5, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
-  }
-  if (ldb < N || ldb < 1) {
-    ierr = -8;
-    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
-  }
   if (ierr) return ierr;
   if (N == 0) return 0;
   c = rt_ctx();

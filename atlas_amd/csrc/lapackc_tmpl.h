@@ -1,0 +1,227 @@
+/*
+ * lapackc_tmpl.h -- precision template of the complex LU family (included
+ * by cblas3c.c after cblas3c_tmpl.h, T=double/PFX=z and T=float/PFX=c).
+ *
+ *   ?getrf_dev / ?getrs_dev       : stream-ordered drivers on device pointers
+ *   clapack_?getrf / ?getrs / ?gesv : the ATLAS C ABI
+ *
+ * The LU is right-looking over panels of CLU_NB columns; each panel is
+ * factored by the recursive halving of src/lapack/ATL_getrfC.c:242-361
+ * (left half, its interchanges and TRSM/GEMM on the right half, right half,
+ * its interchanges on the left half) down to leaves of at most CLU_LEAF
+ * columns, which one workgroup factors unblocked (k_cplx.hip, the column
+ * loop of ATL_getf2.c).  Every TRSM and GEMM is the complex one of
+ * cblas3c_tmpl.h, i.e. the real MFMA GEMM on expanded operands.  Row
+ * interchanges of a complex matrix are those of its interleaved real view
+ * with each pivot doubled (rows 2k, 2k+1 <-> 2p, 2p+1), so the real laswp
+ * kernel applies them.
+ * ---------------------------------------------------------------------- */
+#define CLU_NB 128
+#define CLU_LEAF 8
+
+/* the interchanges ipiv[k1..k2) (forward for inc > 0, reverse for inc < 0)
+ * on ncols columns of the complex matrix A, rows below rows_end */
+static void NM(PFX, laswp_dev)(atlmx_ctx *c, int ncols, T *A, long lda, int k1, int k2,
+                               const int *d_ipiv, int inc, int rows_end, int *ipiv2,
+                               const char *rout)
+{
+  if (ncols <= 0 || k2 <= k1) return;
+  RT_CHECK(rout, atlmx_k_cpiv2(k1, k2, d_ipiv, ipiv2, c->stream));
+  RT_CHECK(rout, K_RLASWP(ncols, A, 2 * lda, 2 * k1, 2 * k2, ipiv2, inc, 2 * rows_end,
+                          c->stream));
+}
+
+/* columns [c0, c0 + w) of the M-row matrix A, rows [c0, M); the columns
+ * left of c0 in the panel are already factored and their interchanges
+ * applied to these columns */
+static void NM(PFX, getrf_rec)(atlmx_ctx *c, int M, int c0, int w, T *A, long lda,
+                               int *d_ipiv, int *ipiv2, int *d_info, const char *rout)
+{
+  static const T one[2] = {(T)1, (T)0}, mone[2] = {(T)-1, (T)0};
+  T *Acc = A + 2 * ((size_t)c0 * lda + c0);
+  int w1;
+  T *A12, *A22;
+  if (w <= CLU_LEAF) {
+    RT_CHECK(rout, K_CGETF2(M - c0, w, Acc, lda, c0, d_ipiv + c0, d_info, c->stream));
+    return;
+  }
+  w1 = w / 2;
+  A12 = A + 2 * ((size_t)(c0 + w1) * lda + c0);
+  A22 = A12 + 2 * (size_t)w1;
+  NM(PFX, getrf_rec)(c, M, c0, w1, A, lda, d_ipiv, ipiv2, d_info, rout);
+  NM(PFX, laswp_dev)(c, w - w1, A + 2 * (size_t)(c0 + w1) * lda, lda, c0, c0 + w1, d_ipiv, 1,
+                     M, ipiv2, rout);
+  NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, w1, w - w1, one, Acc, lda, A12,
+                    lda, rout);
+  if (M - c0 - w1 > 0)
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - c0 - w1, w - w1, w1, mone,
+                      Acc + 2 * (size_t)w1, lda, A12, lda, one, A22, lda, rout);
+  NM(PFX, getrf_rec)(c, M, c0 + w1, w - w1, A, lda, d_ipiv, ipiv2, d_info, rout);
+  NM(PFX, laswp_dev)(c, w1, A + 2 * (size_t)c0 * lda, lda, c0 + w1, c0 + w, d_ipiv, 1, M,
+                     ipiv2, rout);
+}
+
+/* A = P L U in place (L unit lower, U upper), 0-based pivots in d_ipiv
+ * (min(M,N) of them); returns ATLAS's info: the 1-based index of the first
+ * exactly zero pivot, else 0. */
+int NM(PFX, getrf_dev)(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv,
+                       const char *rout)
+{
+  static const T one[2] = {(T)1, (T)0}, mone[2] = {(T)-1, (T)0};
+  const int mn = M < N ? M : N;
+  int *d_info, *ipiv2, info = 0, k;
+  if (mn <= 0) return 0;
+  d_info = (int *)rt_ws(c, WS_INFO, sizeof(int), rout);
+  ipiv2 = (int *)rt_ws(c, WS_CPIV, (size_t)2 * mn * sizeof(int), rout);
+  RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
+  for (k = 0; k < mn; k += CLU_NB) {
+    const int jb = (mn - k < CLU_NB) ? mn - k : CLU_NB;
+    T *Akk = A + 2 * ((size_t)k * lda + k);
+    NM(PFX, getrf_rec)(c, M, k, jb, A, lda, d_ipiv, ipiv2, d_info, rout);
+    NM(PFX, laswp_dev)(c, k, A, lda, k, k + jb, d_ipiv, 1, M, ipiv2, rout);
+    if (k + jb < N) {
+      T *A12 = A + 2 * ((size_t)(k + jb) * lda + k);
+      NM(PFX, laswp_dev)(c, N - k - jb, A + 2 * (size_t)(k + jb) * lda, lda, k, k + jb,
+                         d_ipiv, 1, M, ipiv2, rout);
+      NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, jb, N - k - jb, one, Akk,
+                        lda, A12, lda, rout);
+      if (k + jb < M)
+        NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - k - jb, N - k - jb, jb, mone,
+                          Akk + 2 * (size_t)jb, lda, A12, lda, one, A12 + 2 * (size_t)jb, lda,
+                          rout);
+    }
+  }
+  RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+/* op(A) X = B with A = P L U from getrf_dev (src/lapack/ATL_getrs.c:17-45,
+ * column-major): NoTrans = interchanges, L (unit), U; Trans / ConjTrans =
+ * op(U), op(L), reverse interchanges. */
+void NM(PFX, getrs_dev)(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,
+                        const int *d_ipiv, T *B, long ldb, const char *rout)
+{
+  static const T one[2] = {(T)1, (T)0};
+  int *ipiv2;
+  if (N <= 0 || NRHS <= 0) return;
+  ipiv2 = (int *)rt_ws(c, WS_CPIV, (size_t)2 * N * sizeof(int), rout);
+  if (trans == KX_NOTRANS) {
+    NM(PFX, laswp_dev)(c, NRHS, B, ldb, 0, N, d_ipiv, 1, N, ipiv2, rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, N, NRHS, one, A, lda, B, ldb,
+                      rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, one, A, lda, B,
+                      ldb, rout);
+  } else {
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, trans, KX_NONUNIT, N, NRHS, one, A, lda, B, ldb,
+                      rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, trans, KX_UNIT, N, NRHS, one, A, lda, B, ldb,
+                      rout);
+    NM(PFX, laswp_dev)(c, NRHS, B, ldb, 0, N, d_ipiv, -1, N, ipiv2, rout);
+  }
+}
+
+/* ====================================================================== */
+/* ATLAS C ABI                                                              */
+/* ====================================================================== */
+
+/* interfaces/lapack/C/src/clapack_zgetrf.c; RowMajor factors the stored
+ * A^T column-major (the real routines do the same) */
+int NM3(clapack_, PFX, getrf)(const enum CBLAS_ORDER Order, const int M, const int N,
+                              void *A, const int lda, int *ipiv)
+{
+  static const char *rout = "clapack_" STR(PFX) "getrf";
+  const int ierr = atlmx_getrf_check(rout, Order, M, N, lda);
+  const size_t esz = 2 * sizeof(T);
+  int m, n, mn, info;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  int *dp;
+  if (ierr) return ierr;
+  m = (Order == CblasColMajor) ? M : N;
+  n = (Order == CblasColMajor) ? N : M;
+  mn = m < n ? m : n;
+  if (mn == 0) return 0;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, m, n, lda, esz, 1, &dla, rout);
+  dp = rt_is_device(ipiv) ? ipiv : (int *)rt_ws(c, WS_IPIV, (size_t)mn * sizeof(int), rout);
+  info = NM(PFX, getrf_dev)(c, m, n, dA, dla, dp, rout);
+  rt_stage_out(c, A, dA, m, n, lda, dla, esz, rout);
+  if (dp != ipiv)
+    RT_CHECK(rout, hipMemcpyAsync(ipiv, dp, (size_t)mn * sizeof(int), hipMemcpyDeviceToHost,
+                                  c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+/* interfaces/lapack/C/src/clapack_zgetrs.c.  RowMajor holds the factor of
+ * A^T, so NoTrans is the column-major Trans solve and Trans the NoTrans
+ * one; ConjTrans goes the way of Trans, as src/lapack/ATL_getrs.c:46-69
+ * does (its RowMajor branches test only Trans == CblasNoTrans). */
+int NM3(clapack_, PFX, getrs)(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE Trans,
+                              const int N, const int NRHS, const void *A, const int lda,
+                              const int *ipiv, void *B, const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "getrs";
+  const int ierr = atlmx_getrs_check(rout, Order, Trans, N, NRHS, lda, ldb);
+  const size_t esz = 2 * sizeof(T);
+  int t;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  const int *dp;
+  if (ierr) return ierr;
+  if (N == 0 || NRHS == 0) return 0;
+  if (Order == CblasColMajor)
+    t = (Trans == CblasNoTrans) ? KX_NOTRANS : (Trans == CblasTrans ? KX_TRANS : KX_CONJTRANS);
+  else
+    t = (Trans == CblasNoTrans) ? KX_TRANS : KX_NOTRANS;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, esz, 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, esz, 1, &dlb, rout);
+  if (rt_is_device(ipiv)) {
+    dp = ipiv;
+  } else {
+    int *d = (int *)rt_ws(c, WS_IPIV, (size_t)N * sizeof(int), rout);
+    RT_CHECK(rout, hipMemcpyAsync(d, ipiv, (size_t)N * sizeof(int), hipMemcpyHostToDevice,
+                                  c->stream));
+    dp = d;
+  }
+  NM(PFX, getrs_dev)(c, t, N, NRHS, dA, dla, dp, dB, dlb, rout);
+  rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, esz, rout);
+  rt_sync(c, rout);
+  return 0;
+}
+
+/* interfaces/lapack/C/src/clapack_zgesv.c: getrf, then (info = 0) getrs */
+int NM3(clapack_, PFX, gesv)(const enum CBLAS_ORDER Order, const int N, const int NRHS,
+                             void *A, const int lda, int *ipiv, void *B, const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "gesv";
+  const int ierr = atlmx_gesv_check(rout, Order, N, NRHS, lda, ldb);
+  const size_t esz = 2 * sizeof(T);
+  int info;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  int *dp;
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, esz, 1, &dla, rout);
+  dp = rt_is_device(ipiv) ? ipiv : (int *)rt_ws(c, WS_IPIV, (size_t)N * sizeof(int), rout);
+  info = NM(PFX, getrf_dev)(c, N, N, dA, dla, dp, rout);
+  if (!info && NRHS > 0) {
+    dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, esz, 1, &dlb, rout);
+    NM(PFX, getrs_dev)(c, (Order == CblasRowMajor) ? KX_TRANS : KX_NOTRANS, N, NRHS, dA, dla,
+                       dp, dB, dlb, rout);
+    rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, esz, rout);
+  }
+  rt_stage_out(c, A, dA, N, N, lda, dla, esz, rout);
+  if (dp != ipiv)
+    RT_CHECK(rout, hipMemcpyAsync(ipiv, dp, (size_t)N * sizeof(int), hipMemcpyDeviceToHost,
+                                  c->stream));
+  rt_sync(c, rout);
+  return info;
+}

@@ -38,6 +38,7 @@ enum {
   WS_TAU,           /* QR: device copy of TAU                   */
   WS_LASWP,         /* getrs: row-interchange gather map        */
   WS_LASWP_B,       /* getrs: right-hand sides being permuted   */
+  WS_CPIV,          /* complex LU: pivots of the interleaved real view */
   WS_NSLOTS
 };
 

@@ -83,6 +83,26 @@ int clapack_sgeqrf(const enum CBLAS_ORDER Order, const int M, const int N, float
 int clapack_dgeqrf(const enum CBLAS_ORDER Order, const int M, const int N, double *A,
                    const int lda, double *TAU);
 
+/* ---- complex LU family (c: float, z: double; interleaved real/imaginary
+ * parts, leading dimensions in complex elements, 0-based pivots; pivot =
+ * first row of largest |Re| + |Im|).  Replace ATLAS include/clapack.h:99-106
+ * (c) and :138-145 (z), interfaces/lapack/C/src/clapack_zgetrf.c,
+ * clapack_zgetrs.c, clapack_zgesv.c. */
+int clapack_cgesv(const enum CBLAS_ORDER Order, const int N, const int NRHS, void *A,
+                  const int lda, int *ipiv, void *B, const int ldb);
+int clapack_cgetrf(const enum CBLAS_ORDER Order, const int M, const int N, void *A,
+                   const int lda, int *ipiv);
+int clapack_cgetrs(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE Trans,
+                   const int N, const int NRHS, const void *A, const int lda,
+                   const int *ipiv, void *B, const int ldb);
+int clapack_zgesv(const enum CBLAS_ORDER Order, const int N, const int NRHS, void *A,
+                  const int lda, int *ipiv, void *B, const int ldb);
+int clapack_zgetrf(const enum CBLAS_ORDER Order, const int M, const int N, void *A,
+                   const int lda, int *ipiv);
+int clapack_zgetrs(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE Trans,
+                   const int N, const int NRHS, const void *A, const int lda,
+                   const int *ipiv, void *B, const int ldb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -148,11 +148,14 @@ void orc_track_pivot_gaps(double *gaps, int n)
 #define T double
 #define P orc_z
 #define ORC_ABS(x) fabs(x)
+#define ORC_SAFMIN DBL_MIN
 #include "orc_cplx.h"
 #undef T
 #undef P
 #undef ORC_ABS
+#undef ORC_SAFMIN
 #define T float
 #define P orc_c
 #define ORC_ABS(x) fabsf(x)
+#define ORC_SAFMIN FLT_MIN
 #include "orc_cplx.h"

@@ -131,7 +131,8 @@ void P##gemm_ref(int TA, int TB, int M, int N, int K, const T *alpha,        \
                   const T *A, int lda, const T *B, int ldb, const T *beta,    \
                   T *C, int ldc);                                             \
 void P##trsm_ref(int side, int uplo, int TA, int diag, int M, int N,         \
-                  const T *alpha, const T *A, int lda, T *B, int ldb);
+                  const T *alpha, const T *A, int lda, T *B, int ldb);        \
+int  P##getf2(int M, int N, T *A, int lda, int *ipiv, T *mingap);
 
 ORC_CDECLARE(orc_z, double)
 ORC_CDECLARE(orc_c, float)

@@ -130,6 +130,8 @@ def _declare(L):
     for p in ("orc_z", "orc_c"):
         getattr(L, p + "gemm_ref").argtypes = [_I] * 5 + [_P, _P, _I, _P, _I, _P, _P, _I]
         getattr(L, p + "trsm_ref").argtypes = [_I] * 6 + [_P, _P, _I, _P, _I]
+        getattr(L, p + "getf2").argtypes = [_I, _I, _P, _I, _P, _P]
+        getattr(L, p + "getf2").restype = _I
 
 
 def _p(a):
@@ -249,6 +251,18 @@ def ctrsm(side, uplo, ta, diag, alpha, A, B):
     getattr(lib(), p + "trsm_ref")(side, uplo, ta, diag, M, N, _p(al), _p(A), _ld(A), _p(B),
                                    _ld(B))
     return B
+
+
+def cgetf2(A):
+    """Complex LU of the Fortran array A in place (ATL_getf2's column loop).
+    Returns (info, ipiv (0-based), mingap): mingap is the smallest relative
+    margin between a chosen pivot and the runner-up over all columns."""
+    p, rt = _cpre(A.dtype)
+    M, N = A.shape
+    ipiv = np.zeros(max(min(M, N), 1), dtype=np.int32)
+    gap = np.zeros(1, dtype=rt)
+    info = getattr(lib(), p + "getf2")(M, N, _p(A), _ld(A), _p(ipiv), _p(gap))
+    return info, ipiv[:min(M, N)], float(gap[0])
 
 
 def trsm(side, uplo, ta, diag, alpha, A, B, order=COL):

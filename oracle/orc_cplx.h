@@ -177,4 +177,76 @@ void F_(trsm_ref)(int side, int uplo, int TA, int diag, int M, int N, const T *a
    }
 }
 
+/* Complex LU, src/lapack/ATL_getf2.c:60-95 with LU1 (:10-58), complex
+ * branch.  Per column j: the pivot is the first row of largest |Re| + |Im|
+ * at or below j (cblas_izamax = ATL_izrefamax.c: strict >, starting from 0,
+ * so NaN never wins and an all-zero column gives row j); ipiv[j] = that
+ * row.  A nonzero pivot (|Re| + |Im| != 0) swaps rows across all N columns
+ * and the rows below are multiplied by its reciprocal (cplxinvert, here
+ * Smith's division of 1), or divided by it when |Re| + |Im| < SAFMIN; a
+ * zero pivot makes the return value j + 1 if none was zero before.  The
+ * rank-1 update (geru with -1) of the trailing block follows in every case.
+ * mingap (may be NULL) receives the smallest relative margin
+ * (best - runner-up) / best between the pivot and the next candidate over
+ * all columns: tests compare pivots only where it is comfortable. */
+int F_(getf2)(int M, int N, T *A, int lda, int *ipiv, T *mingap)
+{
+   const int MN = M < N ? M : N;
+   int i, j, c, info = 0;
+   T gap = 1;
+   for (j = 0; j < MN; j++)
+   {
+      T *Aj = A + 2*(size_t)j*lda, best = 0, second = 0, pr, pi, pv;
+      int p = j;
+      for (i = j; i < M; i++)
+      {
+         const T v = ORC_ABS(Aj[2*i]) + ORC_ABS(Aj[2*i+1]);
+         if (v > best) { second = best; best = v; p = i; }
+         else if (v > second) second = v;
+      }
+      if (best > 0 && M - j > 1 && (best - second) / best < gap) gap = (best - second) / best;
+      ipiv[j] = p;
+      pr = Aj[2*p]; pi = Aj[2*p+1];
+      pv = ORC_ABS(pr) + ORC_ABS(pi);
+      if (pv != 0)
+      {
+         if (p != j)
+            for (c = 0; c < N; c++)
+            {
+               T *a = A + 2*((size_t)c*lda + j), *b = A + 2*((size_t)c*lda + p), t;
+               t = a[0]; a[0] = b[0]; b[0] = t;
+               t = a[1]; a[1] = b[1]; b[1] = t;
+            }
+         if (pv >= ORC_SAFMIN)
+         {
+            T rr = 1, ri = 0;
+            F_(cdiv)(pr, pi, &rr, &ri);
+            for (i = j + 1; i < M; i++)
+            {
+               const T xr = Aj[2*i], xi = Aj[2*i+1];
+               Aj[2*i] = xr * rr - xi * ri;
+               Aj[2*i+1] = xr * ri + xi * rr;
+            }
+         }
+         else
+            for (i = j + 1; i < M; i++) F_(cdiv)(pr, pi, &Aj[2*i], &Aj[2*i+1]);
+      }
+      else if (!info)
+         info = j + 1;
+      for (c = j + 1; c < N; c++)
+      {
+         T *ac = A + 2*(size_t)c*lda;
+         const T ur = ac[2*j], ui = ac[2*j+1];
+         for (i = j + 1; i < M; i++)
+         {
+            const T lr = Aj[2*i], li = Aj[2*i+1];
+            ac[2*i] -= lr * ur - li * ui;
+            ac[2*i+1] -= lr * ui + li * ur;
+         }
+      }
+   }
+   if (mingap) *mingap = gap;
+   return info;
+}
+
 #undef F_

@@ -216,3 +216,137 @@ def test_ztrsm_alpha_zero_rowmajor_and_large():
     B0 = _cplx((n, nr), np.complex128)
     D = X.ctrsm(X.Left, X.Lower, X.ConjTrans, X.NonUnit, 1.0, A, B0.copy(order="F"))
     assert _trsm_resid(X.Left, X.Lower, X.ConjTrans, X.NonUnit, 1.0, A, D, B0) < 1.0
+
+
+# ---------------------------------------------------------------- LU family
+def _getrf(A):
+    p = "z" if A.dtype == np.complex128 else "c"
+    M, N = A.shape
+    ipiv = np.zeros(max(min(M, N), 1), dtype=np.int32)
+    info = getattr(X.lib(), "clapack_%sgetrf" % p)(X.ColMajor, M, N, A.ctypes.data,
+                                                   X._ld(A), ipiv.ctypes.data)
+    return info, ipiv[:min(M, N)]
+
+
+def _lu_resid(LU, ipiv, A0):
+    """lutst's ||P A - L U|| / (||A|| eps n) with the factors unpacked."""
+    M, N = A0.shape
+    k = min(M, N)
+    L = np.tril(LU[:, :k].astype(np.complex128), -1) + np.eye(M, k)
+    U = np.triu(LU[:k, :].astype(np.complex128))
+    P = np.arange(M)
+    for j, q in enumerate(ipiv):
+        P[[j, q]] = P[[q, j]]
+    R = A0[P].astype(np.complex128) - L @ U
+    return np.abs(R).sum(0).max() / (np.abs(A0).sum(0).max() * _eps(LU.dtype) * max(M, N))
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("M,N", [(1, 1), (5, 5), (8, 8), (9, 9), (64, 64), (130, 100),
+                                 (100, 130), (300, 300), (517, 517)])
+def test_cgetrf_pivots_and_residual_vs_oracle(dt, M, N):
+    rng = np.random.default_rng(M * 1000 + N)
+    A0 = np.asfortranarray((rng.uniform(-1, 1, (M, N)) + 1j * rng.uniform(-1, 1, (M, N)))
+                           .astype(dt))
+    A = A0.copy(order="F")
+    info, ipiv = _getrf(A)
+    T = A0.copy(order="F")
+    tinfo, tpiv, gap = O.cgetf2(T)
+    assert info == tinfo == 0
+    assert _lu_resid(A, ipiv, A0) < 10.0
+    if gap > 1e-5:
+        assert np.array_equal(ipiv, tpiv)
+        tol = 1e-9 if dt == np.complex128 else 5e-3
+        assert np.max(np.abs(A - T)) <= tol * max(1.0, np.max(np.abs(T)))
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_cgetrf_zero_pivot_info(dt):
+    """info = first exactly zero pivot (1-based), at leaf and panel edges."""
+    for col in (0, 7, 8, 63, 127, 128, 200):
+        n = 260
+        rng = np.random.default_rng(col)
+        A = np.asfortranarray((rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)))
+                              .astype(dt))
+        # rows col.. are zero in columns 0..col: the first col eliminations
+        # pivot above them and leave them exactly zero, so U(col,col) = 0
+        A[col:, :col + 1] = 0
+        T = A.copy(order="F")
+        info, ipiv = _getrf(A)
+        tinfo, tpiv, _ = O.cgetf2(T)
+        assert info == tinfo == col + 1
+        assert ipiv[col] == tpiv[col] == col
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("trans", OPS)
+def test_cgetrs_and_gesv(dt, trans):
+    p = "z" if dt == np.complex128 else "c"
+    L = X.lib()
+    for n, nrhs in [(1, 1), (37, 3), (200, 17), (700, 64)]:
+        rng = np.random.default_rng(n + nrhs)
+        A0 = np.asfortranarray((rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)))
+                               .astype(dt))
+        B0 = np.asfortranarray((rng.uniform(-1, 1, (n, nrhs)) +
+                                1j * rng.uniform(-1, 1, (n, nrhs))).astype(dt))
+        A = A0.copy(order="F")
+        info, ipiv = _getrf(A)
+        assert info == 0
+        B = B0.copy(order="F")
+        assert getattr(L, "clapack_%sgetrs" % p)(X.ColMajor, trans, n, nrhs, A.ctypes.data, n,
+                                                  ipiv.ctypes.data, B.ctypes.data, n) == 0
+        E = _op(A0, trans)
+        R = E @ B.astype(np.complex128) - B0
+        # slvtst: ||A x - b|| / (||A|| ||x|| eps n)
+        r = np.abs(R).sum(0).max() / (np.abs(E).sum(0).max() * np.abs(B).sum(0).max() *
+                                      _eps(dt) * n)
+        assert r < 10.0, (n, nrhs, r)
+        if trans == X.NoTrans:
+            A = A0.copy(order="F")
+            B = B0.copy(order="F")
+            piv = np.zeros(n, dtype=np.int32)
+            assert getattr(L, "clapack_%sgesv" % p)(X.ColMajor, n, nrhs, A.ctypes.data, n,
+                                                     piv.ctypes.data, B.ctypes.data, n) == 0
+            assert np.array_equal(piv, ipiv)
+            r = np.abs(A0 @ B.astype(np.complex128) - B0).sum(0).max() / (
+                np.abs(A0).sum(0).max() * np.abs(B).sum(0).max() * _eps(dt) * n)
+            assert r < 10.0
+
+
+def test_zgesv_rowmajor_and_getrs_rowmajor_conjtrans_as_trans():
+    """RowMajor factors A^T; getrs ConjTrans in RowMajor solves with A^T, as
+    ATLAS's ATL_getrs.c RowMajor branches do."""
+    L = X.lib()
+    n, nrhs = 150, 4
+    rng = np.random.default_rng(5)
+    A0 = np.ascontiguousarray(rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)))
+    B0 = np.ascontiguousarray(rng.uniform(-1, 1, (n, nrhs)) + 1j * rng.uniform(-1, 1, (n, nrhs)))
+    A = A0.copy()
+    B = B0.copy()
+    piv = np.zeros(n, dtype=np.int32)
+    # RowMajor B is n x nrhs row-major: ldb = nrhs ... ATLAS's clapack takes B
+    # as NRHS columns of length N in the column-major sense for both orders
+    Bc = np.asfortranarray(B0)
+    assert L.clapack_zgesv(X.RowMajor, n, nrhs, A.ctypes.data, n, piv.ctypes.data,
+                           Bc.ctypes.data, n) == 0
+    assert np.max(np.abs(A0 @ Bc - B0)) < 1e-9
+    for trans, E in [(X.NoTrans, A0), (X.Trans, A0.T), (X.ConjTrans, A0.T)]:
+        Bt = np.asfortranarray(B0.copy())
+        assert L.clapack_zgetrs(X.RowMajor, trans, n, nrhs, A.ctypes.data, n, piv.ctypes.data,
+                                Bt.ctypes.data, n) == 0
+        assert np.max(np.abs(E @ Bt - B0)) < 1e-9, trans
+
+
+def test_zgesv_4096():
+    n, nrhs = 4096, 8
+    rng = np.random.default_rng(11)
+    A0 = np.asfortranarray(rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)))
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)) + 1j * rng.uniform(-1, 1, (n, nrhs)))
+    A, B = A0.copy(order="F"), B0.copy(order="F")
+    piv = np.zeros(n, dtype=np.int32)
+    assert X.lib().clapack_zgesv(X.ColMajor, n, nrhs, A.ctypes.data, n, piv.ctypes.data,
+                                 B.ctypes.data, n) == 0
+    r = np.abs(A0 @ B - B0).sum(0).max() / (np.abs(A0).sum(0).max() * np.abs(B).sum(0).max() *
+                                            _eps(np.complex128) * n)
+    assert r < 10.0
+    assert _lu_resid(A, piv, A0) < 10.0

@@ -9,7 +9,11 @@
   against numpy int64 arithmetic -- this pins the conjugation and transpose
   conventions without the reference;
 * TRSM against ATL_ztrsm's reference (ATL_zreftrsm / ATL_creftrsm) within
-  rounding, all 24 cases, and against numpy's dense solve of op(A).
+  rounding, all 24 cases, and against numpy's dense solve of op(A);
+* the unblocked complex LU (ATL_getf2) against scipy's LAPACK ?getrf, which
+  chooses pivots by the same |Re| + |Im| rule: pivots equal away from near
+  ties, factors within rounding, P A = L U always; zero-pivot info and the
+  below-SAFMIN division branch by hand.
 """
 import ctypes as C
 import itertools
@@ -154,3 +158,51 @@ def test_ctrsm_ref_vs_atlas_reftrsm_and_numpy(dt):
             else:
                 want = np.linalg.solve(E.T, (alpha * B0.astype(np.complex128)).T).T
             assert np.max(np.abs(b1 - want)) <= tol, (side, uplo, ta, diag, alpha)
+
+
+def _perm_from_ipiv(ipiv, m):
+    p = np.arange(m)
+    for k, q in enumerate(ipiv):
+        p[[k, q]] = p[[q, k]]
+    return p
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("M,N", [(1, 1), (7, 7), (40, 25), (25, 40), (120, 120)])
+def test_cgetf2_vs_scipy_lu_factor(dt, M, N):
+    """scipy's LAPACK ?getrf picks pivots by izamax (|Re| + |Im|, first max),
+    the rule ATL_getf2 uses: pivots agree where no near tie exists, and the
+    factors agree within rounding."""
+    import scipy.linalg as sl
+    tol = 1e-12 if dt == np.complex128 else 2e-4
+    for seed in range(3):
+        rng = np.random.default_rng(seed + 10 * M + N)
+        A0 = np.asfortranarray((rng.uniform(-1, 1, (M, N)) +
+                                1j * rng.uniform(-1, 1, (M, N))).astype(dt))
+        A = A0.copy(order="F")
+        info, ipiv, gap = O.cgetf2(A)
+        assert info == 0
+        lu, piv = sl.lu_factor(A0.astype(np.complex128), check_finite=False)
+        if gap > 1e-6:
+            assert np.array_equal(ipiv, piv[:min(M, N)]), seed
+            assert np.max(np.abs(A - lu)) <= tol * 10, seed
+        # P A = L U whatever the ties
+        k = min(M, N)
+        L = np.tril(A[:, :k].astype(np.complex128), -1) + np.eye(M, k)
+        U = np.triu(A[:k, :].astype(np.complex128))
+        P = _perm_from_ipiv(ipiv, M)
+        assert np.max(np.abs(A0[P].astype(np.complex128) - L @ U)) <= tol * max(M, N)
+
+
+def test_cgetf2_zero_pivot_info_and_safmin_division():
+    # an exactly zero column 3: info = 4, the elimination carries on
+    A = np.asfortranarray(RNG.uniform(-1, 1, (6, 6)) + 1j * RNG.uniform(-1, 1, (6, 6)))
+    A[:, 3] = 0
+    info, ipiv, _ = O.cgetf2(A)
+    assert info == 4 and ipiv[3] == 3
+    # a pivot below SAFMIN is divided into the column, not inverted
+    tiny = np.finfo(np.float64).tiny
+    B = np.asfortranarray(np.array([[tiny * 0.25 + 0j, 1.0], [tiny * 0.125j, 2.0]]))
+    info, ipiv, _ = O.cgetf2(B)
+    assert info == 0 and ipiv[0] == 0
+    assert abs(B[1, 0] - 0.5j) < 1e-15
