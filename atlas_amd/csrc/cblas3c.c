@@ -23,6 +23,8 @@
 #define K_CTRI_INV atlmx_k_ztri_inv
 #define K_CGETF2 atlmx_k_zgetf2
 #define K_RLASWP atlmx_k_dlaswp
+#define K_RLASWP_APPLY atlmx_k_dlaswp_apply
+#define K_CTRSM_LLNU atlmx_k_ztrsm_llnu
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"
 #undef T
@@ -33,6 +35,8 @@
 #undef K_CTRI_INV
 #undef K_CGETF2
 #undef K_RLASWP
+#undef K_RLASWP_APPLY
+#undef K_CTRSM_LLNU
 
 #define T float
 #define PFX c
@@ -42,5 +46,7 @@
 #define K_CTRI_INV atlmx_k_ctri_inv
 #define K_CGETF2 atlmx_k_cgetf2
 #define K_RLASWP atlmx_k_slaswp
+#define K_RLASWP_APPLY atlmx_k_slaswp_apply
+#define K_CTRSM_LLNU atlmx_k_ctrsm_llnu
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"

@@ -210,6 +210,7 @@ constexpr int CLEAF_MAXW = 16;
 template <typename T>
 __global__ __launch_bounds__(1024) void cgetf2_kernel(int m, int n, T *__restrict__ A, long lda,
                                                       int row0, int *__restrict__ ipiv,
+                                                      int *__restrict__ ipiv2,
                                                       int *__restrict__ info) {
   __shared__ T s_v[16];
   __shared__ int s_i[16];
@@ -243,6 +244,7 @@ __global__ __launch_bounds__(1024) void cgetf2_kernel(int m, int n, T *__restric
       s_pr = Aj[2 * p];
       s_pi = Aj[2 * p + 1];
       ipiv[j] = row0 + p;
+      if (ipiv2) { ipiv2[2 * j] = 2 * (row0 + p); ipiv2[2 * j + 1] = 2 * (row0 + p) + 1; }
     }
     __syncthreads();
     const int p = s_p;
@@ -290,6 +292,190 @@ __global__ __launch_bounds__(1024) void cgetf2_kernel(int m, int n, T *__restric
       }
     }
     __syncthreads();
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * The same leaf with its rows in registers: thread t holds rows t,
+ * t + 1024, ... (R of them) of the W columns, so a column step costs two
+ * LDS exchanges and three barriers instead of dependent global round trips.
+ * Rows j and p are exchanged through LDS (the owners of rows j and p write
+ * them, then take each other's); the update reads the pivot row from LDS.
+ * m <= 1024 R, n <= W.  Also writes ipiv2 (the pivots of the interleaved
+ * real view, rows 2p and 2p + 1) when given.
+ * ---------------------------------------------------------------------- */
+template <typename T, int R, int W>
+__global__ __launch_bounds__(1024) void cgetf2_reg_kernel(int m, int n, T *__restrict__ A,
+                                                          long lda, int row0,
+                                                          int *__restrict__ ipiv,
+                                                          int *__restrict__ ipiv2,
+                                                          int *__restrict__ info) {
+  __shared__ T s_v[16];
+  __shared__ int s_i[16];
+  __shared__ int s_p;
+  __shared__ T s_pr[W], s_pi[W], s_jr[W], s_ji[W];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  T ar[R][W], ai[R][W];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = tid + 1024 * q;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+      const bool in = (i < m && c < n);
+      ar[q][c] = in ? A[2 * ((size_t)c * lda + i)] : T(0);
+      ai[q][c] = in ? A[2 * ((size_t)c * lda + i) + 1] : T(0);
+    }
+  }
+  const int mn = min(m, n);
+#pragma unroll
+  for (int j = 0; j < W; ++j) {
+    if (j >= mn) break;
+    T bv = T(0);
+    int bi = j;
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = tid + 1024 * q;
+      if (i >= j && i < m) {
+        const T v = fabs(ar[q][j]) + fabs(ai[q][j]);
+        if (v > bv) { bv = v; bi = i; }
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const T ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+    }
+    if (lane == 0) { s_v[wid] = bv; s_i[wid] = bi; }
+    __syncthreads();
+    if (tid == 0) {
+      T v = s_v[0];
+      int p = s_i[0];
+      for (int w = 1; w < 16; ++w)
+        if (s_v[w] > v || (s_v[w] == v && s_i[w] < p)) { v = s_v[w]; p = s_i[w]; }
+      s_p = p;
+      ipiv[j] = row0 + p;
+      if (ipiv2) { ipiv2[2 * j] = 2 * (row0 + p); ipiv2[2 * j + 1] = 2 * (row0 + p) + 1; }
+    }
+    __syncthreads();
+    const int p = s_p;
+    /* the owners of rows p and j publish them */
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = tid + 1024 * q;
+      if (i == p) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) { s_pr[c] = ar[q][c]; s_pi[c] = ai[q][c]; }
+      }
+      if (i == j) {
+#pragma unroll
+        for (int c = 0; c < W; ++c) { s_jr[c] = ar[q][c]; s_ji[c] = ai[q][c]; }
+      }
+    }
+    __syncthreads();
+    const T pr = s_pr[j], pi = s_pi[j];
+    const T pv = fabs(pr) + fabs(pi);   /* the same in every thread */
+    /* the row the update uses: the pivot row after a swap, else row j */
+    const T *ur = (pv != T(0)) ? s_pr : s_jr;
+    const T *ui = (pv != T(0)) ? s_pi : s_ji;
+    if (pv != T(0)) {
+      T rr = T(0), ri = T(0);
+      const bool inv = (pv >= SafMin<T>::v);
+      if (inv) crecip(pr, pi, rr, ri);
+#pragma unroll
+      for (int q = 0; q < R; ++q) {
+        const int i = tid + 1024 * q;
+        if (p != j && i == j) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) { ar[q][c] = s_pr[c]; ai[q][c] = s_pi[c]; }
+        } else if (p != j && i == p) {
+#pragma unroll
+          for (int c = 0; c < W; ++c) { ar[q][c] = s_jr[c]; ai[q][c] = s_ji[c]; }
+        }
+        if (i > j && i < m) {
+          T xr = ar[q][j], xi = ai[q][j];
+          if (inv) {
+            const T yr = xr * rr - xi * ri;
+            xi = xr * ri + xi * rr;
+            xr = yr;
+          } else {
+            cdivide(pr, pi, xr, xi);
+          }
+          ar[q][j] = xr; ai[q][j] = xi;
+        }
+      }
+    } else if (tid == 0 && *info == 0) {
+      *info = row0 + j + 1;
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      const int i = tid + 1024 * q;
+      if (i > j && i < m) {
+        const T lr = ar[q][j], li = ai[q][j];
+#pragma unroll
+        for (int c = j + 1; c < W; ++c) {
+          ar[q][c] -= lr * ur[c] - li * ui[c];
+          ai[q][c] -= lr * ui[c] + li * ur[c];
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = tid + 1024 * q;
+#pragma unroll
+    for (int c = 0; c < W; ++c)
+      if (i < m && c < n) {
+        A[2 * ((size_t)c * lda + i)] = ar[q][c];
+        A[2 * ((size_t)c * lda + i) + 1] = ai[q][c];
+      }
+  }
+}
+
+/* ------------------------------------------------------------------------
+ * B := inv(L) B for a unit lower triangular L of order n1 <= 64 and B of
+ * n1 rows, in place; one workgroup per 64 columns of B, with L and its
+ * column block in LDS.  Right-looking: after row t of the solution is final,
+ * every row below takes its L(r,t) multiple, one barrier per t.  This is
+ * the TRSM(Left, Lower, NoTrans, Unit) of the LU recursion and of the
+ * panel's U12, in one launch.
+ * ---------------------------------------------------------------------- */
+constexpr int STRSM_N = 64;
+template <typename T>
+__global__ __launch_bounds__(1024) void ctrsm_llnu_kernel(int n1, int nr, const T *__restrict__ L,
+                                                          long lda, T *__restrict__ B, long ldb) {
+  __shared__ T Lr[STRSM_N][STRSM_N + 1], Li[STRSM_N][STRSM_N + 1];
+  __shared__ T Br[STRSM_N][STRSM_N + 1], Bi[STRSM_N][STRSM_N + 1];
+  const int tid = threadIdx.x;
+  const int c0 = blockIdx.x * STRSM_N, nc = min(STRSM_N, nr - c0);
+  for (int e = tid; e < n1 * n1; e += 1024) {
+    const int r = e % n1, c = e / n1;
+    if (r > c) {
+      const T *l = L + 2 * ((size_t)c * lda + r);
+      Lr[r][c] = l[0]; Li[r][c] = l[1];
+    }
+  }
+  for (int e = tid; e < n1 * nc; e += 1024) {
+    const int r = e % n1, c = e / n1;
+    const T *b = B + 2 * ((size_t)(c0 + c) * ldb + r);
+    Br[r][c] = b[0]; Bi[r][c] = b[1];
+  }
+  __syncthreads();
+  for (int t = 0; t + 1 < n1; ++t) {
+    const int rows = n1 - 1 - t;
+    for (int e = tid; e < rows * nc; e += 1024) {
+      const int r = t + 1 + e % rows, c = e / rows;
+      const T lr = Lr[r][t], li = Li[r][t], xr = Br[t][c], xi = Bi[t][c];
+      Br[r][c] -= lr * xr - li * xi;
+      Bi[r][c] -= lr * xi + li * xr;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < n1 * nc; e += 1024) {
+    const int r = e % n1, c = e / n1;
+    T *b = B + 2 * ((size_t)(c0 + c) * ldb + r);
+    b[0] = Br[r][c]; b[1] = Bi[r][c];
   }
 }
 
@@ -370,24 +556,56 @@ extern "C" hipError_t atlmx_k_ctri_inv(int lower, int n, const float *H, long ld
 }
 
 namespace {
+/* the register leaf where its rows fit (8 columns up to 2048 rows, 4 up to
+ * 4096), else the global-memory one */
 template <typename T>
-hipError_t launch_cgetf2(int m, int n, T *A, long lda, int row0, int *ipiv, int *info,
-                         hipStream_t s) {
+hipError_t launch_cgetf2(int m, int n, T *A, long lda, int row0, int *ipiv, int *ipiv2,
+                         int *info, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
   if (n > CLEAF_MAXW || !A || !ipiv || !info || lda < m) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((cgetf2_kernel<T>), dim3(1), dim3(1024), 0, s, m, n, A, lda, row0, ipiv,
-                     info);
+  const dim3 g(1), b(1024);
+  if (n <= 8 && m <= 1024)
+    hipLaunchKernelGGL((cgetf2_reg_kernel<T, 1, 8>), g, b, 0, s, m, n, A, lda, row0, ipiv,
+                       ipiv2, info);
+  else if (n <= 8 && m <= 2048)
+    hipLaunchKernelGGL((cgetf2_reg_kernel<T, 2, 8>), g, b, 0, s, m, n, A, lda, row0, ipiv,
+                       ipiv2, info);
+  else if (n <= 4 && m <= 4096)
+    hipLaunchKernelGGL((cgetf2_reg_kernel<T, 4, 4>), g, b, 0, s, m, n, A, lda, row0, ipiv,
+                       ipiv2, info);
+  else
+    hipLaunchKernelGGL((cgetf2_kernel<T>), g, b, 0, s, m, n, A, lda, row0, ipiv, ipiv2, info);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_ctrsm_llnu(int n1, int nr, const T *L, long lda, T *B, long ldb,
+                             hipStream_t s) {
+  if (n1 <= 0 || nr <= 0) return hipSuccess;
+  if (n1 > STRSM_N || !L || !B || lda < n1 || ldb < n1) return hipErrorInvalidValue;
+  const long nb = (nr + STRSM_N - 1) / STRSM_N;
+  if (nb > (1L << 30)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((ctrsm_llnu_kernel<T>), dim3((unsigned)nb), dim3(1024), 0, s, n1, nr, L,
+                     lda, B, ldb);
   return hipGetLastError();
 }
 }  // namespace
 
 extern "C" hipError_t atlmx_k_zgetf2(int m, int n, double *A, long lda, int row0, int *ipiv,
-                                     int *info, hipStream_t s) {
-  return launch_cgetf2<double>(m, n, A, lda, row0, ipiv, info, s);
+                                     int *ipiv2, int *info, hipStream_t s) {
+  return launch_cgetf2<double>(m, n, A, lda, row0, ipiv, ipiv2, info, s);
 }
 extern "C" hipError_t atlmx_k_cgetf2(int m, int n, float *A, long lda, int row0, int *ipiv,
-                                     int *info, hipStream_t s) {
-  return launch_cgetf2<float>(m, n, A, lda, row0, ipiv, info, s);
+                                     int *ipiv2, int *info, hipStream_t s) {
+  return launch_cgetf2<float>(m, n, A, lda, row0, ipiv, ipiv2, info, s);
+}
+extern "C" hipError_t atlmx_k_ztrsm_llnu(int n1, int nr, const double *L, long lda, double *B,
+                                         long ldb, hipStream_t s) {
+  return launch_ctrsm_llnu<double>(n1, nr, L, lda, B, ldb, s);
+}
+extern "C" hipError_t atlmx_k_ctrsm_llnu(int n1, int nr, const float *L, long lda, float *B,
+                                         long ldb, hipStream_t s) {
+  return launch_ctrsm_llnu<float>(n1, nr, L, lda, B, ldb, s);
 }
 extern "C" hipError_t atlmx_k_cpiv2(int k1, int k2, const int *ipiv, int *ipiv2,
                                     hipStream_t s) {

@@ -390,13 +390,20 @@ hipError_t atlmx_k_ctri_inv(int lower, int n, const float *H, long ldh, float *i
                             hipStream_t s);
 /* Unblocked complex LU (ATL_getf2's column loop) of an m x n leaf, n <=
  * ATLMX_CLEAF_MAXW, whose top row is global row row0: ipiv[j] receives
- * row0 + the pivot row, *info = row0 + j + 1 at the first zero pivot (when
- * *info is 0).  One workgroup. */
+ * row0 + the pivot row (and, when ipiv2 is not NULL, ipiv2[2j + s] = 2 (row0
+ * + pivot row) + s), *info = row0 + j + 1 at the first zero pivot (when
+ * *info is 0).  One workgroup, the rows in registers where they fit. */
 #define ATLMX_CLEAF_MAXW 16
-hipError_t atlmx_k_zgetf2(int m, int n, double *A, long lda, int row0, int *ipiv, int *info,
-                          hipStream_t s);
-hipError_t atlmx_k_cgetf2(int m, int n, float *A, long lda, int row0, int *ipiv, int *info,
-                          hipStream_t s);
+hipError_t atlmx_k_zgetf2(int m, int n, double *A, long lda, int row0, int *ipiv, int *ipiv2,
+                          int *info, hipStream_t s);
+hipError_t atlmx_k_cgetf2(int m, int n, float *A, long lda, int row0, int *ipiv, int *ipiv2,
+                          int *info, hipStream_t s);
+/* B := inv(L) B, L unit lower of order n1 <= 64 (its strict lower triangle
+ * read), B n1 x nr, in place; one launch. */
+hipError_t atlmx_k_ztrsm_llnu(int n1, int nr, const double *L, long lda, double *B, long ldb,
+                              hipStream_t s);
+hipError_t atlmx_k_ctrsm_llnu(int n1, int nr, const float *L, long lda, float *B, long ldb,
+                              hipStream_t s);
 /* ipiv2[2k + s] = 2 ipiv[k] + s, k in [k1, k2): a complex matrix's row
  * interchanges as those of its interleaved real view */
 hipError_t atlmx_k_cpiv2(int k1, int k2, const int *ipiv, int *ipiv2, hipStream_t s);

@@ -9,24 +9,29 @@
  * factored by the recursive halving of src/lapack/ATL_getrfC.c:242-361
  * (left half, its interchanges and TRSM/GEMM on the right half, right half,
  * its interchanges on the left half) down to leaves of at most CLU_LEAF
- * columns, which one workgroup factors unblocked (k_cplx.hip, the column
- * loop of ATL_getf2.c).  Every TRSM and GEMM is the complex one of
- * cblas3c_tmpl.h, i.e. the real MFMA GEMM on expanded operands.  Row
- * interchanges of a complex matrix are those of its interleaved real view
- * with each pivot doubled (rows 2k, 2k+1 <-> 2p, 2p+1), so the real laswp
- * kernel applies them.
+ * columns, which one workgroup factors unblocked with the rows in registers
+ * (k_cplx.hip, the column loop of ATL_getf2.c).  The unit-lower TRSMs of
+ * the recursion and the panel's U12 are one LDS kernel each; the GEMM
+ * updates are the complex GEMM of cblas3c_tmpl.h (the real MFMA GEMM on an
+ * expanded operand).  Row interchanges of a complex matrix are those of its
+ * interleaved real view with each pivot doubled (rows 2k, 2k+1 <-> 2p,
+ * 2p+1; the leaves write both forms): the real laswp kernels apply them,
+ * and a panel's interchanges reach the rest of the matrix as one move list
+ * (plan) applied to every other column in one launch.
  * ---------------------------------------------------------------------- */
-#define CLU_NB 128
+#define CLU_NB 64
 #define CLU_LEAF 8
 
 /* the interchanges ipiv[k1..k2) (forward for inc > 0, reverse for inc < 0)
- * on ncols columns of the complex matrix A, rows below rows_end */
+ * on ncols columns of the complex matrix A, rows below rows_end; ipiv2
+ * holds the doubled pivots for [2 k1, 2 k2) already when d_ipiv is NULL,
+ * else they are formed from d_ipiv first */
 static void NM(PFX, laswp_dev)(atlmx_ctx *c, int ncols, T *A, long lda, int k1, int k2,
                                const int *d_ipiv, int inc, int rows_end, int *ipiv2,
                                const char *rout)
 {
   if (ncols <= 0 || k2 <= k1) return;
-  RT_CHECK(rout, atlmx_k_cpiv2(k1, k2, d_ipiv, ipiv2, c->stream));
+  if (d_ipiv) RT_CHECK(rout, atlmx_k_cpiv2(k1, k2, d_ipiv, ipiv2, c->stream));
   RT_CHECK(rout, K_RLASWP(ncols, A, 2 * lda, 2 * k1, 2 * k2, ipiv2, inc, 2 * rows_end,
                           c->stream));
 }
@@ -39,25 +44,28 @@ static void NM(PFX, getrf_rec)(atlmx_ctx *c, int M, int c0, int w, T *A, long ld
 {
   static const T one[2] = {(T)1, (T)0}, mone[2] = {(T)-1, (T)0};
   T *Acc = A + 2 * ((size_t)c0 * lda + c0);
+  /* leaves whose rows fit the registers of one workgroup: 8 columns up to
+   * 2048 rows, 4 up to 4096 (wider leaves above that run from memory) */
+  const int leaf = (M - c0 > 2048 && M - c0 <= 4096) ? CLU_LEAF / 2 : CLU_LEAF;
   int w1;
   T *A12, *A22;
-  if (w <= CLU_LEAF) {
-    RT_CHECK(rout, K_CGETF2(M - c0, w, Acc, lda, c0, d_ipiv + c0, d_info, c->stream));
+  if (w <= leaf) {
+    RT_CHECK(rout, K_CGETF2(M - c0, w, Acc, lda, c0, d_ipiv + c0, ipiv2 + 2 * (size_t)c0,
+                            d_info, c->stream));
     return;
   }
   w1 = w / 2;
   A12 = A + 2 * ((size_t)(c0 + w1) * lda + c0);
   A22 = A12 + 2 * (size_t)w1;
   NM(PFX, getrf_rec)(c, M, c0, w1, A, lda, d_ipiv, ipiv2, d_info, rout);
-  NM(PFX, laswp_dev)(c, w - w1, A + 2 * (size_t)(c0 + w1) * lda, lda, c0, c0 + w1, d_ipiv, 1,
+  NM(PFX, laswp_dev)(c, w - w1, A + 2 * (size_t)(c0 + w1) * lda, lda, c0, c0 + w1, NULL, 1,
                      M, ipiv2, rout);
-  NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, w1, w - w1, one, Acc, lda, A12,
-                    lda, rout);
+  RT_CHECK(rout, K_CTRSM_LLNU(w1, w - w1, Acc, lda, A12, lda, c->stream));
   if (M - c0 - w1 > 0)
     NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - c0 - w1, w - w1, w1, mone,
                       Acc + 2 * (size_t)w1, lda, A12, lda, one, A22, lda, rout);
   NM(PFX, getrf_rec)(c, M, c0 + w1, w - w1, A, lda, d_ipiv, ipiv2, d_info, rout);
-  NM(PFX, laswp_dev)(c, w1, A + 2 * (size_t)c0 * lda, lda, c0 + w1, c0 + w, d_ipiv, 1, M,
+  NM(PFX, laswp_dev)(c, w1, A + 2 * (size_t)c0 * lda, lda, c0 + w1, c0 + w, NULL, 1, M,
                      ipiv2, rout);
 }
 
@@ -69,22 +77,29 @@ int NM(PFX, getrf_dev)(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv,
 {
   static const T one[2] = {(T)1, (T)0}, mone[2] = {(T)-1, (T)0};
   const int mn = M < N ? M : N;
-  int *d_info, *ipiv2, info = 0, k;
+  int *d_info, *ipiv2, *plan, info = 0, k;
   if (mn <= 0) return 0;
   d_info = (int *)rt_ws(c, WS_INFO, sizeof(int), rout);
   ipiv2 = (int *)rt_ws(c, WS_CPIV, (size_t)2 * mn * sizeof(int), rout);
+  plan = (int *)rt_ws(c, WS_PLAN, ATLMX_LASWP_PLAN_INTS * sizeof(int), rout);
   RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
   for (k = 0; k < mn; k += CLU_NB) {
     const int jb = (mn - k < CLU_NB) ? mn - k : CLU_NB;
     T *Akk = A + 2 * ((size_t)k * lda + k);
     NM(PFX, getrf_rec)(c, M, k, jb, A, lda, d_ipiv, ipiv2, d_info, rout);
-    NM(PFX, laswp_dev)(c, k, A, lda, k, k + jb, d_ipiv, 1, M, ipiv2, rout);
+    /* the panel's interchanges on every other column */
+    if (2 * jb <= 2 * ATLMX_PANEL_MAXW && 2L * (M - k) <= ATLMX_LASWP_PLAN_MAXROWS) {
+      RT_CHECK(rout, atlmx_k_laswp_plan(2 * k, 2 * (k + jb), 2 * M, ipiv2, plan, c->stream));
+      RT_CHECK(rout, K_RLASWP_APPLY(A, 2 * lda, 0, N, k, k + jb, plan, c->stream));
+    } else {
+      NM(PFX, laswp_dev)(c, k, A, lda, k, k + jb, NULL, 1, M, ipiv2, rout);
+      if (k + jb < N)
+        NM(PFX, laswp_dev)(c, N - k - jb, A + 2 * (size_t)(k + jb) * lda, lda, k, k + jb, NULL,
+                           1, M, ipiv2, rout);
+    }
     if (k + jb < N) {
       T *A12 = A + 2 * ((size_t)(k + jb) * lda + k);
-      NM(PFX, laswp_dev)(c, N - k - jb, A + 2 * (size_t)(k + jb) * lda, lda, k, k + jb,
-                         d_ipiv, 1, M, ipiv2, rout);
-      NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_UNIT, jb, N - k - jb, one, Akk,
-                        lda, A12, lda, rout);
+      RT_CHECK(rout, K_CTRSM_LLNU(jb, N - k - jb, Akk, lda, A12, lda, c->stream));
       if (k + jb < M)
         NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - k - jb, N - k - jb, jb, mone,
                           Akk + 2 * (size_t)jb, lda, A12, lda, one, A12 + 2 * (size_t)jb, lda,

@@ -325,6 +325,61 @@ def bench_lapack(n, seed, torch, X, log, reps=3):
     return out
 
 
+def bench_complex(n, seed, torch, X, log, reps=3):
+    """The complex routines at order n on device operands: zgemm and cgemm
+    (NoTrans x ConjTrans, alpha = 1 - 0.5i, beta = 0.25i; 8 n^3 flops), and
+    zgetrf (A ~ U[-1,1) + i U[-1,1); 8/3 n^3 flops) with zgetrs on 64
+    right-hand sides.  Best of reps."""
+    import numpy as np
+    L = X.lib()
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    out = {"n": n}
+
+    def best(fn, prep):
+        ts = []
+        for _ in range(reps):
+            prep()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    for p, dt, peak in (("z", torch.complex128, PEAK_FP64_MFMA),
+                        ("c", torch.complex64, PEAK_FP32_MFMA)):
+        rt = np.float64 if p == "z" else np.float32
+        A = torch.randn(n, n, dtype=dt, device="cuda", generator=g)
+        B = torch.randn(n, n, dtype=dt, device="cuda", generator=g)
+        Cm = torch.randn(n, n, dtype=dt, device="cuda", generator=g)
+        al = np.array([1.0, -0.5], dtype=rt)
+        be = np.array([0.0, 0.25], dtype=rt)
+        f = getattr(L, "cblas_%sgemm" % p)
+        t = best(lambda: f(X.ColMajor, X.NoTrans, X.ConjTrans, n, n, n, al.ctypes.data,
+                           A.data_ptr(), n, B.data_ptr(), n, be.ctypes.data, Cm.data_ptr(), n),
+                 lambda: None)
+        fl = 8.0 * n ** 3
+        out["%sgemm" % p] = {"ms": round(t * 1e3, 2), "tflops": round(fl / t / 1e12, 2),
+                             "frac_of_real_mfma_peak": round(fl / t / peak, 3)}
+        del B, Cm
+    A0 = torch.rand(n, n, dtype=torch.complex128, device="cuda", generator=g) * 2 - (1 + 1j)
+    Aw = torch.empty_like(A0)
+    ipiv = torch.zeros(n, dtype=torch.int32, device="cuda")
+    t = best(lambda: L.clapack_zgetrf(X.ColMajor, n, n, Aw.data_ptr(), n, ipiv.data_ptr()),
+             lambda: Aw.copy_(A0))
+    fl = 8.0 * n ** 3 / 3
+    out["zgetrf"] = {"ms": round(t * 1e3, 2), "tflops": round(fl / t / 1e12, 3)}
+    rhs = torch.rand(64, n, dtype=torch.complex128, device="cuda", generator=g)
+    Xw = torch.empty_like(rhs)
+    t = best(lambda: L.clapack_zgetrs(X.ColMajor, X.NoTrans, n, 64, Aw.data_ptr(), n,
+                                      ipiv.data_ptr(), Xw.data_ptr(), n),
+             lambda: Xw.copy_(rhs))
+    out["zgetrs_64"] = {"ms": round(t * 1e3, 2), "tflops": round(8.0 * n * n * 64 / t / 1e12, 3)}
+    log("complex n=%d: %s" % (n, json.dumps(out)))
+    return out
+
+
 def bench_sweep(n, seed, A6, b6, torch, X, log):
     """The dense-solve configuration's other cases, one solve each: kappa
     1e2 and 1e4 (and the headline's 1e6) with the FP32 tournament LU and
@@ -402,6 +457,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=20000)
     ap.add_argument("--no-sweep", action="store_true",
                     help="skip the other kappa / precision / 16-RHS cases")
+    ap.add_argument("--no-complex", action="store_true",
+                    help="skip the complex GEMM / LU timings")
     ap.add_argument("--profile-run", action="store_true",
                     help="one warm mixed solve plus a 1 GiB device copy (the byte-counter "
                          "calibration) only: the run rocprofv3 --pmc passes wrap")
@@ -559,6 +616,10 @@ def main():
     if not args.no_sweep:
         sweep = bench_sweep(n, args.seed, A, b, torch, X, log)
 
+    cplx = None
+    if not args.no_complex:
+        cplx = bench_complex(4096, args.seed + 4096, torch, X, log)
+
     batched = None
     if not args.no_batched:
         batched = bench_batched(args.batch_count, args.seed, torch, X, log)
@@ -590,6 +651,7 @@ def main():
         "dgemm_pct_peak": round(100 * dgemm_tflops * 1e12 / PEAK_FP64_MFMA, 2),
         "kernel_time_per_step": classes, "profiled_ms_per_step": round(tmp_prof * 1e3, 3),
         "roofline": roofline, "roofline_update": roof_update, "cpu_baseline": cpu, "lapack_n8192": lapack, "sweep": sweep,
+        "complex_n4096": cplx,
         "batched": batched,
     }
     print(json.dumps(out), flush=True)

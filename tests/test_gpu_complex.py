@@ -350,3 +350,21 @@ def test_zgesv_4096():
                                             _eps(np.complex128) * n)
     assert r < 10.0
     assert _lu_resid(A, piv, A0) < 10.0
+
+
+def test_cgetrf_1500_pivots_vs_oracle():
+    """Order 1500: the leaves hold two rows per thread in registers; pivots
+    equal to the oracle's (the smallest margin between a pivot and its
+    runner-up is 2e-5 on this input, far above double rounding), residual."""
+    n = 1500
+    rng = np.random.default_rng(1500)
+    A0 = np.asfortranarray(rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)))
+    A = A0.copy(order="F")
+    info, ipiv = _getrf(A)
+    T = A0.copy(order="F")
+    tinfo, tpiv, gap = O.cgetf2(T)
+    assert info == tinfo == 0
+    assert gap > 1e-9
+    assert np.array_equal(ipiv, tpiv)
+    assert _lu_resid(A, ipiv, A0) < 10.0
+    assert np.max(np.abs(A - T)) <= 1e-8 * np.max(np.abs(T))
