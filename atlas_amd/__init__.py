@@ -135,6 +135,13 @@ def _declare(L):
         f = getattr(L, "clapack_%sgesv" % p)
         f.argtypes = [_I, _I, _I, _P, _I, _P, _P, _I]
         f.restype = _I
+        f = getattr(L, "clapack_%spotrf" % p)
+        f.argtypes = [_I, _I, _I, _P, _I]
+        f.restype = _I
+        for n in ("potrs", "posv"):
+            f = getattr(L, "clapack_%s%s" % (p, n))
+            f.argtypes = [_I, _I, _I, _I, _P, _I, _P, _I]
+            f.restype = _I
     L.atlmx_init.restype = _I
     L.atlmx_set_stream.argtypes = [_P]
     L.atlmx_get_stream.restype = _P

@@ -1,6 +1,6 @@
 /*
  * cblas3c.c -- complex Level-3 BLAS and LU (C host code) of libatlas_mi355x:
- * cblas_{c,z}{gemm,trsm}, clapack_{c,z}{getrf,getrs,gesv} and their
+ * cblas_{c,z}{gemm,trsm}, clapack_{c,z}{getrf,getrs,gesv,potrf,potrs,posv} and their
  * device-pointer twins, instantiated from cblas3c_tmpl.h and lapackc_tmpl.h
  * per precision over the real MFMA GEMM kernels.
  */
@@ -25,6 +25,8 @@
 #define K_RLASWP atlmx_k_dlaswp
 #define K_RLASWP_APPLY atlmx_k_dlaswp_apply
 #define K_CTRSM_LLNU atlmx_k_ztrsm_llnu
+#define K_CPOTF2 atlmx_k_zpotf2
+#define K_CCONJ atlmx_k_zconj
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"
 #undef T
@@ -37,6 +39,8 @@
 #undef K_RLASWP
 #undef K_RLASWP_APPLY
 #undef K_CTRSM_LLNU
+#undef K_CPOTF2
+#undef K_CCONJ
 
 #define T float
 #define PFX c
@@ -48,5 +52,7 @@
 #define K_RLASWP atlmx_k_slaswp
 #define K_RLASWP_APPLY atlmx_k_slaswp_apply
 #define K_CTRSM_LLNU atlmx_k_ctrsm_llnu
+#define K_CPOTF2 atlmx_k_cpotf2
+#define K_CCONJ atlmx_k_cconj
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"

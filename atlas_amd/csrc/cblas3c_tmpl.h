@@ -33,11 +33,12 @@ static void NM(PFX, scal_dev)(atlmx_ctx *c, int m, int n, const T *beta, T *C, l
  * complex pass); a real beta goes to the real kernel.  op(B) other than N
  * is gathered into a K x N complex workspace.  The expanded left operand
  * (4*rows*K reals) is built and used in row chunks of at most
- * ATLMX_CPLX_WS_BYTES.
+ * ATLMX_CPLX_WS_BYTES.  tri = KX_LOWER / KX_UPPER updates only C's lower /
+ * upper triangle, diagonal included (the HERK shape; one chunk then).
  * ---------------------------------------------------------------------- */
 void NM(PFX, gemm_dev)(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const T *alpha,
                        const T *A, long lda, const T *B, long ldb, const T *beta, T *C,
-                       long ldc, const char *rout)
+                       long ldc, int tri, const char *rout)
 {
   const int a_zero = (alpha[0] == (T)0 && alpha[1] == (T)0);
   const int b_one = (beta[0] == (T)1 && beta[1] == (T)0);
@@ -65,7 +66,7 @@ void NM(PFX, gemm_dev)(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const 
   mc = (long)(ATLMX_CPLX_WS_BYTES / (4.0 * K * sizeof(T)));
   mc = (mc / 64) * 64;
   if (mc < 64) mc = 64;
-  if (mc > M) mc = M;
+  if (mc > M || tri) mc = M;
   ldh = (2 * mc + 7) & ~7L;
   H = (T *)rt_ws(c, WS_L3A, (size_t)ldh * 2 * K * sizeof(T), rout);
   for (i0 = 0; i0 < M; i0 += (int)mc) {
@@ -77,7 +78,9 @@ void NM(PFX, gemm_dev)(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const 
                          2.0 * ((double)mi * K + (double)K * N +
                                 (double)mi * N * (breal != (T)0 ? 2 : 1)) * sizeof(T));
     RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * mi, N, 2 * K, (T)1, H, ldh, Bv, ldbv,
-                          breal, C + 2 * (size_t)i0, 2 * ldc, 0, NULL, c->stream));
+                          breal, C + 2 * (size_t)i0, 2 * ldc,
+                          tri == KX_LOWER ? KX_CLOWER : (tri == KX_UPPER ? KX_CUPPER : 0),
+                          NULL, c->stream));
     rt_prof_end(c, pid);
   }
 }
@@ -163,7 +166,7 @@ void NM(PFX, trsm_dev)(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M
 void NM3(atlmx_, PFX, gemm)(int ta, int tb, int M, int N, int K, const T *alpha, const T *A,
                             int lda, const T *B, int ldb, const T *beta, T *C, int ldc)
 {
-  NM(PFX, gemm_dev)(rt_ctx(), ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc,
+  NM(PFX, gemm_dev)(rt_ctx(), ta, tb, M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, 0,
                     "atlmx_gemm");
 }
 
@@ -218,7 +221,7 @@ void NM3(cblas_, PFX, gemm)(const enum CBLAS_ORDER Order, const enum CBLAS_TRANS
   dA = (T *)rt_stage_in(c, WS_STAGE_A, pa, ar, ac, la, esz, 1, &dla, rout);
   dB = (T *)rt_stage_in(c, WS_STAGE_B, pb, br, bc, lb, esz, 1, &dlb, rout);
   dC = (T *)rt_stage_in(c, WS_STAGE_C, C, m, n, ldc, esz, !b_zero, &dlc, rout);
-  NM(PFX, gemm_dev)(c, ta, tb, m, n, K, al, dA, dla, dB, dlb, be, dC, dlc, rout);
+  NM(PFX, gemm_dev)(c, ta, tb, m, n, K, al, dA, dla, dB, dlb, be, dC, dlc, 0, rout);
   rt_stage_out(c, C, dC, m, n, ldc, dlc, esz, rout);
   rt_sync(c, rout);
 }

@@ -66,6 +66,8 @@ int atlmx_getrf_check(const char *rout, int Order, int M, int N, int lda);
 int atlmx_getrs_check(const char *rout, int Order, int Trans, int N, int NRHS, int lda,
                       int ldb);
 int atlmx_gesv_check(const char *rout, int Order, int N, int NRHS, int lda, int ldb);
+int atlmx_potrf_check(const char *rout, int Order, int Uplo, int N, int lda);
+int atlmx_posv_check(const char *rout, int Order, int Uplo, int N, int NRHS, int lda, int ldb);
 
 /* complex drivers (defined in cblas3c.c); alpha and beta point at
  * {real, imaginary}; matrices interleaved, leading dimensions in complex
@@ -73,14 +75,17 @@ int atlmx_gesv_check(const char *rout, int Order, int N, int NRHS, int lda, int 
 #define ATLMX_CDRV_DECLARE(P, T)                                                      \
 void P##gemm_dev(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const T *alpha,   \
                  const T *A, long lda, const T *B, long ldb, const T *beta, T *C,     \
-                 long ldc, const char *rout);                                         \
+                 long ldc, int tri, const char *rout);                                \
 void P##trsm_dev(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,    \
                  const T *alpha, const T *A, long lda, T *B, long ldb,                \
                  const char *rout);                                                   \
 int  P##getrf_dev(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv,            \
                   const char *rout);                                                  \
 void P##getrs_dev(atlmx_ctx *c, int trans, int N, int NRHS, const T *A, long lda,     \
-                  const int *d_ipiv, T *B, long ldb, const char *rout);
+                  const int *d_ipiv, T *B, long ldb, const char *rout);               \
+int  P##potrf_dev(atlmx_ctx *c, int uplo, int N, T *A, long lda, const char *rout);   \
+void P##potrs_dev(atlmx_ctx *c, int uplo, int N, int NRHS, const T *A, long lda,      \
+                  T *B, long ldb, const char *rout);
 ATLMX_CDRV_DECLARE(z, double)
 ATLMX_CDRV_DECLARE(c, float)
 

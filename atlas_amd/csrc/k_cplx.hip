@@ -479,6 +479,75 @@ __global__ __launch_bounds__(1024) void ctrsm_llnu_kernel(int n1, int nr, const 
   }
 }
 
+/* ------------------------------------------------------------------------
+ * Cholesky of one n x n (n <= 64) Hermitian diagonal block in place: A = L
+ * L^H from the lower triangle, or A = U^H U from the upper one, which is
+ * read into LDS as its conjugate transpose (the lower triangle of A^H) and
+ * written back the same way.  Right-looking, one barrier-separated step per
+ * column: the pivot is the real part of the diagonal (the imaginary part is
+ * not referenced; the factor's diagonal is real), the column below it is
+ * divided by the pivot's square root, and the lower triangle of the trailing
+ * block takes L(i,j) conj(L(c,j)).  A pivot that is not positive (or NaN)
+ * stops the factorization and sets *info = base + j + 1 (when *info is 0);
+ * the block is written back as far as it got (src/lapack/ATL_potrfL.c: the
+ * info of the first failing column).  One 256-thread workgroup.
+ * ---------------------------------------------------------------------- */
+constexpr int CPOTF2_N = 64;
+template <typename T>
+__global__ __launch_bounds__(256) void cpotf2_kernel(int upper, int n, T *__restrict__ A,
+                                                     long lda, int base, int *__restrict__ info) {
+  __shared__ T Lr[CPOTF2_N][CPOTF2_N + 1], Li[CPOTF2_N][CPOTF2_N + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += 256) {
+    const int r = e % n, c = e / n;
+    if (r < c) continue;
+    const T *a = upper ? A + 2 * ((size_t)r * lda + c) : A + 2 * ((size_t)c * lda + r);
+    Lr[r][c] = a[0];
+    Li[r][c] = upper ? -a[1] : a[1];
+  }
+  __syncthreads();
+  for (int j = 0; j < n; ++j) {
+    const T d = Lr[j][j];
+    if (!(d > T(0))) {                          /* uniform: every thread reads d */
+      if (tid == 0 && *info == 0) *info = base + j + 1;
+      break;
+    }
+    const T s = sqrt(d), rs = T(1) / s;
+    for (int i = j + 1 + tid; i < n; i += 256) { Lr[i][j] *= rs; Li[i][j] *= rs; }
+    __syncthreads();
+    if (tid == 0) { Lr[j][j] = s; Li[j][j] = T(0); }
+    const int rows = n - 1 - j;
+    for (int e = tid; e < rows * rows; e += 256) {
+      const int i = j + 1 + e % rows, c = j + 1 + e / rows;
+      if (i < c) continue;
+      /* L(i,c) -= L(i,j) * conj(L(c,j)) */
+      const T ar = Lr[i][j], ai = Li[i][j], br = Lr[c][j], bi = -Li[c][j];
+      Lr[i][c] -= ar * br - ai * bi;
+      Li[i][c] -= ar * bi + ai * br;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int e = tid; e < n * n; e += 256) {
+    const int r = e % n, c = e / n;
+    if (r < c) continue;
+    T *a = upper ? A + 2 * ((size_t)r * lda + c) : A + 2 * ((size_t)c * lda + r);
+    a[0] = Lr[r][c];
+    a[1] = upper ? -Li[r][c] : Li[r][c];
+  }
+}
+
+/* C = conj(C), m x n complex, in place */
+template <typename T>
+__global__ __launch_bounds__(256) void cconj_kernel(int m, int n, T *C, long ldc) {
+  const long total = (long)m * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long j = e / m, i = e - j * m;
+    T *c = C + 2 * (j * ldc + i) + 1;
+    *c = -*c;
+  }
+}
+
 /* ipiv2[2k + s] = 2 ipiv[k] + s for k in [k1, k2): the row interchanges of
  * a complex matrix as those of its interleaved real view. */
 __global__ __launch_bounds__(256) void cpiv2_kernel(int k1, int k2, const int *__restrict__ ipiv,
@@ -579,6 +648,24 @@ hipError_t launch_cgetf2(int m, int n, T *A, long lda, int row0, int *ipiv, int 
 }
 
 template <typename T>
+hipError_t launch_cpotf2(int upper, int n, T *A, long lda, int base, int *info, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (n > CPOTF2_N || !A || !info || lda < n) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((cpotf2_kernel<T>), dim3(1), dim3(256), 0, s, upper, n, A, lda, base, info);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_cconj(int m, int n, T *C, long ldc, hipStream_t s) {
+  if (m <= 0 || n <= 0) return hipSuccess;
+  if (!C || ldc < m) return hipErrorInvalidValue;
+  const long want = ((long)m * n + 255) / 256;
+  hipLaunchKernelGGL((cconj_kernel<T>), dim3((unsigned)(want < 8192 ? want : 8192)), dim3(256),
+                     0, s, m, n, C, ldc);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_ctrsm_llnu(int n1, int nr, const T *L, long lda, T *B, long ldb,
                              hipStream_t s) {
   if (n1 <= 0 || nr <= 0) return hipSuccess;
@@ -614,4 +701,18 @@ extern "C" hipError_t atlmx_k_cpiv2(int k1, int k2, const int *ipiv, int *ipiv2,
   hipLaunchKernelGGL(cpiv2_kernel, dim3((k2 - k1 + 255) / 256), dim3(256), 0, s, k1, k2, ipiv,
                      ipiv2);
   return hipGetLastError();
+}
+extern "C" hipError_t atlmx_k_zpotf2(int upper, int n, double *A, long lda, int base, int *info,
+                                     hipStream_t s) {
+  return launch_cpotf2<double>(upper, n, A, lda, base, info, s);
+}
+extern "C" hipError_t atlmx_k_cpotf2(int upper, int n, float *A, long lda, int base, int *info,
+                                     hipStream_t s) {
+  return launch_cpotf2<float>(upper, n, A, lda, base, info, s);
+}
+extern "C" hipError_t atlmx_k_zconj(int m, int n, double *C, long ldc, hipStream_t s) {
+  return launch_cconj<double>(m, n, C, ldc, s);
+}
+extern "C" hipError_t atlmx_k_cconj(int m, int n, float *C, long ldc, hipStream_t s) {
+  return launch_cconj<float>(m, n, C, ldc, s);
 }

@@ -171,8 +171,12 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
   const int tn = (tile % per_group) / gsz;
   const int m0 = tm * BM, n0 = tn * BN;
 
+  /* the triangle masks; for the real view of a complex C (KX_CLOWER /
+   * KX_CUPPER) real row r is complex row r >> 1 */
   if (g.tri == KX_LOWER && m0 + BM - 1 < n0) return;   /* tile above diagonal */
   if (g.tri == KX_UPPER && n0 + BN - 1 < m0) return;   /* tile below diagonal */
+  if (g.tri == KX_CLOWER && ((m0 + BM - 1) >> 1) < n0) return;
+  if (g.tri == KX_CUPPER && n0 + BN - 1 < (m0 >> 1)) return;
 
   typedef SliceLoader<T, A_KCONT, BM, BK> LA;
   typedef SliceLoader<T, B_KCONT, BN, BK> LB;
@@ -250,6 +254,8 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
         if (row < g.M && col < g.N) {
           if (g.tri == KX_LOWER && row < col) continue;
           if (g.tri == KX_UPPER && row > col) continue;
+          if (g.tri == KX_CLOWER && (row >> 1) < col) continue;
+          if (g.tri == KX_CUPPER && (row >> 1) > col) continue;
           T *c = g.C + (size_t)col * g.ldc + row;
           T v = g.alpha * acc[i][j][r];
           if (g.beta != T(0)) v += g.beta * *c;

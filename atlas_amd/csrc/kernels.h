@@ -24,10 +24,14 @@ extern "C" {
 #define KX_UNIT    132
 #define KX_LEFT    141
 #define KX_RIGHT   142
+#define KX_CUPPER  1121
+#define KX_CLOWER  1122
 
 /* C = alpha*op(A)*op(B) + beta*C on the MFMA GEMM kernel.
  * tri: 0 = whole C; KX_LOWER / KX_UPPER = only elements on or below / above
- * the diagonal of C are read and written (the SYRK shape).
+ * the diagonal of C are read and written (the SYRK shape); KX_CLOWER /
+ * KX_CUPPER = the same for C the interleaved real view of a complex matrix
+ * (real row r is complex row r / 2: the HERK shape; gemm_kernel only).
  * abort_flag: NULL, or a device int; the kernel does nothing when it is
  * nonzero (lets a stopped Cholesky skip the rest of its queued work). */
 hipError_t atlmx_k_dgemm(int ta, int tb, int M, int N, int K, double alpha,
@@ -404,6 +408,17 @@ hipError_t atlmx_k_ztrsm_llnu(int n1, int nr, const double *L, long lda, double 
                               hipStream_t s);
 hipError_t atlmx_k_ctrsm_llnu(int n1, int nr, const float *L, long lda, float *B, long ldb,
                               hipStream_t s);
+/* Cholesky of one n x n (n <= 64) Hermitian diagonal block in place, lower
+ * (A = L L^H) or upper (A = U^H U); the real part of the diagonal is the
+ * pivot.  A non-positive pivot in column j sets *info = base + j + 1 (when
+ * *info is 0) and stops.  One workgroup. */
+hipError_t atlmx_k_zpotf2(int upper, int n, double *A, long lda, int base, int *info,
+                          hipStream_t s);
+hipError_t atlmx_k_cpotf2(int upper, int n, float *A, long lda, int base, int *info,
+                          hipStream_t s);
+/* C = conj(C) in place, m x n */
+hipError_t atlmx_k_zconj(int m, int n, double *C, long ldc, hipStream_t s);
+hipError_t atlmx_k_cconj(int m, int n, float *C, long ldc, hipStream_t s);
 /* ipiv2[2k + s] = 2 ipiv[k] + s, k in [k1, k2): a complex matrix's row
  * interchanges as those of its interleaved real view */
 hipError_t atlmx_k_cpiv2(int k1, int k2, const int *ipiv, int *ipiv2, hipStream_t s);

@@ -111,6 +111,65 @@ int atlmx_gesv_check(const char *rout, int Order, int N, int NRHS, int lda, int 
   return ierr;
 }
 
+/* clapack_?potrf (interfaces/lapack/C/src/clapack_dpotrf.c) */
+int atlmx_potrf_check(const char *rout, int Order, int Uplo, int N, int lda)
+{
+  int ierr = 0;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
+                 CblasLower, Uplo);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -5;
+    cblas_xerbla(5, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  return ierr;
+}
+
+/* clapack_?potrs and clapack_?posv, whose checks are the same
+ * (clapack_dpotrs.c, clapack_dposv.c:19-58) */
+int atlmx_posv_check(const char *rout, int Order, int Uplo, int N, int NRHS, int lda, int ldb)
+{
+  int ierr = 0;
+  if (Order != CblasRowMajor && Order != CblasColMajor) {
+    ierr = -1;
+    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
+                 CblasColMajor, Order);
+  }
+  if (Uplo != CblasUpper && Uplo != CblasLower) {
+    ierr = -2;
+    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
+                 CblasLower, Uplo);
+  }
+  if (N < 0) {
+    ierr = -3;
+    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
+  }
+  if (NRHS < 0) {
+    ierr = -4;
+    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
+  }
+  if (lda < N || lda < 1) {
+    ierr = -6;
+    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
+  }
+  if (ldb < N || ldb < 1) {
+    ierr = -8;
+    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
+  }
+  return ierr;
+}
+
 #define T double
 #define PFX d
 #define K_PANEL atlmx_k_dgetf2_panel

@@ -517,28 +517,11 @@ int NM3(clapack_, PFX, potrf)(const enum CBLAS_ORDER Order, const enum CBLAS_UPL
                               const int N, T *A, const int lda)
 {
   static const char *rout = "clapack_" STR(PFX) "potrf";
-  int ierr = 0, low, info;
+  const int ierr = atlmx_potrf_check(rout, Order, Uplo, N, lda);
+  int low, info;
   atlmx_ctx *c;
   long dla;
   T *dA;
-  if (Order != CblasRowMajor && Order != CblasColMajor) {
-    ierr = -1;
-    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
-                 CblasColMajor, Order);
-  }
-  if (Uplo != CblasUpper && Uplo != CblasLower) {
-    ierr = -2;
-    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
-                 CblasLower, Uplo);
-  }
-  if (N < 0) {
-    ierr = -3;
-    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
-  }
-  if (lda < N || lda < 1) {
-    ierr = -5;
-    cblas_xerbla(5, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
-  }
   if (ierr) return ierr;
   if (N == 0) return 0;
   low = (Uplo == CblasLower);
@@ -557,36 +540,11 @@ int NM3(clapack_, PFX, potrs)(const enum CBLAS_ORDER Order, const enum CBLAS_UPL
                               const int ldb)
 {
   static const char *rout = "clapack_" STR(PFX) "potrs";
-  int ierr = 0, low;
+  const int ierr = atlmx_posv_check(rout, Order, Uplo, N, NRHS, lda, ldb);
+  int low;
   atlmx_ctx *c;
   long dla, dlb;
   T *dA, *dB;
-  if (Order != CblasRowMajor && Order != CblasColMajor) {
-    ierr = -1;
-    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
-                 CblasColMajor, Order);
-  }
-  if (Uplo != CblasUpper && Uplo != CblasLower) {
-    ierr = -2;
-    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
-                 CblasLower, Uplo);
-  }
-  if (N < 0) {
-    ierr = -3;
-    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
-  }
-  if (NRHS < 0) {
-    ierr = -4;
-    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
-  }
-  if (lda < N || lda < 1) {
-    ierr = -6;
-    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
-  }
-  if (ldb < N || ldb < 1) {
-    ierr = -8;
-    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
-  }
   if (ierr) return ierr;
   if (N == 0 || NRHS == 0) return 0;
   low = (Uplo == CblasLower);
@@ -637,36 +595,11 @@ int NM3(clapack_, PFX, posv)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO
                              const int ldb)
 {
   static const char *rout = "clapack_" STR(PFX) "posv";
-  int ierr = 0, info, low;
+  const int ierr = atlmx_posv_check(rout, Order, Uplo, N, NRHS, lda, ldb);
+  int info, low;
   atlmx_ctx *c;
   long dla, dlb;
   T *dA, *dB;
-  if (Order != CblasRowMajor && Order != CblasColMajor) {
-    ierr = -1;
-    cblas_xerbla(1, rout, "Order must be %d or %d, but is set to %d\n", CblasRowMajor,
-                 CblasColMajor, Order);
-  }
-  if (Uplo != CblasUpper && Uplo != CblasLower) {
-    ierr = -2;
-    cblas_xerbla(2, rout, "Uplo must be %d or %d, but is set to %d\n", CblasUpper,
-                 CblasLower, Uplo);
-  }
-  if (N < 0) {
-    ierr = -3;
-    cblas_xerbla(3, rout, "N cannot be less than zero 0,; is set to %d.\n", N);
-  }
-  if (NRHS < 0) {
-    ierr = -4;
-    cblas_xerbla(4, rout, "NRHS cannot be less than zero 0,; is set to %d.\n", NRHS);
-  }
-  if (lda < N || lda < 1) {
-    ierr = -6;
-    cblas_xerbla(6, rout, "lda must be >= MAX(N,1): lda=%d N=%d\n", lda, N);
-  }
-  if (ldb < N || ldb < 1) {
-    ierr = -8;
-    cblas_xerbla(8, rout, "ldb must be >= MAX(N,1): ldb=%d N=%d\n", ldb, N);
-  }
   if (ierr) return ierr;
   if (N == 0) return 0;
   low = (Uplo == CblasLower);

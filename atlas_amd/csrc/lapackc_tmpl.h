@@ -1,9 +1,12 @@
 /*
- * lapackc_tmpl.h -- precision template of the complex LU family (included
- * by cblas3c.c after cblas3c_tmpl.h, T=double/PFX=z and T=float/PFX=c).
+ * lapackc_tmpl.h -- precision template of the complex LU and Cholesky
+ * families (included by cblas3c.c after cblas3c_tmpl.h, T=double/PFX=z and
+ * T=float/PFX=c).
  *
- *   ?getrf_dev / ?getrs_dev       : stream-ordered drivers on device pointers
- *   clapack_?getrf / ?getrs / ?gesv : the ATLAS C ABI
+ *   ?getrf_dev / ?getrs_dev / ?potrf_dev / ?potrs_dev : stream-ordered
+ *                                       drivers on device pointers
+ *   clapack_?getrf / ?getrs / ?gesv / ?potrf / ?potrs / ?posv : the ATLAS
+ *                                       C ABI
  *
  * The LU is right-looking over panels of CLU_NB columns; each panel is
  * factored by the recursive halving of src/lapack/ATL_getrfC.c:242-361
@@ -63,7 +66,7 @@ static void NM(PFX, getrf_rec)(atlmx_ctx *c, int M, int c0, int w, T *A, long ld
   RT_CHECK(rout, K_CTRSM_LLNU(w1, w - w1, Acc, lda, A12, lda, c->stream));
   if (M - c0 - w1 > 0)
     NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - c0 - w1, w - w1, w1, mone,
-                      Acc + 2 * (size_t)w1, lda, A12, lda, one, A22, lda, rout);
+                      Acc + 2 * (size_t)w1, lda, A12, lda, one, A22, lda, 0, rout);
   NM(PFX, getrf_rec)(c, M, c0 + w1, w - w1, A, lda, d_ipiv, ipiv2, d_info, rout);
   NM(PFX, laswp_dev)(c, w1, A + 2 * (size_t)c0 * lda, lda, c0 + w1, c0 + w, NULL, 1, M,
                      ipiv2, rout);
@@ -103,7 +106,7 @@ int NM(PFX, getrf_dev)(atlmx_ctx *c, int M, int N, T *A, long lda, int *d_ipiv,
       if (k + jb < M)
         NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M - k - jb, N - k - jb, jb, mone,
                           Akk + 2 * (size_t)jb, lda, A12, lda, one, A12 + 2 * (size_t)jb, lda,
-                          rout);
+                          0, rout);
     }
   }
   RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
@@ -237,6 +240,162 @@ int NM3(clapack_, PFX, gesv)(const enum CBLAS_ORDER Order, const int N, const in
   if (dp != ipiv)
     RT_CHECK(rout, hipMemcpyAsync(ipiv, dp, (size_t)N * sizeof(int), hipMemcpyDeviceToHost,
                                   c->stream));
+  rt_sync(c, rout);
+  return info;
+}
+
+/* ------------------------------------------------------------------------
+ * Cholesky, A = L L^H (Lower) or U^H U (Upper), right-looking over blocks of
+ * CCHOL_NB: the diagonal block by one workgroup in LDS (k_cplx.hip
+ * cpotf2), the panel by the complex TRSM (L21 = A21 L11^-H, or U12 =
+ * U11^-H A12), and the trailing Hermitian update as the complex GEMM masked
+ * to the referenced triangle (the real MFMA kernel with the complex-view
+ * triangle mask): the blocked right-looking form of src/lapack/ATL_potrfL.c
+ * / ATL_potrfU.c.  The real part of the diagonal is the pivot; info is the
+ * 1-based column of the first non-positive one, after which the
+ * factorization stops (each block's info is read back before the next).
+ * ---------------------------------------------------------------------- */
+#define CCHOL_NB 64
+int NM(PFX, potrf_dev)(atlmx_ctx *c, int uplo, int N, T *A, long lda, const char *rout)
+{
+  static const T one[2] = {(T)1, (T)0}, mone[2] = {(T)-1, (T)0};
+  const int upper = (uplo == KX_UPPER);
+  int *d_info, info = 0, k;
+  if (N <= 0) return 0;
+  d_info = (int *)rt_ws(c, WS_INFO, sizeof(int), rout);
+  RT_CHECK(rout, atlmx_k_fill_i32(d_info, 1, 0, c->stream));
+  for (k = 0; k < N; k += CCHOL_NB) {
+    const int jb = (N - k < CCHOL_NB) ? N - k : CCHOL_NB, nr = N - k - jb;
+    T *Akk = A + 2 * ((size_t)k * lda + k);
+    RT_CHECK(rout, K_CPOTF2(upper, jb, Akk, lda, k, d_info, c->stream));
+    RT_CHECK(rout, hipMemcpyAsync(&info, d_info, sizeof(int), hipMemcpyDeviceToHost,
+                                  c->stream));
+    rt_sync(c, rout);
+    if (info) return info;
+    if (nr <= 0) break;
+    if (!upper) {
+      T *L21 = Akk + 2 * (size_t)jb;
+      NM(PFX, trsm_dev)(c, KX_RIGHT, KX_LOWER, KX_CONJTRANS, KX_NONUNIT, nr, jb, one, Akk, lda,
+                        L21, lda, rout);
+      NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_CONJTRANS, nr, nr, jb, mone, L21, lda, L21, lda, one,
+                        L21 + 2 * (size_t)jb * lda, lda, KX_LOWER, rout);
+    } else {
+      T *U12 = Akk + 2 * (size_t)jb * lda;
+      NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_CONJTRANS, KX_NONUNIT, jb, nr, one, Akk, lda,
+                        U12, lda, rout);
+      NM(PFX, gemm_dev)(c, KX_CONJTRANS, KX_NOTRANS, nr, nr, jb, mone, U12, lda, U12, lda, one,
+                        U12 + 2 * (size_t)jb, lda, KX_UPPER, rout);
+    }
+  }
+  return 0;
+}
+
+/* A X = B from potrf_dev's factor (src/lapack/ATL_potrs.c, column-major):
+ * L (L^H X) = B, or U^H (U X) = B */
+void NM(PFX, potrs_dev)(atlmx_ctx *c, int uplo, int N, int NRHS, const T *A, long lda, T *B,
+                        long ldb, const char *rout)
+{
+  static const T one[2] = {(T)1, (T)0};
+  if (N <= 0 || NRHS <= 0) return;
+  if (uplo == KX_LOWER) {
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_NOTRANS, KX_NONUNIT, N, NRHS, one, A, lda, B,
+                      ldb, rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_LOWER, KX_CONJTRANS, KX_NONUNIT, N, NRHS, one, A, lda, B,
+                      ldb, rout);
+  } else {
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_CONJTRANS, KX_NONUNIT, N, NRHS, one, A, lda, B,
+                      ldb, rout);
+    NM(PFX, trsm_dev)(c, KX_LEFT, KX_UPPER, KX_NOTRANS, KX_NONUNIT, N, NRHS, one, A, lda, B,
+                      ldb, rout);
+  }
+}
+
+/* interfaces/lapack/C/src/clapack_zpotrf.c; RowMajor holds A^T = conj(A)
+ * column-major, whose Upper factor is the RowMajor Lower one (and the
+ * other way round), as for the real routines */
+int NM3(clapack_, PFX, potrf)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                              const int N, void *A, const int lda)
+{
+  static const char *rout = "clapack_" STR(PFX) "potrf";
+  const int ierr = atlmx_potrf_check(rout, Order, Uplo, N, lda);
+  const size_t esz = 2 * sizeof(T);
+  int low, info;
+  atlmx_ctx *c;
+  long dla;
+  T *dA;
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, esz, 1, &dla, rout);
+  info = NM(PFX, potrf_dev)(c, low ? KX_LOWER : KX_UPPER, N, dA, dla, rout);
+  rt_stage_out(c, A, dA, N, N, lda, dla, esz, rout);
+  rt_sync(c, rout);
+  return info;
+}
+
+/* the solve for clapack_?potrs / ?posv: RowMajor solves conj(A) conj(X) =
+ * conj(B) with the column-major factor, so B is conjugated before and after,
+ * as src/lapack/ATL_potrs.c:46-69 does */
+static void NM(PFX, potrs_order)(atlmx_ctx *c, int rowmajor, int low, int N, int NRHS,
+                                 const T *dA, long dla, T *dB, long dlb, const char *rout)
+{
+  if (rowmajor) RT_CHECK(rout, K_CCONJ(N, NRHS, dB, dlb, c->stream));
+  NM(PFX, potrs_dev)(c, low ? KX_LOWER : KX_UPPER, N, NRHS, dA, dla, dB, dlb, rout);
+  if (rowmajor) RT_CHECK(rout, K_CCONJ(N, NRHS, dB, dlb, c->stream));
+}
+
+/* interfaces/lapack/C/src/clapack_zpotrs.c */
+int NM3(clapack_, PFX, potrs)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                              const int N, const int NRHS, const void *A, const int lda, void *B,
+                              const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "potrs";
+  const int ierr = atlmx_posv_check(rout, Order, Uplo, N, NRHS, lda, ldb);
+  const size_t esz = 2 * sizeof(T);
+  int low;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  if (ierr) return ierr;
+  if (N == 0 || NRHS == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, esz, 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, esz, 1, &dlb, rout);
+  NM(PFX, potrs_order)(c, Order == CblasRowMajor, low, N, NRHS, dA, dla, dB, dlb, rout);
+  rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, esz, rout);
+  rt_sync(c, rout);
+  return 0;
+}
+
+/* interfaces/lapack/C/src/clapack_zposv.c: potrf, then (info = 0) potrs */
+int NM3(clapack_, PFX, posv)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                             const int N, const int NRHS, void *A, const int lda, void *B,
+                             const int ldb)
+{
+  static const char *rout = "clapack_" STR(PFX) "posv";
+  const int ierr = atlmx_posv_check(rout, Order, Uplo, N, NRHS, lda, ldb);
+  const size_t esz = 2 * sizeof(T);
+  int info, low;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  if (ierr) return ierr;
+  if (N == 0) return 0;
+  low = (Uplo == CblasLower);
+  if (Order == CblasRowMajor) low = !low;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, N, N, lda, esz, 1, &dla, rout);
+  info = NM(PFX, potrf_dev)(c, low ? KX_LOWER : KX_UPPER, N, dA, dla, rout);
+  if (!info && NRHS > 0) {
+    dB = (T *)rt_stage_in(c, WS_STAGE_B, B, N, NRHS, ldb, esz, 1, &dlb, rout);
+    NM(PFX, potrs_order)(c, Order == CblasRowMajor, low, N, NRHS, dA, dla, dB, dlb, rout);
+    rt_stage_out(c, B, dB, N, NRHS, ldb, dlb, esz, rout);
+  }
+  rt_stage_out(c, A, dA, N, N, lda, dla, esz, rout);
   rt_sync(c, rout);
   return info;
 }

@@ -102,6 +102,23 @@ int clapack_zgetrf(const enum CBLAS_ORDER Order, const int M, const int N, void 
 int clapack_zgetrs(const enum CBLAS_ORDER Order, const enum CBLAS_TRANSPOSE Trans,
                    const int N, const int NRHS, const void *A, const int lda,
                    const int *ipiv, void *B, const int ldb);
+/* Complex Cholesky family (Hermitian positive definite: A = L L^H or U^H U;
+ * the real part of the diagonal is the pivot).  Replace ATLAS
+ * include/clapack.h:107-115 (c) and :146-154 (z),
+ * interfaces/lapack/C/src/clapack_zpotrf.c, clapack_zpotrs.c,
+ * clapack_zposv.c. */
+int clapack_cposv(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                  const int NRHS, void *A, const int lda, void *B, const int ldb);
+int clapack_cpotrf(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   void *A, const int lda);
+int clapack_cpotrs(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   const int NRHS, const void *A, const int lda, void *B, const int ldb);
+int clapack_zposv(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                  const int NRHS, void *A, const int lda, void *B, const int ldb);
+int clapack_zpotrf(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   void *A, const int lda);
+int clapack_zpotrs(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo, const int N,
+                   const int NRHS, const void *A, const int lda, void *B, const int ldb);
 
 #ifdef __cplusplus
 }

@@ -149,13 +149,16 @@ void orc_track_pivot_gaps(double *gaps, int n)
 #define P orc_z
 #define ORC_ABS(x) fabs(x)
 #define ORC_SAFMIN DBL_MIN
+#define ORC_SQRT(x) sqrt(x)
 #include "orc_cplx.h"
 #undef T
 #undef P
 #undef ORC_ABS
 #undef ORC_SAFMIN
+#undef ORC_SQRT
 #define T float
 #define P orc_c
 #define ORC_ABS(x) fabsf(x)
 #define ORC_SAFMIN FLT_MIN
+#define ORC_SQRT(x) sqrtf(x)
 #include "orc_cplx.h"

@@ -132,7 +132,8 @@ void P##gemm_ref(int TA, int TB, int M, int N, int K, const T *alpha,        \
                   T *C, int ldc);                                             \
 void P##trsm_ref(int side, int uplo, int TA, int diag, int M, int N,         \
                   const T *alpha, const T *A, int lda, T *B, int ldb);        \
-int  P##getf2(int M, int N, T *A, int lda, int *ipiv, T *mingap);
+int  P##getf2(int M, int N, T *A, int lda, int *ipiv, T *mingap);           \
+int  P##potrf(int uplo, int N, T *A, int lda);
 
 ORC_CDECLARE(orc_z, double)
 ORC_CDECLARE(orc_c, float)

@@ -132,6 +132,8 @@ def _declare(L):
         getattr(L, p + "trsm_ref").argtypes = [_I] * 6 + [_P, _P, _I, _P, _I]
         getattr(L, p + "getf2").argtypes = [_I, _I, _P, _I, _P, _P]
         getattr(L, p + "getf2").restype = _I
+        getattr(L, p + "potrf").argtypes = [_I, _I, _P, _I]
+        getattr(L, p + "potrf").restype = _I
 
 
 def _p(a):
@@ -263,6 +265,13 @@ def cgetf2(A):
     gap = np.zeros(1, dtype=rt)
     info = getattr(lib(), p + "getf2")(M, N, _p(A), _ld(A), _p(ipiv), _p(gap))
     return info, ipiv[:min(M, N)], float(gap[0])
+
+
+def cpotrf(A, uplo=LOWER):
+    """Complex Cholesky of the Fortran array A in place (the referenced
+    triangle); returns info."""
+    p, _ = _cpre(A.dtype)
+    return getattr(lib(), p + "potrf")(uplo, A.shape[0], _p(A), _ld(A))
 
 
 def trsm(side, uplo, ta, diag, alpha, A, B, order=COL):

@@ -249,4 +249,50 @@ int F_(getf2)(int M, int N, T *A, int lda, int *ipiv, T *mingap)
    return info;
 }
 
+/* Complex Cholesky, src/lapack/ATL_potrfL.c / ATL_potrfU.c for TCPLX: A =
+ * L L^H (Lower) or U^H U (Upper, handled as the lower factor of A^H, U =
+ * L^H), the real part of the diagonal being the pivot.  Column j: d =
+ * Re A(j,j) - sum_t |L(j,t)|^2; a d that is not positive (or NaN) returns
+ * j + 1 with the columns before j factored; else L(j,j) = sqrt(d) (imaginary
+ * part 0) and L(i,j) = (A(i,j) - sum_t L(i,t) conj(L(j,t))) / L(j,j).  The
+ * value is restated (dot-product order, not ATLAS's recursion); the tests
+ * pin it against numpy's Cholesky. */
+int F_(potrf)(int uplo, int N, T *A, int lda)
+{
+   const int up = (uplo == ORC_UPPER);
+   int i, j, t;
+   /* element (i,j), i >= j, of the lower view */
+#define LV(i, j) (up ? A + 2*((size_t)(i)*lda + (j)) : A + 2*((size_t)(j)*lda + (i)))
+#define LVI(p) (up ? -(p)[1] : (p)[1])
+   for (j = 0; j < N; j++)
+   {
+      T d = LV(j, j)[0];
+      for (t = 0; t < j; t++)
+      {
+         const T *l = LV(j, t);
+         d -= l[0] * l[0] + LVI(l) * LVI(l);
+      }
+      if (!(d > 0)) return j + 1;
+      d = ORC_SQRT(d);
+      LV(j, j)[0] = d;
+      LV(j, j)[1] = 0;
+      for (i = j + 1; i < N; i++)
+      {
+         T *a = LV(i, j), sr = a[0], si = LVI(a);
+         for (t = 0; t < j; t++)
+         {
+            const T *x = LV(i, t), *y = LV(j, t);
+            const T xr = x[0], xi = LVI(x), yr = y[0], yi = -LVI(y);   /* conj(L(j,t)) */
+            sr -= xr * yr - xi * yi;
+            si -= xr * yi + xi * yr;
+         }
+         a[0] = sr / d;
+         a[1] = up ? -(si / d) : si / d;
+      }
+   }
+#undef LV
+#undef LVI
+   return 0;
+}
+
 #undef F_

@@ -324,14 +324,14 @@ def test_zgesv_rowmajor_and_getrs_rowmajor_conjtrans_as_trans():
     A = A0.copy()
     B = B0.copy()
     piv = np.zeros(n, dtype=np.int32)
-    # RowMajor B is n x nrhs row-major: ldb = nrhs ... ATLAS's clapack takes B
-    # as NRHS columns of length N in the column-major sense for both orders
-    Bc = np.asfortranarray(B0)
+    # for both orders ATLAS's clapack reads B as NRHS vectors of length N
+    # stored ldb apart, i.e. the column-major N x NRHS array
+    Bc = B0.copy(order="F")
     assert L.clapack_zgesv(X.RowMajor, n, nrhs, A.ctypes.data, n, piv.ctypes.data,
                            Bc.ctypes.data, n) == 0
     assert np.max(np.abs(A0 @ Bc - B0)) < 1e-9
     for trans, E in [(X.NoTrans, A0), (X.Trans, A0.T), (X.ConjTrans, A0.T)]:
-        Bt = np.asfortranarray(B0.copy())
+        Bt = B0.copy(order="F")
         assert L.clapack_zgetrs(X.RowMajor, trans, n, nrhs, A.ctypes.data, n, piv.ctypes.data,
                                 Bt.ctypes.data, n) == 0
         assert np.max(np.abs(E @ Bt - B0)) < 1e-9, trans
@@ -368,3 +368,102 @@ def test_cgetrf_1500_pivots_vs_oracle():
     assert np.array_equal(ipiv, tpiv)
     assert _lu_resid(A, ipiv, A0) < 10.0
     assert np.max(np.abs(A - T)) <= 1e-8 * np.max(np.abs(T))
+
+
+# ---------------------------------------------------------------- Cholesky
+def _hpd(n, dt, seed):
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return np.asfortranarray((B @ B.conj().T / n + np.eye(n)).astype(dt))
+
+
+def _llt_resid(A0, F, uplo):
+    """llttst's ||A - L L^H|| / (||A|| eps n) from the referenced triangle."""
+    n = A0.shape[0]
+    if uplo == X.Lower:
+        L = np.tril(F).astype(np.complex128)
+    else:
+        L = np.triu(F).astype(np.complex128).conj().T
+    R = A0.astype(np.complex128) - L @ L.conj().T
+    return np.abs(R).sum(0).max() / (np.abs(A0).sum(0).max() * _eps(F.dtype) * n)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+def test_cpotrf_vs_oracle(dt, uplo):
+    p = "z" if dt == np.complex128 else "c"
+    L = X.lib()
+    for n in (1, 5, 64, 65, 130, 300, 700):
+        A0 = _hpd(n, dt, n + uplo)
+        # junk in the other triangle must stay untouched
+        other = np.triu(np.ones((n, n), bool), 1) if uplo == X.Lower else \
+            np.tril(np.ones((n, n), bool), -1)
+        A0[other] = np.asarray(7.0 - 3.0j, dtype=dt)
+        A = A0.copy(order="F")
+        assert getattr(L, "clapack_%spotrf" % p)(X.ColMajor, uplo, n, A.ctypes.data, n) == 0
+        T = A0.copy(order="F")
+        assert O.cpotrf(T, uplo) == 0
+        assert np.array_equal(A[other], A0[other]), n
+        tri = ~other
+        tol = 1e-10 if dt == np.complex128 else 1e-3
+        assert np.max(np.abs(A[tri] - T[tri])) <= tol * np.max(np.abs(T[tri])), n
+        full = np.where(other, np.conj(A0.T), A0)          # the Hermitian matrix
+        assert _llt_resid(full, A, uplo) < 10.0, n
+
+
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+def test_zpotrf_not_positive_definite_info(uplo):
+    L = X.lib()
+    for col in (0, 1, 63, 64, 65, 129):
+        n = 200
+        A = _hpd(n, np.complex128, col)
+        A[col, col] = -1.0
+        T = A.copy(order="F")
+        info = L.clapack_zpotrf(X.ColMajor, uplo, n, A.ctypes.data, n)
+        assert info == O.cpotrf(T, uplo), col
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("uplo", [X.Lower, X.Upper])
+def test_cpotrs_posv_and_rowmajor(dt, uplo):
+    p = "z" if dt == np.complex128 else "c"
+    L = X.lib()
+    for n, nrhs in [(1, 1), (50, 3), (300, 17), (700, 64)]:
+        A0 = _hpd(n, dt, n)
+        rng = np.random.default_rng(n)
+        B0 = np.asfortranarray((rng.uniform(-1, 1, (n, nrhs)) +
+                                1j * rng.uniform(-1, 1, (n, nrhs))).astype(dt))
+        A, B = A0.copy(order="F"), B0.copy(order="F")
+        assert getattr(L, "clapack_%sposv" % p)(X.ColMajor, uplo, n, nrhs, A.ctypes.data, n,
+                                                 B.ctypes.data, n) == 0
+        r = np.abs(A0.astype(np.complex128) @ B - B0).sum(0).max() / (
+            np.abs(A0).sum(0).max() * np.abs(B).sum(0).max() * _eps(dt) * n)
+        assert r < 10.0, (n, r)
+        B2 = B0.copy(order="F")
+        assert getattr(L, "clapack_%spotrs" % p)(X.ColMajor, uplo, n, nrhs, A.ctypes.data, n,
+                                                  B2.ctypes.data, n) == 0
+        assert np.max(np.abs(B2 - B)) <= (1e-12 if dt == np.complex128 else 1e-4) * \
+            max(1.0, np.max(np.abs(B)))
+        # RowMajor: the stored array is A^T (= conj(A) for Hermitian A); B as
+        # NRHS vectors of length n
+        Ar = A0.copy(order="C")      # a copy: A0 stays the matrix for the residual
+        Br = B0.copy(order="F")
+        assert getattr(L, "clapack_%sposv" % p)(X.RowMajor, uplo, n, nrhs, Ar.ctypes.data, n,
+                                                 Br.ctypes.data, n) == 0
+        r = np.abs(A0.astype(np.complex128) @ Br - B0).sum(0).max() / (
+            np.abs(A0).sum(0).max() * np.abs(Br).sum(0).max() * _eps(dt) * n)
+        assert r < 10.0, ("rowmajor", n, r)
+
+
+def test_zposv_4096():
+    n, nrhs = 4096, 8
+    A0 = _hpd(n, np.complex128, 4096)
+    rng = np.random.default_rng(9)
+    B0 = np.asfortranarray(rng.uniform(-1, 1, (n, nrhs)) + 1j * rng.uniform(-1, 1, (n, nrhs)))
+    A, B = A0.copy(order="F"), B0.copy(order="F")
+    assert X.lib().clapack_zposv(X.ColMajor, X.Lower, n, nrhs, A.ctypes.data, n, B.ctypes.data,
+                                 n) == 0
+    r = np.abs(A0 @ B - B0).sum(0).max() / (np.abs(A0).sum(0).max() * np.abs(B).sum(0).max() *
+                                            _eps(np.complex128) * n)
+    assert r < 10.0
+    assert _llt_resid(A0, A, X.Lower) < 10.0

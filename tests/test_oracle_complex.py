@@ -206,3 +206,37 @@ def test_cgetf2_zero_pivot_info_and_safmin_division():
     info, ipiv, _ = O.cgetf2(B)
     assert info == 0 and ipiv[0] == 0
     assert abs(B[1, 0] - 0.5j) < 1e-15
+
+
+def _hpd(n, dt, seed):
+    rng = np.random.default_rng(seed)
+    B = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return np.asfortranarray((B @ B.conj().T / n + np.eye(n)).astype(dt))
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("uplo", [O.LOWER, O.UPPER])
+def test_cpotrf_vs_numpy_cholesky(dt, uplo):
+    tol = 1e-12 if dt == np.complex128 else 2e-5
+    for n in (1, 2, 7, 40, 130):
+        A0 = _hpd(n, dt, n)
+        A = A0.copy(order="F")
+        assert O.cpotrf(A, uplo) == 0
+        L = np.linalg.cholesky(A0.astype(np.complex128))
+        if uplo == O.LOWER:
+            got = np.tril(A)
+            assert np.array_equal(np.triu(A, 1), np.triu(A0, 1))    # other triangle untouched
+        else:
+            got = np.triu(A).conj().T
+            assert np.array_equal(np.tril(A, -1), np.tril(A0, -1))
+        assert np.max(np.abs(got - L)) <= tol * n
+
+
+def test_cpotrf_not_positive_definite_info():
+    A = _hpd(12, np.complex128, 3)
+    A[5, 5] = -1.0                       # leading minor of order 6 fails
+    assert O.cpotrf(A.copy(order="F"), O.LOWER) == 6
+    assert O.cpotrf(A.copy(order="F"), O.UPPER) == 6
+    A = _hpd(4, np.complex128, 4)
+    A[0, 0] = 0.0
+    assert O.cpotrf(A, O.LOWER) == 1
