@@ -27,6 +27,8 @@
 #define K_CTRSM_LLNU atlmx_k_ztrsm_llnu
 #define K_CPOTF2 atlmx_k_zpotf2
 #define K_CCONJ atlmx_k_zconj
+#define K_RTRSV atlmx_k_dtrsv
+#define K_RTRSM_TAG atlmx_k_dtrsm_tag
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"
 #undef T
@@ -41,6 +43,8 @@
 #undef K_CTRSM_LLNU
 #undef K_CPOTF2
 #undef K_CCONJ
+#undef K_RTRSV
+#undef K_RTRSM_TAG
 
 #define T float
 #define PFX c
@@ -54,5 +58,7 @@
 #define K_CTRSM_LLNU atlmx_k_ctrsm_llnu
 #define K_CPOTF2 atlmx_k_cpotf2
 #define K_CCONJ atlmx_k_cconj
+#define K_RTRSV atlmx_k_strsv
+#define K_RTRSM_TAG atlmx_k_strsm_tag
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"

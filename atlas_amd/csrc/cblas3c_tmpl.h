@@ -128,32 +128,55 @@ void NM(PFX, trsm_dev)(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M
   pid = rt_prof_begin(c, PROF_TRSM_INV, 2.0 * na * NB * sizeof(T));
   RT_CHECK(rout, K_CTRI_INV(lower_e, na, H, ldh, inv, c->stream));
   rt_prof_end(c, pid);
-  for (step = 0; step < nblk; ++step) {
-    const int k = lower_e ? step : nblk - 1 - step;
-    const int r0 = k * NB, rb = (na - r0 < NB) ? na - r0 : NB;
-    T *Xk = X + 2 * (size_t)r0;
-    pid = rt_prof_begin2(c, PROF_GEMM, 8.0 * rb * rb * nrhs,
-                         2.0 * ((double)rb * rb + 2.0 * rb * nrhs) * sizeof(T));
-    RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * rb, nrhs, 2 * rb, (T)1,
-                          inv + (size_t)k * 64 * 64, 64, Xk, 2 * ldx, (T)0, Xk, 2 * ldx, 0, NULL,
-                          c->stream));
+  /* Up to a few hundred right-hand sides: the real one-launch chained
+   * solves on the expanded system.  They read the matrix only off its 64x64
+   * diagonal blocks and take those blocks' inverses from inv, so the 2x2
+   * blocks on the expanded diagonal (not triangular in the real sense) are
+   * no obstacle. */
+  if (nrhs == 1 || (nrhs <= ATLMX_TRSV_MAXRHS && nblk < 2)) {
+    const size_t nctl = atlmx_k_trsv_ctl_ints(2 * na, sizeof(T) == sizeof(double));
+    int *ctl = (int *)rt_ws(c, WS_TRSV, nctl * sizeof(int), rout);
+    RT_CHECK(rout, atlmx_k_fill_i32(ctl, (int)nctl, 0, c->stream));
+    pid = rt_prof_begin(c, PROF_TRSV, 2.0 * na * na * sizeof(T));
+    RT_CHECK(rout, K_RTRSV(lower_e ? KX_LOWER : KX_UPPER, KX_NOTRANS, 2 * na, nrhs, H, ldh, inv,
+                           X, 2 * ldx, ctl, c->stream));
     rt_prof_end(c, pid);
-    if (lower_e && r0 + rb < na) {
-      const int R0 = r0 + rb;
-      pid = rt_prof_begin2(c, PROF_GEMM, 8.0 * (na - R0) * rb * nrhs,
-                           2.0 * ((double)(na - R0) * rb + rb * nrhs + 2.0 * (na - R0) * nrhs) *
-                               sizeof(T));
-      RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * (na - R0), nrhs, 2 * rb, (T)-1,
-                            H + 2 * (size_t)R0 + (size_t)(2 * r0) * ldh, ldh, Xk, 2 * ldx, (T)1,
-                            X + 2 * (size_t)R0, 2 * ldx, 0, NULL, c->stream));
+  } else if (nrhs <= ATLMX_TRSM_TAG_MAXRHS && nblk >= 2) {
+    const size_t nctl = atlmx_k_trsm_tag_ctl_ints(2 * na, nrhs, sizeof(T) == sizeof(double));
+    int *ctl = (int *)rt_ws(c, WS_TRSV, nctl * sizeof(int), rout);
+    RT_CHECK(rout, atlmx_k_fill_i32(ctl, (int)nctl, 0, c->stream));
+    pid = rt_prof_begin(c, PROF_TRSV, 2.0 * na * na * sizeof(T));
+    RT_CHECK(rout, K_RTRSM_TAG(lower_e ? KX_LOWER : KX_UPPER, KX_NOTRANS, 2 * na, nrhs, H, ldh,
+                               inv, X, 2 * ldx, ctl, c->stream));
+    rt_prof_end(c, pid);
+  } else {
+    for (step = 0; step < nblk; ++step) {
+      const int k = lower_e ? step : nblk - 1 - step;
+      const int r0 = k * NB, rb = (na - r0 < NB) ? na - r0 : NB;
+      T *Xk = X + 2 * (size_t)r0;
+      pid = rt_prof_begin2(c, PROF_GEMM, 8.0 * rb * rb * nrhs,
+                           2.0 * ((double)rb * rb + 2.0 * rb * nrhs) * sizeof(T));
+      RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * rb, nrhs, 2 * rb, (T)1,
+                            inv + (size_t)k * 64 * 64, 64, Xk, 2 * ldx, (T)0, Xk, 2 * ldx, 0, NULL,
+                            c->stream));
       rt_prof_end(c, pid);
-    } else if (!lower_e && r0 > 0) {
-      pid = rt_prof_begin2(c, PROF_GEMM, 8.0 * r0 * rb * nrhs,
-                           2.0 * ((double)r0 * rb + rb * nrhs + 2.0 * r0 * nrhs) * sizeof(T));
-      RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * r0, nrhs, 2 * rb, (T)-1,
-                            H + (size_t)(2 * r0) * ldh, ldh, Xk, 2 * ldx, (T)1, X, 2 * ldx, 0,
-                            NULL, c->stream));
-      rt_prof_end(c, pid);
+      if (lower_e && r0 + rb < na) {
+        const int R0 = r0 + rb;
+        pid = rt_prof_begin2(c, PROF_GEMM, 8.0 * (na - R0) * rb * nrhs,
+                             2.0 * ((double)(na - R0) * rb + rb * nrhs + 2.0 * (na - R0) * nrhs) *
+                                 sizeof(T));
+        RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * (na - R0), nrhs, 2 * rb, (T)-1,
+                              H + 2 * (size_t)R0 + (size_t)(2 * r0) * ldh, ldh, Xk, 2 * ldx, (T)1,
+                              X + 2 * (size_t)R0, 2 * ldx, 0, NULL, c->stream));
+        rt_prof_end(c, pid);
+      } else if (!lower_e && r0 > 0) {
+        pid = rt_prof_begin2(c, PROF_GEMM, 8.0 * r0 * rb * nrhs,
+                             2.0 * ((double)r0 * rb + rb * nrhs + 2.0 * r0 * nrhs) * sizeof(T));
+        RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * r0, nrhs, 2 * rb, (T)-1,
+                              H + (size_t)(2 * r0) * ldh, ldh, Xk, 2 * ldx, (T)1, X, 2 * ldx, 0,
+                              NULL, c->stream));
+        rt_prof_end(c, pid);
+      }
     }
   }
   if (side == KX_RIGHT)

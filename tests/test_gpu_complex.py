@@ -2,7 +2,8 @@
 through the C ABI against the oracle (whose GEMM is bit-exact with ATLAS's
 ATL_zrefgemm / ATL_crefgemm), with the complex form of gemmtst's elementwise
 bound (bin/gemmtst.c:395): |D - T| <= 4 K eps (|alpha| |op A| |op B| +
-|beta| |C|) element by element, and l3blastst's TRSM residual."""
+|beta| |C|) element by element, and l3blastst's TRSM residual (bound 4 for
+complex data, where one complex multiply rounds up to ~2*sqrt(2) eps)."""
 import ctypes as C
 import itertools
 
@@ -179,14 +180,18 @@ def _trsm_resid(side, uplo, ta, diag, alpha, A, Xs, B0):
 @pytest.mark.parametrize("ta", OPS)
 @pytest.mark.parametrize("diag", [X.NonUnit, X.Unit])
 def test_ctrsm_all_cases_vs_oracle(dt, side, uplo, ta, diag):
-    for M, N in [(1, 1), (37, 5), (100, 70), (257, 33)]:
+    # one RHS over many blocks and 2-4 RHS (tagged TRSV), up to 256 (chained
+    # MFMA solve), more (GEMM loop)
+    for M, N in [(1, 1), (37, 5), (100, 70), (257, 33), (300, 1), (300, 3), (130, 4), (70, 300)]:
         na = M if side == X.Left else N
         A = _tri(na, uplo, dt)
         B0 = _cplx((M, N), dt)
         alpha = 0.75 - 0.25j
         D = X.ctrsm(side, uplo, ta, diag, alpha, A, B0.copy(order="F"))
         T = O.ctrsm(side, uplo, ta, diag, alpha, A, B0.copy(order="F"))
-        assert _trsm_resid(side, uplo, ta, diag, alpha, A, D, B0) < 1.0
+        # l3blastst's bound is 1 for real data; a complex product or quotient
+        # rounds with up to about 2*sqrt(2) eps relative error, so 4 here
+        assert _trsm_resid(side, uplo, ta, diag, alpha, A, D, B0) < 4.0
         tol = 1e-9 if dt == np.complex128 else 2e-3
         assert np.max(np.abs(D - T)) <= tol * max(1.0, np.max(np.abs(T))), (M, N)
 
@@ -215,7 +220,7 @@ def test_ztrsm_alpha_zero_rowmajor_and_large():
     A[np.tril_indices(n, -1)] /= n    # well conditioned
     B0 = _cplx((n, nr), np.complex128)
     D = X.ctrsm(X.Left, X.Lower, X.ConjTrans, X.NonUnit, 1.0, A, B0.copy(order="F"))
-    assert _trsm_resid(X.Left, X.Lower, X.ConjTrans, X.NonUnit, 1.0, A, D, B0) < 1.0
+    assert _trsm_resid(X.Left, X.Lower, X.ConjTrans, X.NonUnit, 1.0, A, D, B0) < 4.0
 
 
 # ---------------------------------------------------------------- LU family
