@@ -327,9 +327,10 @@ def bench_lapack(n, seed, torch, X, log, reps=3):
 
 def bench_complex(n, seed, torch, X, log, reps=3):
     """The complex routines at order n on device operands: zgemm and cgemm
-    (NoTrans x ConjTrans, alpha = 1 - 0.5i, beta = 0.25i; 8 n^3 flops), and
+    (NoTrans x ConjTrans, alpha = 1 - 0.5i, beta = 0.25i; 8 n^3 flops),
     zgetrf (A ~ U[-1,1) + i U[-1,1); 8/3 n^3 flops) with zgetrs on 64
-    right-hand sides.  Best of reps."""
+    right-hand sides, and zpotrf (Lower, A = B B^H / n + I; 4/3 n^3 flops)
+    with zpotrs on 64.  Best of reps."""
     import numpy as np
     L = X.lib()
     g = torch.Generator(device="cuda")
@@ -376,6 +377,22 @@ def bench_complex(n, seed, torch, X, log, reps=3):
                                       ipiv.data_ptr(), Xw.data_ptr(), n),
              lambda: Xw.copy_(rhs))
     out["zgetrs_64"] = {"ms": round(t * 1e3, 2), "tflops": round(8.0 * n * n * 64 / t / 1e12, 3)}
+    # Cholesky of the Hermitian positive definite B B^H / n + I (4/3 n^3 flops)
+    Bm = torch.randn(n, n, dtype=torch.complex128, device="cuda", generator=g)
+    S0 = torch.empty_like(Bm)
+    one = np.array([1.0 / n, 0.0])
+    zero = np.array([0.0, 0.0])
+    L.cblas_zgemm(X.ColMajor, X.NoTrans, X.ConjTrans, n, n, n, one.ctypes.data, Bm.data_ptr(), n,
+                  Bm.data_ptr(), n, zero.ctypes.data, S0.data_ptr(), n)
+    S0 += torch.eye(n, dtype=torch.complex128, device="cuda")
+    del Bm
+    t = best(lambda: L.clapack_zpotrf(X.ColMajor, X.Lower, n, Aw.data_ptr(), n),
+             lambda: Aw.copy_(S0))
+    out["zpotrf"] = {"ms": round(t * 1e3, 2), "tflops": round(4.0 * n ** 3 / 3 / t / 1e12, 3)}
+    t = best(lambda: L.clapack_zpotrs(X.ColMajor, X.Lower, n, 64, Aw.data_ptr(), n,
+                                      Xw.data_ptr(), n),
+             lambda: Xw.copy_(rhs))
+    out["zpotrs_64"] = {"ms": round(t * 1e3, 2), "tflops": round(8.0 * n * n * 64 / t / 1e12, 3)}
     log("complex n=%d: %s" % (n, json.dumps(out)))
     return out
 
