@@ -62,3 +62,22 @@ static int imax1(int a) { return a > 1 ? a : 1; }
 #define P s
 #define PU "S"
 #include "f77_tmpl.h"
+#undef T
+#undef P
+#undef PU
+
+/* complex: zgemm_, ztrsm_, zgetrf_ ... (interfaces/blas/F77/src/zgemm.f,
+ * ztrsm.f, interfaces/lapack/F77/src/zgetrf.f, ... : the same checks) */
+#define F77_COMPLEX
+#define T double
+#define P z
+#define PU "Z"
+#include "f77_tmpl.h"
+#undef T
+#undef P
+#undef PU
+
+#define T float
+#define P c
+#define PU "C"
+#include "f77_tmpl.h"

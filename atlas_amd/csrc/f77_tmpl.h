@@ -1,6 +1,9 @@
 /*
  * f77_tmpl.h -- precision template of the Fortran 77 entry points
- * (included by f77.c with T=double/P=d and T=float/P=s).  Check order and
+ * (included by f77.c with T=double/P=d and T=float/P=s, and with
+ * F77_COMPLEX defined for T=double/P=z and T=float/P=c: then the GEMM and
+ * TRSM take alpha/beta as {real, imaginary} pairs and only the routines the
+ * complex C ABI has are built).  Check order and
  * XERBLA positions follow interfaces/blas/F77/src/dgemm.f, dtrsm.f,
  * dtrmm.f, dsyrk.f, dsyr2k.f, dsymm.f, idamax.f and
  * interfaces/lapack/F77/src/dgetrf.f, dgetrs.f, dgesv.f, dpotrf.f,
@@ -29,7 +32,11 @@ void F77(gemm)(const char *ta, const char *tb, const int *m, const int *n, const
   else if (*ldb < imax1(nrb)) info = 10;
   else if (*ldc < imax1(*m)) info = 13;
   if (info) { f77_err(PU "GEMM ", info); return; }
+#ifdef F77_COMPLEX
+  CB(gemm)(CblasColMajor, ita, itb, *m, *n, *k, alpha, a, *lda, b, *ldb, beta, c, *ldc);
+#else
   CB(gemm)(CblasColMajor, ita, itb, *m, *n, *k, *alpha, a, *lda, b, *ldb, *beta, c, *ldc);
+#endif
 }
 
 /* TRSM and TRMM share their checks (dtrsm.f / dtrmm.f) */
@@ -59,9 +66,16 @@ void F77(trsm)(const char *side, const char *uplo, const char *ta, const char *d
   if (info) { f77_err(PU "TRSM ", info); return; }
   CB(trsm)(CblasColMajor, lsame(side, 'L') ? CblasLeft : CblasRight,
            lsame(uplo, 'U') ? CblasUpper : CblasLower, tr_code(ta),
-           lsame(diag, 'U') ? CblasUnit : CblasNonUnit, *m, *n, *alpha, a, *lda, b, *ldb);
+           lsame(diag, 'U') ? CblasUnit : CblasNonUnit, *m, *n,
+#ifdef F77_COMPLEX
+           alpha,
+#else
+           *alpha,
+#endif
+           a, *lda, b, *ldb);
 }
 
+#ifndef F77_COMPLEX
 void F77(trmm)(const char *side, const char *uplo, const char *ta, const char *diag,
                const int *m, const int *n, const T *alpha, const T *a, const int *lda, T *b,
                const int *ldb, size_t l1, size_t l2, size_t l3, size_t l4)
@@ -142,6 +156,8 @@ int FN(i, P, amax_)(const int *n, const T *x, const int *incx)
 }
 
 /* ---- LAPACK ---- */
+#endif /* !F77_COMPLEX: TRMM, SYRK, SYR2K, SYMM, I?AMAX */
+
 void F77(getrf)(const int *m, const int *n, T *a, const int *lda, int *ipiv, int *info)
 {
   int i, mn;
@@ -244,6 +260,7 @@ void F77(posv)(const char *uplo, const int *n, const int *nrhs, T *a, const int 
   *info = CL(posv)(CblasColMajor, u, *n, *nrhs, a, *lda, b, *ldb);
 }
 
+#ifndef F77_COMPLEX
 /* dgetri.f: LWORK = -1 asks for the workspace size (WORK(1) = N*64); the
  * GPU routine keeps its own workspace, so WORK is otherwise unused */
 void F77(getri)(const int *n, T *a, const int *lda, const int *ipiv, T *work, const int *lwork,
@@ -307,3 +324,4 @@ void F77(trtri)(const char *uplo, const char *diag, const int *n, T *a, const in
 #undef F77
 #undef CB
 #undef CL
+#endif /* !F77_COMPLEX: GETRI, POTRI, LAUUM, TRTRI */

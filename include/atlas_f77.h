@@ -104,6 +104,44 @@ void slauum_(const char *uplo, const int *n, float *a, const int *lda, int *info
 void strtri_(const char *uplo, const char *diag, const int *n, float *a, const int *lda,
              int *info, size_t l1, size_t l2);
 
+/* Complex (z: COMPLEX*16, c: COMPLEX; arrays and alpha/beta as interleaved
+ * {real, imaginary} pairs): ZGEMM, ZTRSM (interfaces/blas/F77/src/zgemm.f,
+ * ztrsm.f), ZGETRF, ZGETRS, ZGESV, ZPOTRF, ZPOTRS, ZPOSV
+ * (interfaces/lapack/F77/src/zgetrf.f ... zposv.f), and the c twins;
+ * TRANSA/TRANSB/TRANS may be 'C'. */
+void zgemm_(const char *ta, const char *tb, const int *m, const int *n, const int *k,
+            const double *alpha, const double *a, const int *lda, const double *b, const int *ldb,
+            const double *beta, double *c, const int *ldc, size_t lta, size_t ltb);
+void ztrsm_(const char *side, const char *uplo, const char *ta, const char *diag,
+            const int *m, const int *n, const double *alpha, const double *a, const int *lda,
+            double *b, const int *ldb, size_t l1, size_t l2, size_t l3, size_t l4);
+void zgetrf_(const int *m, const int *n, double *a, const int *lda, int *ipiv, int *info);
+void zgetrs_(const char *tr, const int *n, const int *nrhs, const double *a, const int *lda,
+             const int *ipiv, double *b, const int *ldb, int *info, size_t l1);
+void zgesv_(const int *n, const int *nrhs, double *a, const int *lda, int *ipiv, double *b,
+            const int *ldb, int *info);
+void zpotrf_(const char *uplo, const int *n, double *a, const int *lda, int *info, size_t l1);
+void zpotrs_(const char *uplo, const int *n, const int *nrhs, const double *a, const int *lda,
+             double *b, const int *ldb, int *info, size_t l1);
+void zposv_(const char *uplo, const int *n, const int *nrhs, double *a, const int *lda,
+            double *b, const int *ldb, int *info, size_t l1);
+void cgemm_(const char *ta, const char *tb, const int *m, const int *n, const int *k,
+            const float *alpha, const float *a, const int *lda, const float *b, const int *ldb,
+            const float *beta, float *c, const int *ldc, size_t lta, size_t ltb);
+void ctrsm_(const char *side, const char *uplo, const char *ta, const char *diag,
+            const int *m, const int *n, const float *alpha, const float *a, const int *lda,
+            float *b, const int *ldb, size_t l1, size_t l2, size_t l3, size_t l4);
+void cgetrf_(const int *m, const int *n, float *a, const int *lda, int *ipiv, int *info);
+void cgetrs_(const char *tr, const int *n, const int *nrhs, const float *a, const int *lda,
+             const int *ipiv, float *b, const int *ldb, int *info, size_t l1);
+void cgesv_(const int *n, const int *nrhs, float *a, const int *lda, int *ipiv, float *b,
+            const int *ldb, int *info);
+void cpotrf_(const char *uplo, const int *n, float *a, const int *lda, int *info, size_t l1);
+void cpotrs_(const char *uplo, const int *n, const int *nrhs, const float *a, const int *lda,
+             float *b, const int *ldb, int *info, size_t l1);
+void cposv_(const char *uplo, const int *n, const int *nrhs, float *a, const int *lda,
+            float *b, const int *ldb, int *info, size_t l1);
+
 #ifdef __cplusplus
 }
 #endif

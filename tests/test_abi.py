@@ -109,6 +109,14 @@ def test_f77_xerbla_messages_without_gpu():
          "SPOTRF parameter number  1"),
         ("L.dgetri_(i(5), None, i(5), None, None, i(2), C.byref(C.c_int(0)))",
          "DGETRI parameter number  6"),
+        ("L.zgemm_(b'N', b'C', i(3), i(2), i(2), None, None, i(2), None, i(2), None, None,"
+         " i(3), 1, 1)", "ZGEMM  parameter number  8"),
+        ("L.ctrsm_(b'L', b'U', b'Q', b'N', i(5), i(2), None, None, i(5), None, i(5), 1, 1, 1, 1)",
+         "CTRSM  parameter number  3"),
+        ("L.zgesv_(i(4), i(-1), None, i(4), None, None, i(4), C.byref(C.c_int(0)))",
+         "ZGESV  parameter number  2"),
+        ("L.cposv_(b'U', i(6), i(1), None, i(6), None, i(5), C.byref(C.c_int(0)), 1)",
+         "CPOSV  parameter number  7"),
     ]
     for call, msg in cases:
         code = ("import ctypes as C, sys; sys.path.insert(0, %r); import atlas_amd as a;"

@@ -148,3 +148,73 @@ def test_lapack_f77_pivots_solves_inverses(dt):
         fn("trtri")(_c(u), _c("N"), _i(n), _p(Ti), _i(n), C.byref(info), C.c_size_t(1),
                     C.c_size_t(1))
         assert info.value == 0 and O.trtri_resid(uplo, X.NonUnit, Ti, Tm) < 100
+
+
+def _z(v, dt):
+    """a complex scalar by reference, as {real, imaginary}"""
+    return np.array([complex(v).real, complex(v).imag],
+                    dtype=np.float64 if dt == np.complex128 else np.float32)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_complex_f77_gemm_trsm_lu_cholesky(dt):
+    """zgemm_/ztrsm_ (and c) against the oracle; zgetrf_ pivots = clapack's
+    plus one; zgetrs_ ('C'), zgesv_, zpotrf_/zpotrs_/zposv_ under slvtst."""
+    p = "z" if dt == np.complex128 else "c"
+    L = X.lib()
+    fn = lambda name: getattr(L, p + name + "_")  # noqa: E731
+    tol = 1e-10 if dt == np.complex128 else 2e-3
+    cz = lambda shape: _f((RNG.uniform(-0.5, 0.5, shape) +  # noqa: E731
+                           1j * RNG.uniform(-0.5, 0.5, shape)).astype(dt))
+    M, N, K = 70, 50, 40
+    A, B, C0 = cz((K, M)), cz((K, N)), cz((M, N))
+    D = C0.copy(order="F")
+    al, be = _z(1 - 0.5j, dt), _z(0.25j, dt)
+    fn("gemm")(_c("C"), _c("N"), _i(M), _i(N), _i(K), _p(al), _p(A), _i(K), _p(B), _i(K),
+               _p(be), _p(D), _i(M), C.c_size_t(1), C.c_size_t(1))
+    T = O.cgemm(O.CONJTRANS, O.NOTRANS, 1 - 0.5j, A, B, 0.25j, C0.copy(order="F"))
+    assert np.max(np.abs(D - T)) <= tol
+    Tr = cz((M, M))
+    Tr[np.diag_indices(M)] += 2
+    Bs = cz((M, 5))
+    D = Bs.copy(order="F")
+    fn("trsm")(_c("L"), _c("L"), _c("C"), _c("N"), _i(M), _i(5), _p(_z(1.0, dt)), _p(Tr), _i(M),
+               _p(D), _i(M), C.c_size_t(1), C.c_size_t(1), C.c_size_t(1), C.c_size_t(1))
+    T = O.ctrsm(O.LEFT, O.LOWER, O.CONJTRANS, O.NONUNIT, 1.0, Tr, Bs.copy(order="F"))
+    assert np.max(np.abs(D - T)) <= tol * max(1.0, np.max(np.abs(T)))
+    n, nrhs = 120, 3
+    A0 = cz((n, n))
+    A0[np.diag_indices(n)] += 1
+    B0 = cz((n, nrhs))
+    A1 = A0.copy(order="F")
+    ip = np.zeros(n, dtype=np.int32)
+    info = C.c_int(-7)
+    fn("getrf")(_i(n), _i(n), _p(A1), _i(n), _p(ip), C.byref(info))
+    A2 = A0.copy(order="F")
+    ip0 = np.zeros(n, dtype=np.int32)
+    assert getattr(L, "clapack_%sgetrf" % p)(X.ColMajor, n, n, A2.ctypes.data, n,
+                                              ip0.ctypes.data) == 0
+    assert info.value == 0 and np.array_equal(ip, ip0 + 1)
+    Bx = B0.copy(order="F")
+    fn("getrs")(_c("C"), _i(n), _i(nrhs), _p(A1), _i(n), _p(ip), _p(Bx), _i(n), C.byref(info),
+                C.c_size_t(1))
+    assert info.value == 0
+    assert np.max(np.abs(A0.conj().T.astype(np.complex128) @ Bx - B0)) <= tol * n
+    A3, Bx = A0.copy(order="F"), B0.copy(order="F")
+    fn("gesv")(_i(n), _i(nrhs), _p(A3), _i(n), _p(ip), _p(Bx), _i(n), C.byref(info))
+    assert info.value == 0 and np.array_equal(ip, ip0 + 1)
+    assert np.max(np.abs(A0.astype(np.complex128) @ Bx - B0)) <= tol * n
+    G = cz((n, n)).astype(np.complex128)
+    H0 = _f((G @ G.conj().T / n + np.eye(n)).astype(dt))
+    for uplo in ("L", "U"):
+        H, Bx = H0.copy(order="F"), B0.copy(order="F")
+        fn("posv")(_c(uplo), _i(n), _i(nrhs), _p(H), _i(n), _p(Bx), _i(n), C.byref(info),
+                   C.c_size_t(1))
+        assert info.value == 0
+        assert np.max(np.abs(H0.astype(np.complex128) @ Bx - B0)) <= tol * n
+        H2, By = H0.copy(order="F"), B0.copy(order="F")
+        fn("potrf")(_c(uplo), _i(n), _p(H2), _i(n), C.byref(info), C.c_size_t(1))
+        assert info.value == 0
+        fn("potrs")(_c(uplo), _i(n), _i(nrhs), _p(H2), _i(n), _p(By), _i(n), C.byref(info),
+                    C.c_size_t(1))
+        assert np.max(np.abs(By - Bx)) <= tol * max(1.0, np.max(np.abs(Bx)))
