@@ -142,6 +142,26 @@ def _declare(L):
             f = getattr(L, "clapack_%s%s" % (p, n))
             f.argtypes = [_I, _I, _I, _I, _P, _I, _P, _I]
             f.restype = _I
+        rt = _D if p == "z" else _F
+        f = getattr(L, "cblas_%sherk" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, rt, _P, _I, rt, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%sher2k" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _P, _P, _I, _P, _I, rt, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%ssyrk" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _P, _P, _I, _P, _P, _I]
+        f.restype = None
+        f = getattr(L, "cblas_%ssyr2k" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I]
+        f.restype = None
+        for n in ("hemm", "symm"):
+            f = getattr(L, "cblas_%s%s" % (p, n))
+            f.argtypes = [_I, _I, _I, _I, _I, _P, _P, _I, _P, _I, _P, _P, _I]
+            f.restype = None
+        f = getattr(L, "cblas_%strmm" % p)
+        f.argtypes = [_I, _I, _I, _I, _I, _I, _I, _P, _P, _I, _P, _I]
+        f.restype = None
     L.atlmx_init.restype = _I
     L.atlmx_set_stream.argtypes = [_P]
     L.atlmx_get_stream.restype = _P

@@ -123,6 +123,41 @@ int atlmx_trxm_check(int Order, int Side, int Uplo, int TA, int Diag, int M, int
   return info;
 }
 
+/* Argument checks shared by cblas_?symm and the complex cblas_?symm /
+ * cblas_?hemm, whose checks are the same (interfaces/blas/C/src/
+ * cblas_dsymm.c, cblas_zsymm.c, cblas_zhemm.c:25-72).  2000 = valid. */
+int atlmx_symm_check(int Order, int Side, int Uplo, int M, int N, int lda, int ldb, int ldc)
+{
+  int info = 2000;
+  if (Order == CblasColMajor || Order == CblasRowMajor) {
+    const int rows = (Order == CblasColMajor) ? M : N;
+    const char *rn = (Order == CblasColMajor) ? "M" : "N";
+    if (Side == CblasLeft) {
+      if (lda < M || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
+    } else if (Side == CblasRight) {
+      if (lda < N || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
+    } else
+      info = cblas_errprn(2, info, "SIDE must be %d or %d, but is set to %d", CblasRight,
+                          CblasLeft, Side);
+    if (ldb < rows || ldb < 1)
+      info = cblas_errprn(10, info, "ldb must be >= MAX(%s,1): ldb=%d %s=%d", rn, ldb, rn,
+                          rows);
+    if (ldc < rows || ldc < 1)
+      info = cblas_errprn(13, info, "ldc must be >= MAX(%s,1): ldc=%d %s=%d", rn, ldc, rn,
+                          rows);
+  } else
+    info = cblas_errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
+                        CblasColMajor, Order);
+  if (Uplo != CblasUpper && Uplo != CblasLower)
+    info = cblas_errprn(3, info, "UPLO must be %d or %d, but is set to %d", CblasUpper,
+                        CblasLower, Uplo);
+  if (M < 0) info = cblas_errprn(4, info, "M cannot be less than zero; it is set to %d.", M);
+  if (N < 0) info = cblas_errprn(5, info, "N cannot be less than zero; it is set to %d.", N);
+  return info;
+}
+
 #define T double
 #define PFX d
 #define K_GEMM atlmx_k_dgemm

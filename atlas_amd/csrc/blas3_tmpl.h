@@ -461,36 +461,11 @@ void NM3(cblas_, PFX, symm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE 
                             const int ldb, const T beta, T *C, const int ldc)
 {
   static const char *rout = "cblas_" STR(PFX) "symm";
-  int info = 2000, side, uplo, m, n, na;
+  const int info = atlmx_symm_check(Order, Side, Uplo, M, N, lda, ldb, ldc);
+  int side, uplo, m, n, na;
   atlmx_ctx *c;
   long dla, dlb, dlc;
   T *dA, *dB, *dC;
-  if (Order == CblasColMajor || Order == CblasRowMajor) {
-    const int rows = (Order == CblasColMajor) ? M : N;
-    const char *rn = (Order == CblasColMajor) ? "M" : "N";
-    if (Side == CblasLeft) {
-      if (lda < M || lda < 1)
-        info = cblas_errprn(8, info, "lda must be >= MAX(M,1): lda=%d M=%d", lda, M);
-    } else if (Side == CblasRight) {
-      if (lda < N || lda < 1)
-        info = cblas_errprn(8, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
-    } else
-      info = cblas_errprn(2, info, "SIDE must be %d or %d, but is set to %d", CblasRight,
-                          CblasLeft, Side);
-    if (ldb < rows || ldb < 1)
-      info = cblas_errprn(10, info, "ldb must be >= MAX(%s,1): ldb=%d %s=%d", rn, ldb, rn,
-                          rows);
-    if (ldc < rows || ldc < 1)
-      info = cblas_errprn(13, info, "ldc must be >= MAX(%s,1): ldc=%d %s=%d", rn, ldc, rn,
-                          rows);
-  } else
-    info = cblas_errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
-                        CblasColMajor, Order);
-  if (Uplo != CblasUpper && Uplo != CblasLower)
-    info = cblas_errprn(3, info, "UPLO must be %d or %d, but is set to %d", CblasUpper,
-                        CblasLower, Uplo);
-  if (M < 0) info = cblas_errprn(4, info, "M cannot be less than zero; it is set to %d.", M);
-  if (N < 0) info = cblas_errprn(5, info, "N cannot be less than zero; it is set to %d.", N);
   if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
   if (M == 0 || N == 0 || (alpha == (T)0 && beta == (T)1)) return;
   if (Order == CblasColMajor) {

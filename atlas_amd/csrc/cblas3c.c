@@ -1,6 +1,7 @@
 /*
  * cblas3c.c -- complex Level-3 BLAS and LU (C host code) of libatlas_mi355x:
- * cblas_{c,z}{gemm,trsm}, clapack_{c,z}{getrf,getrs,gesv,potrf,potrs,posv} and their
+ * cblas_{c,z}{gemm,trsm,trmm,herk,syrk,her2k,syr2k,hemm,symm},
+ * clapack_{c,z}{getrf,getrs,gesv,potrf,potrs,posv} and their
  * device-pointer twins, instantiated from cblas3c_tmpl.h and lapackc_tmpl.h
  * per precision over the real MFMA GEMM kernels.
  */
@@ -14,6 +15,49 @@
 
 #define STR_(x) #x
 #define STR(x) STR_(x)
+
+/* Argument checks of cblas_?herk / ?her2k (herm: Trans N or C) and
+ * cblas_?syrk / ?syr2k (Trans N or T) for c and z, in ATLAS's order,
+ * positions and messages (interfaces/blas/C/src/cblas_zherk.c:20-75,
+ * cblas_zsyrk.c, cblas_zher2k.c, cblas_zsyr2k.c; two_k adds ldb at
+ * position 10 and moves ldc to 13).  2000 = valid. */
+static int crk_check(int Order, int Uplo, int Trans, int N, int K, int lda, int ldb, int ldc,
+                     int herm, int two_k)
+{
+  int info = 2000;
+  if (Uplo != CblasUpper && Uplo != CblasLower)
+    info = cblas_errprn(2, info, "UPLO must be %d or %d, but is set to %d", CblasUpper,
+                        CblasLower, Uplo);
+  if (N < 0) info = cblas_errprn(4, info, "N cannot be less than zero; it is set to %d.", N);
+  if (K < 0) info = cblas_errprn(5, info, "K cannot be less than zero; it is set to %d.", K);
+  if (Order == CblasColMajor || Order == CblasRowMajor) {
+    const int nt = (Order == CblasColMajor) == (Trans == CblasNoTrans);
+    if (Trans != CblasNoTrans && Trans != (herm ? CblasConjTrans : CblasTrans)) {
+      if (herm)
+        info = cblas_errprn(3, info, "Trans must be %d or %d, but is set to %d", CblasNoTrans,
+                            CblasConjTrans, Trans);
+      else
+        info = cblas_errprn(3, info, "Trans must be %d, %d or %d, but is set to %d",
+                            CblasNoTrans, CblasTrans, CblasConjTrans, Trans);
+    }
+    if (nt) {
+      if (lda < N || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(N,1): lda=%d N=%d", lda, N);
+      if (two_k && (ldb < N || ldb < 1))
+        info = cblas_errprn(10, info, "ldb must be >= MAX(N,1): ldb=%d N=%d", ldb, N);
+    } else {
+      if (lda < K || lda < 1)
+        info = cblas_errprn(8, info, "lda must be >= MAX(K,1): lda=%d K=%d", lda, K);
+      if (two_k && (ldb < K || ldb < 1))
+        info = cblas_errprn(10, info, "ldb must be >= MAX(K,1): ldb=%d K=%d", ldb, K);
+    }
+  } else
+    info = cblas_errprn(1, info, "Order must be %d or %d, but is set to %d", CblasRowMajor,
+                        CblasColMajor, Order);
+  if (ldc < N || ldc < 1)
+    info = cblas_errprn(two_k ? 13 : 11, info, "ldc must be >= MAX(N,1): ldc=%d N=%d", ldc, N);
+  return info;
+}
 
 #define T double
 #define PFX z
@@ -29,6 +73,8 @@
 #define K_CCONJ atlmx_k_zconj
 #define K_RTRSV atlmx_k_dtrsv
 #define K_RTRSM_TAG atlmx_k_dtrsm_tag
+#define K_CTRI_SCALE atlmx_k_ztri_scale
+#define K_CSYM_EXPAND atlmx_k_zsym_expand
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"
 #undef T
@@ -45,6 +91,8 @@
 #undef K_CCONJ
 #undef K_RTRSV
 #undef K_RTRSM_TAG
+#undef K_CTRI_SCALE
+#undef K_CSYM_EXPAND
 
 #define T float
 #define PFX c
@@ -60,5 +108,7 @@
 #define K_CCONJ atlmx_k_cconj
 #define K_RTRSV atlmx_k_strsv
 #define K_RTRSM_TAG atlmx_k_strsm_tag
+#define K_CTRI_SCALE atlmx_k_ctri_scale
+#define K_CSYM_EXPAND atlmx_k_csym_expand
 #include "cblas3c_tmpl.h"
 #include "lapackc_tmpl.h"

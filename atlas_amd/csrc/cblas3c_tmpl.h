@@ -287,3 +287,266 @@ void NM3(cblas_, PFX, trsm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE 
   rt_stage_out(c, B, dB, m, n, ldb, dlb, esz, rout);
   rt_sync(c, rout);
 }
+
+/* ------------------------------------------------------------------------
+ * The rest of the complex Level-3 BLAS on the complex GEMM:
+ *   rank-k / rank-2k (HERK, SYRK, HER2K, SYR2K): one or two GEMMs masked to
+ *     the referenced triangle (the complex-view mask of the real kernel)
+ *     after beta has scaled that triangle, as src/blas/reference/level3/
+ *     ATL_zrefherk.c / _zrefsyrk.c / _zrefher2k.c / _zrefsyr2k.c define
+ *     them; the Hermitian forms keep the diagonal real;
+ *   HEMM / SYMM: A's triangle expanded to the full matrix, then one GEMM;
+ *   TRMM: op(A)'s triangle expanded (zeros outside, ones on a unit
+ *     diagonal) and B copied aside, then one GEMM into B.
+ * ---------------------------------------------------------------------- */
+/* herm: Hermitian forms (beta's imaginary part is ignored, conj(alpha) on
+ * the second product of the 2k form); B NULL: rank-k.  trans: KX_NOTRANS,
+ * or KX_CONJTRANS (herm) / KX_TRANS (symmetric). */
+static void NM(PFX, rk_dev)(atlmx_ctx *c, int herm, int uplo, int trans, int N, int K,
+                            const T *alpha, const T *A, long lda, const T *B, long ldb,
+                            const T *beta, T *C, long ldc, const char *rout)
+{
+  static const T one[2] = {(T)1, (T)0};
+  const int lower = (uplo == KX_LOWER);
+  const int op = herm ? KX_CONJTRANS : KX_TRANS;
+  const T br = beta[0], bi = herm ? (T)0 : beta[1];
+  const int a_zero = (alpha[0] == (T)0 && alpha[1] == (T)0);
+  if (N <= 0) return;
+  if (!(br == (T)1 && bi == (T)0) || herm)
+    RT_CHECK(rout, K_CTRI_SCALE(N, lower, br, bi, herm, C, ldc, c->stream));
+  if (a_zero || K == 0) return;
+  {
+    const T a2[2] = {alpha[0], herm ? -alpha[1] : alpha[1]};
+    const T *X = B ? B : A;
+    const long ldx = B ? ldb : lda;
+    if (trans == KX_NOTRANS) {
+      NM(PFX, gemm_dev)(c, KX_NOTRANS, op, N, N, K, alpha, A, lda, X, ldx, one, C, ldc, uplo,
+                        rout);
+      if (B)
+        NM(PFX, gemm_dev)(c, KX_NOTRANS, op, N, N, K, a2, B, ldb, A, lda, one, C, ldc, uplo,
+                          rout);
+    } else {
+      NM(PFX, gemm_dev)(c, op, KX_NOTRANS, N, N, K, alpha, A, lda, X, ldx, one, C, ldc, uplo,
+                        rout);
+      if (B)
+        NM(PFX, gemm_dev)(c, op, KX_NOTRANS, N, N, K, a2, B, ldb, A, lda, one, C, ldc, uplo,
+                          rout);
+    }
+  }
+  if (herm) RT_CHECK(rout, K_CTRI_SCALE(N, lower, (T)1, (T)0, 1, C, ldc, c->stream));
+}
+
+/* C = alpha A B + beta C (Left) or alpha B A + beta C (Right), A Hermitian
+ * (herm) or symmetric in its uplo triangle */
+static void NM(PFX, hemm_dev)(atlmx_ctx *c, int herm, int side, int uplo, int M, int N,
+                              const T *alpha, const T *A, long lda, const T *B, long ldb,
+                              const T *beta, T *C, long ldc, const char *rout)
+{
+  const int na = (side == KX_LEFT) ? M : N;
+  T *Ad;
+  if (M <= 0 || N <= 0) return;
+  Ad = (T *)rt_ws(c, WS_CPLX_X, (size_t)na * na * 2 * sizeof(T), rout);
+  RT_CHECK(rout, K_CSYM_EXPAND(na, uplo == KX_LOWER, herm, A, lda, Ad, na, c->stream));
+  if (side == KX_LEFT)
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, M, alpha, Ad, na, B, ldb, beta, C, ldc,
+                      0, rout);
+  else
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, N, alpha, B, ldb, Ad, na, beta, C, ldc,
+                      0, rout);
+}
+
+/* B = alpha op(A) B (Left) or alpha B op(A) (Right), A triangular */
+static void NM(PFX, trmm_dev)(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
+                              const T *alpha, const T *A, long lda, T *B, long ldb,
+                              const char *rout)
+{
+  static const T zero[2] = {(T)0, (T)0};
+  const int na = (side == KX_LEFT) ? M : N;
+  const int tr = (ta != KX_NOTRANS);
+  const int lower_op = ((uplo == KX_LOWER) != tr);
+  T *Td, *Bc;
+  if (M <= 0 || N <= 0) return;
+  Td = (T *)rt_ws(c, WS_CPLX_X, (size_t)na * na * 2 * sizeof(T), rout);
+  Bc = (T *)rt_ws(c, WS_CPLX_Y, (size_t)M * N * 2 * sizeof(T), rout);
+  RT_CHECK(rout, K_CGATHER(na, na, A, lda, tr, ta == KX_CONJTRANS, (T)1, (T)0, 0, Td, na,
+                           lower_op ? KX_LOWER : KX_UPPER, diag == KX_UNIT, c->stream));
+  RT_CHECK(rout, K_CGATHER(M, N, B, ldb, 0, 0, (T)1, (T)0, 0, Bc, M, 0, 0, c->stream));
+  if (side == KX_LEFT)
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, M, alpha, Td, na, Bc, M, zero, B, ldb, 0,
+                      rout);
+  else
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, N, alpha, Bc, M, Td, na, zero, B, ldb, 0,
+                      rout);
+}
+
+/* the rank-k / rank-2k C ABI: RowMajor holds C^T, so Uplo and Trans flip
+ * (interfaces/blas/C/src/cblas_zherk.c, _zher2k.c with conj(alpha),
+ * _zsyrk.c, _zsyr2k.c) */
+static void NM(PFX, rk_abi)(const char *rout, int herm, int two, int Order, int Uplo, int Trans,
+                            int N, int K, const T *alpha, const void *A, int lda, const void *B,
+                            int ldb, const T *beta, void *C, int ldc)
+{
+  const int info = crk_check(Order, Uplo, Trans, N, K, lda, ldb, ldc, herm, two);
+  const size_t esz = 2 * sizeof(T);
+  const int alt = herm ? KX_CONJTRANS : KX_TRANS;
+  int uplo, trans, ar, ac;
+  T al[2];
+  atlmx_ctx *c;
+  long dla, dlb = 0, dlc;
+  T *dA, *dB = NULL, *dC;
+  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
+  if (N == 0 || (((alpha[0] == (T)0 && alpha[1] == (T)0) || K == 0) && beta[0] == (T)1 &&
+                 (herm || beta[1] == (T)0)))
+    return;
+  al[0] = alpha[0]; al[1] = alpha[1];
+  if (Order == CblasColMajor) {
+    uplo = (Uplo == CblasUpper) ? KX_UPPER : KX_LOWER;
+    trans = (Trans == CblasNoTrans) ? KX_NOTRANS : alt;
+  } else {
+    uplo = (Uplo == CblasUpper) ? KX_LOWER : KX_UPPER;
+    trans = (Trans == CblasNoTrans) ? alt : KX_NOTRANS;
+    if (herm && two) al[1] = -al[1];
+  }
+  ar = (trans == KX_NOTRANS) ? N : K;
+  ac = (trans == KX_NOTRANS) ? K : N;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, ar, ac, lda, esz, 1, &dla, rout);
+  if (two) dB = (T *)rt_stage_in(c, WS_STAGE_B, B, ar, ac, ldb, esz, 1, &dlb, rout);
+  dC = (T *)rt_stage_in(c, WS_STAGE_C, C, N, N, ldc, esz, 1, &dlc, rout);
+  NM(PFX, rk_dev)(c, herm, uplo, trans, N, K, al, dA, dla, dB, dlb, beta, dC, dlc, rout);
+  rt_stage_out(c, C, dC, N, N, ldc, dlc, esz, rout);
+  rt_sync(c, rout);
+}
+
+/* interfaces/blas/C/src/cblas_zherk.c: alpha and beta real */
+void NM3(cblas_, PFX, herk)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const T alpha, const void *A, const int lda, const T beta, void *C,
+                            const int ldc)
+{
+  const T al[2] = {alpha, (T)0}, be[2] = {beta, (T)0};
+  NM(PFX, rk_abi)("cblas_" STR(PFX) "herk", 1, 0, Order, Uplo, Trans, N, K, al, A, lda, NULL, 0,
+                  be, C, ldc);
+}
+
+/* interfaces/blas/C/src/cblas_zher2k.c: beta real */
+void NM3(cblas_, PFX, her2k)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                             const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                             const void *alpha, const void *A, const int lda, const void *B,
+                             const int ldb, const T beta, void *C, const int ldc)
+{
+  const T be[2] = {beta, (T)0};
+  NM(PFX, rk_abi)("cblas_" STR(PFX) "her2k", 1, 1, Order, Uplo, Trans, N, K, (const T *)alpha, A,
+                  lda, B, ldb, be, C, ldc);
+}
+
+/* interfaces/blas/C/src/cblas_zsyrk.c */
+void NM3(cblas_, PFX, syrk)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                            const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                            const void *alpha, const void *A, const int lda, const void *beta,
+                            void *C, const int ldc)
+{
+  NM(PFX, rk_abi)("cblas_" STR(PFX) "syrk", 0, 0, Order, Uplo, Trans, N, K, (const T *)alpha, A,
+                  lda, NULL, 0, (const T *)beta, C, ldc);
+}
+
+/* interfaces/blas/C/src/cblas_zsyr2k.c */
+void NM3(cblas_, PFX, syr2k)(const enum CBLAS_ORDER Order, const enum CBLAS_UPLO Uplo,
+                             const enum CBLAS_TRANSPOSE Trans, const int N, const int K,
+                             const void *alpha, const void *A, const int lda, const void *B,
+                             const int ldb, const void *beta, void *C, const int ldc)
+{
+  NM(PFX, rk_abi)("cblas_" STR(PFX) "syr2k", 0, 1, Order, Uplo, Trans, N, K, (const T *)alpha, A,
+                  lda, B, ldb, (const T *)beta, C, ldc);
+}
+
+/* HEMM / SYMM C ABI: RowMajor flips Side and Uplo and swaps M and N
+ * (interfaces/blas/C/src/cblas_zhemm.c, cblas_zsymm.c) */
+static void NM(PFX, hemm_abi)(const char *rout, int herm, int Order, int Side, int Uplo, int M,
+                              int N, const T *alpha, const void *A, int lda, const void *B,
+                              int ldb, const T *beta, void *C, int ldc)
+{
+  const int info = atlmx_symm_check(Order, Side, Uplo, M, N, lda, ldb, ldc);
+  const size_t esz = 2 * sizeof(T);
+  int side, uplo, m, n, na;
+  atlmx_ctx *c;
+  long dla, dlb, dlc;
+  T *dA, *dB, *dC;
+  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
+  if (M == 0 || N == 0 ||
+      (alpha[0] == (T)0 && alpha[1] == (T)0 && beta[0] == (T)1 && beta[1] == (T)0))
+    return;
+  if (Order == CblasColMajor) {
+    side = (Side == CblasLeft) ? KX_LEFT : KX_RIGHT;
+    uplo = (Uplo == CblasUpper) ? KX_UPPER : KX_LOWER;
+    m = M; n = N;
+  } else {
+    side = (Side == CblasLeft) ? KX_RIGHT : KX_LEFT;
+    uplo = (Uplo == CblasUpper) ? KX_LOWER : KX_UPPER;
+    m = N; n = M;
+  }
+  na = (side == KX_LEFT) ? m : n;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, na, na, lda, esz, 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, m, n, ldb, esz, 1, &dlb, rout);
+  dC = (T *)rt_stage_in(c, WS_STAGE_C, C, m, n, ldc, esz,
+                        !(beta[0] == (T)0 && beta[1] == (T)0), &dlc, rout);
+  NM(PFX, hemm_dev)(c, herm, side, uplo, m, n, alpha, dA, dla, dB, dlb, beta, dC, dlc, rout);
+  rt_stage_out(c, C, dC, m, n, ldc, dlc, esz, rout);
+  rt_sync(c, rout);
+}
+
+void NM3(cblas_, PFX, hemm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                            const enum CBLAS_UPLO Uplo, const int M, const int N,
+                            const void *alpha, const void *A, const int lda, const void *B,
+                            const int ldb, const void *beta, void *C, const int ldc)
+{
+  NM(PFX, hemm_abi)("cblas_" STR(PFX) "hemm", 1, Order, Side, Uplo, M, N, (const T *)alpha, A,
+                    lda, B, ldb, (const T *)beta, C, ldc);
+}
+
+void NM3(cblas_, PFX, symm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                            const enum CBLAS_UPLO Uplo, const int M, const int N,
+                            const void *alpha, const void *A, const int lda, const void *B,
+                            const int ldb, const void *beta, void *C, const int ldc)
+{
+  NM(PFX, hemm_abi)("cblas_" STR(PFX) "symm", 0, Order, Side, Uplo, M, N, (const T *)alpha, A,
+                    lda, B, ldb, (const T *)beta, C, ldc);
+}
+
+/* interfaces/blas/C/src/cblas_ztrmm.c (RowMajor flips Side and Uplo and
+ * swaps M and N, as for TRSM) */
+void NM3(cblas_, PFX, trmm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE Side,
+                            const enum CBLAS_UPLO Uplo, const enum CBLAS_TRANSPOSE TA,
+                            const enum CBLAS_DIAG Diag, const int M, const int N,
+                            const void *alpha, const void *A, const int lda, void *B,
+                            const int ldb)
+{
+  static const char *rout = "cblas_" STR(PFX) "trmm";
+  const int info = atlmx_trxm_check(Order, Side, Uplo, TA, Diag, M, N, lda, ldb);
+  const T *al = (const T *)alpha;
+  const size_t esz = 2 * sizeof(T);
+  int side, uplo, m, n, na;
+  atlmx_ctx *c;
+  long dla, dlb;
+  T *dA, *dB;
+  if (info != 2000) { cblas_xerbla(info, rout, ""); return; }
+  if (M == 0 || N == 0) return;
+  if (Order == CblasColMajor) {
+    side = (Side == CblasLeft) ? KX_LEFT : KX_RIGHT;
+    uplo = (Uplo == CblasUpper) ? KX_UPPER : KX_LOWER;
+    m = M; n = N;
+  } else {
+    side = (Side == CblasLeft) ? KX_RIGHT : KX_LEFT;
+    uplo = (Uplo == CblasUpper) ? KX_LOWER : KX_UPPER;
+    m = N; n = M;
+  }
+  na = (side == KX_LEFT) ? m : n;
+  c = rt_ctx();
+  dA = (T *)rt_stage_in(c, WS_STAGE_A, A, na, na, lda, esz, 1, &dla, rout);
+  dB = (T *)rt_stage_in(c, WS_STAGE_B, B, m, n, ldb, esz, 1, &dlb, rout);
+  NM(PFX, trmm_dev)(c, side, uplo, NM(PFX, op)(TA), Diag == CblasUnit ? KX_UNIT : KX_NONUNIT, m,
+                    n, al, dA, dla, dB, dlb, rout);
+  rt_stage_out(c, B, dB, m, n, ldb, dlb, esz, rout);
+  rt_sync(c, rout);
+}

@@ -60,6 +60,7 @@ int atlmx_gemm_check(int Order, int TA, int TB, int M, int N, int K, int lda, in
                      int ldc);
 int atlmx_trxm_check(int Order, int Side, int Uplo, int TA, int Diag, int M, int N, int lda,
                      int ldb);
+int atlmx_symm_check(int Order, int Side, int Uplo, int M, int N, int lda, int ldb, int ldc);
 /* clapack argument checks shared by the real and complex routines (defined
  * in lapack.c): 0, or the negated position of the last bad argument */
 int atlmx_getrf_check(const char *rout, int Order, int M, int N, int lda);

@@ -537,6 +537,59 @@ __global__ __launch_bounds__(256) void cpotf2_kernel(int upper, int n, T *__rest
   }
 }
 
+/* C = beta*C on one triangle of an n x n complex C (lower: i >= j), beta = 0
+ * storing zeros; herm also stores 0 in the diagonal's imaginary parts (a
+ * Hermitian C's diagonal is real: src/blas/reference/level3/ATL_zrefherk*.c
+ * set it so at every scaling and update).  With beta = 1 and herm only the
+ * diagonal is touched. */
+template <typename T>
+__global__ __launch_bounds__(256) void ctri_scale_kernel(int n, int lower, T br, T bi, int herm,
+                                                         T *C, long ldc) {
+  const bool diag_only = (br == T(1) && bi == T(0));
+  const long total = diag_only ? (long)n : (long)n * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long j = diag_only ? e : e / n, i = diag_only ? e : e - j * n;
+    if (lower ? (i < j) : (i > j)) continue;
+    T *c = C + 2 * (j * ldc + i);
+    if (!diag_only) {
+      if (br == T(0) && bi == T(0)) {
+        c[0] = T(0); c[1] = T(0);
+      } else if (herm) {
+        /* a real beta scales each part; the diagonal's imaginary part is
+         * not referenced */
+        c[0] = br * c[0];
+        c[1] = (i == j) ? T(0) : br * c[1];
+      } else {
+        const T cr = c[0], ci = c[1];
+        c[0] = br * cr - bi * ci;
+        c[1] = br * ci + bi * cr;
+      }
+    }
+    if (herm && i == j) c[1] = T(0);
+  }
+}
+
+/* The full n x n matrix of a Hermitian (herm) or symmetric A held in one
+ * triangle: out(i,j) = A(i,j) in the triangle, else A(j,i), conjugated when
+ * herm; a Hermitian diagonal's imaginary part is taken as 0 (it is not
+ * referenced: ATL_zrefhemm*.c). */
+template <typename T>
+__global__ __launch_bounds__(256) void csym_expand_kernel(int n, int lower, int herm,
+                                                          const T *__restrict__ A, long lda,
+                                                          T *__restrict__ out, long ldo) {
+  const long total = (long)n * n;
+  for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < total; e += (long)gridDim.x * 256) {
+    const long j = e / n, i = e - j * n;
+    const bool in = lower ? (i >= j) : (i <= j);
+    const T *a = in ? A + 2 * (j * lda + i) : A + 2 * (i * lda + j);
+    T vr = a[0], vi = a[1];
+    if (herm && !in) vi = -vi;
+    if (herm && i == j) vi = T(0);
+    T *o = out + 2 * (j * ldo + i);
+    o[0] = vr; o[1] = vi;
+  }
+}
+
 /* C = conj(C), m x n complex, in place */
 template <typename T>
 __global__ __launch_bounds__(256) void cconj_kernel(int m, int n, T *C, long ldc) {
@@ -656,6 +709,29 @@ hipError_t launch_cpotf2(int upper, int n, T *A, long lda, int base, int *info, 
 }
 
 template <typename T>
+hipError_t launch_ctri_scale(int n, int lower, T br, T bi, int herm, T *C, long ldc,
+                             hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!C || ldc < n) return hipErrorInvalidValue;
+  const long total = (br == T(1) && bi == T(0)) ? (long)n : (long)n * n;
+  const long want = (total + 255) / 256;
+  hipLaunchKernelGGL((ctri_scale_kernel<T>), dim3((unsigned)(want < 8192 ? want : 8192)),
+                     dim3(256), 0, s, n, lower, br, bi, herm, C, ldc);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_csym_expand(int n, int lower, int herm, const T *A, long lda, T *out, long ldo,
+                              hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!A || !out || lda < n || ldo < n) return hipErrorInvalidValue;
+  const long want = ((long)n * n + 255) / 256;
+  hipLaunchKernelGGL((csym_expand_kernel<T>), dim3((unsigned)(want < 8192 ? want : 8192)),
+                     dim3(256), 0, s, n, lower, herm, A, lda, out, ldo);
+  return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_cconj(int m, int n, T *C, long ldc, hipStream_t s) {
   if (m <= 0 || n <= 0) return hipSuccess;
   if (!C || ldc < m) return hipErrorInvalidValue;
@@ -715,4 +791,20 @@ extern "C" hipError_t atlmx_k_zconj(int m, int n, double *C, long ldc, hipStream
 }
 extern "C" hipError_t atlmx_k_cconj(int m, int n, float *C, long ldc, hipStream_t s) {
   return launch_cconj<float>(m, n, C, ldc, s);
+}
+extern "C" hipError_t atlmx_k_ztri_scale(int n, int lower, double br, double bi, int herm,
+                                         double *C, long ldc, hipStream_t s) {
+  return launch_ctri_scale<double>(n, lower, br, bi, herm, C, ldc, s);
+}
+extern "C" hipError_t atlmx_k_ctri_scale(int n, int lower, float br, float bi, int herm, float *C,
+                                         long ldc, hipStream_t s) {
+  return launch_ctri_scale<float>(n, lower, br, bi, herm, C, ldc, s);
+}
+extern "C" hipError_t atlmx_k_zsym_expand(int n, int lower, int herm, const double *A, long lda,
+                                          double *out, long ldo, hipStream_t s) {
+  return launch_csym_expand<double>(n, lower, herm, A, lda, out, ldo, s);
+}
+extern "C" hipError_t atlmx_k_csym_expand(int n, int lower, int herm, const float *A, long lda,
+                                          float *out, long ldo, hipStream_t s) {
+  return launch_csym_expand<float>(n, lower, herm, A, lda, out, ldo, s);
 }

@@ -416,6 +416,19 @@ hipError_t atlmx_k_zpotf2(int upper, int n, double *A, long lda, int base, int *
                           hipStream_t s);
 hipError_t atlmx_k_cpotf2(int upper, int n, float *A, long lda, int base, int *info,
                           hipStream_t s);
+/* beta * C on the lower (lower = 1) or upper triangle of an n x n complex C,
+ * beta = 0 storing zeros; herm zeroes the diagonal's imaginary parts (with
+ * beta = 1 that is all it does) */
+hipError_t atlmx_k_ztri_scale(int n, int lower, double br, double bi, int herm, double *C,
+                              long ldc, hipStream_t s);
+hipError_t atlmx_k_ctri_scale(int n, int lower, float br, float bi, int herm, float *C,
+                              long ldc, hipStream_t s);
+/* out = the full n x n Hermitian (herm) or symmetric matrix held in A's
+ * lower (lower = 1) or upper triangle */
+hipError_t atlmx_k_zsym_expand(int n, int lower, int herm, const double *A, long lda,
+                               double *out, long ldo, hipStream_t s);
+hipError_t atlmx_k_csym_expand(int n, int lower, int herm, const float *A, long lda, float *out,
+                               long ldo, hipStream_t s);
 /* C = conj(C) in place, m x n */
 hipError_t atlmx_k_zconj(int m, int n, double *C, long ldc, hipStream_t s);
 hipError_t atlmx_k_cconj(int m, int n, float *C, long ldc, hipStream_t s);

@@ -39,6 +39,8 @@ enum {
   WS_LASWP,         /* getrs: row-interchange gather map        */
   WS_LASWP_B,       /* getrs: right-hand sides being permuted   */
   WS_CPIV,          /* complex LU: pivots of the interleaved real view */
+  WS_CPLX_X,        /* complex HEMM/SYMM/TRMM: the dense matrix */
+  WS_CPLX_Y,        /* complex TRMM: copy of B */
   WS_NSLOTS
 };
 

@@ -63,6 +63,36 @@ def test_xerbla_exits_on_bad_argument():
     assert "Parameter 6 to routine clapack_dgetrf was incorrect" in p.stderr
 
 
+def test_complex_level3_xerbla_without_gpu():
+    """The complex rank-k, HEMM/SYMM and TRMM argument checks report ATLAS's
+    parameter numbers (interfaces/blas/C/src/cblas_zherk.c, cblas_zsyrk.c,
+    cblas_zher2k.c, cblas_zhemm.c, cblas_ztrmm.c) before any GPU work."""
+    cases = [
+        ("L.cblas_zherk(102, 121, 112, 2, 2, C.c_double(1.0), None, 2, C.c_double(0.0), None, 2)",
+         "Parameter 3 to routine cblas_zherk"),
+        ("L.cblas_csyrk(102, 121, 113, 2, 2, None, None, 2, None, None, 2)",
+         "Parameter 3 to routine cblas_csyrk"),
+        ("L.cblas_zher2k(102, 122, 111, 4, 2, None, None, 4, None, 3, C.c_double(0.0), None, 4)",
+         "Parameter 10 to routine cblas_zher2k"),
+        ("L.cblas_csyr2k(101, 121, 111, 3, 2, None, None, 2, None, 2, None, None, 2)",
+         "Parameter 13 to routine cblas_csyr2k"),
+        ("L.cblas_zhemm(102, 141, 121, 5, 2, None, None, 4, None, 5, None, None, 5)",
+         "Parameter 8 to routine cblas_zhemm"),
+        ("L.cblas_csymm(102, 141, 123, 2, 2, None, None, 2, None, 2, None, None, 2)",
+         "Parameter 3 to routine cblas_csymm"),
+        ("L.cblas_ztrmm(102, 141, 121, 111, 133, 2, 2, None, None, 2, None, 2)",
+         "Parameter 5 to routine cblas_ztrmm"),
+    ]
+    for call, msg in cases:
+        code = ("import ctypes as C, sys; sys.path.insert(0, %r); import atlas_amd as a;"
+                "L = a.lib(); %s; print('RETURNED')" % (ROOT, call))
+        p = subprocess.run(["python", "-c", code], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                           text=True, timeout=120)
+        assert p.returncode == 255, (call, p.stderr)
+        assert msg + " was incorrect" in p.stderr, (call, p.stderr)
+        assert "RETURNED" not in p.stdout
+
+
 def test_batch_plan_order_limits_without_gpu():
     """The batched plans refuse orders the factor kernels cannot hold in
     LDS before any device call: a NULL plan and a message."""

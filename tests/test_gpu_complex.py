@@ -1,6 +1,6 @@
-"""GPU parity of the complex Level-3 BLAS (cblas_{z,c}gemm, cblas_{z,c}trsm)
-through the C ABI against the oracle (whose GEMM is bit-exact with ATLAS's
-ATL_zrefgemm / ATL_crefgemm), with the complex form of gemmtst's elementwise
+"""GPU parity of the complex Level-3 BLAS (cblas_{z,c}gemm, cblas_{z,c}trsm,
+and HERK/HER2K/SYRK/SYR2K/HEMM/SYMM/TRMM) through the C ABI against the
+oracle (whose GEMM is bit-exact with ATLAS's ATL_zrefgemm / ATL_crefgemm), with the complex form of gemmtst's elementwise
 bound (bin/gemmtst.c:395): |D - T| <= 4 K eps (|alpha| |op A| |op B| +
 |beta| |C|) element by element, and l3blastst's TRSM residual (bound 4 for
 complex data, where one complex multiply rounds up to ~2*sqrt(2) eps)."""
@@ -472,3 +472,136 @@ def test_zposv_4096():
                                             _eps(np.complex128) * n)
     assert r < 10.0
     assert _llt_resid(A0, A, X.Lower) < 10.0
+
+
+# --- HERK / HER2K / SYRK / SYR2K / HEMM / SYMM / TRMM through the C ABI
+# against the closed forms of tests/checks.py (pinned there against ATLAS's
+# own ATL_zref* / ATL_cref* routines), ColMajor and RowMajor, with an
+# elementwise bound of gemmtst's form.
+from tests.checks import crk_np, chemm_np, ctrmm_np, csym_full   # noqa: E402
+
+
+def _arr(a, order):
+    return a.copy(order="F" if order == X.ColMajor else "C")
+
+
+def _ldx(a, order):
+    return max(1, a.shape[0] if order == X.ColMajor else a.shape[1])
+
+
+def _pp(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("herm,two", [(1, 0), (1, 1), (0, 0), (0, 1)])
+@pytest.mark.parametrize("n,K", [(1, 1), (37, 70), (200, 33), (130, 0), (129, 300)])
+def test_crk_vs_closed_form(dt, herm, two, n, K):
+    p, rt = X._cpre(dt)
+    nm = ("her" if herm else "syr") + ("2k" if two else "k")
+    f = getattr(X.lib(), "cblas_%s%s" % (p, nm))
+    eps = _eps(dt)
+    alt = X.ConjTrans if herm else X.Trans
+    for order, uplo, trans in itertools.product([X.ColMajor, X.RowMajor], [X.Upper, X.Lower],
+                                                [X.NoTrans, alt]):
+        shp = (n, K) if trans == X.NoTrans else (K, n)
+        A = _arr(_cplx(shp, dt), order)
+        B = _arr(_cplx(shp, dt), order) if two else None
+        for alpha, beta in [(0.7 - 0.4j, 0.5 + 0.3j), (-1.1 + 0.0j, 0.0), (0.0, -0.5)]:
+            if herm and not two:
+                alpha = complex(alpha).real
+            if herm:
+                beta = complex(beta).real
+            C0 = _cplx((n, n), dt)
+            off = np.triu(np.ones((n, n), bool), 1) if uplo == X.Lower else \
+                np.tril(np.ones((n, n), bool), -1)
+            C0[off] = np.asarray(3e30 - 1e30j, dtype=dt)
+            if beta == 0:
+                C0[~off] = np.nan
+            got = _arr(C0, order)
+            al, be = X._cscalar(alpha, rt), X._cscalar(beta, rt)
+            args = [order, uplo, trans, n, K, al[0] if (herm and not two) else _pp(al), _pp(A),
+                    _ldx(A, order)]
+            if two:
+                args += [_pp(B), _ldx(B, order)]
+            f(*(args + [be[0] if herm else _pp(be), _pp(got), _ldx(got, order)]))
+            want = crk_np(herm, uplo, trans, alpha, A, B, beta, C0)
+            opA = _op(A, X.NoTrans if trans == X.NoTrans else alt)
+            mag = np.abs(opA) @ np.abs(opA).T
+            if two:
+                opB = _op(B, X.NoTrans if trans == X.NoTrans else alt)
+                mag = np.abs(opA) @ np.abs(opB).T + np.abs(opB) @ np.abs(opA).T
+            bound = 4 * (K + 2) * eps * (abs(alpha) * mag +
+                                         abs(beta) * np.abs(np.nan_to_num(C0))) + 1e-300
+            case = (order, uplo, trans, alpha, beta)
+            assert np.array_equal(got[off], C0[off]), case
+            err = np.abs(got.astype(np.complex128) - want)[~off]
+            assert np.all(err <= bound[~off]), (float(np.max(err / bound[~off])),) + case
+            if herm and n:
+                assert np.all(np.diag(got).imag == 0), case
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("herm", [1, 0])
+@pytest.mark.parametrize("M,N", [(1, 1), (45, 70), (257, 33)])
+def test_chemm_vs_closed_form(dt, herm, M, N):
+    p, rt = X._cpre(dt)
+    f = getattr(X.lib(), "cblas_%s%s" % (p, "hemm" if herm else "symm"))
+    eps = _eps(dt)
+    for order, side, uplo in itertools.product([X.ColMajor, X.RowMajor], [X.Left, X.Right],
+                                               [X.Upper, X.Lower]):
+        na = M if side == X.Left else N
+        A = _cplx((na, na), dt)
+        off = np.triu(np.ones((na, na), bool), 1) if uplo == X.Lower else \
+            np.tril(np.ones((na, na), bool), -1)
+        A[off] = np.asarray(1e30 + 1e30j, dtype=dt)       # never read
+        if herm:
+            A[np.diag_indices(na)] += np.asarray(1e30j, dtype=dt)  # never read
+        A = _arr(A, order)
+        B = _arr(_cplx((M, N), dt), order)
+        for alpha, beta in [(0.6 + 0.2j, -0.3 + 0.9j), (1.0, 0.0)]:
+            C0 = _cplx((M, N), dt)
+            if beta == 0:
+                C0[:] = np.nan
+            got = _arr(C0, order)
+            al, be = X._cscalar(alpha, rt), X._cscalar(beta, rt)
+            f(order, side, uplo, M, N, _pp(al), _pp(A), _ldx(A, order), _pp(B), _ldx(B, order),
+              _pp(be), _pp(got), _ldx(got, order))
+            want = chemm_np(herm, side, uplo, alpha, A, B, beta, C0)
+            F = np.abs(csym_full(herm, uplo, A))
+            mag = F @ np.abs(B) if side == X.Left else np.abs(B) @ F
+            bound = 4 * (na + 2) * eps * (abs(alpha) * mag +
+                                          abs(beta) * np.abs(np.nan_to_num(C0))) + 1e-300
+            err = np.abs(got.astype(np.complex128) - want)
+            assert np.all(err <= bound), (float(np.max(err / bound)), order, side, uplo, alpha)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("M,N", [(1, 1), (45, 70), (257, 33)])
+def test_ctrmm_vs_closed_form(dt, M, N):
+    p, rt = X._cpre(dt)
+    f = getattr(X.lib(), "cblas_%strmm" % p)
+    eps = _eps(dt)
+    for order, side, uplo, ta, diag in itertools.product(
+            [X.ColMajor, X.RowMajor], [X.Left, X.Right], [X.Upper, X.Lower], OPS,
+            [X.NonUnit, X.Unit]):
+        na = M if side == X.Left else N
+        A = _cplx((na, na), dt)
+        off = np.triu(np.ones((na, na), bool), 1) if uplo == X.Lower else \
+            np.tril(np.ones((na, na), bool), -1)
+        A[off] = np.asarray(1e30 + 1e30j, dtype=dt)       # never read
+        if diag == X.Unit:
+            A[np.diag_indices(na)] = np.asarray(1e30, dtype=dt)   # never read
+        A = _arr(A, order)
+        B0 = _cplx((M, N), dt)
+        for alpha in (1.0, -0.6 + 0.8j, 0.0):
+            got = _arr(B0, order)
+            al = X._cscalar(alpha, rt)
+            f(order, side, uplo, ta, diag, M, N, _pp(al), _pp(A), _ldx(A, order), _pp(got),
+              _ldx(got, order))
+            want = ctrmm_np(side, uplo, ta, diag, alpha, A, B0)
+            T = np.abs(ctrmm_np(side, uplo, ta, diag, 1.0, np.abs(A), np.eye(na)))  # |op(A)|
+            mag = T @ np.abs(B0) if side == X.Left else np.abs(B0) @ T
+            bound = 4 * (na + 2) * eps * abs(alpha) * mag + 1e-300
+            err = np.abs(got.astype(np.complex128) - want)
+            assert np.all(err <= bound), (float(np.max(err / bound)), order, side, uplo, ta, diag)

@@ -10,6 +10,8 @@
   conventions without the reference;
 * TRSM against ATL_ztrsm's reference (ATL_zreftrsm / ATL_creftrsm) within
   rounding, all 24 cases, and against numpy's dense solve of op(A);
+* the closed forms of HERK, HER2K, SYRK, SYR2K, HEMM, SYMM and TRMM that
+  the GPU tests expect (tests/checks.py) against ATL_zref* / ATL_cref*;
 * the unblocked complex LU (ATL_getf2) against scipy's LAPACK ?getrf, which
   chooses pivots by the same |Re| + |Im| rule: pivots equal away from near
   ties, factors within rounding, P A = L U always; zero-pivot info and the
@@ -240,3 +242,120 @@ def test_cpotrf_not_positive_definite_info():
     A = _hpd(4, np.complex128, 4)
     A[0, 0] = 0.0
     assert O.cpotrf(A, O.LOWER) == 1
+
+
+# --- the closed forms the complex rank-k / HEMM / SYMM / TRMM GPU tests
+# expect (tests/checks.py) against ATLAS's own reference routines: the
+# conjugate on her2k's second alpha, the real Hermitian diagonal (whose
+# imaginary part is never read), beta on one triangle only, the quick
+# returns and beta = 0 not reading C.
+from tests.checks import crk_np, chemm_np, ctrmm_np   # noqa: E402
+
+CPTR = C.c_void_p
+
+
+def _rk_ref(R, dt, herm, two):
+    z = dt == np.complex128
+    name = ("ATL_%sref" % ("z" if z else "c")) + ("her" if herm else "syr") + \
+        ("2k" if two else "k")
+    f = getattr(R, name)
+    rt = C.c_double if z else C.c_float
+    al = rt if (herm and not two) else CPTR
+    be = rt if herm else CPTR
+    f.argtypes = [C.c_int] * 4 + [al, CPTR, C.c_int] + ([CPTR, C.c_int] if two else []) + \
+        [be, CPTR, C.c_int]
+    return f
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("herm,two", [(1, 0), (1, 1), (0, 0), (0, 1)])
+def test_crk_closed_form_vs_atlas_ref(dt, herm, two):
+    R = _ref_or_skip()
+    f = _rk_ref(R, dt, herm, two)
+    rt = np.float64 if dt == np.complex128 else np.float32
+    tol = 1e-12 if dt == np.complex128 else 2e-5
+    alt = O.CONJTRANS if herm else O.TRANS
+    n, K = 9, 6
+    for uplo, trans in itertools.product([O.UPPER, O.LOWER], [O.NOTRANS, alt]):
+        shp = (n, K) if trans == O.NOTRANS else (K, n)
+        A, B = _cplx(shp, dt), (_cplx(shp, dt) if two else None)
+        for alpha, beta in [(0.7 - 0.4j, 0.5 + 0.3j), (1.0, 1.0), (0.0, 1.0), (0.0, -0.5),
+                            (0.0, 0.0), (-1.2 + 0.0j, 0.0), (0.25j, 1.0)]:
+            if herm and not two:
+                alpha = alpha.real if isinstance(alpha, complex) else alpha
+            if herm:
+                beta = complex(beta).real
+            C0 = _cplx((n, n), dt)
+            off = np.triu(np.ones((n, n), bool), 1) if uplo == O.LOWER else \
+                np.tril(np.ones((n, n), bool), -1)
+            C0[off] = np.asarray(1e30 - 2e30j, dtype=dt)
+            if beta == 0:
+                C0[~off] = np.nan
+            want = crk_np(herm, uplo, trans, alpha, A, B, beta, C0)
+            got = C0.copy(order="F")
+            al_keep = _scal(alpha, rt)      # held until the call returns
+            a = rt(complex(alpha).real) if (herm and not two) else al_keep.ctypes.data
+            be_keep = _scal(beta, rt)
+            b = rt(complex(beta).real) if herm else be_keep.ctypes.data
+            args = [uplo, trans, n, K, a, A.ctypes.data, A.shape[0]]
+            if two:
+                args += [B.ctypes.data, B.shape[0]]
+            f(*(args + [b, got.ctypes.data, n]))
+            assert np.array_equal(got[off], C0[off]), (uplo, trans, alpha, beta)
+            assert np.all(np.isfinite(got[~off])), (uplo, trans, alpha, beta)
+            assert np.max(np.abs(got - want)[~off]) <= tol, (uplo, trans, alpha, beta)
+            if herm:
+                assert np.all(np.diag(got).imag == 0) or (alpha == 0 and beta == 1)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+@pytest.mark.parametrize("herm", [1, 0])
+def test_chemm_closed_form_vs_atlas_ref(dt, herm):
+    R = _ref_or_skip()
+    z = dt == np.complex128
+    f = getattr(R, "ATL_%sref%s" % ("z" if z else "c", "hemm" if herm else "symm"))
+    f.argtypes = [C.c_int] * 4 + [CPTR, CPTR, C.c_int, CPTR, C.c_int, CPTR, CPTR, C.c_int]
+    rt = np.float64 if z else np.float32
+    tol = 1e-12 if z else 2e-5
+    M, N = 8, 5
+    for side, uplo in itertools.product([O.LEFT, O.RIGHT], [O.UPPER, O.LOWER]):
+        na = M if side == O.LEFT else N
+        A = _cplx((na, na), dt)
+        off = np.triu(np.ones((na, na), bool), 1) if uplo == O.LOWER else \
+            np.tril(np.ones((na, na), bool), -1)
+        A[off] = np.asarray(1e30 + 1e30j, dtype=dt)     # never read
+        if herm:
+            A[np.diag_indices(na)] += np.asarray(1e30j, dtype=dt)   # never read
+        B = _cplx((M, N), dt)
+        for alpha, beta in [(0.6 + 0.2j, -0.3 + 0.9j), (1.0, 0.0), (0.0, 0.5j)]:
+            C0 = _cplx((M, N), dt)
+            if beta == 0:
+                C0[:] = np.nan
+            want = chemm_np(herm, side, uplo, alpha, A, B, beta, C0)
+            got = C0.copy(order="F")
+            al, be = _scal(alpha, rt), _scal(beta, rt)
+            f(side, uplo, M, N, al.ctypes.data, A.ctypes.data, na, B.ctypes.data, M,
+              be.ctypes.data, got.ctypes.data, M)
+            assert np.max(np.abs(got - want)) <= tol, (side, uplo, alpha, beta)
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_ctrmm_closed_form_vs_atlas_ref(dt):
+    R = _ref_or_skip()
+    z = dt == np.complex128
+    f = R.ATL_zreftrmm if z else R.ATL_creftrmm
+    f.argtypes = [C.c_int] * 6 + [CPTR, CPTR, C.c_int, CPTR, C.c_int]
+    rt = np.float64 if z else np.float32
+    tol = 1e-12 if z else 2e-5
+    M, N = 7, 6
+    for side, uplo, ta, diag in itertools.product([O.LEFT, O.RIGHT], [O.UPPER, O.LOWER], OPS,
+                                                  [O.NONUNIT, O.UNIT]):
+        na = M if side == O.LEFT else N
+        A = _tri(na, uplo, dt)
+        B0 = _cplx((M, N), dt)
+        for alpha in (1.0, -0.6 + 0.8j):
+            want = ctrmm_np(side, uplo, ta, diag, alpha, A, B0)
+            got = B0.copy(order="F")
+            al = _scal(alpha, rt)
+            f(side, uplo, ta, diag, M, N, al.ctypes.data, A.ctypes.data, na, got.ctypes.data, M)
+            assert np.max(np.abs(got - want)) <= tol * 10, (side, uplo, ta, diag, alpha)
