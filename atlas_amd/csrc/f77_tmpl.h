@@ -2,8 +2,8 @@
  * f77_tmpl.h -- precision template of the Fortran 77 entry points
  * (included by f77.c with T=double/P=d and T=float/P=s, and with
  * F77_COMPLEX defined for T=double/P=z and T=float/P=c: then the GEMM and
- * TRSM take alpha/beta as {real, imaginary} pairs and only the routines the
- * complex C ABI has are built).  Check order and
+ * Level-3 BLAS take alpha/beta as {real, imaginary} pairs, HEMM/HERK/HER2K
+ * are added and the LAPACK inverses and I?AMAX are left out).  Check order and
  * XERBLA positions follow interfaces/blas/F77/src/dgemm.f, dtrsm.f,
  * dtrmm.f, dsyrk.f, dsyr2k.f, dsymm.f, idamax.f and
  * interfaces/lapack/F77/src/dgetrf.f, dgetrs.f, dgesv.f, dpotrf.f,
@@ -14,6 +14,13 @@
 #define F77(name) FN(P, name, _)
 #define CB(name) FN(cblas_, P, name)
 #define CL(name) FN(clapack_, P, name)
+/* a BLAS scalar as the C ABI takes it: by address for complex types */
+#undef SC
+#ifdef F77_COMPLEX
+#define SC(x) (x)
+#else
+#define SC(x) (*(x))
+#endif
 
 void F77(gemm)(const char *ta, const char *tb, const int *m, const int *n, const int *k,
                const T *alpha, const T *a, const int *lda, const T *b, const int *ldb,
@@ -75,7 +82,6 @@ void F77(trsm)(const char *side, const char *uplo, const char *ta, const char *d
            a, *lda, b, *ldb);
 }
 
-#ifndef F77_COMPLEX
 void F77(trmm)(const char *side, const char *uplo, const char *ta, const char *diag,
                const int *m, const int *n, const T *alpha, const T *a, const int *lda, T *b,
                const int *ldb, size_t l1, size_t l2, size_t l3, size_t l4)
@@ -85,14 +91,25 @@ void F77(trmm)(const char *side, const char *uplo, const char *ta, const char *d
   if (info) { f77_err(PU "TRMM ", info); return; }
   CB(trmm)(CblasColMajor, lsame(side, 'L') ? CblasLeft : CblasRight,
            lsame(uplo, 'U') ? CblasUpper : CblasLower, tr_code(ta),
-           lsame(diag, 'U') ? CblasUnit : CblasNonUnit, *m, *n, *alpha, a, *lda, b, *ldb);
+           lsame(diag, 'U') ? CblasUnit : CblasNonUnit, *m, *n, SC(alpha), a, *lda, b, *ldb);
+}
+
+/* TRANS of ?SYRK / ?SYR2K: N, T or C for real types; N or T for complex
+ * types (zsyrk.f, zsyr2k.f) */
+static int FN(P, syrk_f77, _tr)(const char *tr)
+{
+  const int itr = tr_code(tr);
+#ifdef F77_COMPLEX
+  if (itr == CblasConjTrans) return 0;
+#endif
+  return itr;
 }
 
 void F77(syrk)(const char *uplo, const char *tr, const int *n, const int *k, const T *alpha,
                const T *a, const int *lda, const T *beta, T *c, const int *ldc, size_t l1,
                size_t l2)
 {
-  const int itr = tr_code(tr);
+  const int itr = FN(P, syrk_f77, _tr)(tr);
   const int nra = (itr == CblasNoTrans) ? *n : *k;
   int info = 0;
   (void)l1; (void)l2;
@@ -103,15 +120,15 @@ void F77(syrk)(const char *uplo, const char *tr, const int *n, const int *k, con
   else if (*lda < imax1(nra)) info = 7;
   else if (*ldc < imax1(*n)) info = 10;
   if (info) { f77_err(PU "SYRK ", info); return; }
-  CB(syrk)(CblasColMajor, lsame(uplo, 'U') ? CblasUpper : CblasLower, itr, *n, *k, *alpha, a,
-           *lda, *beta, c, *ldc);
+  CB(syrk)(CblasColMajor, lsame(uplo, 'U') ? CblasUpper : CblasLower, itr, *n, *k, SC(alpha),
+           a, *lda, SC(beta), c, *ldc);
 }
 
 void F77(syr2k)(const char *uplo, const char *tr, const int *n, const int *k, const T *alpha,
                 const T *a, const int *lda, const T *b, const int *ldb, const T *beta, T *c,
                 const int *ldc, size_t l1, size_t l2)
 {
-  const int itr = tr_code(tr);
+  const int itr = FN(P, syrk_f77, _tr)(tr);
   const int nra = (itr == CblasNoTrans) ? *n : *k;
   int info = 0;
   (void)l1; (void)l2;
@@ -123,30 +140,95 @@ void F77(syr2k)(const char *uplo, const char *tr, const int *n, const int *k, co
   else if (*ldb < imax1(nra)) info = 9;
   else if (*ldc < imax1(*n)) info = 12;
   if (info) { f77_err(PU "SYR2K", info); return; }
-  CB(syr2k)(CblasColMajor, lsame(uplo, 'U') ? CblasUpper : CblasLower, itr, *n, *k, *alpha, a,
-            *lda, b, *ldb, *beta, c, *ldc);
+  CB(syr2k)(CblasColMajor, lsame(uplo, 'U') ? CblasUpper : CblasLower, itr, *n, *k, SC(alpha),
+            a, *lda, b, *ldb, SC(beta), c, *ldc);
+}
+
+/* SYMM and the complex HEMM share their checks (dsymm.f, zhemm.f) */
+static int FN(P, symm_f77, _check)(const char *side, const char *uplo, const int *m,
+                                   const int *n, const int *lda, const int *ldb, const int *ldc)
+{
+  const int left = lsame(side, 'L');
+  const int nra = left ? *m : *n;
+  if (!left && !lsame(side, 'R')) return 1;
+  if (!lsame(uplo, 'U') && !lsame(uplo, 'L')) return 2;
+  if (*m < 0) return 3;
+  if (*n < 0) return 4;
+  if (*lda < imax1(nra)) return 7;
+  if (*ldb < imax1(*m)) return 9;
+  if (*ldc < imax1(*m)) return 12;
+  return 0;
 }
 
 void F77(symm)(const char *side, const char *uplo, const int *m, const int *n, const T *alpha,
                const T *a, const int *lda, const T *b, const int *ldb, const T *beta, T *c,
                const int *ldc, size_t l1, size_t l2)
 {
-  const int left = lsame(side, 'L');
-  const int nra = left ? *m : *n;
+  const int info = FN(P, symm_f77, _check)(side, uplo, m, n, lda, ldb, ldc);
+  (void)l1; (void)l2;
+  if (info) { f77_err(PU "SYMM ", info); return; }
+  CB(symm)(CblasColMajor, lsame(side, 'L') ? CblasLeft : CblasRight,
+           lsame(uplo, 'U') ? CblasUpper : CblasLower, *m, *n, SC(alpha), a, *lda, b, *ldb,
+           SC(beta), c, *ldc);
+}
+
+#ifdef F77_COMPLEX
+/* zhemm.f */
+void F77(hemm)(const char *side, const char *uplo, const int *m, const int *n, const T *alpha,
+               const T *a, const int *lda, const T *b, const int *ldb, const T *beta, T *c,
+               const int *ldc, size_t l1, size_t l2)
+{
+  const int info = FN(P, symm_f77, _check)(side, uplo, m, n, lda, ldb, ldc);
+  (void)l1; (void)l2;
+  if (info) { f77_err(PU "HEMM ", info); return; }
+  CB(hemm)(CblasColMajor, lsame(side, 'L') ? CblasLeft : CblasRight,
+           lsame(uplo, 'U') ? CblasUpper : CblasLower, *m, *n, alpha, a, *lda, b, *ldb, beta,
+           c, *ldc);
+}
+
+/* zherk.f / zher2k.f: TRANS N or C; HERK's alpha and beta and HER2K's
+ * beta are real */
+void F77(herk)(const char *uplo, const char *tr, const int *n, const int *k, const T *alpha,
+               const T *a, const int *lda, const T *beta, T *c, const int *ldc, size_t l1,
+               size_t l2)
+{
+  const int itr = lsame(tr, 'N') ? CblasNoTrans : lsame(tr, 'C') ? CblasConjTrans : 0;
+  const int nra = (itr == CblasNoTrans) ? *n : *k;
   int info = 0;
   (void)l1; (void)l2;
-  if (!left && !lsame(side, 'R')) info = 1;
-  else if (!lsame(uplo, 'U') && !lsame(uplo, 'L')) info = 2;
-  else if (*m < 0) info = 3;
-  else if (*n < 0) info = 4;
+  if (!lsame(uplo, 'L') && !lsame(uplo, 'U')) info = 1;
+  else if (!itr) info = 2;
+  else if (*n < 0) info = 3;
+  else if (*k < 0) info = 4;
   else if (*lda < imax1(nra)) info = 7;
-  else if (*ldb < imax1(*m)) info = 9;
-  else if (*ldc < imax1(*m)) info = 12;
-  if (info) { f77_err(PU "SYMM ", info); return; }
-  CB(symm)(CblasColMajor, left ? CblasLeft : CblasRight,
-           lsame(uplo, 'U') ? CblasUpper : CblasLower, *m, *n, *alpha, a, *lda, b, *ldb,
-           *beta, c, *ldc);
+  else if (*ldc < imax1(*n)) info = 10;
+  if (info) { f77_err(PU "HERK ", info); return; }
+  CB(herk)(CblasColMajor, lsame(uplo, 'U') ? CblasUpper : CblasLower, itr, *n, *k, *alpha, a,
+           *lda, *beta, c, *ldc);
 }
+
+void F77(her2k)(const char *uplo, const char *tr, const int *n, const int *k, const T *alpha,
+                const T *a, const int *lda, const T *b, const int *ldb, const T *beta, T *c,
+                const int *ldc, size_t l1, size_t l2)
+{
+  const int itr = lsame(tr, 'N') ? CblasNoTrans : lsame(tr, 'C') ? CblasConjTrans : 0;
+  const int nra = (itr == CblasNoTrans) ? *n : *k;
+  int info = 0;
+  (void)l1; (void)l2;
+  if (!lsame(uplo, 'L') && !lsame(uplo, 'U')) info = 1;
+  else if (!itr) info = 2;
+  else if (*n < 0) info = 3;
+  else if (*k < 0) info = 4;
+  else if (*lda < imax1(nra)) info = 7;
+  else if (*ldb < imax1(nra)) info = 9;
+  else if (*ldc < imax1(*n)) info = 12;
+  if (info) { f77_err(PU "HER2K", info); return; }
+  CB(her2k)(CblasColMajor, lsame(uplo, 'U') ? CblasUpper : CblasLower, itr, *n, *k, alpha, a,
+            *lda, b, *ldb, *beta, c, *ldc);
+}
+#endif /* F77_COMPLEX: HEMM, HERK, HER2K */
+
+#ifndef F77_COMPLEX
 
 /* idamax.f: 0 when N < 1 or INCX < 1, else the 1-based index */
 int FN(i, P, amax_)(const int *n, const T *x, const int *incx)
@@ -156,7 +238,7 @@ int FN(i, P, amax_)(const int *n, const T *x, const int *incx)
 }
 
 /* ---- LAPACK ---- */
-#endif /* !F77_COMPLEX: TRMM, SYRK, SYR2K, SYMM, I?AMAX */
+#endif /* !F77_COMPLEX: I?AMAX */
 
 void F77(getrf)(const int *m, const int *n, T *a, const int *lda, int *ipiv, int *info)
 {

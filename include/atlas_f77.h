@@ -141,6 +141,52 @@ void cpotrs_(const char *uplo, const int *n, const int *nrhs, const float *a, co
              float *b, const int *ldb, int *info, size_t l1);
 void cposv_(const char *uplo, const int *n, const int *nrhs, float *a, const int *lda,
             float *b, const int *ldb, int *info, size_t l1);
+/* complex TRMM, SYRK, SYR2K, SYMM, HEMM, HERK, HER2K (interfaces/blas/F77/src/
+ * ztrmm.f, zsyrk.f, zsyr2k.f, zsymm.f, zhemm.f, zherk.f, zher2k.f) and the c
+ * twins: ?SYRK/?SYR2K take TRANS N or T, ?HERK/?HER2K N or C; HERK's alpha
+ * and beta and HER2K's beta are real scalars (one double / float). */
+void ztrmm_(const char *side, const char *uplo, const char *ta, const char *diag,
+            const int *m, const int *n, const double *alpha, const double *a, const int *lda,
+            double *b, const int *ldb, size_t l1, size_t l2, size_t l3, size_t l4);
+void zsyrk_(const char *uplo, const char *tr, const int *n, const int *k, const double *alpha,
+            const double *a, const int *lda, const double *beta, double *c, const int *ldc, size_t l1,
+            size_t l2);
+void zsyr2k_(const char *uplo, const char *tr, const int *n, const int *k, const double *alpha,
+             const double *a, const int *lda, const double *b, const int *ldb, const double *beta,
+             double *c, const int *ldc, size_t l1, size_t l2);
+void zsymm_(const char *side, const char *uplo, const int *m, const int *n, const double *alpha,
+            const double *a, const int *lda, const double *b, const int *ldb, const double *beta,
+            double *c, const int *ldc, size_t l1, size_t l2);
+void zhemm_(const char *side, const char *uplo, const int *m, const int *n, const double *alpha,
+            const double *a, const int *lda, const double *b, const int *ldb, const double *beta,
+            double *c, const int *ldc, size_t l1, size_t l2);
+void zherk_(const char *uplo, const char *tr, const int *n, const int *k, const double *alpha,
+            const double *a, const int *lda, const double *beta, double *c, const int *ldc, size_t l1,
+            size_t l2);
+void zher2k_(const char *uplo, const char *tr, const int *n, const int *k, const double *alpha,
+             const double *a, const int *lda, const double *b, const int *ldb, const double *beta,
+             double *c, const int *ldc, size_t l1, size_t l2);
+void ctrmm_(const char *side, const char *uplo, const char *ta, const char *diag,
+            const int *m, const int *n, const float *alpha, const float *a, const int *lda,
+            float *b, const int *ldb, size_t l1, size_t l2, size_t l3, size_t l4);
+void csyrk_(const char *uplo, const char *tr, const int *n, const int *k, const float *alpha,
+            const float *a, const int *lda, const float *beta, float *c, const int *ldc, size_t l1,
+            size_t l2);
+void csyr2k_(const char *uplo, const char *tr, const int *n, const int *k, const float *alpha,
+             const float *a, const int *lda, const float *b, const int *ldb, const float *beta,
+             float *c, const int *ldc, size_t l1, size_t l2);
+void csymm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,
+            const float *a, const int *lda, const float *b, const int *ldb, const float *beta,
+            float *c, const int *ldc, size_t l1, size_t l2);
+void chemm_(const char *side, const char *uplo, const int *m, const int *n, const float *alpha,
+            const float *a, const int *lda, const float *b, const int *ldb, const float *beta,
+            float *c, const int *ldc, size_t l1, size_t l2);
+void cherk_(const char *uplo, const char *tr, const int *n, const int *k, const float *alpha,
+            const float *a, const int *lda, const float *beta, float *c, const int *ldc, size_t l1,
+            size_t l2);
+void cher2k_(const char *uplo, const char *tr, const int *n, const int *k, const float *alpha,
+             const float *a, const int *lda, const float *b, const int *ldb, const float *beta,
+             float *c, const int *ldc, size_t l1, size_t l2);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,16 @@ def test_f77_xerbla_messages_without_gpu():
          "ZGESV  parameter number  2"),
         ("L.cposv_(b'U', i(6), i(1), None, i(6), None, i(5), C.byref(C.c_int(0)), 1)",
          "CPOSV  parameter number  7"),
+        ("L.zherk_(b'U', b'T', i(2), i(2), None, None, i(2), None, None, i(2), 1, 1)",
+         "ZHERK  parameter number  2"),
+        ("L.csyrk_(b'L', b'C', i(2), i(2), None, None, i(2), None, None, i(2), 1, 1)",
+         "CSYRK  parameter number  2"),
+        ("L.zher2k_(b'L', b'N', i(4), i(2), None, None, i(4), None, i(3), None, None, i(4),"
+         " 1, 1)", "ZHER2K parameter number  9"),
+        ("L.chemm_(b'R', b'U', i(3), i(5), None, None, i(4), None, i(3), None, None, i(3),"
+         " 1, 1)", "CHEMM  parameter number  7"),
+        ("L.ztrmm_(b'L', b'U', b'N', b'X', i(2), i(2), None, None, i(2), None, i(2), 1, 1, 1, 1)",
+         "ZTRMM  parameter number  4"),
     ]
     for call, msg in cases:
         code = ("import ctypes as C, sys; sys.path.insert(0, %r); import atlas_amd as a;"

@@ -218,3 +218,68 @@ def test_complex_f77_gemm_trsm_lu_cholesky(dt):
         fn("potrs")(_c(uplo), _i(n), _i(nrhs), _p(H2), _i(n), _p(By), _i(n), C.byref(info),
                     C.c_size_t(1))
         assert np.max(np.abs(By - Bx)) <= tol * max(1.0, np.max(np.abs(Bx)))
+
+
+@pytest.mark.parametrize("dt", [np.complex128, np.complex64])
+def test_complex_f77_rank_k_hemm_symm_trmm(dt):
+    """ztrmm_, zsyrk_, zsyr2k_, zsymm_, zhemm_, zherk_, zher2k_ (and c)
+    against cblas_z* through the C ABI: the F77 wrappers map their CHARACTER
+    arguments and scalars onto the same calls, so the results agree bit for
+    bit; HERK's alpha/beta and HER2K's beta go by reference as one real."""
+    p, rt = X._cpre(dt)
+    L = X.lib()
+    fn = lambda name: getattr(L, p + name + "_")  # noqa: E731
+    cb = lambda name: getattr(L, "cblas_" + p + name)  # noqa: E731
+    cz = lambda shape: _f((RNG.uniform(-0.5, 0.5, shape) +  # noqa: E731
+                           1j * RNG.uniform(-0.5, 0.5, shape)).astype(dt))
+    rs = lambda v: C.byref(C.c_double(v) if dt == np.complex128 else C.c_float(v))  # noqa: E731
+    n, K, M = 60, 37, 45
+    al, be = _z(0.8 - 0.3j, dt), _z(-0.4 + 0.2j, dt)
+    s1 = C.c_size_t(1)
+    for uplo, tr, herm in [("U", "N", 1), ("L", "C", 1), ("L", "T", 0), ("U", "N", 0)]:
+        A = cz((n, K) if tr == "N" else (K, n))
+        B = cz(A.shape)
+        lda = A.shape[0]
+        U = X.Upper if uplo == "U" else X.Lower
+        T = {"N": X.NoTrans, "T": X.Trans, "C": X.ConjTrans}[tr]
+        C0 = cz((n, n))
+        g1, g2 = C0.copy(order="F"), C0.copy(order="F")
+        if herm:
+            fn("herk")(_c(uplo), _c(tr), _i(n), _i(K), rs(0.7), _p(A), _i(lda), rs(-0.5), _p(g1),
+                       _i(n), s1, s1)
+            cb("herk")(X.ColMajor, U, T, n, K, rt(0.7), _p(A), lda, rt(-0.5), _p(g2), n)
+        else:
+            fn("syrk")(_c(uplo), _c(tr), _i(n), _i(K), _p(al), _p(A), _i(lda), _p(be), _p(g1),
+                       _i(n), s1, s1)
+            cb("syrk")(X.ColMajor, U, T, n, K, _p(al), _p(A), lda, _p(be), _p(g2), n)
+        assert np.array_equal(g1, g2), ("rk", uplo, tr, herm)
+        g1, g2 = C0.copy(order="F"), C0.copy(order="F")
+        if herm:
+            fn("her2k")(_c(uplo), _c(tr), _i(n), _i(K), _p(al), _p(A), _i(lda), _p(B), _i(lda),
+                        rs(0.25), _p(g1), _i(n), s1, s1)
+            cb("her2k")(X.ColMajor, U, T, n, K, _p(al), _p(A), lda, _p(B), lda, rt(0.25), _p(g2),
+                        n)
+        else:
+            fn("syr2k")(_c(uplo), _c(tr), _i(n), _i(K), _p(al), _p(A), _i(lda), _p(B), _i(lda),
+                        _p(be), _p(g1), _i(n), s1, s1)
+            cb("syr2k")(X.ColMajor, U, T, n, K, _p(al), _p(A), lda, _p(B), lda, _p(be), _p(g2), n)
+        assert np.array_equal(g1, g2), ("r2k", uplo, tr, herm)
+    for side, uplo in [("L", "U"), ("R", "L")]:
+        na = M if side == "L" else n
+        A, B, C0 = cz((na, na)), cz((M, n)), cz((M, n))
+        S = X.Left if side == "L" else X.Right
+        U = X.Upper if uplo == "U" else X.Lower
+        for name in ("hemm", "symm"):
+            g1, g2 = C0.copy(order="F"), C0.copy(order="F")
+            fn(name)(_c(side), _c(uplo), _i(M), _i(n), _p(al), _p(A), _i(na), _p(B), _i(M),
+                     _p(be), _p(g1), _i(M), s1, s1)
+            cb(name)(X.ColMajor, S, U, M, n, _p(al), _p(A), na, _p(B), M, _p(be), _p(g2), M)
+            assert np.array_equal(g1, g2), (name, side, uplo)
+        for ta, dg in [("N", "N"), ("C", "U"), ("T", "N")]:
+            g1, g2 = B.copy(order="F"), B.copy(order="F")
+            fn("trmm")(_c(side), _c(uplo), _c(ta), _c(dg), _i(M), _i(n), _p(al), _p(A), _i(na),
+                       _p(g1), _i(M), s1, s1, s1, s1)
+            cb("trmm")(X.ColMajor, S, U, {"N": X.NoTrans, "T": X.Trans, "C": X.ConjTrans}[ta],
+                       X.Unit if dg == "U" else X.NonUnit, M, n, _p(al), _p(A), na, _p(g2), M)
+            assert np.array_equal(g1, g2), ("trmm", side, uplo, ta, dg)
+            assert np.all(np.isfinite(g1))
