@@ -158,9 +158,14 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
 
   if (g.abort_flag && *g.abort_flag) return;
 
-  /* ---- tile scheduling: XCD-compact numbering, then grouped raster ---- */
+  /* ---- tile scheduling: XCD-compact numbering, then grouped raster ----
+   * Workgroups go to the XCDs round-robin, so the compact numbering gives
+   * each XCD one contiguous chunk of tiles.  Under a triangle mask that
+   * would leave the chunks of one side all live and those of the other all
+   * skipped; there the tiles are dealt round-robin instead (tile = bid),
+   * which spreads the live ones evenly over the XCDs. */
   const int bid = blockIdx.x;
-  const int tile = (bid & 7) * g.per_xcd + (bid >> 3);
+  const int tile = g.tri ? bid : (bid & 7) * g.per_xcd + (bid >> 3);
   const int ntiles = g.tiles_m * g.tiles_n;
   if (tile >= ntiles) return;
   const int per_group = GROUP_M * g.tiles_n;
@@ -392,7 +397,9 @@ __global__ __launch_bounds__(NT, 1) void update_d_kernel(GemmArgs<double> g) {
   typedef SliceLoader<double, B_KCONT, BN, BK> LB;
   if (g.abort_flag && *g.abort_flag) return;
   const int bid = blockIdx.x;
-  const int tile = (bid & 7) * g.per_xcd + (bid >> 3);
+  /* masked updates deal their tiles round-robin over the XCDs, as in
+   * gemm_kernel */
+  const int tile = g.tri ? bid : (bid & 7) * g.per_xcd + (bid >> 3);
   const int ntiles = g.tiles_m * g.tiles_n;
   if (tile >= ntiles) return;
   const int per_group = GROUP_M * g.tiles_n;

@@ -330,7 +330,7 @@ def bench_complex(n, seed, torch, X, log, reps=3):
     (NoTrans x ConjTrans, alpha = 1 - 0.5i, beta = 0.25i; 8 n^3 flops),
     zgetrf (A ~ U[-1,1) + i U[-1,1); 8/3 n^3 flops) with zgetrs on 64
     right-hand sides, and zpotrf (Lower, A = B B^H / n + I; 4/3 n^3 flops)
-    with zpotrs on 64.  Best of reps."""
+    with zpotrs on 64, then zherk, zher2k, zhemm and ztrmm.  Best of reps."""
     import numpy as np
     L = X.lib()
     g = torch.Generator(device="cuda")
@@ -393,6 +393,29 @@ def bench_complex(n, seed, torch, X, log, reps=3):
                                       Xw.data_ptr(), n),
              lambda: Xw.copy_(rhs))
     out["zpotrs_64"] = {"ms": round(t * 1e3, 2), "tflops": round(8.0 * n * n * 64 / t / 1e12, 3)}
+    # the other complex Level-3 routines on the same operands: zherk
+    # (Lower, NoTrans, K = n: 4 n^3 flops), zher2k (8 n^3), zhemm (Left,
+    # Lower: 8 n^3), ztrmm (Left, Lower, NoTrans, NonUnit: 4 n^3)
+    W = torch.empty_like(S0)
+    al = np.array([0.5, -0.25])
+    be = np.array([0.25, 0.0])
+    runs = [
+        ("zherk", 4.0, lambda: L.cblas_zherk(X.ColMajor, X.Lower, X.NoTrans, n, n, 0.5,
+                                             S0.data_ptr(), n, 0.25, W.data_ptr(), n)),
+        ("zher2k", 8.0, lambda: L.cblas_zher2k(X.ColMajor, X.Lower, X.NoTrans, n, n,
+                                               al.ctypes.data, S0.data_ptr(), n, Aw.data_ptr(), n,
+                                               0.25, W.data_ptr(), n)),
+        ("zhemm", 8.0, lambda: L.cblas_zhemm(X.ColMajor, X.Left, X.Lower, n, n, al.ctypes.data,
+                                             S0.data_ptr(), n, Aw.data_ptr(), n, be.ctypes.data,
+                                             W.data_ptr(), n)),
+        ("ztrmm", 4.0, lambda: L.cblas_ztrmm(X.ColMajor, X.Left, X.Lower, X.NoTrans, X.NonUnit,
+                                             n, n, al.ctypes.data, Aw.data_ptr(), n,
+                                             W.data_ptr(), n)),
+    ]
+    for name, c3, fn in runs:
+        t = best(fn, lambda: W.copy_(S0))
+        out[name] = {"ms": round(t * 1e3, 2), "tflops": round(c3 * n ** 3 / t / 1e12, 2),
+                     "frac_of_real_mfma_peak": round(c3 * n ** 3 / t / PEAK_FP64_MFMA, 3)}
     log("complex n=%d: %s" % (n, json.dumps(out)))
     return out
 
