@@ -34,7 +34,9 @@ static void NM(PFX, scal_dev)(atlmx_ctx *c, int m, int n, const T *beta, T *C, l
  * is gathered into a K x N complex workspace.  The expanded left operand
  * (4*rows*K reals) is built and used in row chunks of at most
  * ATLMX_CPLX_WS_BYTES.  tri = KX_LOWER / KX_UPPER updates only C's lower /
- * upper triangle, diagonal included (the HERK shape; one chunk then).
+ * upper triangle, diagonal included (the HERK shape; one chunk then);
+ * tri = KX_KBAND_* (TRMM: op(A) or op(B) triangular with zeros stored)
+ * skips the K slices that meet only the triangle's zeros (one chunk).
  * ---------------------------------------------------------------------- */
 void NM(PFX, gemm_dev)(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const T *alpha,
                        const T *A, long lda, const T *B, long ldb, const T *beta, T *C,
@@ -79,7 +81,7 @@ void NM(PFX, gemm_dev)(atlmx_ctx *c, int ta, int tb, int M, int N, int K, const 
                                 (double)mi * N * (breal != (T)0 ? 2 : 1)) * sizeof(T));
     RT_CHECK(rout, K_GEMM(KX_NOTRANS, KX_NOTRANS, 2 * mi, N, 2 * K, (T)1, H, ldh, Bv, ldbv,
                           breal, C + 2 * (size_t)i0, 2 * ldc,
-                          tri == KX_LOWER ? KX_CLOWER : (tri == KX_UPPER ? KX_CUPPER : 0),
+                          tri == KX_LOWER ? KX_CLOWER : (tri == KX_UPPER ? KX_CUPPER : tri),
                           NULL, c->stream));
     rt_prof_end(c, pid);
   }
@@ -296,8 +298,9 @@ void NM3(cblas_, PFX, trsm)(const enum CBLAS_ORDER Order, const enum CBLAS_SIDE 
  *     ATL_zrefherk.c / _zrefsyrk.c / _zrefher2k.c / _zrefsyr2k.c define
  *     them; the Hermitian forms keep the diagonal real;
  *   HEMM / SYMM: A's triangle expanded to the full matrix, then one GEMM;
- *   TRMM: op(A)'s triangle expanded (zeros outside, ones on a unit
- *     diagonal) and B copied aside, then one GEMM into B.
+ *   TRMM: op(A)'s triangle gathered (zeros outside, ones on a unit
+ *     diagonal) and B copied aside, then one GEMM into B whose tiles skip
+ *     the K slices that meet only the triangle's zeros (about half).
  * ---------------------------------------------------------------------- */
 /* herm: Hermitian forms (beta's imaginary part is ignored, conj(alpha) on
  * the second product of the 2k form); B NULL: rank-k.  trans: KX_NOTRANS,
@@ -372,11 +375,11 @@ static void NM(PFX, trmm_dev)(atlmx_ctx *c, int side, int uplo, int ta, int diag
                            lower_op ? KX_LOWER : KX_UPPER, diag == KX_UNIT, c->stream));
   RT_CHECK(rout, K_CGATHER(M, N, B, ldb, 0, 0, (T)1, (T)0, 0, Bc, M, 0, 0, c->stream));
   if (side == KX_LEFT)
-    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, M, alpha, Td, na, Bc, M, zero, B, ldb, 0,
-                      rout);
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, M, alpha, Td, na, Bc, M, zero, B, ldb,
+                      lower_op ? KX_KBAND_AL : KX_KBAND_AU, rout);
   else
-    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, N, alpha, Bc, M, Td, na, zero, B, ldb, 0,
-                      rout);
+    NM(PFX, gemm_dev)(c, KX_NOTRANS, KX_NOTRANS, M, N, N, alpha, Bc, M, Td, na, zero, B, ldb,
+                      lower_op ? KX_KBAND_BL : KX_KBAND_BU, rout);
 }
 
 /* the rank-k / rank-2k C ABI: RowMajor holds C^T, so Uplo and Trans flip

@@ -147,7 +147,9 @@ struct SliceLoader {
  * one workgroup's prologue, barriers and C epilogue overlap the other's
  * MFMA stream, which matters most for the short-K trailing updates of the
  * LU and Cholesky. */
-template <typename T, int TA, int TB>
+/* KBAND: the K-band variant (tri = KX_KBAND_*), a separate instance so
+ * that the other launches keep the plain main loop's code */
+template <typename T, int TA, int TB, bool KBAND = false>
 __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
   typedef Mfma<T> MF;
   typedef typename MF::acc_t acc_t;
@@ -201,11 +203,24 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
 
   LA la;
   LB lb;
-  const int nk = (g.K + BK - 1) / BK;
+  /* the K slices this tile reads: all, or under a K band only those that
+   * meet the triangle's nonzeros.  AL / AU: A's real row r and column k
+   * are complex row r >> 1 and column k >> 1, so rows [m0, m0 + BM) (m0
+   * even) need k < m0 + BM (lower) or k >= m0 (upper); BL / BU: B's real
+   * row k is complex row k >> 1, so columns [n0, n0 + BN) need k >= 2 n0
+   * (lower) or k < 2 (n0 + BN) (upper).  The skipped slices hold zeros. */
+  int kb = 0, ke = g.K;
+  if constexpr (KBAND) {
+    if (g.tri == KX_KBAND_AL) ke = min(ke, m0 + BM);
+    if (g.tri == KX_KBAND_AU) kb = m0;
+    if (g.tri == KX_KBAND_BL) kb = 2 * n0;
+    if (g.tri == KX_KBAND_BU) ke = min(ke, 2 * (n0 + BN));
+  }
+  const int kt0 = kb / BK, kt1 = (ke + BK - 1) / BK;
 
-  if (nk > 0) {
-    la.load(g.A, g.lda, m0, 0, g.M, g.K, g.vec_a, tid);
-    lb.load(g.B, g.ldb, n0, 0, g.N, g.K, g.vec_b, tid);
+  if (kt0 < kt1) {
+    la.load(g.A, g.lda, m0, kt0 * BK, g.M, g.K, g.vec_a, tid);
+    lb.load(g.B, g.ldb, n0, kt0 * BK, g.N, g.K, g.vec_b, tid);
     la.store(As[0], tid);
     lb.store(Bs[0], tid);
   }
@@ -217,9 +232,9 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
    * elements run down a column of C (coalesced stores). */
   const int kq = (lane >> 4) * 4;
   const int lr = lane & 15;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int cur = kt & 1;
-    if (kt + 1 < nk) {
+  for (int kt = kt0; kt < kt1; ++kt) {
+    const int cur = (kt - kt0) & 1;
+    if (kt + 1 < kt1) {
       la.load(g.A, g.lda, m0, (kt + 1) * BK, g.M, g.K, g.vec_a, tid);
       lb.load(g.B, g.ldb, n0, (kt + 1) * BK, g.N, g.K, g.vec_b, tid);
     }
@@ -239,7 +254,7 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
           for (int j = 0; j < 4; ++j) acc[i][j] = MF::mma(b[j], a[i], acc[i][j]);
       }
     }
-    if (kt + 1 < nk) {
+    if (kt + 1 < kt1) {
       la.store(As[cur ^ 1], tid);
       lb.store(Bs[cur ^ 1], tid);
     }
@@ -651,7 +666,10 @@ hipError_t launch_gemm(int ta, int tb, int M, int N, int K, T alpha, const T *A,
   g.vec_a = (((uintptr_t)A % 16) == 0) && (lda % VEC == 0);
   g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % VEC == 0);
   const dim3 grid((unsigned)(8 * g.per_xcd)), block(NT);
-  if (ta == KX_NOTRANS && tb == KX_NOTRANS)
+  if (tri >= KX_KBAND_AL && tri <= KX_KBAND_BU) {
+    if (ta != KX_NOTRANS || tb != KX_NOTRANS) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_NOTRANS, true>), grid, block, 0, s, g);
+  } else if (ta == KX_NOTRANS && tb == KX_NOTRANS)
     hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_NOTRANS>), grid, block, 0, s, g);
   else if (ta == KX_NOTRANS)
     hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_TRANS>), grid, block, 0, s, g);

@@ -26,12 +26,21 @@ extern "C" {
 #define KX_RIGHT   142
 #define KX_CUPPER  1121
 #define KX_CLOWER  1122
+/* K bands of a triangular operand (complex TRMM on the expanded product,
+ * gemm_kernel only): the K slices that meet only the triangle's zeros are
+ * skipped, C is whole.  AL / AU: A is the 2x2-block expansion of a complex
+ * lower / upper triangle; BL / BU: B is the interleaved real view of one. */
+#define KX_KBAND_AL 1131
+#define KX_KBAND_AU 1132
+#define KX_KBAND_BL 1133
+#define KX_KBAND_BU 1134
 
 /* C = alpha*op(A)*op(B) + beta*C on the MFMA GEMM kernel.
  * tri: 0 = whole C; KX_LOWER / KX_UPPER = only elements on or below / above
  * the diagonal of C are read and written (the SYRK shape); KX_CLOWER /
  * KX_CUPPER = the same for C the interleaved real view of a complex matrix
- * (real row r is complex row r / 2: the HERK shape; gemm_kernel only).
+ * (real row r is complex row r / 2: the HERK shape; gemm_kernel only);
+ * KX_KBAND_* = whole C, a triangular operand's zero K slices skipped.
  * abort_flag: NULL, or a device int; the kernel does nothing when it is
  * nonzero (lets a stopped Cholesky skip the rest of its queued work). */
 hipError_t atlmx_k_dgemm(int ta, int tb, int M, int N, int K, double alpha,
