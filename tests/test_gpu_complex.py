@@ -577,8 +577,11 @@ def test_chemm_vs_closed_form(dt, herm, M, N):
 
 
 @pytest.mark.parametrize("dt", [np.complex128, np.complex64])
-@pytest.mark.parametrize("M,N", [(1, 1), (45, 70), (257, 33)])
+@pytest.mark.parametrize("M,N", [(1, 1), (45, 70), (257, 33), (40, 300)])
 def test_ctrmm_vs_closed_form(dt, M, N):
+    """every side/uplo/trans/diag; M = 257 (Left) and N = 300 (Right) span
+    several 128-row / 128-column GEMM tiles, so the K band of each tile
+    position is exercised"""
     p, rt = X._cpre(dt)
     f = getattr(X.lib(), "cblas_%strmm" % p)
     eps = _eps(dt)
