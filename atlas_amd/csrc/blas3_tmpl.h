@@ -197,13 +197,14 @@ void SYRK_DEV(atlmx_ctx *c, int uplo, int trans, int N, int K, T alpha, const T 
  * place.  Replaces src/blas/ulevel3/ATL_trmm.c:149 (recursive halving on
  * GEMM).  Here A's triangle is expanded into a dense workspace (zeros
  * outside, ones on a unit diagonal) and B copied aside, so the product is
- * one launch of the MFMA GEMM over the whole of B (twice the triangle's
- * flops, at the GEMM's rate).
+ * one launch of the MFMA GEMM over the whole of B whose tiles read only the
+ * K slices that meet the triangle's nonzeros (the K-band instance).
  * ---------------------------------------------------------------------- */
 void NM(PFX, trmm_dev)(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M, int N,
                        T alpha, const T *A, long lda, T *B, long ldb, const char *rout)
 {
   const int na = (side == KX_LEFT) ? M : N;
+  int lower_op;
   T *Tm, *W;
   if (M <= 0 || N <= 0) return;
   if (alpha == (T)0) {                  /* ATL_dreftrmm.c:182: B = 0 */
@@ -215,10 +216,15 @@ void NM(PFX, trmm_dev)(atlmx_ctx *c, int side, int uplo, int ta, int diag, int M
   RT_CHECK(rout, K_TRICOPY(ATLMX_TC_EXPAND_TRI, uplo, diag, na, na, (T *)A, lda, Tm, na,
                            c->stream));
   RT_CHECK(rout, K_TRICOPY(ATLMX_TC_COPY, 0, 0, M, N, W, M, B, ldb, c->stream));
+  /* op(Tm) is lower when exactly one of "lower" and "transposed" holds;
+   * the GEMM's tiles skip the K slices that meet only its zeros */
+  lower_op = ((uplo == KX_LOWER) != (ta != KX_NOTRANS));
   if (side == KX_LEFT)
-    GEMM_DEV(c, ta, KX_NOTRANS, M, N, M, alpha, Tm, na, W, M, (T)0, B, ldb, 0, NULL, rout);
+    GEMM_DEV(c, ta, KX_NOTRANS, M, N, M, alpha, Tm, na, W, M, (T)0, B, ldb,
+             lower_op ? KX_KBAND_AL : KX_KBAND_AU, NULL, rout);
   else
-    GEMM_DEV(c, KX_NOTRANS, ta, M, N, N, alpha, W, M, Tm, na, (T)0, B, ldb, 0, NULL, rout);
+    GEMM_DEV(c, KX_NOTRANS, ta, M, N, N, alpha, W, M, Tm, na, (T)0, B, ldb,
+             lower_op ? KX_KBAND_RBL : KX_KBAND_RBU, NULL, rout);
 }
 
 /* SYMM: C = alpha*A*B + beta*C (Left) or alpha*B*A + beta*C (Right), A

@@ -205,16 +205,20 @@ __global__ __launch_bounds__(NT, 2) void gemm_kernel(GemmArgs<T> g) {
   LB lb;
   /* the K slices this tile reads: all, or under a K band only those that
    * meet the triangle's nonzeros.  AL / AU: A's real row r and column k
-   * are complex row r >> 1 and column k >> 1, so rows [m0, m0 + BM) (m0
-   * even) need k < m0 + BM (lower) or k >= m0 (upper); BL / BU: B's real
-   * row k is complex row k >> 1, so columns [n0, n0 + BN) need k >= 2 n0
-   * (lower) or k < 2 (n0 + BN) (upper).  The skipped slices hold zeros. */
+   * are complex row r >> 1 and column k >> 1 (or real ones), so rows
+   * [m0, m0 + BM) (m0 even) need k < m0 + BM (lower) or k >= m0 (upper);
+   * BL / BU: B's real row k is complex row k >> 1, so columns [n0, n0 + BN)
+   * need k >= 2 n0 (lower) or k < 2 (n0 + BN) (upper); RBL / RBU: the same
+   * for a real op(B), k >= n0 or k < n0 + BN.  The skipped slices hold
+   * zeros. */
   int kb = 0, ke = g.K;
   if constexpr (KBAND) {
     if (g.tri == KX_KBAND_AL) ke = min(ke, m0 + BM);
     if (g.tri == KX_KBAND_AU) kb = m0;
     if (g.tri == KX_KBAND_BL) kb = 2 * n0;
     if (g.tri == KX_KBAND_BU) ke = min(ke, 2 * (n0 + BN));
+    if (g.tri == KX_KBAND_RBL) kb = n0;
+    if (g.tri == KX_KBAND_RBU) ke = min(ke, n0 + BN);
   }
   const int kt0 = kb / BK, kt1 = (ke + BK - 1) / BK;
 
@@ -666,9 +670,15 @@ hipError_t launch_gemm(int ta, int tb, int M, int N, int K, T alpha, const T *A,
   g.vec_a = (((uintptr_t)A % 16) == 0) && (lda % VEC == 0);
   g.vec_b = (((uintptr_t)B % 16) == 0) && (ldb % VEC == 0);
   const dim3 grid((unsigned)(8 * g.per_xcd)), block(NT);
-  if (tri >= KX_KBAND_AL && tri <= KX_KBAND_BU) {
-    if (ta != KX_NOTRANS || tb != KX_NOTRANS) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_NOTRANS, true>), grid, block, 0, s, g);
+  if (tri >= KX_KBAND_AL && tri <= KX_KBAND_RBU) {
+    if (ta == KX_NOTRANS && tb == KX_NOTRANS)
+      hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_NOTRANS, true>), grid, block, 0, s, g);
+    else if (ta == KX_NOTRANS)
+      hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_TRANS, true>), grid, block, 0, s, g);
+    else if (tb == KX_NOTRANS)
+      hipLaunchKernelGGL((gemm_kernel<T, KX_TRANS, KX_NOTRANS, true>), grid, block, 0, s, g);
+    else
+      hipLaunchKernelGGL((gemm_kernel<T, KX_TRANS, KX_TRANS, true>), grid, block, 0, s, g);
   } else if (ta == KX_NOTRANS && tb == KX_NOTRANS)
     hipLaunchKernelGGL((gemm_kernel<T, KX_NOTRANS, KX_NOTRANS>), grid, block, 0, s, g);
   else if (ta == KX_NOTRANS)

@@ -26,14 +26,17 @@ extern "C" {
 #define KX_RIGHT   142
 #define KX_CUPPER  1121
 #define KX_CLOWER  1122
-/* K bands of a triangular operand (complex TRMM on the expanded product,
- * gemm_kernel only): the K slices that meet only the triangle's zeros are
- * skipped, C is whole.  AL / AU: A is the 2x2-block expansion of a complex
- * lower / upper triangle; BL / BU: B is the interleaved real view of one. */
+/* K bands of a triangular operand (TRMM, gemm_kernel only): the K slices
+ * that meet only the triangle's zeros are skipped, C is whole.  AL / AU:
+ * op(A) is a real lower / upper triangle or the 2x2-block expansion of a
+ * complex one; BL / BU: op(B) is the interleaved real view of a complex
+ * lower / upper triangle; RBL / RBU: op(B) is a real lower / upper one. */
 #define KX_KBAND_AL 1131
 #define KX_KBAND_AU 1132
 #define KX_KBAND_BL 1133
 #define KX_KBAND_BU 1134
+#define KX_KBAND_RBL 1135
+#define KX_KBAND_RBU 1136
 
 /* C = alpha*op(A)*op(B) + beta*C on the MFMA GEMM kernel.
  * tri: 0 = whole C; KX_LOWER / KX_UPPER = only elements on or below / above

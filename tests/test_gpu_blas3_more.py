@@ -27,8 +27,10 @@ def _gen(shape, dt):
 @pytest.mark.parametrize("dt", [np.float64, np.float32])
 @pytest.mark.parametrize("side,uplo,ta,diag", list(itertools.product(
     [X.Left, X.Right], [X.Upper, X.Lower], [X.NoTrans, X.Trans], [X.NonUnit, X.Unit])))
-def test_trmm_all_variants_vs_oracle(dt, side, uplo, ta, diag):
-    M, N = 190, 77
+@pytest.mark.parametrize("M,N", [(190, 77), (60, 300)])
+def test_trmm_all_variants_vs_oracle(dt, side, uplo, ta, diag, M, N):
+    """M = 190 (Left) and N = 300 (Right) span several 128-wide GEMM tiles,
+    so each tile position's K band is exercised"""
     na = M if side == X.Left else N
     A = _gen((na, na), dt)
     B0 = _gen((M, N), dt)
